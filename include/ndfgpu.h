@@ -1,0 +1,137 @@
+/*
+ * ndfgpu — MI355X (gfx950) NaN-aware window selection: median and rank
+ * filters over size/footprint/origin windows with reflect/constant/nearest/
+ * mirror/wrap boundaries, float32 and float64, bit-exact with ndfiltrs' CPU
+ * semantics.
+ *
+ * C ABI: plain pointers and sizes only. Host pointers are caller-owned and
+ * never retained after a call returns. Error messages (err, err_len) are
+ * NUL-terminated and byte-identical to the reference's texts where the
+ * reference has one.
+ */
+#ifndef NDFGPU_H
+#define NDFGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. 1..5 mirror ndfiltrs' Error enum (src/error.rs:6-18) and its
+ * Python mapping (src/pyapi/filter.rs:37-45): MEMORY -> MemoryError,
+ * OVERFLOW -> OverflowError, the rest -> ValueError. DEVICE has no reference
+ * counterpart (a HIP runtime failure -> RuntimeError). */
+typedef int32_t ndf_status;
+enum {
+    NDF_OK = 0,
+    NDF_ERR_EMPTY_AXIS = 1,      /* "cannot map an empty axis" */
+    NDF_ERR_INVALID = 2,         /* InvalidConfiguration(message) */
+    NDF_ERR_LENGTH_MISMATCH = 3, /* "input length does not match the configured shape" */
+    NDF_ERR_MEMORY = 4,          /* "unable to allocate filtering workspace" */
+    NDF_ERR_OVERFLOW = 5,        /* "shape or index arithmetic overflowed" */
+    NDF_ERR_DEVICE = 6,          /* HIP runtime error */
+    NDF_ERR_KIND = 7,            /* "unsupported window filter kind" */
+    NDF_ERR_MODE = 8             /* "unsupported boundary mode" */
+};
+
+/* Boundary modes, in the order of BoundaryMode (src/boundary.rs:16-27). */
+enum { NDF_MODE_REFLECT = 0, NDF_MODE_CONSTANT = 1, NDF_MODE_NEAREST = 2, NDF_MODE_MIRROR = 3, NDF_MODE_WRAP = 4 };
+
+/* Window-filter kinds on this path (src/pyapi/filter.rs:289-301). All three
+ * median kinds and both rank kinds give identical bits; the histogram kinds
+ * keep their extra validation (rectangular only, support overflow). */
+enum {
+    NDF_KIND_MEDIAN = 0,
+    NDF_KIND_MEDIAN_NETWORK = 1,
+    NDF_KIND_MEDIAN_HISTOGRAM = 2,
+    NDF_KIND_RANK = 3,
+    NDF_KIND_RANK_HISTOGRAM = 4
+};
+
+/* Device engines. AUTO picks by geometry; the others force one engine and
+ * exist so tests can check engines against each other. */
+enum { NDF_ENGINE_AUTO = 0, NDF_ENGINE_GENERIC = 1, NDF_ENGINE_SLIDE = 2 };
+
+/* One-shot filter over host buffers.
+ * Replaces _core.window_filter(array, kind, size, footprint, origin, mode,
+ * cval, omit_nan, ignore_inf, min_count, parameter)
+ * (src/pyapi/filter.rs:265-314, python/ndfiltrs/_core.pyi:34-46).
+ * dtype_bytes: 4 (float32) or 8 (float64). shape/size/origin/modes have
+ * `rank` entries; footprint is NULL (rectangle of `size`) or prod(size)
+ * bytes, nonzero = selected, C order. `cval` is cast to the dtype.
+ * omit_nan = 0 is InvalidPolicy::Propagate; otherwise Omit{ignore_inf,
+ * min_count} (src/reduction.rs:6-18). `parameter` is the zero-based rank
+ * for the rank kinds. */
+ndf_status ndf_window_filter(int32_t dtype_bytes, const void *input, void *output, size_t rank,
+                             const size_t *shape, const size_t *size, const uint8_t *footprint,
+                             const ptrdiff_t *origin, const int32_t *modes, double cval, int32_t kind,
+                             int32_t omit_nan, int32_t ignore_inf, size_t min_count, size_t parameter,
+                             char *err, size_t err_len);
+
+/* Reusable plan for one shape: replaces WindowFilter::<T>::new
+ * (src/plan.rs:260-324). Validation order and messages follow
+ * Window::new (src/window.rs:44-137), FilterConfig::from_window
+ * (src/filter_nd.rs:32-55), validate_rectangular (src/plan.rs:154-197) and
+ * validate_support (src/histogram.rs:74-88). */
+typedef struct ndf_plan ndf_plan;
+
+ndf_status ndf_plan_create(ndf_plan **plan, int32_t dtype_bytes, size_t rank, const size_t *shape,
+                           const size_t *size, const uint8_t *footprint, const ptrdiff_t *origin,
+                           const int32_t *modes, double cval, int32_t kind, int32_t omit_nan,
+                           int32_t ignore_inf, size_t min_count, size_t parameter, char *err,
+                           size_t err_len);
+
+/* Filter host buffers of `len` elements each: replaces WindowFilter::apply_into
+ * (src/plan.rs:367-427); LENGTH_MISMATCH when len differs from prod(shape).
+ * Copies to the device, runs, copies back; blocking. */
+ndf_status ndf_plan_apply(ndf_plan *plan, const void *input, void *output, size_t len, char *err,
+                          size_t err_len);
+
+/* ndf_plan_apply in two halves, so a caller can copy (snapshot) the input
+ * while it holds its own lock and release that lock for the compute, as the
+ * reference does with the GIL (src/pyapi/filter.rs:219-242).
+ * ndf_plan_upload copies `len` host elements to the plan's device buffer;
+ * ndf_plan_compute_download filters that buffer and copies the result into
+ * `output`. */
+ndf_status ndf_plan_upload(ndf_plan *plan, const void *input, size_t len, char *err, size_t err_len);
+ndf_status ndf_plan_compute_download(ndf_plan *plan, void *output, size_t len, char *err, size_t err_len);
+
+/* Filter device-resident buffers on `stream` (a hipStream_t, NULL = default
+ * stream). Asynchronous: returns after the kernels are enqueued. No
+ * reference counterpart (the reference has no device). */
+ndf_status ndf_plan_apply_device(ndf_plan *plan, const void *d_input, void *d_output, size_t len,
+                                 void *stream, char *err, size_t err_len);
+
+/* Force an engine (NDF_ENGINE_*); returns INVALID when the engine cannot run
+ * this geometry. */
+ndf_status ndf_plan_set_engine(ndf_plan *plan, int32_t engine, char *err, size_t err_len);
+
+/* Writes a short JSON description of the chosen engine and tile shape. */
+ndf_status ndf_plan_describe(const ndf_plan *plan, char *buffer, size_t buffer_len);
+
+void ndf_plan_destroy(ndf_plan *plan);
+
+/* Replaces get_num_threads (src/pyapi/filter.rs:377-380): the host thread
+ * count a CPU caller would use (RAYON_NUM_THREADS, else the CPU count). */
+size_t ndf_get_num_threads(void);
+
+/* Device plumbing for benchmarks and tests (no reference counterpart). */
+ndf_status ndf_device_count(int32_t *count);
+ndf_status ndf_device_malloc(void **ptr, size_t bytes);
+ndf_status ndf_device_free(void *ptr);
+ndf_status ndf_memcpy_to_device(void *d_dst, const void *h_src, size_t bytes);
+ndf_status ndf_memcpy_to_host(void *h_dst, const void *d_src, size_t bytes);
+ndf_status ndf_device_synchronize(void);
+/* Times `iterations` launches of ndf_plan_apply_device with HIP events on
+ * one stream; writes the mean milliseconds per launch. */
+ndf_status ndf_plan_time_device(ndf_plan *plan, const void *d_input, void *d_output, size_t len,
+                                int32_t warmup, int32_t iterations, double *ms_per_launch, char *err,
+                                size_t err_len);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,296 @@
+"""Native layer: the MI355X counterpart of ndfiltrs' private ``_core`` module.
+
+``window_filter`` keeps the reference signature
+(``src/pyapi/filter.rs:265-314``, ``python/ndfiltrs/_core.pyi:34-46``) and its
+error behaviour, and runs on the GPU through ``libndfgpu.so``. The input is
+snapshotted while the GIL is held (``src/pyapi/filter.rs:219-229``); the GIL is
+released while the device computes.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from collections.abc import Sequence
+
+import numpy as np
+
+from . import _lib
+
+_MSG_HIST_FOOTPRINT = "histogram reductions require rectangular size, not footprint"
+_MSG_RANK0 = "filtering requires at least one dimension"
+_MSG_EQUAL_RANK = "array, window, origin, and mode must have equal rank"
+_MSG_SIZE0 = "window sizes must be positive"
+_MSG_LENGTH = "input length does not match the configured shape"
+_MSG_OVERFLOW = "shape or index arithmetic overflowed"
+
+
+def _raise(status: int, message: str) -> None:
+    """Map a status to the exception class the reference raises
+    (``src/pyapi/filter.rs:37-45``)."""
+    if status == _lib.ERR_MEMORY:
+        raise MemoryError(message)
+    if status == _lib.ERR_OVERFLOW:
+        raise OverflowError(message)
+    if status == _lib.ERR_DEVICE:
+        raise RuntimeError(message)
+    raise ValueError(message)
+
+
+def _as_sizes(values: Sequence[int], name: str) -> list[int]:
+    out = []
+    for value in values:
+        value = int(value)
+        if value < 0:
+            raise OverflowError(f"can't convert negative int to unsigned ({name})")
+        out.append(value)
+    return out
+
+
+def _typed_array(array: object) -> np.ndarray:
+    if (
+        not isinstance(array, np.ndarray)
+        or array.dtype not in (np.float32, np.float64)
+        or not array.dtype.isnative
+    ):
+        raise TypeError("argument 'array': expected a float32 or float64 numpy.ndarray")
+    return array
+
+
+class Plan:
+    """A validated filter for one shape (``WindowFilter``, ``src/plan.rs:236-428``).
+
+    Owns a native plan; ``apply`` filters host arrays, ``apply_device``
+    device buffers.
+    """
+
+    def __init__(
+        self,
+        dtype: np.dtype,
+        shape: Sequence[int],
+        kind: str,
+        size: Sequence[int],
+        footprint: Sequence[bool] | None,
+        origin: Sequence[int],
+        mode: Sequence[str],
+        cval: float,
+        omit_nan: bool,
+        ignore_inf: bool,
+        min_count: int,
+        parameter: int,
+    ) -> None:
+        self._handle = None
+        dtype = np.dtype(dtype)
+        if kind not in _lib.KINDS:
+            raise ValueError("unsupported window filter kind")
+        histogram = kind in ("median_histogram", "rank_histogram")
+        ndim = len(shape)
+        size = _as_sizes(size, "size")
+        origin = [int(value) for value in origin]
+        min_count = _as_sizes([min_count], "min_count")[0]
+        parameter = _as_sizes([parameter], "parameter")[0]
+        modes = []
+        for value in mode:
+            if value not in _lib.MODES:
+                raise ValueError("unsupported boundary mode")
+            modes.append(_lib.MODES.index(value))
+        if histogram and footprint is not None:
+            raise ValueError(_MSG_HIST_FOOTPRINT)
+        if ndim == 0:
+            raise ValueError(_MSG_RANK0)
+        if len(size) != ndim or len(origin) != ndim or len(modes) != ndim:
+            raise ValueError(_MSG_EQUAL_RANK)
+        fp = None
+        if footprint is not None:
+            flags = [bool(value) for value in footprint]
+            if any(value == 0 for value in size):
+                raise ValueError(_MSG_SIZE0)
+            volume = 1
+            for value in size:
+                volume *= value
+            if volume >= 2**64:
+                raise OverflowError(_MSG_OVERFLOW)
+            if len(flags) != volume:
+                raise ValueError(_MSG_LENGTH)
+            fp = np.asarray(flags, dtype=np.uint8)
+        self._footprint = fp
+        self.dtype = dtype
+        self.shape = tuple(int(v) for v in shape)
+        self.kind = kind
+        self.len = int(np.prod(self.shape, dtype=np.int64)) if ndim else 0
+        c_shape = (ctypes.c_size_t * ndim)(*self.shape)
+        c_size = (ctypes.c_size_t * ndim)(*size)
+        c_origin = (ctypes.c_ssize_t * ndim)(*origin)
+        c_modes = (ctypes.c_int32 * ndim)(*modes)
+        err = ctypes.create_string_buffer(256)
+        handle = ctypes.c_void_p()
+        status = _lib.lib.ndf_plan_create(
+            ctypes.byref(handle),
+            dtype.itemsize,
+            ndim,
+            c_shape,
+            c_size,
+            None if fp is None else fp.ctypes.data,
+            c_origin,
+            c_modes,
+            float(cval),
+            _lib.KINDS[kind],
+            int(bool(omit_nan)),
+            int(bool(ignore_inf)),
+            min_count,
+            parameter,
+            err,
+            len(err),
+        )
+        if status != _lib.OK:
+            _raise(status, err.value.decode())
+        self._handle = handle
+
+    def __del__(self) -> None:
+        handle = getattr(self, "_handle", None)
+        if handle is not None and handle.value:
+            _lib.lib.ndf_plan_destroy(handle)
+            self._handle = None
+
+    def set_engine(self, engine: str) -> None:
+        err = ctypes.create_string_buffer(256)
+        status = _lib.lib.ndf_plan_set_engine(self._handle, _lib.ENGINES[engine], err, len(err))
+        if status != _lib.OK:
+            _raise(status, err.value.decode())
+
+    def describe(self) -> str:
+        buffer = ctypes.create_string_buffer(512)
+        _lib.lib.ndf_plan_describe(self._handle, buffer, len(buffer))
+        return buffer.value.decode()
+
+    def apply(self, array: np.ndarray) -> np.ndarray:
+        """Filter a C-contiguous host array into a new owned array."""
+        if array.shape != self.shape or array.dtype != self.dtype:
+            raise ValueError(_MSG_LENGTH)
+        err = ctypes.create_string_buffer(256)
+        if self.len:
+            # Snapshot: copied to the device while the GIL is held.
+            status = _lib.lib_gil.ndf_plan_upload(self._handle, array.ctypes.data, self.len, err, len(err))
+            if status != _lib.OK:
+                _raise(status, err.value.decode())
+        # PyArray_Zeros in the reference: a new owned C-contiguous array.
+        output = np.zeros(self.shape, dtype=self.dtype)
+        if self.len:
+            status = _lib.lib.ndf_plan_compute_download(
+                self._handle, output.ctypes.data, self.len, err, len(err)
+            )
+            if status != _lib.OK:
+                _raise(status, err.value.decode())
+        return output
+
+    def apply_device(self, d_input: int, d_output: int, stream: int | None = None) -> None:
+        """Filter device buffers (integer addresses) asynchronously on ``stream``."""
+        err = ctypes.create_string_buffer(256)
+        status = _lib.lib.ndf_plan_apply_device(
+            self._handle, d_input, d_output, self.len, stream, err, len(err)
+        )
+        if status != _lib.OK:
+            _raise(status, err.value.decode())
+
+    def time_device(self, d_input: int, d_output: int, warmup: int, iterations: int) -> float:
+        """Mean milliseconds per launch, timed with HIP events on one stream."""
+        err = ctypes.create_string_buffer(256)
+        ms = ctypes.c_double(0.0)
+        status = _lib.lib.ndf_plan_time_device(
+            self._handle, d_input, d_output, self.len, warmup, iterations, ctypes.byref(ms), err, len(err)
+        )
+        if status != _lib.OK:
+            _raise(status, err.value.decode())
+        return ms.value
+
+
+def window_filter(
+    array: np.ndarray,
+    kind: str,
+    size: Sequence[int],
+    footprint: Sequence[bool] | None,
+    origin: Sequence[int],
+    mode: Sequence[str],
+    cval: float,
+    omit_nan: bool,
+    ignore_inf: bool,
+    min_count: int,
+    parameter: int,
+) -> np.ndarray:
+    """Apply one window reduction (median or rank kinds) on the GPU.
+
+    ``kind`` is one of ``median``, ``median_network``, ``median_histogram``,
+    ``rank`` or ``rank_histogram``; ``parameter`` is the nonnegative rank for
+    the rank kinds. The input must be C-contiguous.
+    """
+    if kind not in _lib.KINDS:
+        raise ValueError("unsupported window filter kind")
+    typed = _typed_array(array)
+    if not typed.flags.c_contiguous:
+        raise ValueError("native input must be C-contiguous")
+    plan = Plan(
+        typed.dtype,
+        typed.shape,
+        kind,
+        size,
+        footprint,
+        origin,
+        mode,
+        cval,
+        omit_nan,
+        ignore_inf,
+        min_count,
+        parameter,
+    )
+    return plan.apply(typed)
+
+
+def get_num_threads() -> int:
+    """Host thread count (``RAYON_NUM_THREADS`` or the CPU count)."""
+    return int(_lib.lib.ndf_get_num_threads())
+
+
+class DeviceArray:
+    """A device allocation owned by Python (benchmark and test plumbing)."""
+
+    def __init__(self, nbytes: int) -> None:
+        self.ptr = ctypes.c_void_p()
+        self.nbytes = int(nbytes)
+        if _lib.lib.ndf_device_malloc(ctypes.byref(self.ptr), max(self.nbytes, 1)) != _lib.OK:
+            raise RuntimeError("device allocation failed")
+
+    @property
+    def address(self) -> int:
+        return int(self.ptr.value or 0)
+
+    def upload(self, array: np.ndarray) -> None:
+        array = np.ascontiguousarray(array)
+        if array.nbytes > self.nbytes:
+            raise ValueError("array larger than the device buffer")
+        if _lib.lib.ndf_memcpy_to_device(self.ptr, array.ctypes.data, array.nbytes) != _lib.OK:
+            raise RuntimeError("host-to-device copy failed")
+
+    def download(self, array: np.ndarray) -> np.ndarray:
+        if array.nbytes > self.nbytes or not array.flags.c_contiguous:
+            raise ValueError("bad destination array")
+        if _lib.lib.ndf_memcpy_to_host(array.ctypes.data, self.ptr, array.nbytes) != _lib.OK:
+            raise RuntimeError("device-to-host copy failed")
+        return array
+
+    def free(self) -> None:
+        if self.ptr.value:
+            _lib.lib.ndf_device_free(self.ptr)
+            self.ptr = ctypes.c_void_p()
+
+    def __del__(self) -> None:
+        self.free()
+
+
+def device_count() -> int:
+    count = ctypes.c_int32(0)
+    _lib.lib.ndf_device_count(ctypes.byref(count))
+    return int(count.value)
+
+
+def synchronize() -> None:
+    if _lib.lib.ndf_device_synchronize() != _lib.OK:
+        raise RuntimeError("device synchronize failed")

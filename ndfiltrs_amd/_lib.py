@@ -1,0 +1,147 @@
+"""Loads ``libndfgpu.so`` (the HIP library) and declares its C ABI.
+
+There is no fallback: if the library is missing the import of the native
+layer fails loudly.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBRARY = os.path.join(HERE, "libndfgpu.so")
+
+OK = 0
+ERR_EMPTY_AXIS = 1
+ERR_INVALID = 2
+ERR_LENGTH_MISMATCH = 3
+ERR_MEMORY = 4
+ERR_OVERFLOW = 5
+ERR_DEVICE = 6
+ERR_KIND = 7
+ERR_MODE = 8
+
+MODES = ("reflect", "constant", "nearest", "mirror", "wrap")
+KINDS = {
+    "median": 0,
+    "median_network": 1,
+    "median_histogram": 2,
+    "rank": 3,
+    "rank_histogram": 4,
+}
+ENGINES = {"auto": 0, "generic": 1, "slide": 2}
+
+_size_p = ctypes.POINTER(ctypes.c_size_t)
+_ssize_p = ctypes.POINTER(ctypes.c_ssize_t)
+_i32_p = ctypes.POINTER(ctypes.c_int32)
+_err = [ctypes.c_char_p, ctypes.c_size_t]
+
+
+class NativeLibraryMissing(ImportError):
+    """libndfgpu.so has not been built."""
+
+
+def _declare(lib: ctypes.CDLL) -> None:
+    plan_p = ctypes.c_void_p
+    lib.ndf_window_filter.argtypes = [
+        ctypes.c_int32,
+        ctypes.c_void_p,
+        ctypes.c_void_p,
+        ctypes.c_size_t,
+        _size_p,
+        _size_p,
+        ctypes.c_void_p,
+        _ssize_p,
+        _i32_p,
+        ctypes.c_double,
+        ctypes.c_int32,
+        ctypes.c_int32,
+        ctypes.c_int32,
+        ctypes.c_size_t,
+        ctypes.c_size_t,
+        *_err,
+    ]
+    lib.ndf_plan_create.argtypes = [
+        ctypes.POINTER(ctypes.c_void_p),
+        ctypes.c_int32,
+        ctypes.c_size_t,
+        _size_p,
+        _size_p,
+        ctypes.c_void_p,
+        _ssize_p,
+        _i32_p,
+        ctypes.c_double,
+        ctypes.c_int32,
+        ctypes.c_int32,
+        ctypes.c_int32,
+        ctypes.c_size_t,
+        ctypes.c_size_t,
+        *_err,
+    ]
+    lib.ndf_plan_apply.argtypes = [plan_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, *_err]
+    lib.ndf_plan_upload.argtypes = [plan_p, ctypes.c_void_p, ctypes.c_size_t, *_err]
+    lib.ndf_plan_compute_download.argtypes = [plan_p, ctypes.c_void_p, ctypes.c_size_t, *_err]
+    lib.ndf_plan_apply_device.argtypes = [
+        plan_p,
+        ctypes.c_void_p,
+        ctypes.c_void_p,
+        ctypes.c_size_t,
+        ctypes.c_void_p,
+        *_err,
+    ]
+    lib.ndf_plan_set_engine.argtypes = [plan_p, ctypes.c_int32, *_err]
+    lib.ndf_plan_describe.argtypes = [plan_p, ctypes.c_char_p, ctypes.c_size_t]
+    lib.ndf_plan_destroy.argtypes = [plan_p]
+    lib.ndf_plan_destroy.restype = None
+    lib.ndf_get_num_threads.argtypes = []
+    lib.ndf_get_num_threads.restype = ctypes.c_size_t
+    lib.ndf_device_count.argtypes = [_i32_p]
+    lib.ndf_device_malloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+    lib.ndf_device_free.argtypes = [ctypes.c_void_p]
+    lib.ndf_memcpy_to_device.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
+    lib.ndf_memcpy_to_host.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
+    lib.ndf_device_synchronize.argtypes = []
+    lib.ndf_plan_time_device.argtypes = [
+        plan_p,
+        ctypes.c_void_p,
+        ctypes.c_void_p,
+        ctypes.c_size_t,
+        ctypes.c_int32,
+        ctypes.c_int32,
+        ctypes.POINTER(ctypes.c_double),
+        *_err,
+    ]
+    for name in (
+        "ndf_window_filter",
+        "ndf_plan_create",
+        "ndf_plan_apply",
+        "ndf_plan_upload",
+        "ndf_plan_compute_download",
+        "ndf_plan_apply_device",
+        "ndf_plan_set_engine",
+        "ndf_plan_describe",
+        "ndf_device_count",
+        "ndf_device_malloc",
+        "ndf_device_free",
+        "ndf_memcpy_to_device",
+        "ndf_memcpy_to_host",
+        "ndf_device_synchronize",
+        "ndf_plan_time_device",
+    ):
+        getattr(lib, name).restype = ctypes.c_int32
+
+
+def _open(loader: type) -> ctypes.CDLL:
+    if not os.path.exists(LIBRARY):
+        raise NativeLibraryMissing(
+            f"{LIBRARY} is not built; run `python -c 'import __graft_entry__ as g; g.build()'`"
+        )
+    lib = loader(LIBRARY)
+    _declare(lib)
+    return lib
+
+
+# CDLL releases the GIL during calls (compute); PyDLL keeps it (snapshot).
+lib = _open(ctypes.CDLL)
+lib_gil = _open(ctypes.PyDLL)

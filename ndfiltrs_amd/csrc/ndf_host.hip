@@ -1,0 +1,804 @@
+// Host side of libndfgpu: validation, planning, device tables and the C ABI.
+//
+// Validation restates, in order and with the same messages:
+//   Window::new                 src/window.rs:44-137
+//   FilterConfig::from_window   src/filter_nd.rs:32-55
+//   WindowFilter::new           src/plan.rs:260-324 (histogram kinds:
+//                               validate_rectangular :154-197 and
+//                               validate_support src/histogram.rs:74-88)
+//   _core.window_filter         src/pyapi/filter.rs:265-314 (kind dispatch)
+#include <hip/hip_runtime.h>
+#include <sched.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <unistd.h>
+
+#include <algorithm>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/ndfgpu.h"
+#include "ndf_device.hpp"
+#include "ndf_internal.hpp"
+
+namespace {
+
+using namespace ndf;
+
+const char *const MSG_RANK0 = "filtering requires at least one dimension";
+const char *const MSG_SIZE0 = "window sizes must be positive";
+const char *const MSG_ORIGIN = "origin must satisfy -(size // 2) <= origin <= (size - 1) // 2";
+const char *const MSG_EMPTY_FOOTPRINT = "footprint must select at least one element";
+const char *const MSG_MIN_COUNT = "min_count must be between 1 and the number of window elements";
+const char *const MSG_HIST_FOOTPRINT = "histogram reductions require rectangular size, not footprint";
+const char *const MSG_EMPTY_AXIS = "cannot map an empty axis";
+const char *const MSG_LENGTH = "input length does not match the configured shape";
+const char *const MSG_MEMORY = "unable to allocate filtering workspace";
+const char *const MSG_OVERFLOW = "shape or index arithmetic overflowed";
+const char *const MSG_KIND = "unsupported window filter kind";
+const char *const MSG_MODE = "unsupported boundary mode";
+
+ndf_status fail(ndf_status code, const char *message, char *err, size_t err_len)
+{
+    if (err && err_len) {
+        snprintf(err, err_len, "%s", message);
+    }
+    return code;
+}
+
+ndf_status fail_code(ndf_status code, char *err, size_t err_len)
+{
+    switch (code) {
+    case NDF_ERR_EMPTY_AXIS: return fail(code, MSG_EMPTY_AXIS, err, err_len);
+    case NDF_ERR_LENGTH_MISMATCH: return fail(code, MSG_LENGTH, err, err_len);
+    case NDF_ERR_MEMORY: return fail(code, MSG_MEMORY, err, err_len);
+    case NDF_ERR_OVERFLOW: return fail(code, MSG_OVERFLOW, err, err_len);
+    case NDF_ERR_KIND: return fail(code, MSG_KIND, err, err_len);
+    case NDF_ERR_MODE: return fail(code, MSG_MODE, err, err_len);
+    default: return fail(code, "invalid configuration", err, err_len);
+    }
+}
+
+ndf_status fail_hip(hipError_t e, char *err, size_t err_len)
+{
+    char buffer[256];
+    snprintf(buffer, sizeof(buffer), "HIP error %d: %s", (int)e, hipGetErrorString(e));
+    return fail(NDF_ERR_DEVICE, buffer, err, err_len);
+}
+
+bool checked_mul(size_t a, size_t b, size_t *out)
+{
+    if (a != 0 && b > SIZE_MAX / a) {
+        return false;
+    }
+    *out = a * b;
+    return true;
+}
+
+// map_index (src/boundary.rs:29-71); returns -1 for a constant-mode sample.
+int64_t map_coordinate(int64_t index, int64_t length, int32_t mode)
+{
+    if (index >= 0 && index < length) {
+        return index;
+    }
+    auto rem = [](int64_t v, int64_t m) {
+        const int64_t r = v % m;
+        return r < 0 ? r + m : r;
+    };
+    switch (mode) {
+    case NDF_MODE_CONSTANT: return -1;
+    case NDF_MODE_NEAREST: return index < 0 ? 0 : length - 1;
+    case NDF_MODE_WRAP: return rem(index, length);
+    case NDF_MODE_REFLECT: {
+        if (length == 1) return 0;
+        const int64_t period = 2 * length;
+        const int64_t r = rem(index, period);
+        return r < length ? r : period - 1 - r;
+    }
+    default: {  // mirror
+        if (length == 1) return 0;
+        const int64_t period = 2 * (length - 1);
+        const int64_t r = rem(index, period);
+        return r < length ? r : period - r;
+    }
+    }
+}
+
+size_t next_pow2(size_t v)
+{
+    size_t p = 1;
+    while (p < v) p <<= 1;
+    return p;
+}
+
+struct DeviceBuffer {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    hipError_t ensure(size_t need)
+    {
+        if (need <= bytes && ptr) return hipSuccess;
+        if (ptr) {
+            (void)hipFree(ptr);
+            ptr = nullptr;
+            bytes = 0;
+        }
+        hipError_t e = hipMalloc(&ptr, need < 16 ? 16 : need);
+        if (e == hipSuccess) bytes = need < 16 ? 16 : need;
+        return e;
+    }
+    void release()
+    {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+    }
+};
+
+template <typename V>
+hipError_t upload(DeviceBuffer &buffer, const std::vector<V> &values)
+{
+    hipError_t e = buffer.ensure(values.size() * sizeof(V));
+    if (e != hipSuccess || values.empty()) return e;
+    return hipMemcpy(buffer.ptr, values.data(), values.size() * sizeof(V), hipMemcpyHostToDevice);
+}
+
+// LDS per workgroup on gfx950 (160 KiB); a present device may report less.
+constexpr int GFX950_LDS_BYTES = 163840;
+
+int lds_limit()
+{
+    static int limit = -1;
+    if (limit < 0) {
+        int count = 0;
+        int device = 0;
+        int value = 0;
+        limit = GFX950_LDS_BYTES;
+        if (hipGetDeviceCount(&count) == hipSuccess && count > 0 && hipGetDevice(&device) == hipSuccess &&
+            hipDeviceGetAttribute(&value, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess &&
+            value > 0 && value < limit) {
+            limit = value;
+        }
+    }
+    return limit;
+}
+
+}  // namespace
+
+struct ndf_plan {
+    int32_t dbytes = 4;
+    size_t rank = 0;
+    std::vector<size_t> shape, size;
+    std::vector<uint8_t> footprint;  // empty = rectangle
+    std::vector<ptrdiff_t> origin;
+    std::vector<int32_t> modes;
+    double cval = 0.0;
+    uint64_t cval_bits = 0;
+    int32_t kind = 0;
+    bool omit = false, ignore_inf = false;
+    size_t min_count = 1, parameter = 0;
+    int32_t stat = STAT_MEDIAN;
+    size_t len = 0, count = 0;
+    std::vector<int64_t> taps;  // count * rank offsets
+    std::vector<int64_t> low, high;
+    // Effective policy: propagate == omit{ignore_inf=false, min_count=count}.
+    bool eff_ignore_inf = false;
+    int64_t eff_min_count = 1;
+    bool all_nan = false;  // rank >= count: every output is NaN
+    int32_t engine = NDF_ENGINE_GENERIC;
+    bool slide_ok = false;
+
+    // Slide geometry (host side).
+    SlideArgs slide{};
+    size_t slide_lds = 0;
+    std::vector<int64_t> slide_mapz, slide_mapy, slide_mapx;
+    std::vector<uint32_t> slide_taps, slide_rm, slide_add;
+    std::vector<uint8_t> slide_mask;
+
+    // Device state.
+    std::mutex mutex;
+    bool tables_ready = false;
+    bool uploaded = false;
+    DeviceBuffer g_maps, g_taps, s_mapz, s_mapy, s_mapx, s_taps, s_rm, s_add, s_mask, d_in, d_out, d_error;
+    GenericArgs generic{};
+
+    ~ndf_plan()
+    {
+        for (DeviceBuffer *b : {&g_maps, &g_taps, &s_mapz, &s_mapy, &s_mapx, &s_taps, &s_rm, &s_add, &s_mask, &d_in,
+                                &d_out, &d_error}) {
+            b->release();
+        }
+    }
+};
+
+namespace {
+
+// Plan the slide engine; false when the geometry cannot be staged.
+bool plan_slide(ndf_plan *p)
+{
+    if (p->rank > 3 || p->count > 65534 || p->len == 0) return false;
+    // Canonical 3-D view: leading singleton axes for rank 1 and 2.
+    int64_t dims[3] = {1, 1, 1}, lo[3] = {0, 0, 0}, box[3] = {1, 1, 1};
+    int32_t modes[3] = {NDF_MODE_NEAREST, NDF_MODE_NEAREST, NDF_MODE_NEAREST};
+    const size_t pad = 3 - p->rank;
+    for (size_t a = 0; a < p->rank; ++a) {
+        dims[pad + a] = (int64_t)p->shape[a];
+        lo[pad + a] = p->low[a];
+        box[pad + a] = p->high[a] - p->low[a] + 1;
+        modes[pad + a] = p->modes[a];
+    }
+    const int64_t kz = box[0], ky = box[1], kx = box[2];
+    const int64_t sx = SLIDE_TILE_X + kx - 1;
+    if (sx > 1000 || ky > 900 || kz > 4000) return false;
+    // Footprint mask over the tight box.
+    std::vector<uint8_t> mask((size_t)(kz * ky * kx), 0);
+    for (size_t t = 0; t < p->count; ++t) {
+        int64_t d[3] = {0, 0, 0};
+        for (size_t a = 0; a < p->rank; ++a) d[pad + a] = p->taps[t * p->rank + a] - p->low[a];
+        mask[(size_t)((d[0] * ky + d[1]) * kx + d[2])] = 1;
+    }
+    const bool rect = p->count == mask.size();
+    const int mask_bytes = rect ? 0 : (int)mask.size();
+    if (mask_bytes > 16384) return false;
+    // Rows per wave: as many as LDS allows, but no more than the image needs.
+    const int limit = lds_limit();
+    auto fits = [&](int cand) {
+        const int64_t ty = (int64_t)SLIDE_WAVES * cand;
+        const int64_t sy = ty + ky - 1;
+        const int64_t s = kz * sy * sx;
+        if (sy > 1023 || s > 65534) return false;
+        const int64_t npad = (int64_t)next_pow2((size_t)s);
+        return slide_lds_bytes(p->dbytes, (int)s, (int)npad, mask_bytes) <= (size_t)limit;
+    };
+    int tw = 0;
+    for (int cand : {16, 8, 4, 2, 1}) {
+        if (fits(cand)) {
+            tw = cand;
+            break;
+        }
+    }
+    if (tw == 0) return false;
+    // A smaller tile is enough when it still covers every row of the image.
+    while (tw > 1 && (int64_t)SLIDE_WAVES * (tw / 2) >= dims[1]) tw /= 2;
+    const int ty = SLIDE_WAVES * tw;
+    const int sy = ty + (int)ky - 1;
+    const int s = (int)(kz * sy * sx);
+    const int npad = (int)next_pow2((size_t)s);
+    const int64_t tiles_x = (dims[2] + SLIDE_TILE_X - 1) / SLIDE_TILE_X;
+    const int64_t tiles_y = (dims[1] + ty - 1) / ty;
+    if (tiles_x * tiles_y * dims[0] > 0x7FFFFFFF) return false;
+
+    const int64_t in_sy = dims[2];
+    const int64_t in_sz = dims[1] * dims[2];
+    // Maps: index i <-> coordinate i + lo; entries past the last valid
+    // window coordinate only feed padded outputs.
+    auto build = [](std::vector<int64_t> &map, int64_t length, int64_t n, int64_t low, int64_t k, int32_t mode,
+                    int64_t stride) {
+        map.assign((size_t)length, MAP_PAD);
+        for (int64_t i = 0; i < length; ++i) {
+            if (i >= n + k - 1) break;
+            const int64_t m = map_coordinate(i + low, n, mode);
+            map[(size_t)i] = m < 0 ? MAP_CVAL : m * stride;
+        }
+    };
+    build(p->slide_mapz, dims[0] + kz - 1, dims[0], lo[0], kz, modes[0], in_sz);
+    build(p->slide_mapy, tiles_y * ty + ky - 1, dims[1], lo[1], ky, modes[1], in_sy);
+    build(p->slide_mapx, tiles_x * SLIDE_TILE_X + kx - 1, dims[2], lo[2], kx, modes[2], 1);
+
+    p->slide_taps.clear();
+    p->slide_rm.clear();
+    p->slide_add.clear();
+    for (int64_t dz = 0; dz < kz; ++dz) {
+        for (int64_t dy = 0; dy < ky; ++dy) {
+            for (int64_t dx = 0; dx < kx; ++dx) {
+                auto at = [&](int64_t yy) { return mask[(size_t)((dz * ky + yy) * kx + dx)] != 0; };
+                if (!at(dy)) continue;
+                const uint32_t off = (uint32_t)((dz * sy + dy) * sx + dx);
+                p->slide_taps.push_back(off);
+                if (dy == 0 || !at(dy - 1)) p->slide_rm.push_back(off);
+                if (dy == ky - 1 || !at(dy + 1)) p->slide_add.push_back(off);
+            }
+        }
+    }
+    if (p->slide_rm.size() != p->slide_add.size()) return false;
+    p->slide_mask = rect ? std::vector<uint8_t>() : mask;
+
+    SlideArgs &s_args = p->slide;
+    s_args = SlideArgs{};
+    s_args.nz = dims[0];
+    s_args.ny = dims[1];
+    s_args.nx = dims[2];
+    s_args.in_sz = in_sz;
+    s_args.in_sy = in_sy;
+    s_args.kz = (int32_t)kz;
+    s_args.ky = (int32_t)ky;
+    s_args.kx = (int32_t)kx;
+    s_args.sx = (int32_t)sx;
+    s_args.sy = sy;
+    s_args.s = s;
+    s_args.npad = npad;
+    s_args.ty = ty;
+    s_args.tw = tw;
+    s_args.tiles_x = (int32_t)tiles_x;
+    s_args.tiles_y = (int32_t)tiles_y;
+    s_args.count = (int32_t)p->count;
+    s_args.nedge = (int32_t)p->slide_rm.size();
+    s_args.cval_bits = p->cval_bits;
+    s_args.ignore_inf = p->eff_ignore_inf;
+    s_args.stat = p->stat;
+    s_args.min_count = (int32_t)p->eff_min_count;
+    s_args.rank_param = (int32_t)std::min(p->parameter, p->count);
+    p->slide_lds = slide_lds_bytes(p->dbytes, s, npad, mask_bytes);
+    return true;
+}
+
+ndf_status prepare_tables(ndf_plan *p, char *err, size_t err_len)
+{
+    if (p->tables_ready) return NDF_OK;
+    hipError_t e = p->d_error.ensure(sizeof(int32_t));
+    if (e != hipSuccess) return fail_hip(e, err, err_len);
+    if (p->engine == NDF_ENGINE_GENERIC) {
+        std::vector<int64_t> maps;
+        GenericArgs &g = p->generic;
+        g = GenericArgs{};
+        g.rank = (int32_t)p->rank;
+        g.count = (int32_t)p->count;
+        g.len = (int64_t)p->len;
+        int64_t stride = 1;
+        for (size_t a = p->rank; a-- > 0;) {
+            g.stride[a] = stride;
+            stride *= (int64_t)p->shape[a];
+        }
+        for (size_t a = 0; a < p->rank; ++a) {
+            g.map_base[a] = (int64_t)maps.size();
+            const int64_t n = (int64_t)p->shape[a];
+            const int64_t length = n + p->high[a] - p->low[a];
+            for (int64_t i = 0; i < length; ++i) {
+                const int64_t m = map_coordinate(i + p->low[a], n, p->modes[a]);
+                maps.push_back(m < 0 ? MAP_CVAL : m * g.stride[a]);
+            }
+        }
+        std::vector<int32_t> taps(p->count * p->rank);
+        for (size_t t = 0; t < p->count; ++t) {
+            for (size_t a = 0; a < p->rank; ++a) {
+                taps[t * p->rank + a] = (int32_t)(p->taps[t * p->rank + a] - p->low[a]);
+            }
+        }
+        if ((e = upload(p->g_maps, maps)) != hipSuccess || (e = upload(p->g_taps, taps)) != hipSuccess) {
+            return fail_hip(e, err, err_len);
+        }
+        g.maps = static_cast<const int64_t *>(p->g_maps.ptr);
+        g.taps = static_cast<const int32_t *>(p->g_taps.ptr);
+        g.cval_bits = p->cval_bits;
+        g.ignore_inf = p->eff_ignore_inf;
+        g.stat = p->stat;
+        g.min_count = p->eff_min_count;
+        g.rank_param = (int64_t)std::min(p->parameter, p->count);
+    } else if (p->engine == NDF_ENGINE_SLIDE) {
+        if ((e = upload(p->s_mapz, p->slide_mapz)) != hipSuccess ||
+            (e = upload(p->s_mapy, p->slide_mapy)) != hipSuccess ||
+            (e = upload(p->s_mapx, p->slide_mapx)) != hipSuccess ||
+            (e = upload(p->s_taps, p->slide_taps)) != hipSuccess ||
+            (e = upload(p->s_rm, p->slide_rm)) != hipSuccess || (e = upload(p->s_add, p->slide_add)) != hipSuccess ||
+            (e = upload(p->s_mask, p->slide_mask)) != hipSuccess) {
+            return fail_hip(e, err, err_len);
+        }
+        SlideArgs &s = p->slide;
+        s.mapz = static_cast<const int64_t *>(p->s_mapz.ptr);
+        s.mapy = static_cast<const int64_t *>(p->s_mapy.ptr);
+        s.mapx = static_cast<const int64_t *>(p->s_mapx.ptr);
+        s.taps = static_cast<const uint32_t *>(p->s_taps.ptr);
+        s.edge_remove = static_cast<const uint32_t *>(p->s_rm.ptr);
+        s.edge_add = static_cast<const uint32_t *>(p->s_add.ptr);
+        s.mask = p->slide_mask.empty() ? nullptr : static_cast<const uint8_t *>(p->s_mask.ptr);
+        s.error_flag = static_cast<int32_t *>(p->d_error.ptr);
+    }
+    p->tables_ready = true;
+    return NDF_OK;
+}
+
+ndf_status launch(ndf_plan *p, const void *d_in, void *d_out, hipStream_t stream, char *err, size_t err_len)
+{
+    hipError_t e = hipSuccess;
+    const uint64_t nan_bits = p->dbytes == 4 ? 0x7FC00000ull : 0x7FF8000000000000ull;
+    if (p->all_nan) {
+        e = launch_fill(p->dbytes, d_out, (int64_t)p->len, nan_bits, stream);
+    } else if (p->engine == NDF_ENGINE_SLIDE) {
+        SlideArgs args = p->slide;
+        args.input = d_in;
+        args.output = d_out;
+        e = launch_slide(p->dbytes, args, p->slide_lds, stream);
+    } else {
+        GenericArgs args = p->generic;
+        args.input = d_in;
+        args.output = d_out;
+        e = launch_generic(p->dbytes, args, stream);
+    }
+    return e == hipSuccess ? NDF_OK : fail_hip(e, err, err_len);
+}
+
+// Validation + tap construction shared by plan creation.
+ndf_status build_plan(ndf_plan *p, int32_t dtype_bytes, size_t rank, const size_t *shape, const size_t *size,
+                      const uint8_t *footprint, const ptrdiff_t *origin, const int32_t *modes, double cval,
+                      int32_t kind, int32_t omit_nan, int32_t ignore_inf, size_t min_count, size_t parameter,
+                      char *err, size_t err_len)
+{
+    if (dtype_bytes != 4 && dtype_bytes != 8) {
+        return fail(NDF_ERR_INVALID, "array dtype must be float32 or float64", err, err_len);
+    }
+    if (kind < NDF_KIND_MEDIAN || kind > NDF_KIND_RANK_HISTOGRAM) return fail_code(NDF_ERR_KIND, err, err_len);
+    for (size_t a = 0; a < rank; ++a) {
+        if (modes[a] < NDF_MODE_REFLECT || modes[a] > NDF_MODE_WRAP) return fail_code(NDF_ERR_MODE, err, err_len);
+    }
+    if (rank > (size_t)MAX_RANK) return fail(NDF_ERR_INVALID, "array rank exceeds 32", err, err_len);
+    const bool histogram = kind == NDF_KIND_MEDIAN_HISTOGRAM || kind == NDF_KIND_RANK_HISTOGRAM;
+    p->dbytes = dtype_bytes;
+    p->rank = rank;
+    p->shape.assign(shape, shape + rank);
+    p->size.assign(size, size + rank);
+    p->origin.assign(origin, origin + rank);
+    p->modes.assign(modes, modes + rank);
+    p->kind = kind;
+    p->omit = omit_nan != 0;
+    p->ignore_inf = ignore_inf != 0;
+    p->min_count = min_count;
+    p->parameter = parameter;
+    p->stat = (kind == NDF_KIND_RANK || kind == NDF_KIND_RANK_HISTOGRAM) ? STAT_RANK : STAT_MEDIAN;
+    if (dtype_bytes == 4) {
+        const float f = (float)cval;
+        uint32_t b;
+        memcpy(&b, &f, 4);
+        p->cval_bits = b;
+    } else {
+        memcpy(&p->cval_bits, &cval, 8);
+    }
+    p->cval = cval;
+
+    if (histogram) {
+        // src/plan.rs:294-312, validate_rectangular, validate_support.
+        if (footprint) return fail(NDF_ERR_INVALID, MSG_HIST_FOOTPRINT, err, err_len);
+        if (rank == 0) return fail(NDF_ERR_INVALID, MSG_RANK0, err, err_len);
+        for (size_t a = 0; a < rank; ++a) {
+            if (size[a] == 0) return fail(NDF_ERR_INVALID, MSG_SIZE0, err, err_len);
+        }
+        size_t len = 1, volume = 1;
+        for (size_t a = 0; a < rank; ++a) {
+            if (!checked_mul(len, shape[a], &len)) return fail_code(NDF_ERR_OVERFLOW, err, err_len);
+        }
+        for (size_t a = 0; a < rank; ++a) {
+            if (!checked_mul(volume, size[a], &volume)) return fail_code(NDF_ERR_OVERFLOW, err, err_len);
+        }
+        for (size_t a = 0; a < rank; ++a) {
+            if (size[a] > (size_t)INT64_MAX) return fail_code(NDF_ERR_OVERFLOW, err, err_len);
+            const int64_t length = (int64_t)size[a];
+            const int64_t shift = (int64_t)origin[a];
+            if (shift < -(length / 2) || shift > (length - 1) / 2) {
+                return fail(NDF_ERR_INVALID, MSG_ORIGIN, err, err_len);
+            }
+        }
+        if (p->omit && (min_count == 0 || min_count > volume)) {
+            return fail(NDF_ERR_INVALID, MSG_MIN_COUNT, err, err_len);
+        }
+        // validate_support: block lengths of max(size, 32) per axis.
+        size_t support = 1;
+        for (size_t a = 0; a < rank; ++a) {
+            const size_t window = std::max<size_t>(size[a], 32);
+            size_t blocks = shape[a] / window;
+            if (blocks < 1) blocks = 1;
+            size_t block = (shape[a] + blocks - 1) / blocks;
+            if (block < 1) block = 1;
+            const size_t length = block + size[a] - 1;
+            if (length < block || !checked_mul(support, length, &support)) {
+                return fail_code(NDF_ERR_OVERFLOW, err, err_len);
+            }
+        }
+        if (support > 0xFFFFFFFEull) return fail_code(NDF_ERR_OVERFLOW, err, err_len);
+    }
+
+    // Window::new (src/window.rs:44-137).
+    if (rank == 0) return fail(NDF_ERR_INVALID, MSG_RANK0, err, err_len);
+    for (size_t a = 0; a < rank; ++a) {
+        if (size[a] == 0) return fail(NDF_ERR_INVALID, MSG_SIZE0, err, err_len);
+    }
+    size_t volume = 1;
+    for (size_t a = 0; a < rank; ++a) {
+        if (!checked_mul(volume, size[a], &volume)) return fail_code(NDF_ERR_OVERFLOW, err, err_len);
+    }
+    std::vector<int64_t> anchors(rank);
+    for (size_t a = 0; a < rank; ++a) {
+        if (size[a] > (size_t)INT64_MAX) return fail_code(NDF_ERR_OVERFLOW, err, err_len);
+        const int64_t length = (int64_t)size[a];
+        const int64_t shift = (int64_t)origin[a];
+        if (shift < -(length / 2) || shift > (length - 1) / 2) {
+            return fail(NDF_ERR_INVALID, MSG_ORIGIN, err, err_len);
+        }
+        anchors[a] = length / 2 + shift;
+    }
+    size_t len = 1;
+    for (size_t a = 0; a < rank; ++a) {
+        if (!checked_mul(len, shape[a], &len)) return fail_code(NDF_ERR_OVERFLOW, err, err_len);
+    }
+    size_t count = 0;
+    if (footprint) {
+        for (size_t i = 0; i < volume; ++i) count += footprint[i] != 0;
+    } else {
+        count = volume;
+    }
+    if (count == 0) return fail(NDF_ERR_INVALID, MSG_EMPTY_FOOTPRINT, err, err_len);
+    if (p->omit && (min_count == 0 || min_count > count)) {
+        return fail(NDF_ERR_INVALID, MSG_MIN_COUNT, err, err_len);
+    }
+    if (count > (size_t)INT32_MAX / (rank ? rank : 1)) return fail_code(NDF_ERR_MEMORY, err, err_len);
+    try {
+        p->taps.resize(count * rank);
+        p->low.assign(rank, INT64_MAX);
+        p->high.assign(rank, INT64_MIN);
+        if (footprint) p->footprint.assign(footprint, footprint + volume);
+    } catch (const std::bad_alloc &) {
+        return fail_code(NDF_ERR_MEMORY, err, err_len);
+    }
+    std::vector<size_t> index(rank, 0);
+    size_t t = 0;
+    for (size_t element = 0; element < volume; ++element) {
+        if (!footprint || footprint[element]) {
+            for (size_t a = 0; a < rank; ++a) {
+                const int64_t offset = (int64_t)index[a] - anchors[a];
+                p->taps[t * rank + a] = offset;
+                p->low[a] = std::min(p->low[a], offset);
+                p->high[a] = std::max(p->high[a], offset);
+            }
+            ++t;
+        }
+        for (size_t a = rank; a-- > 0;) {
+            if (++index[a] < size[a]) break;
+            index[a] = 0;
+        }
+    }
+    p->len = len;
+    p->count = count;
+    p->eff_ignore_inf = p->omit && p->ignore_inf;
+    p->eff_min_count = p->omit ? (int64_t)min_count : (int64_t)count;
+    p->all_nan = p->stat == STAT_RANK && parameter >= count;
+    p->slide_ok = plan_slide(p);
+    p->engine = p->slide_ok ? NDF_ENGINE_SLIDE : NDF_ENGINE_GENERIC;
+    return NDF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+ndf_status ndf_plan_create(ndf_plan **plan, int32_t dtype_bytes, size_t rank, const size_t *shape,
+                           const size_t *size, const uint8_t *footprint, const ptrdiff_t *origin,
+                           const int32_t *modes, double cval, int32_t kind, int32_t omit_nan, int32_t ignore_inf,
+                           size_t min_count, size_t parameter, char *err, size_t err_len)
+{
+    if (!plan) return fail(NDF_ERR_INVALID, "plan pointer is null", err, err_len);
+    *plan = nullptr;
+    ndf_plan *p = new (std::nothrow) ndf_plan();
+    if (!p) return fail_code(NDF_ERR_MEMORY, err, err_len);
+    const ndf_status st = build_plan(p, dtype_bytes, rank, shape, size, footprint, origin, modes, cval, kind,
+                                     omit_nan, ignore_inf, min_count, parameter, err, err_len);
+    if (st != NDF_OK) {
+        delete p;
+        return st;
+    }
+    *plan = p;
+    return NDF_OK;
+}
+
+void ndf_plan_destroy(ndf_plan *plan) { delete plan; }
+
+ndf_status ndf_plan_set_engine(ndf_plan *plan, int32_t engine, char *err, size_t err_len)
+{
+    std::lock_guard<std::mutex> guard(plan->mutex);
+    int32_t chosen;
+    if (engine == NDF_ENGINE_AUTO) {
+        chosen = plan->slide_ok ? NDF_ENGINE_SLIDE : NDF_ENGINE_GENERIC;
+    } else if (engine == NDF_ENGINE_GENERIC) {
+        chosen = NDF_ENGINE_GENERIC;
+    } else if (engine == NDF_ENGINE_SLIDE) {
+        if (!plan->slide_ok) return fail(NDF_ERR_INVALID, "the slide engine cannot stage this geometry", err, err_len);
+        chosen = NDF_ENGINE_SLIDE;
+    } else {
+        return fail(NDF_ERR_INVALID, "unknown engine", err, err_len);
+    }
+    if (chosen != plan->engine) {
+        plan->engine = chosen;
+        plan->tables_ready = false;
+    }
+    return NDF_OK;
+}
+
+ndf_status ndf_plan_describe(const ndf_plan *plan, char *buffer, size_t buffer_len)
+{
+    const char *engine = plan->all_nan ? "fill" : (plan->engine == NDF_ENGINE_SLIDE ? "slide" : "generic");
+    if (plan->engine == NDF_ENGINE_SLIDE) {
+        snprintf(buffer, buffer_len,
+                 "{\"engine\": \"%s\", \"count\": %zu, \"tile\": [%d, %d], \"support\": %d, \"sort_len\": %d, "
+                 "\"lds_bytes\": %zu, \"edge_taps\": %d, \"tiles\": %lld}",
+                 engine, plan->count, plan->slide.ty, SLIDE_TILE_X, plan->slide.s, plan->slide.npad, plan->slide_lds,
+                 plan->slide.nedge,
+                 (long long)plan->slide.tiles_x * plan->slide.tiles_y * plan->slide.nz);
+    } else {
+        snprintf(buffer, buffer_len, "{\"engine\": \"%s\", \"count\": %zu}", engine, plan->count);
+    }
+    return NDF_OK;
+}
+
+ndf_status ndf_plan_apply_device(ndf_plan *plan, const void *d_input, void *d_output, size_t len, void *stream,
+                                 char *err, size_t err_len)
+{
+    if (len != plan->len) return fail_code(NDF_ERR_LENGTH_MISMATCH, err, err_len);
+    if (len == 0) return NDF_OK;
+    std::lock_guard<std::mutex> guard(plan->mutex);
+    const ndf_status st = prepare_tables(plan, err, err_len);
+    if (st != NDF_OK) return st;
+    return launch(plan, d_input, d_output, static_cast<hipStream_t>(stream), err, err_len);
+}
+
+namespace {
+
+ndf_status upload_locked(ndf_plan *plan, const void *input, char *err, size_t err_len)
+{
+    ndf_status st = prepare_tables(plan, err, err_len);
+    if (st != NDF_OK) return st;
+    const size_t bytes = plan->len * (size_t)plan->dbytes;
+    hipError_t e;
+    if ((e = plan->d_in.ensure(bytes)) != hipSuccess || (e = plan->d_out.ensure(bytes)) != hipSuccess) {
+        return fail_hip(e, err, err_len);
+    }
+    if ((e = hipMemcpy(plan->d_in.ptr, input, bytes, hipMemcpyHostToDevice)) != hipSuccess) {
+        return fail_hip(e, err, err_len);
+    }
+    plan->uploaded = true;
+    return NDF_OK;
+}
+
+ndf_status compute_download_locked(ndf_plan *plan, void *output, char *err, size_t err_len)
+{
+    if (!plan->uploaded) return fail(NDF_ERR_INVALID, "no input has been uploaded to this plan", err, err_len);
+    plan->uploaded = false;
+    const size_t bytes = plan->len * (size_t)plan->dbytes;
+    hipError_t e;
+    if ((e = launch_fill(4, plan->d_error.ptr, 1, 0, nullptr)) != hipSuccess) return fail_hip(e, err, err_len);
+    ndf_status st = launch(plan, plan->d_in.ptr, plan->d_out.ptr, nullptr, err, err_len);
+    if (st != NDF_OK) return st;
+    if ((e = hipMemcpy(output, plan->d_out.ptr, bytes, hipMemcpyDeviceToHost)) != hipSuccess) {
+        return fail_hip(e, err, err_len);
+    }
+    int32_t flag = 0;
+    if ((e = hipMemcpy(&flag, plan->d_error.ptr, sizeof(flag), hipMemcpyDeviceToHost)) != hipSuccess) {
+        return fail_hip(e, err, err_len);
+    }
+    if (flag) return fail(NDF_ERR_DEVICE, "internal error: rank walk left its bounds", err, err_len);
+    return NDF_OK;
+}
+
+}  // namespace
+
+ndf_status ndf_plan_upload(ndf_plan *plan, const void *input, size_t len, char *err, size_t err_len)
+{
+    if (len != plan->len) return fail_code(NDF_ERR_LENGTH_MISMATCH, err, err_len);
+    if (len == 0) return NDF_OK;
+    std::lock_guard<std::mutex> guard(plan->mutex);
+    return upload_locked(plan, input, err, err_len);
+}
+
+ndf_status ndf_plan_compute_download(ndf_plan *plan, void *output, size_t len, char *err, size_t err_len)
+{
+    if (len != plan->len) return fail_code(NDF_ERR_LENGTH_MISMATCH, err, err_len);
+    if (len == 0) return NDF_OK;
+    std::lock_guard<std::mutex> guard(plan->mutex);
+    return compute_download_locked(plan, output, err, err_len);
+}
+
+ndf_status ndf_plan_apply(ndf_plan *plan, const void *input, void *output, size_t len, char *err, size_t err_len)
+{
+    if (len != plan->len) return fail_code(NDF_ERR_LENGTH_MISMATCH, err, err_len);
+    if (len == 0) return NDF_OK;
+    std::lock_guard<std::mutex> guard(plan->mutex);
+    const ndf_status st = upload_locked(plan, input, err, err_len);
+    if (st != NDF_OK) return st;
+    return compute_download_locked(plan, output, err, err_len);
+}
+
+ndf_status ndf_window_filter(int32_t dtype_bytes, const void *input, void *output, size_t rank,
+                             const size_t *shape, const size_t *size, const uint8_t *footprint,
+                             const ptrdiff_t *origin, const int32_t *modes, double cval, int32_t kind,
+                             int32_t omit_nan, int32_t ignore_inf, size_t min_count, size_t parameter, char *err,
+                             size_t err_len)
+{
+    ndf_plan *plan = nullptr;
+    ndf_status st = ndf_plan_create(&plan, dtype_bytes, rank, shape, size, footprint, origin, modes, cval, kind,
+                                    omit_nan, ignore_inf, min_count, parameter, err, err_len);
+    if (st != NDF_OK) return st;
+    st = ndf_plan_apply(plan, input, output, plan->len, err, err_len);
+    ndf_plan_destroy(plan);
+    return st;
+}
+
+size_t ndf_get_num_threads(void)
+{
+    const char *env = getenv("RAYON_NUM_THREADS");
+    if (env && *env) {
+        const long v = strtol(env, nullptr, 10);
+        if (v > 0) return (size_t)v;
+    }
+    cpu_set_t set;
+    if (sched_getaffinity(0, sizeof(set), &set) == 0) {
+        const int n = CPU_COUNT(&set);
+        if (n > 0) return (size_t)n;
+    }
+    const long n = sysconf(_SC_NPROCESSORS_ONLN);
+    return n > 0 ? (size_t)n : 1;
+}
+
+ndf_status ndf_device_count(int32_t *count)
+{
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    *count = e == hipSuccess ? n : 0;
+    return e == hipSuccess ? NDF_OK : NDF_ERR_DEVICE;
+}
+
+ndf_status ndf_device_malloc(void **ptr, size_t bytes)
+{
+    return hipMalloc(ptr, bytes) == hipSuccess ? NDF_OK : NDF_ERR_DEVICE;
+}
+
+ndf_status ndf_device_free(void *ptr) { return hipFree(ptr) == hipSuccess ? NDF_OK : NDF_ERR_DEVICE; }
+
+ndf_status ndf_memcpy_to_device(void *d_dst, const void *h_src, size_t bytes)
+{
+    return hipMemcpy(d_dst, h_src, bytes, hipMemcpyHostToDevice) == hipSuccess ? NDF_OK : NDF_ERR_DEVICE;
+}
+
+ndf_status ndf_memcpy_to_host(void *h_dst, const void *d_src, size_t bytes)
+{
+    return hipMemcpy(h_dst, d_src, bytes, hipMemcpyDeviceToHost) == hipSuccess ? NDF_OK : NDF_ERR_DEVICE;
+}
+
+ndf_status ndf_device_synchronize(void) { return hipDeviceSynchronize() == hipSuccess ? NDF_OK : NDF_ERR_DEVICE; }
+
+ndf_status ndf_plan_time_device(ndf_plan *plan, const void *d_input, void *d_output, size_t len, int32_t warmup,
+                                int32_t iterations, double *ms_per_launch, char *err, size_t err_len)
+{
+    if (len != plan->len) return fail_code(NDF_ERR_LENGTH_MISMATCH, err, err_len);
+    std::lock_guard<std::mutex> guard(plan->mutex);
+    ndf_status st = prepare_tables(plan, err, err_len);
+    if (st != NDF_OK) return st;
+    hipStream_t stream = nullptr;
+    hipEvent_t start = nullptr, stop = nullptr;
+    hipError_t e;
+    if ((e = hipStreamCreate(&stream)) != hipSuccess) return fail_hip(e, err, err_len);
+    if ((e = hipEventCreate(&start)) != hipSuccess || (e = hipEventCreate(&stop)) != hipSuccess) {
+        (void)hipStreamDestroy(stream);
+        return fail_hip(e, err, err_len);
+    }
+    if ((e = launch_fill(4, plan->d_error.ptr, 1, 0, stream)) != hipSuccess) st = fail_hip(e, err, err_len);
+    for (int i = 0; st == NDF_OK && i < warmup; ++i) st = launch(plan, d_input, d_output, stream, err, err_len);
+    if (st == NDF_OK && (e = hipEventRecord(start, stream)) != hipSuccess) st = fail_hip(e, err, err_len);
+    for (int i = 0; st == NDF_OK && i < iterations; ++i) st = launch(plan, d_input, d_output, stream, err, err_len);
+    if (st == NDF_OK && (e = hipEventRecord(stop, stream)) != hipSuccess) st = fail_hip(e, err, err_len);
+    if (st == NDF_OK && (e = hipEventSynchronize(stop)) != hipSuccess) st = fail_hip(e, err, err_len);
+    float ms = 0.0f;
+    if (st == NDF_OK && (e = hipEventElapsedTime(&ms, start, stop)) != hipSuccess) st = fail_hip(e, err, err_len);
+    if (st == NDF_OK) {
+        *ms_per_launch = iterations > 0 ? (double)ms / iterations : 0.0;
+        int32_t flag = 0;
+        if ((e = hipMemcpy(&flag, plan->d_error.ptr, sizeof(flag), hipMemcpyDeviceToHost)) != hipSuccess) {
+            st = fail_hip(e, err, err_len);
+        } else if (flag) {
+            st = fail(NDF_ERR_DEVICE, "internal error: rank walk left its bounds", err, err_len);
+        }
+    }
+    (void)hipEventDestroy(start);
+    (void)hipEventDestroy(stop);
+    (void)hipStreamDestroy(stream);
+    return st;
+}
+
+}  // extern "C"

@@ -1,0 +1,83 @@
+// Internal interfaces between the host planner (ndf_host.hip) and the device
+// engines (ndf_generic.hip, ndf_slide.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace ndf {
+
+constexpr int MAX_RANK = 32;
+
+// Per-axis boundary maps hold premultiplied input offsets:
+//   >= 0  offset of the mapped coordinate (index * stride)
+//   MAP_CVAL   the sample is cval (constant mode, outside the axis)
+//   MAP_PAD    the coordinate only feeds outputs beyond the array (tile
+//              padding); the sample is omitted and never reaches a result.
+constexpr int64_t MAP_CVAL = -1;
+constexpr int64_t MAP_PAD = -2;
+
+// Engine 1: one thread per output, any rank and footprint. Samples are
+// gathered from global memory through the maps; selection is a bitwise
+// binary search over the key (count of keys below a candidate, one bit per
+// pass). Used for ranks above 3, geometries the tiled engine cannot stage,
+// and as an independent cross-check of the tiled engine.
+struct GenericArgs {
+    const void *input;
+    void *output;
+    int64_t len;
+    int32_t rank;
+    int32_t count;
+    int64_t stride[MAX_RANK];   // output/input strides in elements (C order)
+    int64_t map_base[MAX_RANK]; // start of axis a's map in `maps`
+    const int64_t *maps;        // entry (map_base[a] + coord + rel) for tap rel
+    const int32_t *taps;        // count * rank entries: tap offset - low offset
+    uint64_t cval_bits;
+    int32_t ignore_inf;
+    int32_t stat;
+    int64_t min_count;
+    int64_t rank_param;
+};
+
+// Engine 2: LDS-staged tile, block sort of the support, per-lane column slide
+// with a rank walk. Canonical 3-D geometry (nz planes of ny x nx); a tile is
+// 64 output columns x ty rows of one plane; wave w of the block slides down
+// rows [w*tw, (w+1)*tw).
+struct SlideArgs {
+    const void *input;
+    void *output;
+    int64_t nz, ny, nx;
+    int64_t in_sz, in_sy;  // input strides of z and y (x stride is 1)
+    const int64_t *mapz;   // index z0 + z, z in [0, kz)
+    const int64_t *mapy;   // index y0 + y, y in [0, sy)
+    const int64_t *mapx;   // index x0 + x, x in [0, sx)
+    int32_t kz, ky, kx;    // window bounding box
+    int32_t sx, sy, s;     // support: sx = 64 + kx - 1, sy = ty + ky - 1, s = kz*sy*sx
+    int32_t npad;          // power of two >= s (sort length)
+    int32_t ty, tw;        // tile rows and rows per wave
+    int32_t tiles_x, tiles_y;
+    int32_t count;         // taps in the window
+    const uint32_t *taps;  // count support offsets (dz*sy + dy)*sx + dx
+    int32_t nedge;
+    const uint32_t *edge_remove;  // offsets relative to row y-1
+    const uint32_t *edge_add;     // offsets relative to row y
+    const uint8_t *mask;          // kz*ky*kx footprint, nullptr for a rectangle
+    uint64_t cval_bits;
+    int32_t ignore_inf;
+    int32_t stat;
+    int32_t min_count;
+    int32_t rank_param;
+    int32_t *error_flag;  // set when a rank walk leaves its bounds
+};
+
+constexpr int SLIDE_WAVES = 4;
+constexpr int SLIDE_TILE_X = 64;
+constexpr uint32_t SLIDE_NO_POS = 0xFFFFFFFFu;
+
+hipError_t launch_generic(int dtype_bytes, const GenericArgs &args, hipStream_t stream);
+hipError_t launch_slide(int dtype_bytes, const SlideArgs &args, size_t lds_bytes, hipStream_t stream);
+hipError_t launch_fill(int dtype_bytes, void *output, int64_t len, uint64_t bits, hipStream_t stream);
+size_t slide_lds_bytes(int dtype_bytes, int s, int npad, int mask_bytes);
+
+}  // namespace ndf

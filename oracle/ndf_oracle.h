@@ -1,0 +1,72 @@
+/*
+ * CPU oracle for the NaN-aware window-selection path (test infrastructure).
+ *
+ * This library is the checker, never the product: only the test suite, the
+ * smoke check and the CPU-baseline leg of the benchmark load it. It restates
+ * the reference's CPU semantics in plain C:
+ *
+ *   - boundary mapping            src/boundary.rs:29-71
+ *   - window taps / gather        src/window.rs:44-137, 211-294
+ *   - NaN/inf policy + selection  src/reduction.rs:78-130
+ *   - total-order keys            src/median_network.rs:9-37
+ *   - even-count midpoint         docs/semantics.md:74-85, tests/_reference.py:36-47
+ *   - block rank histogram        src/histogram.rs:30-631 (used as the fast
+ *                                 multi-threaded CPU baseline and as a second,
+ *                                 independent algorithm for cross-checking)
+ */
+#ifndef NDF_ORACLE_H
+#define NDF_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes; identical numbering to include/ndfgpu.h. */
+enum {
+    NDFO_OK = 0,
+    NDFO_EMPTY_AXIS = 1,
+    NDFO_INVALID = 2,
+    NDFO_LENGTH_MISMATCH = 3,
+    NDFO_MEMORY = 4,
+    NDFO_OVERFLOW = 5
+};
+
+/* Boundary modes in the order of src/boundary.rs:16-27. */
+enum { NDFO_REFLECT = 0, NDFO_CONSTANT = 1, NDFO_NEAREST = 2, NDFO_MIRROR = 3, NDFO_WRAP = 4 };
+
+enum { NDFO_STAT_MEDIAN = 0, NDFO_STAT_RANK = 1 };
+enum { NDFO_METHOD_DIRECT = 0, NDFO_METHOD_HISTOGRAM = 1 };
+
+/* Maps `index` on an axis of `length`; *mapped = -1 marks a constant
+ * boundary sample. Returns NDFO_EMPTY_AXIS for length 0. */
+int ndfo_map_index(int64_t index, uint64_t length, int32_t mode, int64_t *mapped);
+
+/* Even-count midpoint of two f64 values (the reducers contract). */
+double ndfo_midpoint(double lower, double upper);
+
+/* Unsigned total-order keys and their inverses. */
+uint32_t ndfo_key32(uint32_t bits);
+uint64_t ndfo_key64(uint64_t bits);
+uint32_t ndfo_unkey32(uint32_t key);
+uint64_t ndfo_unkey64(uint64_t key);
+
+/* Full filter. `dtype_bytes` is 4 or 8; buffers are C-ordered with
+ * prod(shape) elements. `footprint` is NULL (rectangle) or prod(size) bytes.
+ * statistic: NDFO_STAT_MEDIAN or NDFO_STAT_RANK (with `parameter` = rank).
+ * omit_nan = 0 selects the propagate policy. method: direct or histogram.
+ * nthreads <= 0 uses one thread. On error, a message is written to err. */
+int ndfo_filter(int dtype_bytes, const void *input, void *output, size_t rank,
+                const size_t *shape, const size_t *size, const uint8_t *footprint,
+                const ptrdiff_t *origin, const int32_t *modes, double cval,
+                int32_t statistic, size_t parameter, int32_t omit_nan, int32_t ignore_inf,
+                size_t min_count, int32_t method, int32_t nthreads, char *err,
+                size_t err_len);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

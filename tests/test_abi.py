@@ -1,0 +1,90 @@
+"""The C-ABI library loads and exports every entry point of include/ndfgpu.h.
+
+Runs without a GPU: symbol lookup and plan construction only (no kernel
+launches). Also checks the engine and tile choice for the benchmark shapes.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import json
+import os
+import re
+
+import numpy as np
+import pytest
+
+from ndfiltrs_amd import _core, _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "ndfgpu.h")
+
+
+def declared_functions() -> list[str]:
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(ndf_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_header_declares_the_entry_points() -> None:
+    names = declared_functions()
+    for required in ("ndf_window_filter", "ndf_plan_create", "ndf_plan_apply", "ndf_plan_apply_device",
+                     "ndf_plan_destroy", "ndf_get_num_threads"):
+        assert required in names
+
+
+def test_library_exports_every_declared_symbol() -> None:
+    lib = ctypes.CDLL(_lib.LIBRARY)
+    missing = [name for name in declared_functions() if not hasattr(lib, name)]
+    assert not missing, missing
+
+
+def test_library_is_built_for_gfx950_only() -> None:
+    data = open(_lib.LIBRARY, "rb").read()
+    assert b"gfx950" in data
+    for other in (b"gfx90a", b"gfx942", b"gfx1100"):
+        assert b"amdgcn-amd-amdhsa--" + other not in data
+
+
+def _plan(dtype, shape, size, **kw) -> dict:
+    ndim = len(shape)
+    plan = _core.Plan(
+        dtype, shape, kw.get("kind", "median"), size, kw.get("footprint"), kw.get("origin", [0] * ndim),
+        kw.get("mode", ["reflect"] * ndim), 0.0, kw.get("omit", True), kw.get("ignore_inf", False), 1,
+        kw.get("parameter", 0),
+    )
+    return json.loads(plan.describe())
+
+
+def test_engine_choice_for_the_benchmark_shapes() -> None:
+    c5 = _plan(np.float32, (8192, 8192), (63, 63))
+    assert c5["engine"] == "slide" and c5["tile"] == [64, 64] and c5["lds_bytes"] <= 163840
+    c2 = _plan(np.float64, (4096, 4096), (31, 31), mode=["nearest"] * 2)
+    assert c2["engine"] == "slide" and c2["lds_bytes"] <= 163840
+    yy, xx = np.mgrid[-6:7, -6:7]
+    ring = (((yy**2 + xx**2) >= 16) & ((yy**2 + xx**2) <= 36)).ravel().tolist()
+    c3 = _plan(np.float32, (8192, 8192), (13, 13), footprint=ring, origin=[1, 0], mode=["wrap"] * 2)
+    assert c3["engine"] == "slide" and c3["count"] == 68
+    cube = _plan(np.float32, (128, 1024, 1024), (3, 7, 7), mode=["wrap"] * 3)
+    assert cube["engine"] == "slide" and cube["count"] == 147
+    five = _plan(np.float64, (2, 2, 2, 2, 3), (3, 3, 3, 3, 5))
+    assert five["engine"] == "generic"
+    fill = _plan(np.float32, (10, 10), (3, 3), kind="rank", parameter=9)
+    assert fill["engine"] == "fill"
+
+
+def test_plan_rejects_length_mismatch_without_a_device() -> None:
+    plan = _core.Plan(np.float32, (4, 4), "median", (3, 3), None, (0, 0), ("reflect",) * 2, 0.0, False, False, 1, 0)
+    err = ctypes.create_string_buffer(128)
+    buf = np.zeros(15, dtype=np.float32)
+    status = _lib.lib.ndf_plan_apply(plan._handle, buf.ctypes.data, buf.ctypes.data, 15, err, len(err))
+    assert status == _lib.ERR_LENGTH_MISMATCH
+    assert err.value.decode() == "input length does not match the configured shape"
+
+
+@pytest.mark.parametrize("engine", ["generic", "slide", "auto"])
+def test_engines_can_be_forced(engine: str) -> None:
+    plan = _core.Plan(np.float32, (40, 40), "median", (5, 5), None, (0, 0), ("reflect",) * 2, 0.0, False, False, 1, 0)
+    plan.set_engine(engine)
+    described = json.loads(plan.describe())["engine"]
+    assert described == ("slide" if engine == "auto" else engine)
