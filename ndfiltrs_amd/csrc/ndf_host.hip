@@ -107,13 +107,6 @@ int64_t map_coordinate(int64_t index, int64_t length, int32_t mode)
     }
 }
 
-size_t next_pow2(size_t v)
-{
-    size_t p = 1;
-    while (p < v) p <<= 1;
-    return p;
-}
-
 struct DeviceBuffer {
     void *ptr = nullptr;
     size_t bytes = 0;
@@ -244,28 +237,36 @@ bool plan_slide(ndf_plan *p)
     if (mask_bytes > 16384) return false;
     // Rows per wave: as many as LDS allows, but no more than the image needs.
     const int limit = lds_limit();
-    auto fits = [&](int cand) {
-        const int64_t ty = (int64_t)SLIDE_WAVES * cand;
-        const int64_t sy = ty + ky - 1;
-        const int64_t s = kz * sy * sx;
-        if (sy > 1023 || s > 65534) return false;
-        const int64_t npad = (int64_t)next_pow2((size_t)s);
-        return slide_lds_bytes(p->dbytes, (int)s, (int)npad, mask_bytes) <= (size_t)limit;
+    // Tile rows: the support (kz x (ty + ky - 1) x sx samples) must fit the
+    // sort (256 * E <= 16384 entries, u16 positions) and LDS. Larger tiles
+    // amortise the sort and the per-wave start; a support much larger than
+    // the window lengthens every rank walk, so it is capped at 32 windows.
+    const int64_t hist_stride = rect ? (int64_t)16 * (sx + 1) : 0;
+    auto support_of = [&](int64_t ty) { return kz * (ty + ky - 1) * sx; };
+    auto sort_e_of = [](int64_t s) {
+        int e = 4;
+        while ((int64_t)256 * e < s) e <<= 1;
+        return e;
     };
-    int tw = 0;
-    for (int cand : {16, 8, 4, 2, 1}) {
-        if (fits(cand)) {
-            tw = cand;
-            break;
-        }
+    auto fits = [&](int64_t ty) {
+        const int64_t s = support_of(ty);
+        if (ty + ky - 1 > 4096 || s > 16384) return false;
+        const int e = sort_e_of(s);
+        return e <= 64 && slide_lds_bytes(p->dbytes, (int)s, e, (int)hist_stride, mask_bytes) <= (size_t)limit;
+    };
+    const int64_t rows_needed = ((dims[1] + SLIDE_WAVES - 1) / SLIDE_WAVES) * SLIDE_WAVES;
+    int64_t ty = 0;
+    for (int64_t cand = SLIDE_WAVES; cand <= 256; cand += SLIDE_WAVES) {
+        if (!fits(cand)) break;
+        if (cand > SLIDE_WAVES && support_of(cand) > 32 * (int64_t)p->count) break;
+        ty = cand;
+        if (cand >= rows_needed) break;
     }
-    if (tw == 0) return false;
-    // A smaller tile is enough when it still covers every row of the image.
-    while (tw > 1 && (int64_t)SLIDE_WAVES * (tw / 2) >= dims[1]) tw /= 2;
-    const int ty = SLIDE_WAVES * tw;
-    const int sy = ty + (int)ky - 1;
-    const int s = (int)(kz * sy * sx);
-    const int npad = (int)next_pow2((size_t)s);
+    if (ty == 0) return false;
+    const int tw = (int)(ty / SLIDE_WAVES);
+    const int sy = (int)(ty + ky - 1);
+    const int s = (int)support_of(ty);
+    const int sort_e = sort_e_of(s);
     const int64_t tiles_x = (dims[2] + SLIDE_TILE_X - 1) / SLIDE_TILE_X;
     const int64_t tiles_y = (dims[1] + ty - 1) / ty;
     if (tiles_x * tiles_y * dims[0] > 0x7FFFFFFF) return false;
@@ -318,8 +319,11 @@ bool plan_slide(ndf_plan *p)
     s_args.sx = (int32_t)sx;
     s_args.sy = sy;
     s_args.s = s;
-    s_args.npad = npad;
-    s_args.ty = ty;
+    s_args.sort_e = sort_e;
+    s_args.hist_stride = (int32_t)hist_stride;
+    s_args.inv_sx = 1.0f / (float)sx;
+    s_args.inv_sy = 1.0f / (float)sy;
+    s_args.ty = (int32_t)ty;
     s_args.tw = tw;
     s_args.tiles_x = (int32_t)tiles_x;
     s_args.tiles_y = (int32_t)tiles_y;
@@ -330,7 +334,7 @@ bool plan_slide(ndf_plan *p)
     s_args.stat = p->stat;
     s_args.min_count = (int32_t)p->eff_min_count;
     s_args.rank_param = (int32_t)std::min(p->parameter, p->count);
-    p->slide_lds = slide_lds_bytes(p->dbytes, s, npad, mask_bytes);
+    p->slide_lds = slide_lds_bytes(p->dbytes, s, sort_e, (int)hist_stride, mask_bytes);
     return true;
 }
 
@@ -619,7 +623,7 @@ ndf_status ndf_plan_describe(const ndf_plan *plan, char *buffer, size_t buffer_l
         snprintf(buffer, buffer_len,
                  "{\"engine\": \"%s\", \"count\": %zu, \"tile\": [%d, %d], \"support\": %d, \"sort_len\": %d, "
                  "\"lds_bytes\": %zu, \"edge_taps\": %d, \"tiles\": %lld}",
-                 engine, plan->count, plan->slide.ty, SLIDE_TILE_X, plan->slide.s, plan->slide.npad, plan->slide_lds,
+                 engine, plan->count, plan->slide.ty, SLIDE_TILE_X, plan->slide.s, 256 * plan->slide.sort_e, plan->slide_lds,
                  plan->slide.nedge,
                  (long long)plan->slide.tiles_x * plan->slide.tiles_y * plan->slide.nz);
     } else {

@@ -43,7 +43,7 @@ struct GenericArgs {
 // Engine 2: LDS-staged tile, block sort of the support, per-lane column slide
 // with a rank walk. Canonical 3-D geometry (nz planes of ny x nx); a tile is
 // 64 output columns x ty rows of one plane; wave w of the block slides down
-// rows [w*tw, (w+1)*tw).
+// rows [w*tw, (w+1)*tw). Support positions are stored as u16 (s <= 16384).
 struct SlideArgs {
     const void *input;
     void *output;
@@ -54,11 +54,13 @@ struct SlideArgs {
     const int64_t *mapx;   // index x0 + x, x in [0, sx)
     int32_t kz, ky, kx;    // window bounding box
     int32_t sx, sy, s;     // support: sx = 64 + kx - 1, sy = ty + ky - 1, s = kz*sy*sx
-    int32_t npad;          // power of two >= s (sort length)
+    int32_t sort_e;        // sort elements per thread; sort length 256 * sort_e >= s
+    int32_t hist_stride;   // u16 entries of one wave's histogram region
+    float inv_sx, inv_sy;  // reciprocals for position decoding
     int32_t ty, tw;        // tile rows and rows per wave
     int32_t tiles_x, tiles_y;
     int32_t count;         // taps in the window
-    const uint32_t *taps;  // count support offsets (dz*sy + dy)*sx + dx
+    const uint32_t *taps;  // count support offsets (dz*sy + dy)*sx + dx (footprints)
     int32_t nedge;
     const uint32_t *edge_remove;  // offsets relative to row y-1
     const uint32_t *edge_add;     // offsets relative to row y
@@ -78,6 +80,6 @@ constexpr uint32_t SLIDE_NO_POS = 0xFFFFFFFFu;
 hipError_t launch_generic(int dtype_bytes, const GenericArgs &args, hipStream_t stream);
 hipError_t launch_slide(int dtype_bytes, const SlideArgs &args, size_t lds_bytes, hipStream_t stream);
 hipError_t launch_fill(int dtype_bytes, void *output, int64_t len, uint64_t bits, hipStream_t stream);
-size_t slide_lds_bytes(int dtype_bytes, int s, int npad, int mask_bytes);
+size_t slide_lds_bytes(int dtype_bytes, int s, int sort_e, int hist_stride, int mask_bytes);
 
 }  // namespace ndf

@@ -2,18 +2,27 @@
 //
 // This is the reference's block rank histogram (src/histogram.rs:1-16,
 // 302-485) re-designed for a 64-wide wavefront:
-//   * a workgroup stages the support of a 64 x ty output tile (tile + window
-//     halo, mapped through the boundary maps) into LDS with coalesced row
-//     loads, classifying each sample under the policy;
-//   * the retained samples are sorted once (bitonic, in LDS), giving every
-//     support sample a tile rank (codes) and every rank its position;
-//   * each lane owns one output column and slides its window down the tile.
-//     Instead of a bit histogram per window it keeps one cursor: a rank m and
-//     c = number of in-window ranks below m. A step updates c from the removed
-//     and added edge taps only, then walks m to the target rank, testing
-//     window membership of each rank it passes from its position.
-// The selected rank maps back to the original key bits, so the output is the
+//   * a workgroup (4 waves) stages the support of a 64 x ty output tile (tile
+//     plus window halo, mapped through the boundary maps) into LDS with
+//     coalesced row loads; retained samples are compacted, omitted ones
+//     (NaN, and +-inf under ignore_inf) never enter the sort;
+//   * the retained samples are sorted once per tile by a register-resident
+//     bitonic network (each thread owns E consecutive elements; strides
+//     below E stay in registers, larger strides exchange through LDS), which
+//     gives every support sample a tile rank ("code") and every rank its
+//     position;
+//   * each lane owns one output column and slides its window down the rows
+//     of its wave. Instead of a bit histogram per window it keeps a cursor:
+//     a rank m and c = number of in-window ranks below m. A step updates c
+//     from the removed and added edge taps only, then walks m to the target
+//     rank, testing window membership of 16 ranks per LDS batch;
+//   * a wave starts from a two-level rank histogram of its first window
+//     (column counts for rectangles, per-lane counts for footprints), so the
+//     first walk is short.
+// The selected rank maps back to the original key, so the output is the
 // exact sample (or the f64 midpoint of two), independent of the tile shape.
+#include <type_traits>
+
 #include "ndf_device.hpp"
 #include "ndf_internal.hpp"
 
@@ -21,35 +30,378 @@ namespace ndf {
 namespace {
 
 constexpr uint16_t CODE_OMIT = 0xFFFF;
+constexpr uint16_t POS_NONE = 0xFFFF;
+constexpr int NT = SLIDE_WAVES * 64;
+constexpr int BINS = 16;
+constexpr int BATCH = 16;
 
-__device__ inline bool in_window(uint32_t p, int lane, int y, int kz, int ky, int kx, const uint8_t *mask)
-{
-    const int x = (int)(p & 0x3FFu);
-    const int yy = (int)((p >> 10) & 0x3FFu);
-    const int zz = (int)(p >> 20);
-    const unsigned dx = (unsigned)(x - lane);
-    const unsigned dy = (unsigned)(yy - y);
-    if (dx >= (unsigned)kx || dy >= (unsigned)ky || (unsigned)zz >= (unsigned)kz) {
-        return false;
+struct Geometry {
+    int sx, sy, kz, ky, kx;
+    float inv_sx, inv_sy;
+    const uint8_t *mask;
+
+    // Window membership of the support sample at flat position p for the
+    // lane's window whose top-left support sample is (z=0, y, lane).
+    __device__ inline bool contains(uint32_t p, int lane, int y) const
+    {
+        if (p == POS_NONE) {
+            return false;
+        }
+        const int zy = (int)(((float)p + 0.5f) * inv_sx);
+        const int x = (int)p - zy * sx;
+        const unsigned dx = (unsigned)(x - lane);
+        if (dx >= (unsigned)kx) {
+            return false;
+        }
+        int z = 0;
+        int yy = zy;
+        if (kz > 1) {
+            z = (int)(((float)zy + 0.5f) * inv_sy);
+            yy = zy - z * sy;
+        }
+        const unsigned dy = (unsigned)(yy - y);
+        if (dy >= (unsigned)ky) {
+            return false;
+        }
+        return mask == nullptr || mask[(z * ky + (int)dy) * kx + (int)dx] != 0;
     }
-    return mask == nullptr || mask[(zz * ky + (int)dy) * kx + (int)dx] != 0;
+};
+
+// Position of the `need`-th set bit (0-based) of a 16-bit mask holding more
+// than `need` set bits.
+__device__ inline int select_bit16(uint32_t mask, int need)
+{
+    int base = 0;
+    for (int width = 8; width >= 1; width >>= 1) {
+        const int low = __builtin_popcount(mask & ((1u << width) - 1u));
+        if (need >= low) {
+            need -= low;
+            mask >>= width;
+            base += width;
+        }
+    }
+    return base;
 }
 
-template <typename T>
-__global__ void __launch_bounds__(SLIDE_WAVES * 64) k_slide(SlideArgs a)
+// Membership bits of ranks [first, first + BATCH) (bit i <-> rank first + i).
+__device__ inline uint32_t batch_mask(const uint16_t *pos, int first, int retained, const Geometry &g, int lane,
+                                      int y)
+{
+    uint32_t mask = 0;
+#pragma unroll
+    for (int i = 0; i < BATCH; ++i) {
+        const int r = first + i;
+        const bool inside = r >= 0 && r < retained && g.contains(pos[r], lane, y);
+        mask |= (uint32_t)inside << i;
+    }
+    return mask;
+}
+
+// Moves the cursor (m, c) to the rank m with c == k in-window ranks below it
+// and m itself in the window. Returns false if the cursor would leave
+// [0, retained), which a consistent cursor never does.
+__device__ inline bool walk_to(int &m, int &c, int k, const uint16_t *pos, int retained, const Geometry &g, int lane,
+                               int y)
+{
+    const int limit = retained / BATCH + 4;
+    if (c > k) {
+        // The (c - k)-th in-window rank below m, counting downward.
+        for (int step = 0; step <= limit; ++step) {
+            const uint32_t mask = batch_mask(pos, m - BATCH, retained, g, lane, y);
+            const int count = __builtin_popcount(mask);
+            const int need = c - k;
+            if (count >= need) {
+                m = m - BATCH + select_bit16(mask, count - need);
+                c = k;
+                return true;
+            }
+            c -= count;
+            m -= BATCH;
+            if (m <= 0) {
+                return false;
+            }
+        }
+        return false;
+    }
+    // The (k - c)-th in-window rank at or above m, counting upward.
+    for (int step = 0; step <= limit; ++step) {
+        const uint32_t mask = batch_mask(pos, m, retained, g, lane, y);
+        const int count = __builtin_popcount(mask);
+        const int need = k - c;
+        if (count > need) {
+            m += select_bit16(mask, need);
+            c = k;
+            return true;
+        }
+        c += count;
+        m += BATCH;
+        if (m >= retained) {
+            return false;
+        }
+    }
+    return false;
+}
+
+// Largest in-window rank below m, or -1.
+__device__ inline int previous_member(int m, const uint16_t *pos, int retained, const Geometry &g, int lane, int y)
+{
+    for (int first = m - BATCH; first > -BATCH; first -= BATCH) {
+        const uint32_t mask = batch_mask(pos, first, retained, g, lane, y) & ((1u << (m - first)) - 1u);
+        if (mask) {
+            return first + 31 - __builtin_clz(mask);
+        }
+        m = first;
+    }
+    return -1;
+}
+
+// Packed 16-bin counters: 8 bits per bin while counts stay below 256,
+// otherwise 16 bits per bin.
+struct Bins8 {
+    uint64_t lo = 0, hi = 0;
+    __device__ inline void add(int b)
+    {
+        const uint64_t one = 1ull << ((b & 7) * 8);
+        if (b < 8) {
+            lo += one;
+        } else {
+            hi += one;
+        }
+    }
+    __device__ inline int get(int b) const { return (int)(((b < 8 ? lo : hi) >> ((b & 7) * 8)) & 0xFF); }
+};
+
+struct Bins16 {
+    uint64_t q[4] = {0, 0, 0, 0};
+    __device__ inline void add(int b)
+    {
+        const uint64_t one = 1ull << ((b & 3) * 16);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            q[i] += (b >> 2) == i ? one : 0ull;
+        }
+    }
+    __device__ inline int get(int b) const { return (int)((q[b >> 2] >> ((b & 3) * 16)) & 0xFFFF); }
+};
+
+// Counts, for one lane's window at row y, the codes in [lo, lo + 16 << shift)
+// per bin. Rectangles use per-column counts shared through LDS (hist); other
+// footprints scan their taps.
+template <typename Bins>
+__device__ void window_bins(int counts[BINS], int lo, int shift, const uint16_t *codes, uint16_t *hist,
+                            const SlideArgs &a, int lane, int y)
+{
+    const int hi = lo + (BINS << shift);
+    if (a.mask == nullptr) {
+        const int stride = a.sx + 1;
+        for (int x = lane; x < a.sx; x += 64) {
+            Bins bins;
+            for (int z = 0; z < a.kz; ++z) {
+                const uint16_t *column = codes + (z * a.sy + y) * a.sx + x;
+                for (int dy = 0; dy < a.ky; ++dy) {
+                    const int code = column[dy * a.sx];
+                    if (code >= lo && code < hi) {
+                        bins.add((code - lo) >> shift);
+                    }
+                }
+            }
+#pragma unroll
+            for (int b = 0; b < BINS; ++b) {
+                hist[b * stride + x] = (uint16_t)bins.get(b);
+            }
+        }
+        // Exclusive prefix over columns, one bin per lane. LDS accesses of one
+        // wave complete in order; the wave barrier keeps the compiler from
+        // moving them across the phases.
+        __builtin_amdgcn_wave_barrier();
+        if (lane < BINS) {
+            uint16_t *row = hist + lane * stride;
+            int run = 0;
+            for (int x0 = 0; x0 < a.sx; x0 += 8) {
+                int v[8];
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    v[i] = x0 + i < a.sx ? row[x0 + i] : 0;
+                }
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    if (x0 + i < a.sx) {
+                        row[x0 + i] = (uint16_t)run;
+                        run += v[i];
+                    }
+                }
+            }
+            row[a.sx] = (uint16_t)run;
+        }
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int b = 0; b < BINS; ++b) {
+            counts[b] = (int)hist[b * stride + lane + a.kx] - (int)hist[b * stride + lane];
+        }
+        __builtin_amdgcn_wave_barrier();
+    } else {
+        Bins bins;
+        const int base = y * a.sx + lane;
+        for (int t = 0; t < a.count; ++t) {
+            const int code = codes[base + a.taps[t]];
+            if (code >= lo && code < hi) {
+                bins.add((code - lo) >> shift);
+            }
+        }
+#pragma unroll
+        for (int b = 0; b < BINS; ++b) {
+            counts[b] = bins.get(b);
+        }
+    }
+}
+
+// Bitonic sort of NT * E (key, pos) pairs; thread t holds elements
+// [t*E, t*E + E) in registers. Strides below E run in registers; larger
+// strides exchange whole chunks with the partner thread through LDS
+// (xk/xp, rows of E + 1 entries so a chunk-wide access is bank-conflict
+// free).
+// Compile-time loop: calls f(std::integral_constant<int, I>) for I in [B, E).
+template <int B, int E, typename F>
+__device__ __forceinline__ void static_for(F &&f)
+{
+    if constexpr (B < E) {
+        f(std::integral_constant<int, B>{});
+        static_for<B + 1, E>(f);
+    }
+}
+
+template <typename Key, int E>
+struct Chunk {
+    Key k[E];
+    uint32_t p[E];
+
+    // Compare-exchange of local elements I < J: ascending puts the smaller
+    // key at I. Equal keys are never swapped.
+    template <int I, int J>
+    __device__ __forceinline__ void ce(bool asc)
+    {
+        const bool swap = asc ? (k[J] < k[I]) : (k[I] < k[J]);
+        const Key ki = k[I], kj = k[J];
+        const uint32_t pi = p[I], pj = p[J];
+        k[I] = swap ? kj : ki;
+        k[J] = swap ? ki : kj;
+        p[I] = swap ? pj : pi;
+        p[J] = swap ? pi : pj;
+    }
+
+    // One in-register bitonic stage of stride STRIDE; the direction of the
+    // pair (I, I + STRIDE) is ascending when bit SIZE of its global index
+    // (tbase | I) is clear.
+    template <int SIZE, int STRIDE>
+    __device__ __forceinline__ void stage(int tbase)
+    {
+        static_for<0, E>([&](auto ic) {
+            constexpr int I = decltype(ic)::value;
+            if constexpr ((I & STRIDE) == 0) {
+                ce<I, I + STRIDE>(((tbase | I) & SIZE) == 0);
+            }
+        });
+    }
+
+    template <int SIZE, int STRIDE>
+    __device__ __forceinline__ void stages_down(int tbase)
+    {
+        if constexpr (STRIDE > 0) {
+            stage<SIZE, STRIDE>(tbase);
+            stages_down<SIZE, STRIDE / 2>(tbase);
+        }
+    }
+
+    // Sizes 2 .. E: a full bitonic sort of the chunk (direction per the
+    // global index, so neighbouring chunks form bitonic sequences).
+    template <int SIZE>
+    __device__ __forceinline__ void local_sort(int tbase)
+    {
+        if constexpr (SIZE <= E) {
+            stages_down<SIZE, SIZE / 2>(tbase);
+            local_sort<SIZE * 2>(tbase);
+        }
+    }
+
+    // Strides E/2 .. 1 of a merge level larger than the chunk: one direction
+    // for the whole chunk.
+    __device__ __forceinline__ void merge_local(bool asc)
+    {
+        merge_down<E / 2>(asc);
+    }
+
+    template <int STRIDE>
+    __device__ __forceinline__ void merge_down(bool asc)
+    {
+        if constexpr (STRIDE > 0) {
+            static_for<0, E>([&](auto ic) {
+                constexpr int I = decltype(ic)::value;
+                if constexpr ((I & STRIDE) == 0) {
+                    ce<I, I + STRIDE>(asc);
+                }
+            });
+            merge_down<STRIDE / 2>(asc);
+        }
+    }
+};
+
+// Bitonic sort of NT * E (key, pos) pairs; thread t holds elements
+// [t*E, t*E + E) in registers. Strides below E run in registers; larger
+// strides exchange whole chunks with the partner thread through LDS
+// (xk/xp, rows of E + 1 entries so a chunk-wide access is bank-conflict
+// free).
+template <typename Key, int E>
+__device__ void block_sort(Chunk<Key, E> &c, Key *xk, uint16_t *xp, int t)
+{
+    constexpr int N = NT * E;
+    const int tbase = t * E;
+    c.template local_sort<2>(tbase);
+    for (int size = 2 * E; size <= N; size <<= 1) {
+        const bool asc = (tbase & size) == 0;
+        for (int stride = size >> 1; stride >= E; stride >>= 1) {
+            const int partner = t ^ (stride / E);
+            const bool keep_min = ((t & (stride / E)) == 0) == asc;
+            static_for<0, E>([&](auto ic) {
+                constexpr int I = decltype(ic)::value;
+                xk[t * (E + 1) + I] = c.k[I];
+                xp[t * (E + 1) + I] = (uint16_t)c.p[I];
+            });
+            __syncthreads();
+            static_for<0, E>([&](auto ic) {
+                constexpr int I = decltype(ic)::value;
+                const Key other = xk[partner * (E + 1) + I];
+                const uint32_t op = xp[partner * (E + 1) + I];
+                const bool take = keep_min ? (other < c.k[I]) : (c.k[I] < other);
+                c.k[I] = take ? other : c.k[I];
+                c.p[I] = take ? op : c.p[I];
+            });
+            __syncthreads();
+        }
+        c.merge_local(asc);
+    }
+}
+
+template <typename T, int E>
+__global__ void __launch_bounds__(NT, 1) k_slide(SlideArgs a)
 {
     using KT = KeyTraits<T>;
     using Key = typename KT::Key;
+    constexpr int N = NT * E;
     extern __shared__ __align__(16) unsigned char lds[];
+    // Layout: keys [N] | pos [N] | codes [s] | hist [4 waves] | mask.
+    // The sort's exchange rows (NT x (E + 1)) alias keys + pos + codes.
     Key *keys = reinterpret_cast<Key *>(lds);
-    uint32_t *pos = reinterpret_cast<uint32_t *>(keys + a.npad);
-    uint16_t *codes = reinterpret_cast<uint16_t *>(pos + a.npad);
-    uint8_t *mask = a.mask ? reinterpret_cast<uint8_t *>(codes + ((a.s + 1) & ~1)) : nullptr;
+    uint16_t *pos = reinterpret_cast<uint16_t *>(keys + N);
+    uint16_t *codes = pos + N;
+    uint16_t *hist_all = codes + ((a.s + 7) & ~7);
+    uint8_t *mask = a.mask ? reinterpret_cast<uint8_t *>(hist_all + SLIDE_WAVES * a.hist_stride) : nullptr;
+    Key *xk = keys;
+    uint16_t *xp = reinterpret_cast<uint16_t *>(xk + NT * (E + 1));
+    __shared__ int retained_shared;
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
-    constexpr int NT = SLIDE_WAVES * 64;
 
     int tile = blockIdx.x;
     const int tx = tile % a.tiles_x;
@@ -59,7 +411,12 @@ __global__ void __launch_bounds__(SLIDE_WAVES * 64) k_slide(SlideArgs a)
     const int64_t x0 = (int64_t)tx * SLIDE_TILE_X;
     const int64_t y0 = (int64_t)tyi * a.ty;
 
-    // ---- stage: one support row per wave iteration, lanes along x ----
+    if (tid == 0) {
+        retained_shared = 0;
+    }
+    __syncthreads();
+
+    // ---- stage: rows of the support, lanes along x; compact retained ----
     const Key *input = static_cast<const Key *>(a.input);
     const Key cval_key = classify<T>((Key)a.cval_bits, a.ignore_inf);
     const int rows = a.kz * a.sy;
@@ -68,212 +425,276 @@ __global__ void __launch_bounds__(SLIDE_WAVES * 64) k_slide(SlideArgs a)
         const int y = row - z * a.sy;
         const int64_t mz = a.mapz[z0 + z];
         const int64_t my = a.mapy[y0 + y];
-        for (int x = lane; x < a.sx; x += 64) {
-            const int64_t mx = a.mapx[x0 + x];
-            Key key;
-            if (mz == MAP_PAD || my == MAP_PAD || mx == MAP_PAD) {
-                key = KT::SENTINEL;
-            } else if (mz < 0 || my < 0 || mx < 0) {
-                key = cval_key;
-            } else {
-                key = classify<T>(input[mz + my + mx], a.ignore_inf);
-            }
-            const int p = row * a.sx + x;
-            keys[p] = key;
-            pos[p] = key == KT::SENTINEL ? SLIDE_NO_POS : ((uint32_t)z << 20) | ((uint32_t)y << 10) | (uint32_t)x;
-            codes[p] = CODE_OMIT;
-        }
-    }
-    for (int p = a.s + tid; p < a.npad; p += NT) {
-        keys[p] = KT::SENTINEL;
-        pos[p] = SLIDE_NO_POS;
-    }
-    if (mask) {
-        const int mbytes = a.kz * a.ky * a.kx;
-        for (int i = tid; i < mbytes; i += NT) {
-            mask[i] = a.mask[i];
-        }
-    }
-    __syncthreads();
-
-    // ---- bitonic sort of (key, pos) over npad entries ----
-    const int half = a.npad >> 1;
-    for (int k = 2; k <= a.npad; k <<= 1) {
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            for (int q = tid; q < half; q += NT) {
-                const int i = ((q & ~(j - 1)) << 1) | (q & (j - 1));
-                const int l = i + j;
-                const bool up = (i & k) == 0;
-                const Key ki = keys[i];
-                const Key kl = keys[l];
-                if ((ki > kl) == up) {
-                    keys[i] = kl;
-                    keys[l] = ki;
-                    const uint32_t pi = pos[i];
-                    pos[i] = pos[l];
-                    pos[l] = pi;
+        for (int x0l = 0; x0l < a.sx; x0l += 64) {
+            const int x = x0l + lane;
+            Key key = KT::SENTINEL;
+            if (x < a.sx) {
+                const int64_t mx = a.mapx[x0 + x];
+                if (mz == MAP_PAD || my == MAP_PAD || mx == MAP_PAD) {
+                    key = KT::SENTINEL;
+                } else if (mz < 0 || my < 0 || mx < 0) {
+                    key = cval_key;
+                } else {
+                    key = classify<T>(input[mz + my + mx], a.ignore_inf);
                 }
             }
-            __syncthreads();
-        }
-    }
-
-    // ---- ranks: codes[support position] = rank; R = retained count ----
-    __shared__ int retained_shared;
-    if (tid == 0) {
-        retained_shared = 0;
-    }
-    __syncthreads();
-    for (int r = tid; r < a.npad; r += NT) {
-        const uint32_t p = pos[r];
-        if (p != SLIDE_NO_POS) {
-            const int x = (int)(p & 0x3FFu);
-            const int yy = (int)((p >> 10) & 0x3FFu);
-            const int zz = (int)(p >> 20);
-            codes[(zz * a.sy + yy) * a.sx + x] = (uint16_t)r;
-            if (r + 1 == a.npad || pos[r + 1] == SLIDE_NO_POS) {
-                retained_shared = r + 1;
+            const bool keep = key != KT::SENTINEL;
+            const uint64_t ballot = __ballot(keep);
+            int base = 0;
+            if (lane == 0 && ballot) {
+                base = atomicAdd(&retained_shared, __popcll(ballot));
+            }
+            base = __shfl(base, 0);
+            if (keep) {
+                const int slot = base + (int)__popcll(ballot & ((1ull << lane) - 1ull));
+                keys[slot] = key;
+                pos[slot] = (uint16_t)(row * a.sx + x);
             }
         }
     }
     __syncthreads();
     const int retained = retained_shared;
 
+    // ---- sort the retained samples (padded with sentinels to N) ----
+    {
+        Chunk<Key, E> chunk;
+        static_for<0, E>([&](auto ic) {
+            constexpr int I = decltype(ic)::value;
+            const int g = tid * E + I;
+            chunk.k[I] = g < retained ? keys[g] : KT::SENTINEL;
+            chunk.p[I] = g < retained ? pos[g] : POS_NONE;
+        });
+        __syncthreads();
+        block_sort<Key, E>(chunk, xk, xp, tid);
+        __syncthreads();
+        static_for<0, E>([&](auto ic) {
+            constexpr int I = decltype(ic)::value;
+            keys[tid * E + I] = chunk.k[I];
+            pos[tid * E + I] = (uint16_t)chunk.p[I];
+        });
+        __syncthreads();
+        // Codes and the mask are written after the sort: the exchange rows
+        // may overlap them.
+        for (int q = tid; q < a.s; q += NT) {
+            codes[q] = CODE_OMIT;
+        }
+        if (mask) {
+            const int mbytes = a.kz * a.ky * a.kx;
+            for (int q = tid; q < mbytes; q += NT) {
+                mask[q] = a.mask[q];
+            }
+        }
+        __syncthreads();
+        static_for<0, E>([&](auto ic) {
+            constexpr int I = decltype(ic)::value;
+            if (chunk.p[I] != POS_NONE) {
+                codes[chunk.p[I]] = (uint16_t)(tid * E + I);
+            }
+        });
+    }
+    __syncthreads();
+
     // ---- per-lane slide ----
     const int y_first = wave * a.tw;
-    if (y0 + y_first >= a.ny) {
+    if (y0 + y_first >= a.ny || retained == 0) {
+        if (retained == 0) {
+            // Every window of the tile is empty: NaN.
+            for (int y = y_first; y < y_first + a.tw && y0 + y < a.ny; ++y) {
+                if (x0 + lane < a.nx) {
+                    static_cast<Key *>(a.output)[z0 * a.ny * a.nx + (y0 + y) * a.nx + x0 + lane] = KT::CANONICAL_NAN;
+                }
+            }
+        }
         return;
     }
+    const Geometry g{a.sx, a.sy, a.kz, a.ky, a.kx, a.inv_sx, a.inv_sy, mask};
+    uint16_t *hist = hist_all + wave * a.hist_stride;
     const bool col_valid = x0 + lane < a.nx;
     Key *output = static_cast<Key *>(a.output);
-    int m = retained >> 1;
-    int c = 0;
-    int n = 0;
+    const bool wide = a.kz * a.ky > 255 || (a.mask != nullptr && a.count > 255);
+
+    auto target_of = [&](int n) {
+        if (n < a.min_count || n <= 0) {
+            return -1;
+        }
+        if (a.stat == STAT_RANK) {
+            return a.rank_param < n ? a.rank_param : -1;
+        }
+        return n >> 1;
+    };
+
+    // Initial cursor: two levels of 16-bin histograms over the ranks.
+    int m = 0, c = 0, n = 0;
     {
-        const int base = y_first * a.sx + lane;
-        for (int t = 0; t < a.count; ++t) {
-            const uint16_t code = codes[base + a.taps[t]];
-            n += code != CODE_OMIT;
-            c += code < m;
+        int shift1 = 0;
+        while ((BINS << shift1) < retained) {
+            ++shift1;
+        }
+        int counts[BINS];
+        if (wide) {
+            window_bins<Bins16>(counts, 0, shift1, codes, hist, a, lane, y_first);
+        } else {
+            window_bins<Bins8>(counts, 0, shift1, codes, hist, a, lane, y_first);
+        }
+        n = 0;
+#pragma unroll
+        for (int b = 0; b < BINS; ++b) {
+            n += counts[b];
+        }
+        const int k = target_of(n);
+        int bin = -1, below = 0;
+        if (k >= 0) {
+            int run = 0;
+#pragma unroll
+            for (int b = 0; b < BINS; ++b) {
+                if (bin < 0 && k < run + counts[b]) {
+                    bin = b;
+                    below = run;
+                }
+                run += counts[b];
+            }
+        }
+        // Second level over the union of the lanes' first-level bins.
+        const uint64_t any = __ballot(bin >= 0);
+        int bmin = BINS, bmax = -1;
+        for (int b = 0; b < BINS; ++b) {
+            if (__ballot(bin == b)) {
+                bmin = b < bmin ? b : bmin;
+                bmax = b;
+            }
+        }
+        if (any && shift1 > 2) {
+            const int lo2 = bmin << shift1;
+            const int span = (bmax + 1 - bmin) << shift1;
+            int shift2 = 0;
+            while ((BINS << shift2) < span) {
+                ++shift2;
+            }
+            int counts2[BINS];
+            if (wide) {
+                window_bins<Bins16>(counts2, lo2, shift2, codes, hist, a, lane, y_first);
+            } else {
+                window_bins<Bins8>(counts2, lo2, shift2, codes, hist, a, lane, y_first);
+            }
+            if (bin >= 0) {
+                int below2 = 0;
+                for (int b = 0; b < bmin; ++b) {
+                    below2 += counts[b];
+                }
+                int run = below2, bin2 = -1, base2 = below2;
+#pragma unroll
+                for (int b = 0; b < BINS; ++b) {
+                    if (bin2 < 0 && k < run + counts2[b]) {
+                        bin2 = b;
+                        base2 = run;
+                    }
+                    run += counts2[b];
+                }
+                if (bin2 < 0) {
+                    bin2 = 0;
+                    base2 = below2;
+                }
+                m = lo2 + (bin2 << shift2);
+                c = base2;
+            }
+        } else if (bin >= 0) {
+            m = bin << shift1;
+            c = below;
         }
     }
-    const int walk_limit = 2 * a.npad + 4;
+
+    bool failed = false;
     for (int y = y_first; y < y_first + a.tw && y0 + y < a.ny; ++y) {
         if (y > y_first) {
             const int rbase = (y - 1) * a.sx + lane;
             const int abase = y * a.sx + lane;
+            int dn = 0, dc = 0;
+#pragma unroll 8
             for (int e = 0; e < a.nedge; ++e) {
-                const uint16_t cr = codes[rbase + a.edge_remove[e]];
-                const uint16_t ca = codes[abase + a.edge_add[e]];
-                n += (int)(ca != CODE_OMIT) - (int)(cr != CODE_OMIT);
-                c += (int)(ca < m) - (int)(cr < m);
+                const int cr = codes[rbase + a.edge_remove[e]];
+                const int ca = codes[abase + a.edge_add[e]];
+                dn += (int)(ca != CODE_OMIT) - (int)(cr != CODE_OMIT);
+                dc += (int)(ca < m) - (int)(cr < m);
             }
+            n += dn;
+            c += dc;
         }
         Key out = KT::CANONICAL_NAN;
-        int k = -1;
-        if (n >= a.min_count && n > 0) {
-            if (a.stat == STAT_RANK) {
-                k = a.rank_param < n ? a.rank_param : -1;
-            } else {
-                k = n >> 1;
-            }
-        }
+        const int k = target_of(n);
         if (k >= 0) {
-            // Walk m until it is an in-window rank with exactly k in-window
-            // ranks below it.
-            // Every access is bounds-checked; a cursor that leaves [0, retained)
-            // or a walk longer than the tile raises the error flag.
-            int steps = 0;
-            bool ok = true;
-            for (;;) {
-                if (++steps > walk_limit) {
-                    ok = false;
+            if (!walk_to(m, c, k, pos, retained, g, lane, y)) {
+                failed = true;
+                break;
+            }
+            if (a.stat == STAT_MEDIAN && (n & 1) == 0) {
+                const int m2 = previous_member(m, pos, retained, g, lane, y);
+                if (m2 < 0) {
+                    failed = true;
                     break;
                 }
-                if (c > k) {
-                    if (m <= 0) {
-                        ok = false;
-                        break;
-                    }
-                    --m;
-                    c -= in_window(pos[m], lane, y, a.kz, a.ky, a.kx, mask);
-                } else {
-                    if (m >= retained) {
-                        ok = false;
-                        break;
-                    }
-                    const bool inside = in_window(pos[m], lane, y, a.kz, a.ky, a.kx, mask);
-                    if (c == k && inside) {
-                        break;
-                    }
-                    c += inside;
-                    ++m;
-                }
-            }
-            if (ok && a.stat == STAT_MEDIAN && (n & 1) == 0) {
-                int m2 = m - 1;
-                while (m2 >= 0 && !in_window(pos[m2], lane, y, a.kz, a.ky, a.kx, mask)) {
-                    --m2;
-                }
-                if (m2 >= 0) {
-                    out = median_pair_bits<T>(keys[m2], keys[m]);
-                } else {
-                    ok = false;
-                }
-            } else if (ok) {
+                out = median_pair_bits<T>(keys[m2], keys[m]);
+            } else {
                 out = KT::unkey(keys[m]);
-            }
-            if (!ok) {
-                atomicOr(a.error_flag, 1);
-                m = retained >> 1;
-                k = -1;
-            }
-            if (k < 0) {
-                // Restore a consistent cursor after a failed walk.
-                c = 0;
-                const int base = y * a.sx + lane;
-                for (int t = 0; t < a.count; ++t) {
-                    c += codes[base + a.taps[t]] < m;
-                }
             }
         }
         if (col_valid) {
             output[z0 * a.ny * a.nx + (y0 + y) * a.nx + x0 + lane] = out;
         }
     }
+    if (failed) {
+        atomicOr(a.error_flag, 1);
+    }
+}
+
+template <typename T>
+hipError_t launch_typed(const SlideArgs &args, size_t lds_bytes, hipStream_t stream)
+{
+    const int64_t blocks = (int64_t)args.tiles_x * args.tiles_y * args.nz;
+    const dim3 grid((unsigned)blocks);
+    const dim3 block(NT);
+    const void *fn = nullptr;
+    switch (args.sort_e) {
+    case 4: fn = reinterpret_cast<const void *>(&k_slide<T, 4>); break;
+    case 8: fn = reinterpret_cast<const void *>(&k_slide<T, 8>); break;
+    case 16: fn = reinterpret_cast<const void *>(&k_slide<T, 16>); break;
+    case 32: fn = reinterpret_cast<const void *>(&k_slide<T, 32>); break;
+    case 64: fn = reinterpret_cast<const void *>(&k_slide<T, 64>); break;
+    default: return hipErrorInvalidValue;
+    }
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+    if (e != hipSuccess) {
+        return e;
+    }
+    switch (args.sort_e) {
+    case 4: hipLaunchKernelGGL((k_slide<T, 4>), grid, block, lds_bytes, stream, args); break;
+    case 8: hipLaunchKernelGGL((k_slide<T, 8>), grid, block, lds_bytes, stream, args); break;
+    case 16: hipLaunchKernelGGL((k_slide<T, 16>), grid, block, lds_bytes, stream, args); break;
+    case 32: hipLaunchKernelGGL((k_slide<T, 32>), grid, block, lds_bytes, stream, args); break;
+    default: hipLaunchKernelGGL((k_slide<T, 64>), grid, block, lds_bytes, stream, args); break;
+    }
+    return hipGetLastError();
 }
 
 }  // namespace
 
-size_t slide_lds_bytes(int dtype_bytes, int s, int npad, int mask_bytes)
+size_t slide_lds_bytes(int dtype_bytes, int s, int sort_e, int hist_stride, int mask_bytes)
 {
-    return (size_t)npad * (size_t)(dtype_bytes + 4) + (size_t)((s + 1) & ~1) * 2 + (size_t)mask_bytes;
+    const size_t n = (size_t)NT * (size_t)sort_e;
+    const size_t main = n * (size_t)(dtype_bytes + 2) + (size_t)((s + 7) & ~7) * 2 +
+                        (size_t)SLIDE_WAVES * (size_t)hist_stride * 2 + (size_t)mask_bytes;
+    // The sort's exchange rows (NT x (E + 1) keys and positions) alias the
+    // start of the layout.
+    const size_t exchange = (size_t)NT * (size_t)(sort_e + 1) * (size_t)(dtype_bytes + 2);
+    return main > exchange ? main : exchange;
 }
 
 hipError_t launch_slide(int dtype_bytes, const SlideArgs &args, size_t lds_bytes, hipStream_t stream)
 {
-    const int64_t blocks = (int64_t)args.tiles_x * args.tiles_y * args.nz;
-    if (blocks <= 0) {
+    if ((int64_t)args.tiles_x * args.tiles_y * args.nz <= 0) {
         return hipSuccess;
     }
-    const dim3 grid((unsigned)blocks);
-    const dim3 block(SLIDE_WAVES * 64);
-    if (dtype_bytes == 4) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_slide<float>),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
-        if (e != hipSuccess) {
-            return e;
-        }
-        hipLaunchKernelGGL(k_slide<float>, grid, block, lds_bytes, stream, args);
-    } else {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_slide<double>),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
-        if (e != hipSuccess) {
-            return e;
-        }
-        hipLaunchKernelGGL(k_slide<double>, grid, block, lds_bytes, stream, args);
-    }
-    return hipGetLastError();
+    return dtype_bytes == 4 ? launch_typed<float>(args, lds_bytes, stream)
+                            : launch_typed<double>(args, lds_bytes, stream);
 }
 
 }  // namespace ndf
