@@ -224,7 +224,7 @@ bool plan_slide(ndf_plan *p)
     }
     const int64_t kz = box[0], ky = box[1], kx = box[2];
     const int64_t sx = SLIDE_TILE_X + kx - 1;
-    if (sx > 1000 || ky > 900 || kz > 4000) return false;
+    if (sx > 254 || ky > 254 || kz > 254) return false;
     // Footprint mask over the tight box.
     std::vector<uint8_t> mask((size_t)(kz * ky * kx), 0);
     for (size_t t = 0; t < p->count; ++t) {
@@ -242,7 +242,17 @@ bool plan_slide(ndf_plan *p)
     // amortise the sort and the per-wave start; a support much larger than
     // the window lengthens every rank walk, so it is capped at 32 windows.
     const int64_t hist_stride = rect ? (int64_t)16 * (sx + 1) : 0;
-    auto support_of = [&](int64_t ty) { return kz * (ty + ky - 1) * sx; };
+    // Rows of the support: planes of ty + ky - 1 rows; with several planes
+    // the plane stride is a power of two so positions decode with shifts.
+    auto zrows_of = [&](int64_t ty) {
+        int64_t r = ty + ky - 1;
+        if (kz == 1) return r;
+        int64_t pow2 = 1;
+        while (pow2 < r) pow2 <<= 1;
+        return pow2;
+    };
+    auto rows_of = [&](int64_t ty) { return (kz - 1) * zrows_of(ty) + ty + ky - 1; };
+    auto support_of = [&](int64_t ty) { return rows_of(ty) * sx; };
     auto sort_e_of = [](int64_t s) {
         int e = 4;
         while ((int64_t)256 * e < s) e <<= 1;
@@ -250,7 +260,7 @@ bool plan_slide(ndf_plan *p)
     };
     auto fits = [&](int64_t ty) {
         const int64_t s = support_of(ty);
-        if (ty + ky - 1 > 4096 || s > 16384) return false;
+        if (rows_of(ty) > 254 || s > 16384) return false;
         const int e = sort_e_of(s);
         return e <= 64 && slide_lds_bytes(p->dbytes, (int)s, e, (int)hist_stride, mask_bytes) <= (size_t)limit;
     };
@@ -265,6 +275,13 @@ bool plan_slide(ndf_plan *p)
     if (ty == 0) return false;
     const int tw = (int)(ty / SLIDE_WAVES);
     const int sy = (int)(ty + ky - 1);
+    const int zrows = (int)zrows_of(ty);
+    int zshift = 8;
+    if (kz > 1) {
+        zshift = 0;
+        while ((1 << zshift) < zrows) ++zshift;
+    }
+    const int rows = (int)rows_of(ty);
     const int s = (int)support_of(ty);
     const int sort_e = sort_e_of(s);
     const int64_t tiles_x = (dims[2] + SLIDE_TILE_X - 1) / SLIDE_TILE_X;
@@ -296,7 +313,7 @@ bool plan_slide(ndf_plan *p)
             for (int64_t dx = 0; dx < kx; ++dx) {
                 auto at = [&](int64_t yy) { return mask[(size_t)((dz * ky + yy) * kx + dx)] != 0; };
                 if (!at(dy)) continue;
-                const uint32_t off = (uint32_t)((dz * sy + dy) * sx + dx);
+                const uint32_t off = (uint32_t)((dz * zrows + dy) * sx + dx);
                 p->slide_taps.push_back(off);
                 if (dy == 0 || !at(dy - 1)) p->slide_rm.push_back(off);
                 if (dy == ky - 1 || !at(dy + 1)) p->slide_add.push_back(off);
@@ -318,11 +335,12 @@ bool plan_slide(ndf_plan *p)
     s_args.kx = (int32_t)kx;
     s_args.sx = (int32_t)sx;
     s_args.sy = sy;
+    s_args.zrows = zrows;
+    s_args.zshift = zshift;
+    s_args.rows = rows;
     s_args.s = s;
     s_args.sort_e = sort_e;
     s_args.hist_stride = (int32_t)hist_stride;
-    s_args.inv_sx = 1.0f / (float)sx;
-    s_args.inv_sy = 1.0f / (float)sy;
     s_args.ty = (int32_t)ty;
     s_args.tw = tw;
     s_args.tiles_x = (int32_t)tiles_x;

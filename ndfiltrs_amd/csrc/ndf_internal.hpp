@@ -43,7 +43,7 @@ struct GenericArgs {
 // Engine 2: LDS-staged tile, block sort of the support, per-lane column slide
 // with a rank walk. Canonical 3-D geometry (nz planes of ny x nx); a tile is
 // 64 output columns x ty rows of one plane; wave w of the block slides down
-// rows [w*tw, (w+1)*tw). Support positions are stored as u16 (s <= 16384).
+// rows [w*tw, (w+1)*tw). Support positions are packed u16 (row << 8 | x).
 struct SlideArgs {
     const void *input;
     void *output;
@@ -53,14 +53,15 @@ struct SlideArgs {
     const int64_t *mapy;   // index y0 + y, y in [0, sy)
     const int64_t *mapx;   // index x0 + x, x in [0, sx)
     int32_t kz, ky, kx;    // window bounding box
-    int32_t sx, sy, s;     // support: sx = 64 + kx - 1, sy = ty + ky - 1, s = kz*sy*sx
+    int32_t sx, sy;        // support row length (64 + kx - 1) and rows per plane (ty + ky - 1)
+    int32_t zrows, zshift; // row stride between planes (a power of two 1 << zshift when kz > 1)
+    int32_t rows, s;       // support rows ((kz - 1) * zrows + sy, <= 254) and s = rows * sx
     int32_t sort_e;        // sort elements per thread; sort length 256 * sort_e >= s
     int32_t hist_stride;   // u16 entries of one wave's histogram region
-    float inv_sx, inv_sy;  // reciprocals for position decoding
     int32_t ty, tw;        // tile rows and rows per wave
     int32_t tiles_x, tiles_y;
     int32_t count;         // taps in the window
-    const uint32_t *taps;  // count support offsets (dz*sy + dy)*sx + dx (footprints)
+    const uint32_t *taps;  // count support offsets (dz*zrows + dy)*sx + dx (footprints)
     int32_t nedge;
     const uint32_t *edge_remove;  // offsets relative to row y-1
     const uint32_t *edge_add;     // offsets relative to row y

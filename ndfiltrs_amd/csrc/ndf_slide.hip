@@ -34,65 +34,66 @@ constexpr uint16_t POS_NONE = 0xFFFF;
 constexpr int NT = SLIDE_WAVES * 64;
 constexpr int BINS = 16;
 constexpr int BATCH = 16;
+constexpr int GUARD = BATCH;
 
+// Support positions are packed as (row << 8) | x with row = z * zrows + y
+// (row <= 254, x <= 254); POS_NONE (row 255, x 255) is outside every window.
 struct Geometry {
-    int sx, sy, kz, ky, kx;
-    float inv_sx, inv_sy;
+    int kz, ky, kx;
+    int zshift;
+    uint32_t vmask;  // keeps (y within its plane, x) of a packed position
     const uint8_t *mask;
 
-    // Window membership of the support sample at flat position p for the
-    // lane's window whose top-left support sample is (z=0, y, lane).
-    __device__ inline bool contains(uint32_t p, int lane, int y) const
+    // Window membership of the sample at packed position v for the window
+    // whose top-left support sample is `base` = (y << 8) | lane. Branch-free:
+    // d = v - base holds dx in its low byte when x >= lane (otherwise the low
+    // byte is >= 256 - 63 > kx) and dy in the bits above it (huge when
+    // negative).
+    __device__ __forceinline__ uint32_t contains(uint32_t v, uint32_t base) const
     {
-        if (p == POS_NONE) {
-            return false;
+        const uint32_t d = (v & vmask) - base;
+        const uint32_t dx = d & 0xFFu;
+        const uint32_t dy = d >> 8;
+        const uint32_t box = (uint32_t)(dx < (uint32_t)kx) & (uint32_t)(dy < (uint32_t)ky);
+        if (mask == nullptr) {
+            return box;
         }
-        const int zy = (int)(((float)p + 0.5f) * inv_sx);
-        const int x = (int)p - zy * sx;
-        const unsigned dx = (unsigned)(x - lane);
-        if (dx >= (unsigned)kx) {
-            return false;
-        }
-        int z = 0;
-        int yy = zy;
-        if (kz > 1) {
-            z = (int)(((float)zy + 0.5f) * inv_sy);
-            yy = zy - z * sy;
-        }
-        const unsigned dy = (unsigned)(yy - y);
-        if (dy >= (unsigned)ky) {
-            return false;
-        }
-        return mask == nullptr || mask[(z * ky + (int)dy) * kx + (int)dx] != 0;
+        const uint32_t dz = v >> (8 + zshift);
+        const uint32_t index = box ? (dz * (uint32_t)ky + dy) * (uint32_t)kx + dx : 0u;
+        return box & (uint32_t)(mask[index] != 0);
     }
 };
 
 // Position of the `need`-th set bit (0-based) of a 16-bit mask holding more
 // than `need` set bits.
-__device__ inline int select_bit16(uint32_t mask, int need)
+__device__ __forceinline__ int select_bit16(uint32_t mask, int need)
 {
     int base = 0;
+#pragma unroll
     for (int width = 8; width >= 1; width >>= 1) {
         const int low = __builtin_popcount(mask & ((1u << width) - 1u));
-        if (need >= low) {
-            need -= low;
-            mask >>= width;
-            base += width;
-        }
+        const bool up = need >= low;
+        need -= up ? low : 0;
+        mask >>= up ? width : 0;
+        base += up ? width : 0;
     }
     return base;
 }
 
 // Membership bits of ranks [first, first + BATCH) (bit i <-> rank first + i).
-__device__ inline uint32_t batch_mask(const uint16_t *pos, int first, int retained, const Geometry &g, int lane,
-                                      int y)
+// `pos` has BATCH guard entries (POS_NONE) on both sides, and ranks in
+// [retained, N) hold POS_NONE, so first in [-BATCH, N) needs no clamping.
+__device__ __forceinline__ uint32_t batch_mask(const uint16_t *pos, int first, const Geometry &g, uint32_t base)
 {
+    uint32_t v[BATCH];
+#pragma unroll
+    for (int i = 0; i < BATCH; ++i) {
+        v[i] = pos[first + i];
+    }
     uint32_t mask = 0;
 #pragma unroll
     for (int i = 0; i < BATCH; ++i) {
-        const int r = first + i;
-        const bool inside = r >= 0 && r < retained && g.contains(pos[r], lane, y);
-        mask |= (uint32_t)inside << i;
+        mask |= g.contains(v[i], base) << i;
     }
     return mask;
 }
@@ -100,14 +101,14 @@ __device__ inline uint32_t batch_mask(const uint16_t *pos, int first, int retain
 // Moves the cursor (m, c) to the rank m with c == k in-window ranks below it
 // and m itself in the window. Returns false if the cursor would leave
 // [0, retained), which a consistent cursor never does.
-__device__ inline bool walk_to(int &m, int &c, int k, const uint16_t *pos, int retained, const Geometry &g, int lane,
-                               int y)
+__device__ inline bool walk_to(int &m, int &c, int k, const uint16_t *pos, int retained, const Geometry &g,
+                               uint32_t base)
 {
     const int limit = retained / BATCH + 4;
     if (c > k) {
         // The (c - k)-th in-window rank below m, counting downward.
         for (int step = 0; step <= limit; ++step) {
-            const uint32_t mask = batch_mask(pos, m - BATCH, retained, g, lane, y);
+            const uint32_t mask = batch_mask(pos, m - BATCH, g, base);
             const int count = __builtin_popcount(mask);
             const int need = c - k;
             if (count >= need) {
@@ -125,7 +126,7 @@ __device__ inline bool walk_to(int &m, int &c, int k, const uint16_t *pos, int r
     }
     // The (k - c)-th in-window rank at or above m, counting upward.
     for (int step = 0; step <= limit; ++step) {
-        const uint32_t mask = batch_mask(pos, m, retained, g, lane, y);
+        const uint32_t mask = batch_mask(pos, m, g, base);
         const int count = __builtin_popcount(mask);
         const int need = k - c;
         if (count > need) {
@@ -143,10 +144,10 @@ __device__ inline bool walk_to(int &m, int &c, int k, const uint16_t *pos, int r
 }
 
 // Largest in-window rank below m, or -1.
-__device__ inline int previous_member(int m, const uint16_t *pos, int retained, const Geometry &g, int lane, int y)
+__device__ inline int previous_member(int m, const uint16_t *pos, const Geometry &g, uint32_t base)
 {
     for (int first = m - BATCH; first > -BATCH; first -= BATCH) {
-        const uint32_t mask = batch_mask(pos, first, retained, g, lane, y) & ((1u << (m - first)) - 1u);
+        const uint32_t mask = batch_mask(pos, first, g, base) & ((1u << (m - first)) - 1u);
         if (mask) {
             return first + 31 - __builtin_clz(mask);
         }
@@ -197,7 +198,7 @@ __device__ void window_bins(int counts[BINS], int lo, int shift, const uint16_t 
         for (int x = lane; x < a.sx; x += 64) {
             Bins bins;
             for (int z = 0; z < a.kz; ++z) {
-                const uint16_t *column = codes + (z * a.sy + y) * a.sx + x;
+                const uint16_t *column = codes + (z * a.zrows + y) * a.sx + x;
                 for (int dy = 0; dy < a.ky; ++dy) {
                     const int code = column[dy * a.sx];
                     if (code >= lo && code < hi) {
@@ -388,11 +389,11 @@ __global__ void __launch_bounds__(NT, 1) k_slide(SlideArgs a)
     using Key = typename KT::Key;
     constexpr int N = NT * E;
     extern __shared__ __align__(16) unsigned char lds[];
-    // Layout: keys [N] | pos [N] | codes [s] | hist [4 waves] | mask.
+    // Layout: keys [N] | guard | pos [N] | guard | codes [s] | hist [4 waves] | mask.
     // The sort's exchange rows (NT x (E + 1)) alias keys + pos + codes.
     Key *keys = reinterpret_cast<Key *>(lds);
-    uint16_t *pos = reinterpret_cast<uint16_t *>(keys + N);
-    uint16_t *codes = pos + N;
+    uint16_t *pos = reinterpret_cast<uint16_t *>(keys + N) + GUARD;
+    uint16_t *codes = pos + N + GUARD;
     uint16_t *hist_all = codes + ((a.s + 7) & ~7);
     uint8_t *mask = a.mask ? reinterpret_cast<uint8_t *>(hist_all + SLIDE_WAVES * a.hist_stride) : nullptr;
     Key *xk = keys;
@@ -419,10 +420,12 @@ __global__ void __launch_bounds__(NT, 1) k_slide(SlideArgs a)
     // ---- stage: rows of the support, lanes along x; compact retained ----
     const Key *input = static_cast<const Key *>(a.input);
     const Key cval_key = classify<T>((Key)a.cval_bits, a.ignore_inf);
-    const int rows = a.kz * a.sy;
-    for (int row = wave; row < rows; row += SLIDE_WAVES) {
-        const int z = row / a.sy;
-        const int y = row - z * a.sy;
+    for (int row = wave; row < a.rows; row += SLIDE_WAVES) {
+        const int z = row >> a.zshift;
+        const int y = row & ((1 << a.zshift) - 1);
+        if (y >= a.sy) {
+            continue;  // padding rows between z planes
+        }
         const int64_t mz = a.mapz[z0 + z];
         const int64_t my = a.mapy[y0 + y];
         for (int x0l = 0; x0l < a.sx; x0l += 64) {
@@ -448,7 +451,7 @@ __global__ void __launch_bounds__(NT, 1) k_slide(SlideArgs a)
             if (keep) {
                 const int slot = base + (int)__popcll(ballot & ((1ull << lane) - 1ull));
                 keys[slot] = key;
-                pos[slot] = (uint16_t)(row * a.sx + x);
+                pos[slot] = (uint16_t)((row << 8) | x);
             }
         }
     }
@@ -473,10 +476,14 @@ __global__ void __launch_bounds__(NT, 1) k_slide(SlideArgs a)
             pos[tid * E + I] = (uint16_t)chunk.p[I];
         });
         __syncthreads();
-        // Codes and the mask are written after the sort: the exchange rows
-        // may overlap them.
+        // Codes, guards and the mask are written after the sort: the
+        // exchange rows may overlap them.
         for (int q = tid; q < a.s; q += NT) {
             codes[q] = CODE_OMIT;
+        }
+        if (tid < GUARD) {
+            pos[tid - GUARD] = POS_NONE;
+            pos[N + tid] = POS_NONE;
         }
         if (mask) {
             const int mbytes = a.kz * a.ky * a.kx;
@@ -488,7 +495,7 @@ __global__ void __launch_bounds__(NT, 1) k_slide(SlideArgs a)
         static_for<0, E>([&](auto ic) {
             constexpr int I = decltype(ic)::value;
             if (chunk.p[I] != POS_NONE) {
-                codes[chunk.p[I]] = (uint16_t)(tid * E + I);
+                codes[(chunk.p[I] >> 8) * a.sx + (chunk.p[I] & 0xFFu)] = (uint16_t)(tid * E + I);
             }
         });
     }
@@ -507,7 +514,7 @@ __global__ void __launch_bounds__(NT, 1) k_slide(SlideArgs a)
         }
         return;
     }
-    const Geometry g{a.sx, a.sy, a.kz, a.ky, a.kx, a.inv_sx, a.inv_sy, mask};
+    const Geometry g{a.kz, a.ky, a.kx, a.zshift, ((((uint32_t)1 << a.zshift) - 1u) << 8) | 0xFFu, mask};
     uint16_t *hist = hist_all + wave * a.hist_stride;
     const bool col_valid = x0 + lane < a.nx;
     Key *output = static_cast<Key *>(a.output);
@@ -606,15 +613,31 @@ __global__ void __launch_bounds__(NT, 1) k_slide(SlideArgs a)
     bool failed = false;
     for (int y = y_first; y < y_first + a.tw && y0 + y < a.ny; ++y) {
         if (y > y_first) {
-            const int rbase = (y - 1) * a.sx + lane;
-            const int abase = y * a.sx + lane;
             int dn = 0, dc = 0;
+            if (a.mask == nullptr) {
+                // Rectangle: the top row of the previous window leaves, the
+                // bottom row of this one enters (every z plane).
+                for (int z = 0; z < a.kz; ++z) {
+                    const uint16_t *top = codes + (z * a.zrows + y - 1) * a.sx + lane;
+                    const uint16_t *bottom = top + a.ky * a.sx;
 #pragma unroll 8
-            for (int e = 0; e < a.nedge; ++e) {
-                const int cr = codes[rbase + a.edge_remove[e]];
-                const int ca = codes[abase + a.edge_add[e]];
-                dn += (int)(ca != CODE_OMIT) - (int)(cr != CODE_OMIT);
-                dc += (int)(ca < m) - (int)(cr < m);
+                    for (int dx = 0; dx < a.kx; ++dx) {
+                        const int cr = top[dx];
+                        const int ca = bottom[dx];
+                        dn += (int)(ca != CODE_OMIT) - (int)(cr != CODE_OMIT);
+                        dc += (int)(ca < m) - (int)(cr < m);
+                    }
+                }
+            } else {
+                const int rbase = (y - 1) * a.sx + lane;
+                const int abase = y * a.sx + lane;
+#pragma unroll 8
+                for (int e = 0; e < a.nedge; ++e) {
+                    const int cr = codes[rbase + a.edge_remove[e]];
+                    const int ca = codes[abase + a.edge_add[e]];
+                    dn += (int)(ca != CODE_OMIT) - (int)(cr != CODE_OMIT);
+                    dc += (int)(ca < m) - (int)(cr < m);
+                }
             }
             n += dn;
             c += dc;
@@ -622,12 +645,13 @@ __global__ void __launch_bounds__(NT, 1) k_slide(SlideArgs a)
         Key out = KT::CANONICAL_NAN;
         const int k = target_of(n);
         if (k >= 0) {
-            if (!walk_to(m, c, k, pos, retained, g, lane, y)) {
+            const uint32_t base = ((uint32_t)y << 8) | (uint32_t)lane;
+            if (!walk_to(m, c, k, pos, retained, g, base)) {
                 failed = true;
                 break;
             }
             if (a.stat == STAT_MEDIAN && (n & 1) == 0) {
-                const int m2 = previous_member(m, pos, retained, g, lane, y);
+                const int m2 = previous_member(m, pos, g, base);
                 if (m2 < 0) {
                     failed = true;
                     break;
@@ -680,7 +704,7 @@ hipError_t launch_typed(const SlideArgs &args, size_t lds_bytes, hipStream_t str
 size_t slide_lds_bytes(int dtype_bytes, int s, int sort_e, int hist_stride, int mask_bytes)
 {
     const size_t n = (size_t)NT * (size_t)sort_e;
-    const size_t main = n * (size_t)(dtype_bytes + 2) + (size_t)((s + 7) & ~7) * 2 +
+    const size_t main = n * (size_t)(dtype_bytes + 2) + (size_t)(2 * GUARD) * 2 + (size_t)((s + 7) & ~7) * 2 +
                         (size_t)SLIDE_WAVES * (size_t)hist_stride * 2 + (size_t)mask_bytes;
     // The sort's exchange rows (NT x (E + 1) keys and positions) alias the
     // start of the layout.
