@@ -113,6 +113,34 @@ ndf_status ndf_plan_describe(const ndf_plan *plan, char *buffer, size_t buffer_l
 
 void ndf_plan_destroy(ndf_plan *plan);
 
+/* Learned bracket (2-D images on the slide engine; no reference
+ * counterpart). A tiny MLP (6 -> 32 -> 32 -> 2, ReLU, fp32; NDF_BRACKET_PARAMS
+ * floats: W1[6][32], b1[32], W2[32][32], b2[32], W3[32][2], b3[2]) proposes
+ * a value bracket per tile from box-sum features of its support; only
+ * in-bracket samples are sorted (sort_len entries: 1024..16384) and any tile
+ * whose window targets leave the bracket is recomputed by the plain kernel.
+ * Results are bit-identical with and without a bracket. weights = NULL
+ * disables the bracket. */
+#define NDF_BRACKET_PARAMS 1346
+#define NDF_BRACKET_FEATURES 6
+ndf_status ndf_plan_set_bracket(ndf_plan *plan, const float *weights, size_t n_weights, int32_t sort_len,
+                                char *err, size_t err_len);
+/* Tiles of the plan and tiles that fell back in the last launch (synchronises). */
+ndf_status ndf_plan_bracket_report(ndf_plan *plan, int64_t *tiles, int64_t *fallback_tiles);
+size_t ndf_plan_bracket_tiles(const ndf_plan *plan);
+/* Training data for the bracket model: per-tile features (tiles x 6) of the
+ * device input and per-tile targets (tiles x 2: smallest and largest result
+ * of the tile, standardised by the tile's mean and std) from a device output
+ * of the plain filter; weight is 0 for tiles without a result. Host arrays. */
+ndf_status ndf_plan_bracket_training_data(ndf_plan *plan, const void *d_input, const void *d_output, float *features,
+                                          float *targets, float *weight, char *err, size_t err_len);
+/* Full-batch Adam on the pinball losses (quantile tau_lo of the lower
+ * target, tau_hi of the upper) with the hand-written HIP kernels; weights
+ * are updated in place; losses receives `steps` values (may be NULL). */
+ndf_status ndf_mlp_train(float *weights, size_t n_weights, const float *features, const float *targets,
+                         const float *weight, size_t count, int32_t steps, float learning_rate, float tau_lo,
+                         float tau_hi, float *losses, char *err, size_t err_len);
+
 /* Replaces get_num_threads (src/pyapi/filter.rs:377-380): the host thread
  * count a CPU caller would use (RAYON_NUM_THREADS, else the CPU count). */
 size_t ndf_get_num_threads(void);

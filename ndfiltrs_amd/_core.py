@@ -191,6 +191,50 @@ class Plan:
         if status != _lib.OK:
             _raise(status, err.value.decode())
 
+    def set_bracket(self, weights: np.ndarray | None, sort_len: int = 4096) -> None:
+        """Enable the learned bracket (a 1346-float model) or disable it (None)."""
+        err = ctypes.create_string_buffer(256)
+        if weights is None:
+            status = _lib.lib.ndf_plan_set_bracket(self._handle, None, 0, 0, err, len(err))
+        else:
+            w = np.ascontiguousarray(weights, dtype=np.float32)
+            status = _lib.lib.ndf_plan_set_bracket(
+                self._handle, w.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), w.size, int(sort_len), err, len(err)
+            )
+        if status != _lib.OK:
+            _raise(status, err.value.decode())
+
+    def bracket_report(self) -> tuple[int, int]:
+        """(tiles, tiles recomputed by the plain kernel) of the last launch."""
+        tiles = ctypes.c_int64(0)
+        fallback = ctypes.c_int64(0)
+        status = _lib.lib.ndf_plan_bracket_report(self._handle, ctypes.byref(tiles), ctypes.byref(fallback))
+        if status != _lib.OK:
+            raise RuntimeError("bracket report failed")
+        return int(tiles.value), int(fallback.value)
+
+    def bracket_training_data(self, d_input: int, d_output: int) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+        """Per-tile features, standardised target range and weight (see ndfgpu.h)."""
+        tiles = int(_lib.lib.ndf_plan_bracket_tiles(self._handle))
+        features = np.zeros((tiles, 6), dtype=np.float32)
+        targets = np.zeros((tiles, 2), dtype=np.float32)
+        weight = np.zeros(tiles, dtype=np.float32)
+        err = ctypes.create_string_buffer(256)
+        f32 = ctypes.POINTER(ctypes.c_float)
+        status = _lib.lib.ndf_plan_bracket_training_data(
+            self._handle,
+            d_input,
+            d_output,
+            features.ctypes.data_as(f32),
+            targets.ctypes.data_as(f32),
+            weight.ctypes.data_as(f32),
+            err,
+            len(err),
+        )
+        if status != _lib.OK:
+            _raise(status, err.value.decode())
+        return features, targets, weight
+
     def time_device(self, d_input: int, d_output: int, warmup: int, iterations: int) -> float:
         """Mean milliseconds per launch, timed with HIP events on one stream."""
         err = ctypes.create_string_buffer(256)

@@ -112,7 +112,49 @@ def _declare(lib: ctypes.CDLL) -> None:
         ctypes.POINTER(ctypes.c_double),
         *_err,
     ]
+    lib.ndf_plan_set_bracket.argtypes = [
+        plan_p,
+        ctypes.POINTER(ctypes.c_float),
+        ctypes.c_size_t,
+        ctypes.c_int32,
+        *_err,
+    ]
+    lib.ndf_plan_bracket_report.argtypes = [
+        plan_p,
+        ctypes.POINTER(ctypes.c_int64),
+        ctypes.POINTER(ctypes.c_int64),
+    ]
+    lib.ndf_plan_bracket_tiles.argtypes = [plan_p]
+    lib.ndf_plan_bracket_tiles.restype = ctypes.c_size_t
+    _f32_p = ctypes.POINTER(ctypes.c_float)
+    lib.ndf_plan_bracket_training_data.argtypes = [
+        plan_p,
+        ctypes.c_void_p,
+        ctypes.c_void_p,
+        _f32_p,
+        _f32_p,
+        _f32_p,
+        *_err,
+    ]
+    lib.ndf_mlp_train.argtypes = [
+        _f32_p,
+        ctypes.c_size_t,
+        _f32_p,
+        _f32_p,
+        _f32_p,
+        ctypes.c_size_t,
+        ctypes.c_int32,
+        ctypes.c_float,
+        ctypes.c_float,
+        ctypes.c_float,
+        _f32_p,
+        *_err,
+    ]
     for name in (
+        "ndf_plan_set_bracket",
+        "ndf_plan_bracket_report",
+        "ndf_plan_bracket_training_data",
+        "ndf_mlp_train",
         "ndf_window_filter",
         "ndf_plan_create",
         "ndf_plan_apply",

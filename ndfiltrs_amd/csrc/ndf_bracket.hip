@@ -1,0 +1,413 @@
+// Learned bracket proposal for the slide engine (2-D images).
+//
+// A tile's in-bracket samples are the only ones the slide kernel has to sort.
+// The bracket comes from a tiny MLP (6 -> 32 -> 32 -> 2, ReLU, fp32) that
+// reads cheap box-sum features of the tile's support:
+//   1. k_block_stats: one pass over the image, per 8x8 block: retained
+//      count, sum, sum of squares (f64), min, max;
+//   2. k_tile_features: per tile, the blocks covering its support give
+//      mean mu, std sigma, min, max and the retained fraction; the features
+//      are scale free: asinh((min-mu)/sigma), asinh((max-mu)/sigma),
+//      fraction, target quantile, asinh(sigma/|mu|), log2(taps)/16;
+//   3. k_bracket_predict: the MLP's two outputs are standardised offsets:
+//      bracket = [mu + sigma*o0, mu + sigma*o1], converted to keys.
+// Training (k_tile_labels, k_mlp_grad, k_mlp_adam) fits the offsets of the
+// smallest and largest window result per tile with pinball losses, so that
+// the bracket contains every target of a tile with high probability.
+// Exactness never depends on the model: the slide kernel falls back to the
+// plain kernel for any tile whose targets leave the bracket.
+#include "ndf_bracket.hpp"
+#include "ndf_device.hpp"
+
+namespace ndf {
+namespace {
+
+constexpr int BLOCK = 8;
+
+struct Stats {
+    double n, sum, sumsq, mn, mx;
+};
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_block_stats(const T *image, int64_t ny, int64_t nx, int64_t by, int64_t bx,
+                                                     int ignore_inf, double *out)
+{
+    using KT = KeyTraits<T>;
+    const int64_t block = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (block >= by * bx) {
+        return;
+    }
+    const int lane = threadIdx.x & 63;
+    const int64_t y = (block / bx) * BLOCK + (lane >> 3);
+    const int64_t x = (block % bx) * BLOCK + (lane & 7);
+    double v = 0.0;
+    bool keep = false;
+    if (y < ny && x < nx) {
+        const typename KT::Key bits = KT::bits(image[y * nx + x]);
+        keep = classify<T>(bits, ignore_inf) != KT::SENTINEL;
+        v = KT::to_double(bits);
+    }
+    double n = keep ? 1.0 : 0.0, s = keep ? v : 0.0, q = keep ? v * v : 0.0;
+    double mn = keep ? v : __builtin_huge_val(), mx = keep ? v : -__builtin_huge_val();
+    for (int off = 32; off > 0; off >>= 1) {
+        n += __shfl_xor(n, off);
+        s += __shfl_xor(s, off);
+        q += __shfl_xor(q, off);
+        const double a = __shfl_xor(mn, off), b = __shfl_xor(mx, off);
+        mn = a < mn ? a : mn;
+        mx = b > mx ? b : mx;
+    }
+    if (lane == 0) {
+        double *o = out + block * 5;
+        o[0] = n;
+        o[1] = s;
+        o[2] = q;
+        o[3] = mn;
+        o[4] = mx;
+    }
+}
+
+// asinh(v) = sign(v) * log(|v| + sqrt(v^2 + 1)); log(2|v|) once v^2 would overflow.
+__device__ inline float feature_asinh(double v)
+{
+    const double a = v < 0.0 ? -v : v;
+    const double r = a > 1e150 ? log(2.0 * a) : log(a + sqrt(a * a + 1.0));
+    return (float)(v < 0.0 ? -r : r);
+}
+
+__global__ void __launch_bounds__(64) k_tile_features(BracketArgs b, const double *blocks)
+{
+    const int tile = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int tyi = tile % b.tiles_y;
+    const int tx = tile / b.tiles_y;
+    // Support rows/columns of the tile, clipped to the image.
+    int64_t r0 = (int64_t)tyi * b.ty + b.low_y, r1 = (int64_t)tyi * b.ty + b.ty - 1 + b.high_y;
+    int64_t c0 = (int64_t)tx * 64 + b.low_x, c1 = (int64_t)tx * 64 + 63 + b.high_x;
+    r0 = r0 < 0 ? 0 : r0;
+    c0 = c0 < 0 ? 0 : c0;
+    r1 = r1 >= b.ny ? b.ny - 1 : r1;
+    c1 = c1 >= b.nx ? b.nx - 1 : c1;
+    const int64_t br0 = r0 / BLOCK, br1 = r1 / BLOCK, bc0 = c0 / BLOCK, bc1 = c1 / BLOCK;
+    const int64_t width = bc1 - bc0 + 1;
+    const int64_t count = (br1 - br0 + 1) * width;
+    double n = 0.0, s = 0.0, q = 0.0, mn = __builtin_huge_val(), mx = -__builtin_huge_val();
+    for (int64_t i = lane; i < count; i += 64) {
+        const double *o = blocks + ((br0 + i / width) * b.bx + bc0 + i % width) * 5;
+        n += o[0];
+        s += o[1];
+        q += o[2];
+        mn = o[3] < mn ? o[3] : mn;
+        mx = o[4] > mx ? o[4] : mx;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        n += __shfl_xor(n, off);
+        s += __shfl_xor(s, off);
+        q += __shfl_xor(q, off);
+        const double a = __shfl_xor(mn, off), c = __shfl_xor(mx, off);
+        mn = a < mn ? a : mn;
+        mx = c > mx ? c : mx;
+    }
+    if (lane == 0) {
+        const double cells = (double)(count * BLOCK * BLOCK);
+        double mu = 0.0, sigma = 1.0;
+        if (n > 0.0) {
+            mu = s / n;
+            const double var = q / n - mu * mu;
+            sigma = var > 0.0 ? __builtin_sqrt(var) : 0.0;
+        }
+        const double floor_sigma = 1e-6 * (__builtin_fabs(mu) + 1e-30);
+        sigma = sigma > floor_sigma ? sigma : floor_sigma;
+        float *f = b.features + (int64_t)tile * FEATURES;
+        f[0] = n > 0.0 ? feature_asinh((mn - mu) / sigma) : 0.0f;
+        f[1] = n > 0.0 ? feature_asinh((mx - mu) / sigma) : 0.0f;
+        f[2] = (float)(n / cells);
+        f[3] = b.quantile;
+        f[4] = feature_asinh(sigma / (__builtin_fabs(mu) + 1e-30));
+        f[5] = b.log_taps;
+        double *m = b.moments + (int64_t)tile * 3;
+        m[0] = mu;
+        m[1] = sigma;
+        m[2] = n;
+    }
+}
+
+// MLP forward for one sample; weights in the documented flat layout.
+__device__ inline void mlp_forward(const float *w, const float *x, float h1[HIDDEN], float h2[HIDDEN], float out[2])
+{
+    const float *w1 = w + W1_OFF, *b1 = w + B1_OFF, *w2 = w + W2_OFF, *b2 = w + B2_OFF, *w3 = w + W3_OFF,
+                *b3 = w + B3_OFF;
+    for (int j = 0; j < HIDDEN; ++j) {
+        float acc = b1[j];
+        for (int i = 0; i < FEATURES; ++i) {
+            acc += x[i] * w1[i * HIDDEN + j];
+        }
+        h1[j] = acc > 0.0f ? acc : 0.0f;
+    }
+    for (int j = 0; j < HIDDEN; ++j) {
+        float acc = b2[j];
+        for (int i = 0; i < HIDDEN; ++i) {
+            acc += h1[i] * w2[i * HIDDEN + j];
+        }
+        h2[j] = acc > 0.0f ? acc : 0.0f;
+    }
+    for (int j = 0; j < 2; ++j) {
+        float acc = b3[j];
+        for (int i = 0; i < HIDDEN; ++i) {
+            acc += h2[i] * w3[i * 2 + j];
+        }
+        out[j] = acc;
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(64) k_bracket_predict(BracketArgs b, const float *weights, int64_t tiles)
+{
+    using KT = KeyTraits<T>;
+    using Key = typename KT::Key;
+    __shared__ float w[PARAMS];
+    for (int i = threadIdx.x; i < PARAMS; i += blockDim.x) {
+        w[i] = weights[i];
+    }
+    __syncthreads();
+    const int64_t tile = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (tile >= tiles) {
+        return;
+    }
+    float h1[HIDDEN], h2[HIDDEN], out[2];
+    mlp_forward(w, b.features + tile * FEATURES, h1, h2, out);
+    const double mu = b.moments[tile * 3 + 0];
+    const double sigma = b.moments[tile * 3 + 1];
+    const float o0 = out[0] < out[1] ? out[0] : out[1];
+    const float o1 = out[0] < out[1] ? out[1] : out[0];
+    const double lo = mu + sigma * (double)o0;
+    const double hi = mu + sigma * (double)o1;
+    // The bracket is a key range of the dtype; rounding lo down and hi up
+    // is not needed for exactness (a miss only costs a fallback).
+    Key lo_key = KT::key(KT::from_double(lo));
+    Key hi_key = KT::key(KT::from_double(hi));
+    if (!(lo == lo) || !(hi == hi)) {
+        lo_key = 0;
+        hi_key = KT::SENTINEL;
+    }
+    static_cast<Key *>(b.lo_keys)[tile] = lo_key;
+    static_cast<Key *>(b.hi_keys)[tile] = hi_key;
+    b.predicted[tile * 2 + 0] = o0;
+    b.predicted[tile * 2 + 1] = o1;
+}
+
+// Per tile: standardised smallest and largest non-NaN result of the tile's
+// outputs; tiles without a result get a zero weight.
+template <typename T>
+__global__ void __launch_bounds__(64) k_tile_labels(BracketArgs b, const T *output, float *targets, float *weight)
+{
+    using KT = KeyTraits<T>;
+    const int tile = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int tyi = tile % b.tiles_y;
+    const int tx = tile / b.tiles_y;
+    const int64_t x = (int64_t)tx * 64 + lane;
+    double mn = __builtin_huge_val(), mx = -__builtin_huge_val();
+    if (x < b.nx) {
+        for (int r = 0; r < b.ty; ++r) {
+            const int64_t y = (int64_t)tyi * b.ty + r;
+            if (y >= b.ny) {
+                break;
+            }
+            const double v = KT::to_double(KT::bits(output[y * b.nx + x]));
+            if (v == v) {
+                mn = v < mn ? v : mn;
+                mx = v > mx ? v : mx;
+            }
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        const double a = __shfl_xor(mn, off), c = __shfl_xor(mx, off);
+        mn = a < mn ? a : mn;
+        mx = c > mx ? c : mx;
+    }
+    if (lane == 0) {
+        const double mu = b.moments[tile * 3 + 0];
+        const double sigma = b.moments[tile * 3 + 1];
+        const bool ok = mn <= mx && __builtin_isfinite(mn) && __builtin_isfinite(mx);
+        targets[tile * 2 + 0] = ok ? (float)((mn - mu) / sigma) : 0.0f;
+        targets[tile * 2 + 1] = ok ? (float)((mx - mu) / sigma) : 0.0f;
+        weight[tile] = ok ? 1.0f : 0.0f;
+    }
+}
+
+// One thread per sample: forward, pinball-loss gradient, backward; the
+// gradient is summed per block in LDS (float atomics) and written to
+// partial[block][PARAMS].
+__global__ void __launch_bounds__(256) k_mlp_grad(const float *weights, const float *features, const float *targets,
+                                                 const float *weight, const int32_t *index, int64_t count,
+                                                 float tau_lo, float tau_hi, float *partial, float *loss_partial)
+{
+    __shared__ float w[PARAMS];
+    __shared__ float g[PARAMS];
+    __shared__ float loss_sum;
+    for (int i = threadIdx.x; i < PARAMS; i += blockDim.x) {
+        w[i] = weights[i];
+        g[i] = 0.0f;
+    }
+    if (threadIdx.x == 0) {
+        loss_sum = 0.0f;
+    }
+    __syncthreads();
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < count) {
+        const int64_t s = index ? index[k] : k;
+        const float *x = features + s * FEATURES;
+        const float wt = weight[s];
+        float h1[HIDDEN], h2[HIDDEN], out[2];
+        mlp_forward(w, x, h1, h2, out);
+        // Pinball loss: quantile tau_lo of the lower target, tau_hi of the upper.
+        float dout[2], loss = 0.0f;
+        const float taus[2] = {tau_lo, tau_hi};
+        for (int j = 0; j < 2; ++j) {
+            const float r = targets[s * 2 + j] - out[j];
+            loss += wt * (r >= 0.0f ? taus[j] * r : (taus[j] - 1.0f) * r);
+            dout[j] = wt * (r >= 0.0f ? -taus[j] : 1.0f - taus[j]);
+        }
+        atomicAdd(&loss_sum, loss);
+        float dh2[HIDDEN];
+        for (int i = 0; i < HIDDEN; ++i) {
+            float acc = 0.0f;
+            for (int j = 0; j < 2; ++j) {
+                atomicAdd(&g[W3_OFF + i * 2 + j], h2[i] * dout[j]);
+                acc += w[W3_OFF + i * 2 + j] * dout[j];
+            }
+            dh2[i] = h2[i] > 0.0f ? acc : 0.0f;
+        }
+        for (int j = 0; j < 2; ++j) {
+            atomicAdd(&g[B3_OFF + j], dout[j]);
+        }
+        float dh1[HIDDEN];
+        for (int i = 0; i < HIDDEN; ++i) {
+            float acc = 0.0f;
+            for (int j = 0; j < HIDDEN; ++j) {
+                if (dh2[j] != 0.0f) {
+                    atomicAdd(&g[W2_OFF + i * HIDDEN + j], h1[i] * dh2[j]);
+                }
+                acc += w[W2_OFF + i * HIDDEN + j] * dh2[j];
+            }
+            dh1[i] = h1[i] > 0.0f ? acc : 0.0f;
+        }
+        for (int j = 0; j < HIDDEN; ++j) {
+            if (dh2[j] != 0.0f) {
+                atomicAdd(&g[B2_OFF + j], dh2[j]);
+            }
+            if (dh1[j] != 0.0f) {
+                atomicAdd(&g[B1_OFF + j], dh1[j]);
+                for (int i = 0; i < FEATURES; ++i) {
+                    atomicAdd(&g[W1_OFF + i * HIDDEN + j], x[i] * dh1[j]);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < PARAMS; i += blockDim.x) {
+        partial[(int64_t)blockIdx.x * PARAMS + i] = g[i];
+    }
+    if (threadIdx.x == 0) {
+        loss_partial[blockIdx.x] = loss_sum;
+    }
+}
+
+// One thread per parameter: sum the partial gradients (fixed order), then an
+// Adam step; thread 0 also sums the loss.
+__global__ void __launch_bounds__(256) k_mlp_adam(float *weights, float *m1, float *m2, const float *partial,
+                                                 const float *loss_partial, int blocks, float inv_count, float lr,
+                                                 float beta1, float beta2, float eps, int step, float *loss_out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < PARAMS) {
+        float grad = 0.0f;
+        for (int b = 0; b < blocks; ++b) {
+            grad += partial[(int64_t)b * PARAMS + i];
+        }
+        grad *= inv_count;
+        m1[i] = beta1 * m1[i] + (1.0f - beta1) * grad;
+        m2[i] = beta2 * m2[i] + (1.0f - beta2) * grad * grad;
+        const float c1 = 1.0f - __builtin_powf(beta1, (float)step);
+        const float c2 = 1.0f - __builtin_powf(beta2, (float)step);
+        weights[i] -= lr * (m1[i] / c1) / (__builtin_sqrtf(m2[i] / c2) + eps);
+    }
+    if (i == 0 && loss_out) {
+        float total = 0.0f;
+        for (int b = 0; b < blocks; ++b) {
+            total += loss_partial[b];
+        }
+        *loss_out = total * inv_count;
+    }
+}
+
+}  // namespace
+
+hipError_t launch_block_stats(int dtype_bytes, const void *image, int64_t ny, int64_t nx, int ignore_inf,
+                              double *blocks, hipStream_t stream)
+{
+    const int64_t by = (ny + BLOCK - 1) / BLOCK, bx = (nx + BLOCK - 1) / BLOCK;
+    const dim3 grid((unsigned)((by * bx + 3) / 4));
+    if (dtype_bytes == 4) {
+        hipLaunchKernelGGL(k_block_stats<float>, grid, dim3(256), 0, stream, static_cast<const float *>(image), ny, nx,
+                           by, bx, ignore_inf, blocks);
+    } else {
+        hipLaunchKernelGGL(k_block_stats<double>, grid, dim3(256), 0, stream, static_cast<const double *>(image), ny,
+                           nx, by, bx, ignore_inf, blocks);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_tile_features(const BracketArgs &args, const double *blocks, int64_t tiles, hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_tile_features, dim3((unsigned)tiles), dim3(64), 0, stream, args, blocks);
+    return hipGetLastError();
+}
+
+hipError_t launch_bracket_predict(int dtype_bytes, const BracketArgs &args, const float *weights, int64_t tiles,
+                                  hipStream_t stream)
+{
+    const dim3 grid((unsigned)((tiles + 63) / 64));
+    if (dtype_bytes == 4) {
+        hipLaunchKernelGGL(k_bracket_predict<float>, grid, dim3(64), 0, stream, args, weights, tiles);
+    } else {
+        hipLaunchKernelGGL(k_bracket_predict<double>, grid, dim3(64), 0, stream, args, weights, tiles);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_tile_labels(int dtype_bytes, const BracketArgs &args, const void *output, float *targets,
+                              float *weight, int64_t tiles, hipStream_t stream)
+{
+    if (dtype_bytes == 4) {
+        hipLaunchKernelGGL(k_tile_labels<float>, dim3((unsigned)tiles), dim3(64), 0, stream, args,
+                           static_cast<const float *>(output), targets, weight);
+    } else {
+        hipLaunchKernelGGL(k_tile_labels<double>, dim3((unsigned)tiles), dim3(64), 0, stream, args,
+                           static_cast<const double *>(output), targets, weight);
+    }
+    return hipGetLastError();
+}
+
+int mlp_grad_blocks(int64_t count) { return (int)((count + 255) / 256); }
+
+hipError_t launch_mlp_step(float *weights, float *m1, float *m2, const float *features, const float *targets,
+                           const float *weight, const int32_t *index, int64_t count, float tau_lo, float tau_hi,
+                           float lr, int step, float *partial, float *loss_partial, float *loss_out,
+                           hipStream_t stream)
+{
+    const int blocks = mlp_grad_blocks(count);
+    hipLaunchKernelGGL(k_mlp_grad, dim3((unsigned)blocks), dim3(256), 0, stream, weights, features, targets, weight,
+                       index, count, tau_lo, tau_hi, partial, loss_partial);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        return e;
+    }
+    hipLaunchKernelGGL(k_mlp_adam, dim3((PARAMS + 255) / 256), dim3(256), 0, stream, weights, m1, m2, partial,
+                       loss_partial, blocks, 1.0f / (float)(count > 0 ? count : 1), lr, 0.9f, 0.999f, 1e-8f, step,
+                       loss_out);
+    return hipGetLastError();
+}
+
+}  // namespace ndf

@@ -15,12 +15,14 @@
 #include <unistd.h>
 
 #include <algorithm>
+#include <cmath>
 #include <mutex>
 #include <new>
 #include <string>
 #include <vector>
 
 #include "../../include/ndfgpu.h"
+#include "ndf_bracket.hpp"
 #include "ndf_device.hpp"
 #include "ndf_internal.hpp"
 
@@ -190,6 +192,17 @@ struct ndf_plan {
     std::vector<uint32_t> slide_taps, slide_rm, slide_add;
     std::vector<uint8_t> slide_mask;
 
+    // Bracket mode (2-D images on the slide engine).
+    bool bracket_on = false;
+    bool bracket_ready = false;
+    bool bracket_launched = false;  // the last launch took the bracket path
+    int bracket_e = 16;
+    std::vector<float> bracket_weights;
+    SlideArgs slide_b{};
+    size_t slide_b_lds = 0;
+    BracketArgs bargs{};
+    DeviceBuffer b_weights, b_blocks, b_features, b_moments, b_lo, b_hi, b_pred, b_fallback, b_fcount;
+
     // Device state.
     std::mutex mutex;
     bool tables_ready = false;
@@ -200,7 +213,8 @@ struct ndf_plan {
     ~ndf_plan()
     {
         for (DeviceBuffer *b : {&g_maps, &g_taps, &s_mapz, &s_mapy, &s_mapx, &s_taps, &s_rm, &s_add, &s_mask, &d_in,
-                                &d_out, &d_error}) {
+                                &d_out, &d_error, &b_weights, &b_blocks, &b_features, &b_moments, &b_lo, &b_hi,
+                                &b_pred, &b_fallback, &b_fcount}) {
             b->release();
         }
     }
@@ -241,7 +255,7 @@ bool plan_slide(ndf_plan *p)
     // sort (256 * E <= 16384 entries, u16 positions) and LDS. Larger tiles
     // amortise the sort and the per-wave start; a support much larger than
     // the window lengthens every rank walk, so it is capped at 32 windows.
-    const int64_t hist_stride = rect ? (int64_t)16 * (sx + 1) : 0;
+    const int64_t hist_stride = rect ? (int64_t)18 * (sx + 1) : 0;
     // Rows of the support: planes of ty + ky - 1 rows; with several planes
     // the plane stride is a power of two so positions decode with shifts.
     auto zrows_of = [&](int64_t ty) {
@@ -356,7 +370,19 @@ bool plan_slide(ndf_plan *p)
     return true;
 }
 
+ndf_status prepare_bracket(ndf_plan *p, char *err, size_t err_len);
+ndf_status prepare_engine_tables(ndf_plan *p, char *err, size_t err_len);
+
 ndf_status prepare_tables(ndf_plan *p, char *err, size_t err_len)
+{
+    ndf_status st = prepare_engine_tables(p, err, err_len);
+    if (st == NDF_OK && p->bracket_on && p->engine == NDF_ENGINE_SLIDE && !p->bracket_ready) {
+        st = prepare_bracket(p, err, err_len);
+    }
+    return st;
+}
+
+ndf_status prepare_engine_tables(ndf_plan *p, char *err, size_t err_len)
 {
     if (p->tables_ready) return NDF_OK;
     hipError_t e = p->d_error.ensure(sizeof(int32_t));
@@ -421,17 +447,113 @@ ndf_status prepare_tables(ndf_plan *p, char *err, size_t err_len)
     return NDF_OK;
 }
 
+bool bracket_supported(const ndf_plan *p)
+{
+    return p->slide_ok && p->rank >= 2 && p->slide.nz == 1 && p->slide.kz == 1;
+}
+
+int64_t slide_tiles(const ndf_plan *p) { return (int64_t)p->slide.tiles_x * p->slide.tiles_y * p->slide.nz; }
+
+// Allocates the bracket buffers and fills the bracket launch arguments.
+ndf_status prepare_bracket(ndf_plan *p, char *err, size_t err_len)
+{
+    if (p->bracket_ready) return NDF_OK;
+    const int64_t tiles = slide_tiles(p);
+    const int64_t by = (p->slide.ny + 7) / 8, bx = (p->slide.nx + 7) / 8;
+    hipError_t e;
+    if ((e = p->b_blocks.ensure((size_t)(by * bx * 5) * sizeof(double))) != hipSuccess ||
+        (e = p->b_features.ensure((size_t)tiles * FEATURES * sizeof(float))) != hipSuccess ||
+        (e = p->b_moments.ensure((size_t)tiles * 3 * sizeof(double))) != hipSuccess ||
+        (e = p->b_lo.ensure((size_t)tiles * (size_t)p->dbytes)) != hipSuccess ||
+        (e = p->b_hi.ensure((size_t)tiles * (size_t)p->dbytes)) != hipSuccess ||
+        (e = p->b_pred.ensure((size_t)tiles * 2 * sizeof(float))) != hipSuccess ||
+        (e = p->b_fallback.ensure((size_t)tiles * sizeof(int32_t))) != hipSuccess ||
+        (e = p->b_fcount.ensure(sizeof(int32_t))) != hipSuccess) {
+        return fail_hip(e, err, err_len);
+    }
+    if (!p->bracket_weights.empty() && (e = upload(p->b_weights, p->bracket_weights)) != hipSuccess) {
+        return fail_hip(e, err, err_len);
+    }
+    BracketArgs &b = p->bargs;
+    b = BracketArgs{};
+    b.ny = p->slide.ny;
+    b.nx = p->slide.nx;
+    b.bx = bx;
+    b.ty = p->slide.ty;
+    b.tiles_y = p->slide.tiles_y;
+    b.tiles_x = p->slide.tiles_x;
+    const size_t ay = p->rank - 2, ax = p->rank - 1;
+    b.low_y = (int32_t)p->low[ay];
+    b.high_y = (int32_t)p->high[ay];
+    b.low_x = (int32_t)p->low[ax];
+    b.high_x = (int32_t)p->high[ax];
+    b.quantile = p->stat == STAT_RANK
+                     ? (float)((double)std::min(p->parameter, p->count) / (double)(p->count > 1 ? p->count - 1 : 1))
+                     : 0.5f;
+    b.log_taps = (float)(std::log2((double)p->count) / 16.0);
+    b.features = static_cast<float *>(p->b_features.ptr);
+    b.moments = static_cast<double *>(p->b_moments.ptr);
+    b.lo_keys = p->b_lo.ptr;
+    b.hi_keys = p->b_hi.ptr;
+    b.predicted = static_cast<float *>(p->b_pred.ptr);
+    p->slide_b = p->slide;
+    p->slide_b.sort_e = p->bracket_e;
+    p->slide_b.bracket_lo = p->b_lo.ptr;
+    p->slide_b.bracket_hi = p->b_hi.ptr;
+    p->slide_b.fallback_tiles = static_cast<int32_t *>(p->b_fallback.ptr);
+    p->slide_b.fallback_count = static_cast<int32_t *>(p->b_fcount.ptr);
+    p->slide_b_lds = slide_lds_bytes(p->dbytes, p->slide.s, p->bracket_e, p->slide.hist_stride,
+                                     (int)p->slide_mask.size());
+    p->bracket_ready = true;
+    return NDF_OK;
+}
+
+// Features and bracket keys of every tile for the input in d_in.
+hipError_t bracket_propose(ndf_plan *p, const void *d_in, hipStream_t stream)
+{
+    const int64_t tiles = slide_tiles(p);
+    hipError_t e = launch_block_stats(p->dbytes, d_in, p->slide.ny, p->slide.nx, p->eff_ignore_inf,
+                                      static_cast<double *>(p->b_blocks.ptr), stream);
+    if (e == hipSuccess) e = launch_tile_features(p->bargs, static_cast<const double *>(p->b_blocks.ptr), tiles, stream);
+    if (e == hipSuccess && p->bracket_on) {
+        e = launch_bracket_predict(p->dbytes, p->bargs, static_cast<const float *>(p->b_weights.ptr), tiles, stream);
+    }
+    return e;
+}
+
 ndf_status launch(ndf_plan *p, const void *d_in, void *d_out, hipStream_t stream, char *err, size_t err_len)
 {
     hipError_t e = hipSuccess;
     const uint64_t nan_bits = p->dbytes == 4 ? 0x7FC00000ull : 0x7FF8000000000000ull;
+    p->bracket_launched = false;
     if (p->all_nan) {
         e = launch_fill(p->dbytes, d_out, (int64_t)p->len, nan_bits, stream);
+    } else if (p->engine == NDF_ENGINE_SLIDE && p->bracket_on) {
+        // Learned bracket: propose, filter in-bracket samples only, then
+        // recompute every fallback tile with the plain kernel.
+        const int tiles = (int)slide_tiles(p);
+        p->bracket_launched = true;
+        e = bracket_propose(p, d_in, stream);
+        if (e == hipSuccess) e = launch_fill(4, p->b_fcount.ptr, 1, 0, stream);
+        if (e == hipSuccess) {
+            SlideArgs args = p->slide_b;
+            args.input = d_in;
+            args.output = d_out;
+            e = launch_slide(p->dbytes, args, p->slide_b_lds, tiles, stream);
+        }
+        if (e == hipSuccess) {
+            SlideArgs args = p->slide;
+            args.input = d_in;
+            args.output = d_out;
+            args.tile_list = static_cast<const int32_t *>(p->b_fallback.ptr);
+            args.tile_count = static_cast<const int32_t *>(p->b_fcount.ptr);
+            e = launch_slide(p->dbytes, args, p->slide_lds, tiles, stream);
+        }
     } else if (p->engine == NDF_ENGINE_SLIDE) {
         SlideArgs args = p->slide;
         args.input = d_in;
         args.output = d_out;
-        e = launch_slide(p->dbytes, args, p->slide_lds, stream);
+        e = launch_slide(p->dbytes, args, p->slide_lds, args.tiles_x * args.tiles_y * (int)args.nz, stream);
     } else {
         GenericArgs args = p->generic;
         args.input = d_in;
@@ -740,6 +862,138 @@ ndf_status ndf_window_filter(int32_t dtype_bytes, const void *input, void *outpu
     st = ndf_plan_apply(plan, input, output, plan->len, err, err_len);
     ndf_plan_destroy(plan);
     return st;
+}
+
+ndf_status ndf_plan_set_bracket(ndf_plan *plan, const float *weights, size_t n_weights, int32_t sort_len, char *err,
+                                size_t err_len)
+{
+    std::lock_guard<std::mutex> guard(plan->mutex);
+    if (weights == nullptr) {
+        plan->bracket_on = false;
+        return NDF_OK;
+    }
+    if (n_weights != (size_t)PARAMS) return fail(NDF_ERR_INVALID, "bracket model must have 1346 parameters", err, err_len);
+    if (!bracket_supported(plan)) {
+        return fail(NDF_ERR_INVALID, "the learned bracket needs a 2-D image on the slide engine", err, err_len);
+    }
+    int e = 4;
+    while (256 * e < sort_len && e < 64) e <<= 1;
+    if (sort_len <= 0 || 256 * e != sort_len) {
+        return fail(NDF_ERR_INVALID, "bracket sort length must be 1024, 2048, 4096, 8192 or 16384", err, err_len);
+    }
+    if (slide_lds_bytes(plan->dbytes, plan->slide.s, e, plan->slide.hist_stride, (int)plan->slide_mask.size()) >
+        (size_t)lds_limit()) {
+        return fail(NDF_ERR_INVALID, "bracket sort length does not fit LDS", err, err_len);
+    }
+    plan->bracket_weights.assign(weights, weights + n_weights);
+    plan->bracket_e = e;
+    plan->bracket_on = true;
+    plan->bracket_ready = false;
+    return NDF_OK;
+}
+
+ndf_status ndf_plan_bracket_report(ndf_plan *plan, int64_t *tiles, int64_t *fallback_tiles)
+{
+    std::lock_guard<std::mutex> guard(plan->mutex);
+    *tiles = plan->slide_ok ? slide_tiles(plan) : 0;
+    *fallback_tiles = 0;
+    if (!plan->bracket_launched) return NDF_OK;
+    int32_t count = 0;
+    if (hipDeviceSynchronize() != hipSuccess ||
+        hipMemcpy(&count, plan->b_fcount.ptr, sizeof(count), hipMemcpyDeviceToHost) != hipSuccess) {
+        return NDF_ERR_DEVICE;
+    }
+    *fallback_tiles = count;
+    return NDF_OK;
+}
+
+size_t ndf_plan_bracket_tiles(const ndf_plan *plan)
+{
+    return bracket_supported(plan) ? (size_t)slide_tiles(plan) : 0;
+}
+
+ndf_status ndf_plan_bracket_training_data(ndf_plan *plan, const void *d_input, const void *d_output, float *features,
+                                          float *targets, float *weight, char *err, size_t err_len)
+{
+    std::lock_guard<std::mutex> guard(plan->mutex);
+    if (!bracket_supported(plan)) {
+        return fail(NDF_ERR_INVALID, "the learned bracket needs a 2-D image on the slide engine", err, err_len);
+    }
+    ndf_status st = prepare_tables(plan, err, err_len);
+    if (st != NDF_OK) return st;
+    const bool was_on = plan->bracket_on;
+    plan->bracket_on = false;
+    st = prepare_bracket(plan, err, err_len);
+    plan->bracket_on = was_on;
+    if (st != NDF_OK) return st;
+    const int64_t tiles = slide_tiles(plan);
+    DeviceBuffer d_targets, d_weight;
+    hipError_t e;
+    if ((e = d_targets.ensure((size_t)tiles * 2 * sizeof(float))) != hipSuccess ||
+        (e = d_weight.ensure((size_t)tiles * sizeof(float))) != hipSuccess) {
+        return fail_hip(e, err, err_len);
+    }
+    const bool on = plan->bracket_on;
+    plan->bracket_on = false;  // features only
+    e = bracket_propose(plan, d_input, nullptr);
+    plan->bracket_on = on;
+    if (e == hipSuccess) {
+        e = launch_tile_labels(plan->dbytes, plan->bargs, d_output, static_cast<float *>(d_targets.ptr),
+                               static_cast<float *>(d_weight.ptr), tiles, nullptr);
+    }
+    if (e == hipSuccess) {
+        e = hipMemcpy(features, plan->b_features.ptr, (size_t)tiles * FEATURES * sizeof(float), hipMemcpyDeviceToHost);
+    }
+    if (e == hipSuccess) {
+        e = hipMemcpy(targets, d_targets.ptr, (size_t)tiles * 2 * sizeof(float), hipMemcpyDeviceToHost);
+    }
+    if (e == hipSuccess) {
+        e = hipMemcpy(weight, d_weight.ptr, (size_t)tiles * sizeof(float), hipMemcpyDeviceToHost);
+    }
+    d_targets.release();
+    d_weight.release();
+    return e == hipSuccess ? NDF_OK : fail_hip(e, err, err_len);
+}
+
+ndf_status ndf_mlp_train(float *weights, size_t n_weights, const float *features, const float *targets,
+                         const float *weight, size_t count, int32_t steps, float learning_rate, float tau_lo,
+                         float tau_hi, float *losses, char *err, size_t err_len)
+{
+    if (n_weights != (size_t)PARAMS) return fail(NDF_ERR_INVALID, "bracket model must have 1346 parameters", err, err_len);
+    if (count == 0 || steps <= 0) return fail(NDF_ERR_INVALID, "training needs samples and steps", err, err_len);
+    DeviceBuffer w, m1, m2, x, t, wt, partial, loss_partial, loss;
+    const int blocks = mlp_grad_blocks((int64_t)count);
+    std::vector<float> zeros(PARAMS, 0.0f);
+    hipError_t e;
+    if ((e = w.ensure(PARAMS * sizeof(float))) != hipSuccess || (e = m1.ensure(PARAMS * sizeof(float))) != hipSuccess ||
+        (e = m2.ensure(PARAMS * sizeof(float))) != hipSuccess ||
+        (e = x.ensure(count * FEATURES * sizeof(float))) != hipSuccess ||
+        (e = t.ensure(count * 2 * sizeof(float))) != hipSuccess ||
+        (e = wt.ensure(count * sizeof(float))) != hipSuccess ||
+        (e = partial.ensure((size_t)blocks * PARAMS * sizeof(float))) != hipSuccess ||
+        (e = loss_partial.ensure((size_t)blocks * sizeof(float))) != hipSuccess ||
+        (e = loss.ensure((size_t)steps * sizeof(float))) != hipSuccess ||
+        (e = hipMemcpy(w.ptr, weights, PARAMS * sizeof(float), hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(m1.ptr, zeros.data(), PARAMS * sizeof(float), hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(m2.ptr, zeros.data(), PARAMS * sizeof(float), hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(x.ptr, features, count * FEATURES * sizeof(float), hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(t.ptr, targets, count * 2 * sizeof(float), hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(wt.ptr, weight, count * sizeof(float), hipMemcpyHostToDevice)) != hipSuccess) {
+        return fail_hip(e, err, err_len);
+    }
+    for (int step = 1; step <= steps && e == hipSuccess; ++step) {
+        e = launch_mlp_step(static_cast<float *>(w.ptr), static_cast<float *>(m1.ptr), static_cast<float *>(m2.ptr),
+                            static_cast<const float *>(x.ptr), static_cast<const float *>(t.ptr),
+                            static_cast<const float *>(wt.ptr), nullptr, (int64_t)count, tau_lo, tau_hi,
+                            learning_rate, step, static_cast<float *>(partial.ptr),
+                            static_cast<float *>(loss_partial.ptr), static_cast<float *>(loss.ptr) + (step - 1),
+                            nullptr);
+    }
+    if (e == hipSuccess) e = hipMemcpy(weights, w.ptr, PARAMS * sizeof(float), hipMemcpyDeviceToHost);
+    if (e == hipSuccess && losses) {
+        e = hipMemcpy(losses, loss.ptr, (size_t)steps * sizeof(float), hipMemcpyDeviceToHost);
+    }
+    return e == hipSuccess ? NDF_OK : fail_hip(e, err, err_len);
 }
 
 size_t ndf_get_num_threads(void)

@@ -71,7 +71,16 @@ struct SlideArgs {
     int32_t stat;
     int32_t min_count;
     int32_t rank_param;
-    int32_t *error_flag;  // set when a rank walk leaves its bounds
+    int32_t *error_flag;  // set when a rank walk leaves its bounds (plain mode)
+    // Tile selection: tiles 0 .. tiles_x*tiles_y*nz - 1, or tile_list[0 .. *tile_count).
+    const int32_t *tile_list;
+    const int32_t *tile_count;
+    // Bracket mode: per-tile key bracket [lo, hi]; a tile whose targets leave
+    // it is appended to fallback_tiles (count in fallback_count).
+    const void *bracket_lo;
+    const void *bracket_hi;
+    int32_t *fallback_tiles;
+    int32_t *fallback_count;
 };
 
 constexpr int SLIDE_WAVES = 4;
@@ -79,7 +88,7 @@ constexpr int SLIDE_TILE_X = 64;
 constexpr uint32_t SLIDE_NO_POS = 0xFFFFFFFFu;
 
 hipError_t launch_generic(int dtype_bytes, const GenericArgs &args, hipStream_t stream);
-hipError_t launch_slide(int dtype_bytes, const SlideArgs &args, size_t lds_bytes, hipStream_t stream);
+hipError_t launch_slide(int dtype_bytes, const SlideArgs &args, size_t lds_bytes, int max_tiles, hipStream_t stream);
 hipError_t launch_fill(int dtype_bytes, void *output, int64_t len, uint64_t bits, hipStream_t stream);
 size_t slide_lds_bytes(int dtype_bytes, int s, int sort_e, int hist_stride, int mask_bytes);
 

@@ -29,6 +29,13 @@
 namespace ndf {
 namespace {
 
+// Codes of support samples: 1 + tile rank for sorted (in-bracket) samples,
+// CODE_BELOW / CODE_ABOVE for retained samples outside the tile's bracket,
+// CODE_OMIT for omitted samples; CODE_PENDING marks in-bracket samples
+// between staging and ranking. Every rank code is below CODE_PENDING.
+constexpr uint16_t CODE_BELOW = 0;
+constexpr uint16_t CODE_PENDING = 0xFFFD;
+constexpr uint16_t CODE_ABOVE = 0xFFFE;
 constexpr uint16_t CODE_OMIT = 0xFFFF;
 constexpr uint16_t POS_NONE = 0xFFFF;
 constexpr int NT = SLIDE_WAVES * 64;
@@ -98,9 +105,12 @@ __device__ __forceinline__ uint32_t batch_mask(const uint16_t *pos, int first, c
     return mask;
 }
 
-// Moves the cursor (m, c) to the rank m with c == k in-window ranks below it
-// and m itself in the window. Returns false if the cursor would leave
-// [0, retained), which a consistent cursor never does.
+// Moves the cursor (m, c) to the rank m with c == k in-window samples below it
+// and m itself in the window. Ranks are 1-based (pos[r] for r in
+// [1, retained]); rank 0 stands for the samples below the bracket, which are
+// counted in c but never visited. Returns false if the target is not a
+// ranked sample (below or above the bracket in bracket mode; never in plain
+// mode for a consistent cursor).
 __device__ inline bool walk_to(int &m, int &c, int k, const uint16_t *pos, int retained, const Geometry &g,
                                uint32_t base)
 {
@@ -118,7 +128,7 @@ __device__ inline bool walk_to(int &m, int &c, int k, const uint16_t *pos, int r
             }
             c -= count;
             m -= BATCH;
-            if (m <= 0) {
+            if (m <= 1) {
                 return false;
             }
         }
@@ -136,14 +146,14 @@ __device__ inline bool walk_to(int &m, int &c, int k, const uint16_t *pos, int r
         }
         c += count;
         m += BATCH;
-        if (m >= retained) {
+        if (m > retained) {
             return false;
         }
     }
     return false;
 }
 
-// Largest in-window rank below m, or -1.
+// Largest in-window rank below m, or a value below 1 when none is ranked.
 __device__ inline int previous_member(int m, const uint16_t *pos, const Geometry &g, uint32_t base)
 {
     for (int first = m - BATCH; first > -BATCH; first -= BATCH) {
@@ -186,21 +196,25 @@ struct Bins16 {
 };
 
 // Counts, for one lane's window at row y, the codes in [lo, lo + 16 << shift)
-// per bin. Rectangles use per-column counts shared through LDS (hist); other
-// footprints scan their taps.
+// per bin, in `below` the codes under lo and in `above` the retained codes at
+// or over the range. Rectangles use per-column counts shared through LDS
+// (hist, BINS + 2 rows of sx + 1 entries); other footprints scan their taps.
 template <typename Bins>
-__device__ void window_bins(int counts[BINS], int lo, int shift, const uint16_t *codes, uint16_t *hist,
-                            const SlideArgs &a, int lane, int y)
+__device__ void window_bins(int counts[BINS], int &below, int &above, int lo, int shift, const uint16_t *codes,
+                            uint16_t *hist, const SlideArgs &a, int lane, int y)
 {
     const int hi = lo + (BINS << shift);
     if (a.mask == nullptr) {
         const int stride = a.sx + 1;
         for (int x = lane; x < a.sx; x += 64) {
             Bins bins;
+            int under = 0, over = 0;
             for (int z = 0; z < a.kz; ++z) {
                 const uint16_t *column = codes + (z * a.zrows + y) * a.sx + x;
                 for (int dy = 0; dy < a.ky; ++dy) {
                     const int code = column[dy * a.sx];
+                    under += code < lo;
+                    over += code >= hi && code != CODE_OMIT;
                     if (code >= lo && code < hi) {
                         bins.add((code - lo) >> shift);
                     }
@@ -210,12 +224,14 @@ __device__ void window_bins(int counts[BINS], int lo, int shift, const uint16_t 
             for (int b = 0; b < BINS; ++b) {
                 hist[b * stride + x] = (uint16_t)bins.get(b);
             }
+            hist[BINS * stride + x] = (uint16_t)under;
+            hist[(BINS + 1) * stride + x] = (uint16_t)over;
         }
         // Exclusive prefix over columns, one bin per lane. LDS accesses of one
         // wave complete in order; the wave barrier keeps the compiler from
         // moving them across the phases.
         __builtin_amdgcn_wave_barrier();
-        if (lane < BINS) {
+        if (lane <= BINS + 1) {
             uint16_t *row = hist + lane * stride;
             int run = 0;
             for (int x0 = 0; x0 < a.sx; x0 += 8) {
@@ -239,12 +255,17 @@ __device__ void window_bins(int counts[BINS], int lo, int shift, const uint16_t 
         for (int b = 0; b < BINS; ++b) {
             counts[b] = (int)hist[b * stride + lane + a.kx] - (int)hist[b * stride + lane];
         }
+        below = (int)hist[BINS * stride + lane + a.kx] - (int)hist[BINS * stride + lane];
+        above = (int)hist[(BINS + 1) * stride + lane + a.kx] - (int)hist[(BINS + 1) * stride + lane];
         __builtin_amdgcn_wave_barrier();
     } else {
         Bins bins;
+        int under = 0, over = 0;
         const int base = y * a.sx + lane;
         for (int t = 0; t < a.count; ++t) {
             const int code = codes[base + a.taps[t]];
+            under += code < lo;
+            over += code >= hi && code != CODE_OMIT;
             if (code >= lo && code < hi) {
                 bins.add((code - lo) >> shift);
             }
@@ -253,14 +274,11 @@ __device__ void window_bins(int counts[BINS], int lo, int shift, const uint16_t 
         for (int b = 0; b < BINS; ++b) {
             counts[b] = bins.get(b);
         }
+        below = under;
+        above = over;
     }
 }
 
-// Bitonic sort of NT * E (key, pos) pairs; thread t holds elements
-// [t*E, t*E + E) in registers. Strides below E run in registers; larger
-// strides exchange whole chunks with the partner thread through LDS
-// (xk/xp, rows of E + 1 entries so a chunk-wide access is bank-conflict
-// free).
 // Compile-time loop: calls f(std::integral_constant<int, I>) for I in [B, E).
 template <int B, int E, typename F>
 __device__ __forceinline__ void static_for(F &&f)
@@ -382,6 +400,17 @@ __device__ void block_sort(Chunk<Key, E> &c, Key *xk, uint16_t *xp, int t)
     }
 }
 
+template <typename Key, int E>
+constexpr size_t exchange_bytes()
+{
+    return ((size_t)NT * (E + 1) * (sizeof(Key) + 2) + 15) & ~(size_t)15;
+}
+
+// One workgroup per tile, persistent over tiles. Plain mode (no bracket):
+// every retained sample is ranked. Bracket mode: only samples whose keys lie
+// in the tile's [lo, hi] are ranked; the others count as below or above, and
+// a window whose target rank leaves the bracket sends the tile to the
+// fallback list (the plain kernel then recomputes it).
 template <typename T, int E>
 __global__ void __launch_bounds__(NT, 1) k_slide(SlideArgs a)
 {
@@ -389,292 +418,322 @@ __global__ void __launch_bounds__(NT, 1) k_slide(SlideArgs a)
     using Key = typename KT::Key;
     constexpr int N = NT * E;
     extern __shared__ __align__(16) unsigned char lds[];
-    // Layout: keys [N] | guard | pos [N] | guard | codes [s] | hist [4 waves] | mask.
-    // The sort's exchange rows (NT x (E + 1)) alias keys + pos + codes.
+    // Layout: [exchange rows NT x (E + 1) keys + u16 positions; after the
+    // sort: keys[N] | guard | pos[N] | guard] | codes [s] | hist | mask.
+    Key *xk = reinterpret_cast<Key *>(lds);
+    uint16_t *xp = reinterpret_cast<uint16_t *>(xk + NT * (E + 1));
     Key *keys = reinterpret_cast<Key *>(lds);
-    uint16_t *pos = reinterpret_cast<uint16_t *>(keys + N) + GUARD;
-    uint16_t *codes = pos + N + GUARD;
+    uint16_t *pos0 = reinterpret_cast<uint16_t *>(keys + N) + GUARD;
+    uint16_t *pos1 = pos0 - 1;  // 1-based ranks
+    uint16_t *codes = reinterpret_cast<uint16_t *>(lds + exchange_bytes<Key, E>());
     uint16_t *hist_all = codes + ((a.s + 7) & ~7);
     uint8_t *mask = a.mask ? reinterpret_cast<uint8_t *>(hist_all + SLIDE_WAVES * a.hist_stride) : nullptr;
-    Key *xk = keys;
-    uint16_t *xp = reinterpret_cast<uint16_t *>(xk + NT * (E + 1));
-    __shared__ int retained_shared;
+    __shared__ int count_shared;
+    __shared__ int failed_shared;
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
+    const bool bracket = a.bracket_lo != nullptr;
 
-    int tile = blockIdx.x;
-    const int tx = tile % a.tiles_x;
-    tile /= a.tiles_x;
-    const int tyi = tile % a.tiles_y;
-    const int64_t z0 = tile / a.tiles_y;
-    const int64_t x0 = (int64_t)tx * SLIDE_TILE_X;
-    const int64_t y0 = (int64_t)tyi * a.ty;
-
-    if (tid == 0) {
-        retained_shared = 0;
+    if (mask) {
+        const int mbytes = a.kz * a.ky * a.kx;
+        for (int q = tid; q < mbytes; q += NT) {
+            mask[q] = a.mask[q];
+        }
     }
-    __syncthreads();
-
-    // ---- stage: rows of the support, lanes along x; compact retained ----
+    const int ntiles = a.tile_count ? *a.tile_count : a.tiles_x * a.tiles_y * (int)a.nz;
     const Key *input = static_cast<const Key *>(a.input);
+    Key *output = static_cast<Key *>(a.output);
     const Key cval_key = classify<T>((Key)a.cval_bits, a.ignore_inf);
-    for (int row = wave; row < a.rows; row += SLIDE_WAVES) {
-        const int z = row >> a.zshift;
-        const int y = row & ((1 << a.zshift) - 1);
-        if (y >= a.sy) {
-            continue;  // padding rows between z planes
-        }
-        const int64_t mz = a.mapz[z0 + z];
-        const int64_t my = a.mapy[y0 + y];
-        for (int x0l = 0; x0l < a.sx; x0l += 64) {
-            const int x = x0l + lane;
-            Key key = KT::SENTINEL;
-            if (x < a.sx) {
-                const int64_t mx = a.mapx[x0 + x];
-                if (mz == MAP_PAD || my == MAP_PAD || mx == MAP_PAD) {
-                    key = KT::SENTINEL;
-                } else if (mz < 0 || my < 0 || mx < 0) {
-                    key = cval_key;
-                } else {
-                    key = classify<T>(input[mz + my + mx], a.ignore_inf);
-                }
-            }
-            const bool keep = key != KT::SENTINEL;
-            const uint64_t ballot = __ballot(keep);
-            int base = 0;
-            if (lane == 0 && ballot) {
-                base = atomicAdd(&retained_shared, __popcll(ballot));
-            }
-            base = __shfl(base, 0);
-            if (keep) {
-                const int slot = base + (int)__popcll(ballot & ((1ull << lane) - 1ull));
-                keys[slot] = key;
-                pos[slot] = (uint16_t)((row << 8) | x);
-            }
-        }
-    }
-    __syncthreads();
-    const int retained = retained_shared;
-
-    // ---- sort the retained samples (padded with sentinels to N) ----
-    {
-        Chunk<Key, E> chunk;
-        static_for<0, E>([&](auto ic) {
-            constexpr int I = decltype(ic)::value;
-            const int g = tid * E + I;
-            chunk.k[I] = g < retained ? keys[g] : KT::SENTINEL;
-            chunk.p[I] = g < retained ? pos[g] : POS_NONE;
-        });
-        __syncthreads();
-        block_sort<Key, E>(chunk, xk, xp, tid);
-        __syncthreads();
-        static_for<0, E>([&](auto ic) {
-            constexpr int I = decltype(ic)::value;
-            keys[tid * E + I] = chunk.k[I];
-            pos[tid * E + I] = (uint16_t)chunk.p[I];
-        });
-        __syncthreads();
-        // Codes, guards and the mask are written after the sort: the
-        // exchange rows may overlap them.
-        for (int q = tid; q < a.s; q += NT) {
-            codes[q] = CODE_OMIT;
-        }
-        if (tid < GUARD) {
-            pos[tid - GUARD] = POS_NONE;
-            pos[N + tid] = POS_NONE;
-        }
-        if (mask) {
-            const int mbytes = a.kz * a.ky * a.kx;
-            for (int q = tid; q < mbytes; q += NT) {
-                mask[q] = a.mask[q];
-            }
-        }
-        __syncthreads();
-        static_for<0, E>([&](auto ic) {
-            constexpr int I = decltype(ic)::value;
-            if (chunk.p[I] != POS_NONE) {
-                codes[(chunk.p[I] >> 8) * a.sx + (chunk.p[I] & 0xFFu)] = (uint16_t)(tid * E + I);
-            }
-        });
-    }
-    __syncthreads();
-
-    // ---- per-lane slide ----
-    const int y_first = wave * a.tw;
-    if (y0 + y_first >= a.ny || retained == 0) {
-        if (retained == 0) {
-            // Every window of the tile is empty: NaN.
-            for (int y = y_first; y < y_first + a.tw && y0 + y < a.ny; ++y) {
-                if (x0 + lane < a.nx) {
-                    static_cast<Key *>(a.output)[z0 * a.ny * a.nx + (y0 + y) * a.nx + x0 + lane] = KT::CANONICAL_NAN;
-                }
-            }
-        }
-        return;
-    }
     const Geometry g{a.kz, a.ky, a.kx, a.zshift, ((((uint32_t)1 << a.zshift) - 1u) << 8) | 0xFFu, mask};
     uint16_t *hist = hist_all + wave * a.hist_stride;
-    const bool col_valid = x0 + lane < a.nx;
-    Key *output = static_cast<Key *>(a.output);
     const bool wide = a.kz * a.ky > 255 || (a.mask != nullptr && a.count > 255);
 
-    auto target_of = [&](int n) {
-        if (n < a.min_count || n <= 0) {
-            return -1;
+    // XCD-aware order: block b runs on XCD b % 8, so slot L = b + i * grid
+    // maps to tile (L % 8) * chunk + L / 8 and each XCD walks a contiguous
+    // run of tiles (vertical neighbours share their halo rows in its L2).
+    const int chunk = (ntiles + 7) / 8;
+    for (int slot = blockIdx.x; slot < 8 * chunk; slot += gridDim.x) {
+        const int t = (slot % 8) * chunk + slot / 8;
+        if (t >= ntiles) {
+            continue;
         }
-        if (a.stat == STAT_RANK) {
-            return a.rank_param < n ? a.rank_param : -1;
+        const int tile = a.tile_list ? a.tile_list[t] : t;
+        int rest = tile;
+        const int tyi = rest % a.tiles_y;  // tiles run down a column first
+        rest /= a.tiles_y;
+        const int tx = rest % a.tiles_x;
+        const int64_t z0 = rest / a.tiles_x;
+        const int64_t x0 = (int64_t)tx * SLIDE_TILE_X;
+        const int64_t y0 = (int64_t)tyi * a.ty;
+        Key lo_key = 0, hi_key = KT::SENTINEL;
+        if (bracket) {
+            lo_key = static_cast<const Key *>(a.bracket_lo)[tile];
+            hi_key = static_cast<const Key *>(a.bracket_hi)[tile];
         }
-        return n >> 1;
-    };
+        if (tid == 0) {
+            count_shared = 0;
+            failed_shared = 0;
+        }
+        __syncthreads();
 
-    // Initial cursor: two levels of 16-bin histograms over the ranks.
-    int m = 0, c = 0, n = 0;
-    {
-        int shift1 = 0;
-        while ((BINS << shift1) < retained) {
-            ++shift1;
-        }
-        int counts[BINS];
-        if (wide) {
-            window_bins<Bins16>(counts, 0, shift1, codes, hist, a, lane, y_first);
-        } else {
-            window_bins<Bins8>(counts, 0, shift1, codes, hist, a, lane, y_first);
-        }
-        n = 0;
-#pragma unroll
-        for (int b = 0; b < BINS; ++b) {
-            n += counts[b];
-        }
-        const int k = target_of(n);
-        int bin = -1, below = 0;
-        if (k >= 0) {
-            int run = 0;
-#pragma unroll
-            for (int b = 0; b < BINS; ++b) {
-                if (bin < 0 && k < run + counts[b]) {
-                    bin = b;
-                    below = run;
+        // ---- stage: classify the support, compact in-bracket samples ----
+        for (int row = wave; row < a.rows; row += SLIDE_WAVES) {
+            const int z = row >> a.zshift;
+            const int y = row & ((1 << a.zshift) - 1);
+            if (y >= a.sy) {
+                continue;  // padding rows between z planes
+            }
+            const int64_t mz = a.mapz[z0 + z];
+            const int64_t my = a.mapy[y0 + y];
+            for (int xb = 0; xb < a.sx; xb += 64) {
+                const int x = xb + lane;
+                Key key = KT::SENTINEL;
+                if (x < a.sx) {
+                    const int64_t mx = a.mapx[x0 + x];
+                    if (mz == MAP_PAD || my == MAP_PAD || mx == MAP_PAD) {
+                        key = KT::SENTINEL;
+                    } else if (mz < 0 || my < 0 || mx < 0) {
+                        key = cval_key;
+                    } else {
+                        key = classify<T>(input[mz + my + mx], a.ignore_inf);
+                    }
+                    uint16_t code = CODE_PENDING;
+                    if (key == KT::SENTINEL) {
+                        code = CODE_OMIT;
+                    } else if (key < lo_key) {
+                        code = CODE_BELOW;
+                    } else if (key > hi_key) {
+                        code = CODE_ABOVE;
+                    }
+                    codes[row * a.sx + x] = code;
                 }
-                run += counts[b];
-            }
-        }
-        // Second level over the union of the lanes' first-level bins.
-        const uint64_t any = __ballot(bin >= 0);
-        int bmin = BINS, bmax = -1;
-        for (int b = 0; b < BINS; ++b) {
-            if (__ballot(bin == b)) {
-                bmin = b < bmin ? b : bmin;
-                bmax = b;
-            }
-        }
-        if (any && shift1 > 2) {
-            const int lo2 = bmin << shift1;
-            const int span = (bmax + 1 - bmin) << shift1;
-            int shift2 = 0;
-            while ((BINS << shift2) < span) {
-                ++shift2;
-            }
-            int counts2[BINS];
-            if (wide) {
-                window_bins<Bins16>(counts2, lo2, shift2, codes, hist, a, lane, y_first);
-            } else {
-                window_bins<Bins8>(counts2, lo2, shift2, codes, hist, a, lane, y_first);
-            }
-            if (bin >= 0) {
-                int below2 = 0;
-                for (int b = 0; b < bmin; ++b) {
-                    below2 += counts[b];
+                const bool keep = key != KT::SENTINEL && key >= lo_key && key <= hi_key;
+                const uint64_t ballot = __ballot(keep);
+                int base = 0;
+                if (lane == 0 && ballot) {
+                    base = atomicAdd(&count_shared, __popcll(ballot));
                 }
-                int run = below2, bin2 = -1, base2 = below2;
+                base = __shfl(base, 0);
+                if (keep) {
+                    const int dst = base + (int)__popcll(ballot & ((1ull << lane) - 1ull));
+                    if (dst < N) {
+                        xk[dst] = key;
+                        xp[dst] = (uint16_t)((row << 8) | x);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        const int retained = count_shared;
+        if (retained > N) {
+            // More in-bracket samples than the sort holds (bracket mode only;
+            // the plain sort holds the whole support).
+            if (tid == 0) {
+                if (bracket) {
+                    a.fallback_tiles[atomicAdd(a.fallback_count, 1)] = tile;
+                } else {
+                    atomicOr(a.error_flag, 1);
+                }
+            }
+            __syncthreads();
+            continue;
+        }
+
+        // ---- sort the in-bracket samples (padded with sentinels to N) ----
+        {
+            Chunk<Key, E> chunk;
+            static_for<0, E>([&](auto ic) {
+                constexpr int I = decltype(ic)::value;
+                const int gi = tid * E + I;
+                chunk.k[I] = gi < retained ? xk[gi] : KT::SENTINEL;
+                chunk.p[I] = gi < retained ? xp[gi] : POS_NONE;
+            });
+            __syncthreads();
+            block_sort<Key, E>(chunk, xk, xp, tid);
+            __syncthreads();
+            static_for<0, E>([&](auto ic) {
+                constexpr int I = decltype(ic)::value;
+                keys[tid * E + I] = chunk.k[I];
+                pos0[tid * E + I] = (uint16_t)chunk.p[I];
+                if (chunk.p[I] != POS_NONE) {
+                    codes[(chunk.p[I] >> 8) * a.sx + (chunk.p[I] & 0xFFu)] = (uint16_t)(tid * E + I + 1);
+                }
+            });
+            if (tid < GUARD) {
+                pos0[tid - GUARD] = POS_NONE;
+                pos0[N + tid] = POS_NONE;
+            }
+        }
+        __syncthreads();
+
+        // ---- per-lane slide ----
+        const int y_first = wave * a.tw;
+        if (y0 + y_first < a.ny) {
+            const bool col_valid = x0 + lane < a.nx;
+            auto target_of = [&](int n) {
+                if (n < a.min_count || n <= 0) {
+                    return -1;
+                }
+                if (a.stat == STAT_RANK) {
+                    return a.rank_param < n ? a.rank_param : -1;
+                }
+                return n >> 1;
+            };
+
+            // Initial cursor: two levels of 16-bin histograms over the ranks.
+            int m = 1, c = 0, n = 0;
+            bool failed = false;
+            {
+                int shift1 = 0;
+                while ((BINS << shift1) <= retained) {
+                    ++shift1;
+                }
+                int counts[BINS], below = 0, above = 0;
+                if (wide) {
+                    window_bins<Bins16>(counts, below, above, 1, shift1, codes, hist, a, lane, y_first);
+                } else {
+                    window_bins<Bins8>(counts, below, above, 1, shift1, codes, hist, a, lane, y_first);
+                }
+                n = below + above;
 #pragma unroll
                 for (int b = 0; b < BINS; ++b) {
-                    if (bin2 < 0 && k < run + counts2[b]) {
-                        bin2 = b;
-                        base2 = run;
+                    n += counts[b];
+                }
+                const int k = target_of(n);
+                int bin = -1, base1 = below;
+                if (k >= 0) {
+                    if (k < below || k >= n - above) {
+                        failed = true;  // the target lies outside the bracket
+                    } else {
+                        int run = below;
+#pragma unroll
+                        for (int b = 0; b < BINS; ++b) {
+                            if (bin < 0 && k < run + counts[b]) {
+                                bin = b;
+                                base1 = run;
+                            }
+                            run += counts[b];
+                        }
                     }
-                    run += counts2[b];
                 }
-                if (bin2 < 0) {
-                    bin2 = 0;
-                    base2 = below2;
+                c = below;  // consistent with m = 1 for lanes without a target
+                int bmin = BINS, bmax = -1;
+                for (int b = 0; b < BINS; ++b) {
+                    if (__ballot(bin == b)) {
+                        bmin = b < bmin ? b : bmin;
+                        bmax = b;
+                    }
                 }
-                m = lo2 + (bin2 << shift2);
-                c = base2;
+                if (bmax >= 0 && shift1 > 2) {
+                    const int lo2 = 1 + (bmin << shift1);
+                    const int span = (bmax + 1 - bmin) << shift1;
+                    int shift2 = 0;
+                    while ((BINS << shift2) < span) {
+                        ++shift2;
+                    }
+                    int counts2[BINS], below2 = 0, above2 = 0;
+                    if (wide) {
+                        window_bins<Bins16>(counts2, below2, above2, lo2, shift2, codes, hist, a, lane, y_first);
+                    } else {
+                        window_bins<Bins8>(counts2, below2, above2, lo2, shift2, codes, hist, a, lane, y_first);
+                    }
+                    if (bin >= 0) {
+                        int run = below2, bin2 = -1, base2 = below2;
+#pragma unroll
+                        for (int b = 0; b < BINS; ++b) {
+                            if (bin2 < 0 && k < run + counts2[b]) {
+                                bin2 = b;
+                                base2 = run;
+                            }
+                            run += counts2[b];
+                        }
+                        if (bin2 < 0) {
+                            bin2 = 0;
+                            base2 = below2;
+                        }
+                        m = lo2 + (bin2 << shift2);
+                        c = base2;
+                    }
+                } else if (bin >= 0) {
+                    m = 1 + (bin << shift1);
+                    c = base1;
+                }
             }
-        } else if (bin >= 0) {
-            m = bin << shift1;
-            c = below;
-        }
-    }
 
-    bool failed = false;
-    for (int y = y_first; y < y_first + a.tw && y0 + y < a.ny; ++y) {
-        if (y > y_first) {
-            int dn = 0, dc = 0;
-            if (a.mask == nullptr) {
-                // Rectangle: the top row of the previous window leaves, the
-                // bottom row of this one enters (every z plane).
-                for (int z = 0; z < a.kz; ++z) {
-                    const uint16_t *top = codes + (z * a.zrows + y - 1) * a.sx + lane;
-                    const uint16_t *bottom = top + a.ky * a.sx;
+            for (int y = y_first; y < y_first + a.tw && y0 + y < a.ny && !failed; ++y) {
+                if (y > y_first) {
+                    int dn = 0, dc = 0;
+                    if (a.mask == nullptr) {
+                        // Rectangle: the top row of the previous window leaves,
+                        // the bottom row of this one enters (every z plane).
+                        for (int z = 0; z < a.kz; ++z) {
+                            const uint16_t *top = codes + (z * a.zrows + y - 1) * a.sx + lane;
+                            const uint16_t *bottom = top + a.ky * a.sx;
 #pragma unroll 8
-                    for (int dx = 0; dx < a.kx; ++dx) {
-                        const int cr = top[dx];
-                        const int ca = bottom[dx];
-                        dn += (int)(ca != CODE_OMIT) - (int)(cr != CODE_OMIT);
-                        dc += (int)(ca < m) - (int)(cr < m);
+                            for (int dx = 0; dx < a.kx; ++dx) {
+                                const int cr = top[dx];
+                                const int ca = bottom[dx];
+                                dn += (int)(ca != CODE_OMIT) - (int)(cr != CODE_OMIT);
+                                dc += (int)(ca < m) - (int)(cr < m);
+                            }
+                        }
+                    } else {
+                        const int rbase = (y - 1) * a.sx + lane;
+                        const int abase = y * a.sx + lane;
+#pragma unroll 8
+                        for (int e = 0; e < a.nedge; ++e) {
+                            const int cr = codes[rbase + a.edge_remove[e]];
+                            const int ca = codes[abase + a.edge_add[e]];
+                            dn += (int)(ca != CODE_OMIT) - (int)(cr != CODE_OMIT);
+                            dc += (int)(ca < m) - (int)(cr < m);
+                        }
+                    }
+                    n += dn;
+                    c += dc;
+                }
+                Key out = KT::CANONICAL_NAN;
+                const int k = target_of(n);
+                if (k >= 0) {
+                    const uint32_t base = ((uint32_t)y << 8) | (uint32_t)lane;
+                    if (!walk_to(m, c, k, pos1, retained, g, base)) {
+                        failed = true;
+                        break;
+                    }
+                    if (a.stat == STAT_MEDIAN && (n & 1) == 0) {
+                        const int m2 = previous_member(m, pos1, g, base);
+                        if (m2 < 1) {
+                            failed = true;
+                            break;
+                        }
+                        out = median_pair_bits<T>(keys[m2 - 1], keys[m - 1]);
+                    } else {
+                        out = KT::unkey(keys[m - 1]);
                     }
                 }
-            } else {
-                const int rbase = (y - 1) * a.sx + lane;
-                const int abase = y * a.sx + lane;
-#pragma unroll 8
-                for (int e = 0; e < a.nedge; ++e) {
-                    const int cr = codes[rbase + a.edge_remove[e]];
-                    const int ca = codes[abase + a.edge_add[e]];
-                    dn += (int)(ca != CODE_OMIT) - (int)(cr != CODE_OMIT);
-                    dc += (int)(ca < m) - (int)(cr < m);
+                if (col_valid) {
+                    output[z0 * a.ny * a.nx + (y0 + y) * a.nx + x0 + lane] = out;
                 }
             }
-            n += dn;
-            c += dc;
-        }
-        Key out = KT::CANONICAL_NAN;
-        const int k = target_of(n);
-        if (k >= 0) {
-            const uint32_t base = ((uint32_t)y << 8) | (uint32_t)lane;
-            if (!walk_to(m, c, k, pos, retained, g, base)) {
-                failed = true;
-                break;
+            if (failed) {
+                failed_shared = 1;
             }
-            if (a.stat == STAT_MEDIAN && (n & 1) == 0) {
-                const int m2 = previous_member(m, pos, g, base);
-                if (m2 < 0) {
-                    failed = true;
-                    break;
-                }
-                out = median_pair_bits<T>(keys[m2], keys[m]);
+        }
+        __syncthreads();
+        if (tid == 0 && failed_shared) {
+            if (bracket) {
+                a.fallback_tiles[atomicAdd(a.fallback_count, 1)] = tile;
             } else {
-                out = KT::unkey(keys[m]);
+                atomicOr(a.error_flag, 1);
             }
         }
-        if (col_valid) {
-            output[z0 * a.ny * a.nx + (y0 + y) * a.nx + x0 + lane] = out;
-        }
-    }
-    if (failed) {
-        atomicOr(a.error_flag, 1);
     }
 }
 
 template <typename T>
-hipError_t launch_typed(const SlideArgs &args, size_t lds_bytes, hipStream_t stream)
+hipError_t launch_typed(const SlideArgs &args, size_t lds_bytes, int max_tiles, hipStream_t stream)
 {
-    const int64_t blocks = (int64_t)args.tiles_x * args.tiles_y * args.nz;
-    const dim3 grid((unsigned)blocks);
     const dim3 block(NT);
     const void *fn = nullptr;
     switch (args.sort_e) {
@@ -689,6 +748,16 @@ hipError_t launch_typed(const SlideArgs &args, size_t lds_bytes, hipStream_t str
     if (e != hipSuccess) {
         return e;
     }
+    // Persistent grid: as many workgroups as can be resident at once.
+    int per_cu = 0, device = 0, cus = 0;
+    if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, NT, lds_bytes)) != hipSuccess ||
+        (e = hipGetDevice(&device)) != hipSuccess ||
+        (e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device)) != hipSuccess) {
+        return e;
+    }
+    int64_t blocks = (int64_t)(per_cu < 1 ? 1 : per_cu) * (cus < 1 ? 1 : cus);
+    blocks = blocks < max_tiles ? blocks : max_tiles;
+    const dim3 grid((unsigned)(blocks < 1 ? 1 : blocks));
     switch (args.sort_e) {
     case 4: hipLaunchKernelGGL((k_slide<T, 4>), grid, block, lds_bytes, stream, args); break;
     case 8: hipLaunchKernelGGL((k_slide<T, 8>), grid, block, lds_bytes, stream, args); break;
@@ -703,22 +772,20 @@ hipError_t launch_typed(const SlideArgs &args, size_t lds_bytes, hipStream_t str
 
 size_t slide_lds_bytes(int dtype_bytes, int s, int sort_e, int hist_stride, int mask_bytes)
 {
-    const size_t n = (size_t)NT * (size_t)sort_e;
-    const size_t main = n * (size_t)(dtype_bytes + 2) + (size_t)(2 * GUARD) * 2 + (size_t)((s + 7) & ~7) * 2 +
-                        (size_t)SLIDE_WAVES * (size_t)hist_stride * 2 + (size_t)mask_bytes;
-    // The sort's exchange rows (NT x (E + 1) keys and positions) alias the
-    // start of the layout.
-    const size_t exchange = (size_t)NT * (size_t)(sort_e + 1) * (size_t)(dtype_bytes + 2);
-    return main > exchange ? main : exchange;
+    // Exchange rows (NT x (E + 1) keys and u16 positions, 16-byte aligned);
+    // after the sort the same region holds keys[N], pos[N] and the guards.
+    const size_t exchange = (((size_t)NT * (size_t)(sort_e + 1) * (size_t)(dtype_bytes + 2)) + 15) & ~(size_t)15;
+    return exchange + (size_t)((s + 7) & ~7) * 2 + (size_t)SLIDE_WAVES * (size_t)hist_stride * 2 +
+           (size_t)mask_bytes;
 }
 
-hipError_t launch_slide(int dtype_bytes, const SlideArgs &args, size_t lds_bytes, hipStream_t stream)
+hipError_t launch_slide(int dtype_bytes, const SlideArgs &args, size_t lds_bytes, int max_tiles, hipStream_t stream)
 {
-    if ((int64_t)args.tiles_x * args.tiles_y * args.nz <= 0) {
+    if (max_tiles <= 0) {
         return hipSuccess;
     }
-    return dtype_bytes == 4 ? launch_typed<float>(args, lds_bytes, stream)
-                            : launch_typed<double>(args, lds_bytes, stream);
+    return dtype_bytes == 4 ? launch_typed<float>(args, lds_bytes, max_tiles, stream)
+                            : launch_typed<double>(args, lds_bytes, max_tiles, stream);
 }
 
 }  // namespace ndf

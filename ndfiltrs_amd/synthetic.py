@@ -71,7 +71,7 @@ def cube(shape=(128, 1024, 1024), seed: int = BASE_SEED + 4) -> np.ndarray:
     return image
 
 
-def sky_quadratic(shape=(8192, 8192), seed: int = BASE_SEED + 5) -> np.ndarray:
+def sky_quadratic(shape=(8192, 8192), seed: int = BASE_SEED + 5, sources: int = 2000) -> np.ndarray:
     """63x63 workload: N(1000,10) sky + 2-D quadratic background up to 300,
     2000 sources, 2% NaN, float32."""
     rng = np.random.default_rng(seed)
@@ -80,7 +80,7 @@ def sky_quadratic(shape=(8192, 8192), seed: int = BASE_SEED + 5) -> np.ndarray:
     y = np.linspace(-1.0, 1.0, rows)[:, None]
     x = np.linspace(-1.0, 1.0, cols)[None, :]
     image += 150.0 * (x**2 + y**2)
-    _add_sources(image, rng, 2000)
+    _add_sources(image, rng, sources)
     image[rng.random(shape) < 0.02] = np.nan
     return image.astype(np.float32)
 

@@ -88,3 +88,22 @@ def test_engines_can_be_forced(engine: str) -> None:
     plan.set_engine(engine)
     described = json.loads(plan.describe())["engine"]
     assert described == ("slide" if engine == "auto" else engine)
+
+
+def test_bracket_validation_without_a_device() -> None:
+    plan = _core.Plan(np.float32, (256, 256), "median", (63, 63), None, (0, 0), ("reflect",) * 2, 0.0, True,
+                      False, 1, 0)
+    weights = np.zeros(1346, dtype=np.float32)
+    with pytest.raises(ValueError, match="1346 parameters"):
+        plan.set_bracket(weights[:-1])
+    with pytest.raises(ValueError, match="sort length"):
+        plan.set_bracket(weights, 3000)
+    plan.set_bracket(weights, 4096)
+    plan.set_bracket(None)
+    line = _core.Plan(np.float32, (4096,), "median", (63,), None, (0,), ("reflect",), 0.0, True, False, 1, 0)
+    with pytest.raises(ValueError, match="2-D image"):
+        line.set_bracket(weights)
+    cube = _core.Plan(np.float32, (8, 64, 64), "median", (3, 5, 5), None, (0, 0, 0), ("reflect",) * 3, 0.0, True,
+                      False, 1, 0)
+    with pytest.raises(ValueError, match="2-D image"):
+        cube.set_bracket(weights)

@@ -1,0 +1,203 @@
+"""GPU: the learned bracket never changes a result.
+
+Every bracketed launch is compared bit for bit with the plain slide kernel
+(and, at oracle-friendly sizes, with the CPU oracle). Hand-set models pin the
+fallback logic: a zero-width bracket must send (nearly) every tile to the
+plain kernel, an unbounded bracket none, and a NaN model degenerates to an
+unbounded bracket. The training kernels are pinned by a closed form: with
+constant features the fitted outputs are the tau quantiles of the targets.
+"""
+
+from __future__ import annotations
+
+import os
+
+import numpy as np
+import pytest
+
+from ndfiltrs_amd import _core, bracket, synthetic
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def device() -> None:
+    assert _core.device_count() > 0, "no HIP device visible: GPU tests need an MI355X"
+
+
+def constant_model(lo: float, hi: float) -> np.ndarray:
+    """A model whose outputs are the standardised offsets (lo, hi) everywhere."""
+    weights = np.zeros(bracket.PARAMS, dtype=np.float32)
+    weights[-2:] = (lo, hi)
+    return weights
+
+
+def make_plan(image, kind, size, parameter=0, mode="reflect", footprint=None, omit=True, ignore_inf=False,
+              min_count=1):
+    ndim = image.ndim
+    flat = None
+    if footprint is not None:
+        size = list(footprint.shape)
+        flat = footprint.ravel().tolist()
+    return _core.Plan(image.dtype, image.shape, kind, size, flat, (0,) * ndim, (mode,) * ndim, 0.0, omit,
+                      ignore_inf, min_count, parameter)
+
+
+def run(plan, image, weights=None, sort_len=4096):
+    d_in = _core.DeviceArray(image.nbytes)
+    d_out = _core.DeviceArray(image.nbytes)
+    d_in.upload(image)
+    plan.set_bracket(weights, sort_len) if weights is not None else plan.set_bracket(None)
+    plan.apply_device(d_in.address, d_out.address)
+    out = d_out.download(np.empty_like(image))
+    report = plan.bracket_report()
+    d_in.free()
+    d_out.free()
+    return out, report
+
+
+def assert_same_bits(a, b):
+    view = np.uint32 if a.dtype == np.float32 else np.uint64
+    mismatch = np.flatnonzero(a.view(view).ravel() != b.view(view).ravel())
+    assert mismatch.size == 0, f"{mismatch.size} outputs differ, first at flat index {mismatch[0]}"
+
+
+CASES = [
+    # (name, image factory, kind, size, parameter, mode)
+    ("c5_median", lambda: synthetic.sky_quadratic((384, 512), seed=11, sources=12), "median", (63, 63), 0, "reflect"),
+    ("c5_rank10", lambda: synthetic.sky_quadratic((384, 512), seed=12, sources=12), "rank", (63, 63), 397, "reflect"),
+    ("c2_f64", lambda: synthetic.sky_gradient_f64((300, 260), seed=13), "median", (31, 31), 0, "nearest"),
+    ("c1_small", lambda: synthetic.sky_small((200, 330), seed=14), "median", (5, 5), 0, "reflect"),
+    ("wrap_rank", lambda: synthetic.sky_small((150, 150), seed=15), "rank", (9, 7), 50, "wrap"),
+    ("constant_mode", lambda: synthetic.sky_small((130, 140), seed=16), "median", (15, 15), 0, "constant"),
+]
+
+
+@pytest.mark.parametrize("lo,hi", [(-1.0, 1.0), (-0.2, 0.2), (0.0, 0.0), (-3.0, 3.0), (-1e30, 1e30)])
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_bracket_bits_equal_plain(case, lo, hi) -> None:
+    name, factory, kind, size, parameter, mode = case
+    image = factory()
+    plan = make_plan(image, kind, size, parameter, mode)
+    plain, _ = run(plan, image)
+    bracketed, (tiles, fallback) = run(plan, image, constant_model(lo, hi))
+    assert_same_bits(bracketed, plain)
+    assert tiles > 0 and 0 <= fallback <= tiles
+    if lo == hi == 0.0:
+        assert fallback >= 0.9 * tiles, (fallback, tiles)
+    if hi >= 1e30 and image.dtype == np.float32:
+        # Unbounded bracket: only capacity (more samples than the sort) can fail.
+        support = plan_support(plan)
+        if support <= 4096:
+            assert fallback == 0
+
+
+def plan_support(plan) -> int:
+    import json
+
+    info = json.loads(plan.describe())
+    return int(info.get("support", 1 << 30))
+
+
+def test_bracket_matches_oracle_small() -> None:
+    image = synthetic.sky_quadratic((192, 256), seed=21, sources=6)
+    plan = make_plan(image, "median", (63, 63))
+    out, (tiles, fallback) = run(plan, image, constant_model(-0.5, 0.5))
+    expected = oracle.window_filter(image, size=(63, 63), mode="reflect", statistic="median", omit_nan=True,
+                                    method="histogram")
+    assert_same_bits(out, expected)
+    assert 0 <= fallback <= tiles
+
+
+def test_nan_model_is_an_unbounded_bracket() -> None:
+    image = synthetic.sky_small((160, 192), seed=22)
+    plan = make_plan(image, "median", (11, 11))
+    plain, _ = run(plan, image)
+    weights = constant_model(-1.0, 1.0)
+    weights[-1] = np.nan
+    out, (tiles, fallback) = run(plan, image, weights)
+    assert_same_bits(out, plain)
+    assert fallback == 0
+
+
+@pytest.mark.parametrize("sort_len", [1024, 2048, 8192])
+def test_sort_lengths(sort_len) -> None:
+    image = synthetic.sky_quadratic((256, 320), seed=23, sources=8)
+    plan = make_plan(image, "median", (31, 31))
+    plain, _ = run(plan, image)
+    out, (tiles, fallback) = run(plan, image, constant_model(-0.8, 0.8), sort_len)
+    assert_same_bits(out, plain)
+
+
+def test_footprint_and_propagate() -> None:
+    rng = np.random.default_rng(24)
+    image = synthetic.sky_small((180, 200), seed=24)
+    fp = rng.random((9, 13)) < 0.6
+    fp[4, 6] = True
+    for omit in (True, False):
+        plan = make_plan(image, "median", None, footprint=fp, omit=omit)
+        plain, _ = run(plan, image)
+        out, _ = run(plan, image, constant_model(-0.7, 0.7))
+        assert_same_bits(out, plain)
+
+
+def test_all_nan_tiles_and_min_count() -> None:
+    image = synthetic.sky_small((200, 256), seed=25)
+    image[:, 64:140] = np.nan
+    plan = make_plan(image, "median", (21, 21), min_count=200)
+    plain, _ = run(plan, image)
+    out, _ = run(plan, image, constant_model(-1.0, 1.0))
+    assert_same_bits(out, plain)
+
+
+def test_trained_model_if_present() -> None:
+    if not os.path.exists(bracket.DEFAULT_MODEL):
+        pytest.skip("no trained bracket model in the tree")
+    weights, _ = bracket.load()
+    image = synthetic.sky_quadratic((512, 640), seed=26, sources=20)
+    for kind, parameter in (("median", 0), ("rank", 397)):
+        plan = make_plan(image, kind, (63, 63), parameter)
+        plain, _ = run(plan, image)
+        out, (tiles, fallback) = run(plan, image, weights)
+        assert_same_bits(out, plain)
+        assert fallback <= tiles
+
+
+def test_training_recovers_target_quantiles() -> None:
+    """Constant features: the pinball optimum is the tau quantile of each target."""
+    rng = np.random.default_rng(27)
+    count = 20000
+    features = np.zeros((count, bracket.FEATURES), dtype=np.float32)
+    features[:, 3] = 0.5
+    lower = rng.normal(-2.0, 0.5, size=count).astype(np.float32)
+    upper = rng.normal(2.0, 0.25, size=count).astype(np.float32)
+    targets = np.stack([lower, upper], axis=1)
+    weight = np.ones(count, dtype=np.float32)
+    weight[:100] = 0.0  # ignored samples
+    weights, losses = bracket.fit(bracket.init_weights(1), features, targets, weight, 1500, 1e-2, 0.05, 0.95)
+    assert np.all(np.isfinite(losses)) and losses[-1] < losses[0]
+    pred = bracket.forward(weights, features[:1])[0]
+    q_lo = np.quantile(lower[100:], 0.05)
+    q_hi = np.quantile(upper[100:], 0.95)
+    assert abs(pred[0] - q_lo) < 0.05, (pred, q_lo)
+    assert abs(pred[1] - q_hi) < 0.05, (pred, q_hi)
+
+
+def test_training_data_shapes_and_labels() -> None:
+    image = synthetic.sky_quadratic((256, 256), seed=28, sources=4)
+    plan = make_plan(image, "median", (63, 63))
+    d_in = _core.DeviceArray(image.nbytes)
+    d_out = _core.DeviceArray(image.nbytes)
+    d_in.upload(image)
+    plan.apply_device(d_in.address, d_out.address)
+    features, targets, weight = plan.bracket_training_data(d_in.address, d_out.address)
+    out = d_out.download(np.empty_like(image))
+    d_in.free()
+    d_out.free()
+    assert features.shape[1] == 6 and targets.shape == (features.shape[0], 2)
+    assert np.all(np.isfinite(features)) and np.all(weight == 1.0)
+    assert np.all(targets[:, 0] <= targets[:, 1])
+    # The tile labels bracket the plain outputs of their tile by construction:
+    # check the global extremes (standardisation is monotone per tile).
+    assert np.isfinite(out).any()
