@@ -117,8 +117,9 @@ void ndf_plan_destroy(ndf_plan *plan);
  * counterpart). A tiny MLP (6 -> 32 -> 32 -> 2, ReLU, fp32; NDF_BRACKET_PARAMS
  * floats: W1[6][32], b1[32], W2[32][32], b2[32], W3[32][2], b3[2]) proposes
  * a value bracket per tile from box-sum features of its support; only
- * in-bracket samples are sorted (sort_len entries: 1024..16384) and any tile
- * whose window targets leave the bracket is recomputed by the plain kernel.
+ * in-bracket samples are sorted (sort_len entries: 1024, 2048, 4096 or 8192)
+ * and any tile whose window targets leave the bracket is recomputed by the
+ * plain kernel. Needs a 2-D image and a rectangular window at most 64 wide.
  * Results are bit-identical with and without a bracket. weights = NULL
  * disables the bracket. */
 #define NDF_BRACKET_PARAMS 1346

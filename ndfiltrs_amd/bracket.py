@@ -60,10 +60,8 @@ def forward(weights: np.ndarray, features: np.ndarray) -> np.ndarray:
 
 
 def training_image(seed: int, shape=(2048, 2048)) -> np.ndarray:
-    """The 63x63 workload's law at 2048^2: sources scaled with the area."""
-    rng_sources = int(round(2000 * shape[0] * shape[1] / (8192 * 8192)))
-    image = synthetic.sky_quadratic(shape=shape, seed=seed, sources=rng_sources)
-    return image
+    """A seeded crop of the 63x63 workload's law (see synthetic.sky_quadratic_crop)."""
+    return synthetic.sky_quadratic_crop(shape=shape, seed=seed)
 
 
 def collect(images, size: int, statistics, mode: str = "reflect"):
@@ -171,7 +169,11 @@ def train_main(argv=None) -> int:
         "image_side": args.side,
         "window": [args.size, args.size],
         "statistics": [["median", 0], ["rank", rank]],
-        "law": "sky N(1000,10) + quadratic background up to 300 + scaled sources + 2% NaN (float32)",
+        "law": "2048^2 crops (seeded offsets) of the 8192^2 field: sky N(1000,10) + quadratic background up to "
+               "300 + 2000 sources per 8192^2 + 2% NaN (float32)",
+        "bracket_tile": [int(json.loads(_core.Plan(np.float32, (args.side, args.side), "median", (args.size, args.size),
+                                                   None, (0, 0), ("reflect", "reflect"), 0.0, True, False, 1, 0)
+                                         .describe())["bracket_tile_rows"]), 64],
         "steps": args.steps,
         "learning_rate": args.lr,
         "tau": [args.tau_lo, args.tau_hi],

@@ -1,21 +1,21 @@
 // Learned bracket proposal for the slide engine (2-D images).
 //
-// A tile's in-bracket samples are the only ones the slide kernel has to sort.
-// The bracket comes from a tiny MLP (6 -> 32 -> 32 -> 2, ReLU, fp32) that
-// reads cheap box-sum features of the tile's support:
+// A tile's in-bracket samples are the only ones the bracket kernel sorts and
+// walks. The bracket comes from a tiny MLP (6 -> 32 -> 32 -> 2, ReLU, fp32)
+// that reads cheap box-sum features of the tile's support:
 //   1. k_block_stats: one pass over the image, per 8x8 block: retained
 //      count, sum, sum of squares (f64), min, max;
-//   2. k_tile_features: per tile, the blocks covering its support give
-//      mean mu, std sigma, min, max and the retained fraction; the features
-//      are scale free: asinh((min-mu)/sigma), asinh((max-mu)/sigma),
-//      fraction, target quantile, asinh(sigma/|mu|), log2(taps)/16;
+//   2. k_tile_features: per tile, robust level c and noise scale sigma
+//      (medians of the block means and block std devs), a plane slope of
+//      the block means, the retained fraction and the fraction of bright
+//      blocks, standardised so that the features are scale free;
 //   3. k_bracket_predict: the MLP's two outputs are standardised offsets:
-//      bracket = [mu + sigma*o0, mu + sigma*o1], converted to keys.
+//      bracket = [c + sigma*o0, c + sigma*o1], converted to keys.
 // Training (k_tile_labels, k_mlp_grad, k_mlp_adam) fits the offsets of the
 // smallest and largest window result per tile with pinball losses, so that
 // the bracket contains every target of a tile with high probability.
-// Exactness never depends on the model: the slide kernel falls back to the
-// plain kernel for any tile whose targets leave the bracket.
+// Exactness never depends on the model: the bracket kernel sends any tile
+// whose targets leave the bracket to the plain kernel.
 #include "ndf_bracket.hpp"
 #include "ndf_device.hpp"
 
@@ -23,6 +23,7 @@ namespace ndf {
 namespace {
 
 constexpr int BLOCK = 8;
+constexpr int MAX_TILE_BLOCKS = 1024;
 
 struct Stats {
     double n, sum, sumsq, mn, mx;
@@ -75,59 +76,161 @@ __device__ inline float feature_asinh(double v)
     return (float)(v < 0.0 ? -r : r);
 }
 
+// Ascending bitonic sort of v[0 .. n) in LDS by one 64-lane workgroup
+// (n a power of two, 64 <= n <= MAX_TILE_BLOCKS).
+__device__ void wave_sort(float *v, int n, int lane)
+{
+    for (int size = 2; size <= n; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int i = lane; i < n / 2; i += 64) {
+                const int lo = 2 * i - (i & (stride - 1));
+                const int hi = lo + stride;
+                const bool asc = (lo & size) == 0;
+                const float a = v[lo], b = v[hi];
+                const bool swap = asc ? (b < a) : (a < b);
+                v[lo] = swap ? b : a;
+                v[hi] = swap ? a : b;
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// Per tile, from the 8x8 blocks covering its support (clipped to the image):
+//   c     = median of the block means       (robust level: sources move few blocks)
+//   sigma = median of the block std devs    (robust noise scale)
+//   (gy, gx) = least-squares plane slope of the block means over the blocks
+//           within 0.75 sigma of c (sources excluded), per pixel
+// Features (all scale free): asinh((mean - c) / sigma), asinh(gy * ty / sigma),
+// asinh(gx * 64 / sigma), retained fraction, target quantile, fraction of
+// blocks brighter than c + 0.75 sigma. Targets are standardised as
+// (value - c) / sigma.
 __global__ void __launch_bounds__(64) k_tile_features(BracketArgs b, const double *blocks)
 {
+    __shared__ float means[MAX_TILE_BLOCKS];
+    __shared__ float sds[MAX_TILE_BLOCKS];
     const int tile = blockIdx.x;
     const int lane = threadIdx.x;
     const int tyi = tile % b.tiles_y;
     const int tx = tile / b.tiles_y;
-    // Support rows/columns of the tile, clipped to the image.
     int64_t r0 = (int64_t)tyi * b.ty + b.low_y, r1 = (int64_t)tyi * b.ty + b.ty - 1 + b.high_y;
     int64_t c0 = (int64_t)tx * 64 + b.low_x, c1 = (int64_t)tx * 64 + 63 + b.high_x;
     r0 = r0 < 0 ? 0 : r0;
     c0 = c0 < 0 ? 0 : c0;
     r1 = r1 >= b.ny ? b.ny - 1 : r1;
     c1 = c1 >= b.nx ? b.nx - 1 : c1;
-    const int64_t br0 = r0 / BLOCK, br1 = r1 / BLOCK, bc0 = c0 / BLOCK, bc1 = c1 / BLOCK;
-    const int64_t width = bc1 - bc0 + 1;
-    const int64_t count = (br1 - br0 + 1) * width;
-    double n = 0.0, s = 0.0, q = 0.0, mn = __builtin_huge_val(), mx = -__builtin_huge_val();
-    for (int64_t i = lane; i < count; i += 64) {
-        const double *o = blocks + ((br0 + i / width) * b.bx + bc0 + i % width) * 5;
-        n += o[0];
-        s += o[1];
-        q += o[2];
-        mn = o[3] < mn ? o[3] : mn;
-        mx = o[4] > mx ? o[4] : mx;
+    const int64_t br0 = r0 / BLOCK, bc0 = c0 / BLOCK;
+    const int height = (int)(r1 / BLOCK - br0 + 1), width = (int)(c1 / BLOCK - bc0 + 1);
+    // Subsample block rows/columns when a support spans more blocks than the
+    // sort holds.
+    int step = 1;
+    while (((height + step - 1) / step) * ((width + step - 1) / step) > MAX_TILE_BLOCKS) {
+        step <<= 1;
+    }
+    const int h_used = (height + step - 1) / step, w_used = (width + step - 1) / step;
+    const int count = h_used * w_used;
+    int n_sort = 64;
+    while (n_sort < count) {
+        n_sort <<= 1;
+    }
+    double n = 0.0, s = 0.0, cells = 0.0;
+    int usable = 0;
+    for (int i = lane; i < n_sort; i += 64) {
+        float mean = __builtin_huge_valf(), sd = __builtin_huge_valf();
+        if (i < count) {
+            const int64_t br = br0 + (int64_t)(i / w_used) * step, bc = bc0 + (int64_t)(i % w_used) * step;
+            const double *o = blocks + (br * b.bx + bc) * 5;
+            n += o[0];
+            s += o[1];
+            cells += 64.0;
+            if (o[0] >= 16.0) {
+                const double mu = o[1] / o[0];
+                const double var = o[2] / o[0] - mu * mu;
+                mean = (float)mu;
+                sd = (float)(var > 0.0 ? __builtin_sqrt(var) : 0.0);
+                ++usable;
+            }
+        }
+        means[i] = mean;
+        sds[i] = sd;
     }
     for (int off = 32; off > 0; off >>= 1) {
         n += __shfl_xor(n, off);
         s += __shfl_xor(s, off);
-        q += __shfl_xor(q, off);
-        const double a = __shfl_xor(mn, off), c = __shfl_xor(mx, off);
-        mn = a < mn ? a : mn;
-        mx = c > mx ? c : mx;
+        cells += __shfl_xor(cells, off);
+        usable += __shfl_xor(usable, off);
+    }
+    __syncthreads();
+    wave_sort(means, n_sort, lane);
+    wave_sort(sds, n_sort, lane);
+    float c = 0.0f, sigma = 1.0f;
+    if (usable > 0) {
+        c = means[(usable - 1) / 2];
+        sigma = sds[(usable - 1) / 2];
+    }
+    const float floor_sigma = 1e-6f * (__builtin_fabsf(c) + 1e-30f);
+    sigma = sigma > floor_sigma ? sigma : floor_sigma;
+    // Plane fit over unsorted block means (re-read from the block table).
+    double sw = 0, sr = 0, sq = 0, sm = 0, srr = 0, sqq = 0, srq = 0, srm = 0, sqm = 0, bright = 0;
+    for (int i = lane; i < count; i += 64) {
+        const int64_t br = br0 + (int64_t)(i / w_used) * step, bc = bc0 + (int64_t)(i % w_used) * step;
+        const double *o = blocks + (br * b.bx + bc) * 5;
+        if (o[0] >= 16.0) {
+            const double mu = o[1] / o[0] - (double)c;
+            bright += mu > 0.75 * (double)sigma ? 1.0 : 0.0;
+            if (__builtin_fabs(mu) <= 0.75 * (double)sigma) {
+                const double r = (double)((i / w_used) * step * BLOCK), q = (double)((i % w_used) * step * BLOCK);
+                sw += 1.0;
+                sr += r;
+                sq += q;
+                sm += mu;
+                srr += r * r;
+                sqq += q * q;
+                srq += r * q;
+                srm += r * mu;
+                sqm += q * mu;
+            }
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        sw += __shfl_xor(sw, off);
+        sr += __shfl_xor(sr, off);
+        sq += __shfl_xor(sq, off);
+        sm += __shfl_xor(sm, off);
+        srr += __shfl_xor(srr, off);
+        sqq += __shfl_xor(sqq, off);
+        srq += __shfl_xor(srq, off);
+        srm += __shfl_xor(srm, off);
+        sqm += __shfl_xor(sqm, off);
+        bright += __shfl_xor(bright, off);
     }
     if (lane == 0) {
-        const double cells = (double)(count * BLOCK * BLOCK);
-        double mu = 0.0, sigma = 1.0;
-        if (n > 0.0) {
-            mu = s / n;
-            const double var = q / n - mu * mu;
-            sigma = var > 0.0 ? __builtin_sqrt(var) : 0.0;
+        double gy = 0.0, gx = 0.0;
+        if (sw >= 3.0) {
+            const double vrr = srr - sr * sr / sw, vqq = sqq - sq * sq / sw, vrq = srq - sr * sq / sw;
+            const double vrm = srm - sr * sm / sw, vqm = sqm - sq * sm / sw;
+            const double det = vrr * vqq - vrq * vrq;
+            if (det > 1e-9 * (vrr * vqq + 1e-30)) {
+                gy = (vrm * vqq - vqm * vrq) / det;
+                gx = (vqm * vrr - vrm * vrq) / det;
+            } else if (vrr > 0.0) {
+                gy = vrm / vrr;
+            } else if (vqq > 0.0) {
+                gx = vqm / vqq;
+            }
         }
-        const double floor_sigma = 1e-6 * (__builtin_fabs(mu) + 1e-30);
-        sigma = sigma > floor_sigma ? sigma : floor_sigma;
+        const double sg = (double)sigma;
+        const double mean = n > 0.0 ? s / n : (double)c;
         float *f = b.features + (int64_t)tile * FEATURES;
-        f[0] = n > 0.0 ? feature_asinh((mn - mu) / sigma) : 0.0f;
-        f[1] = n > 0.0 ? feature_asinh((mx - mu) / sigma) : 0.0f;
-        f[2] = (float)(n / cells);
-        f[3] = b.quantile;
-        f[4] = feature_asinh(sigma / (__builtin_fabs(mu) + 1e-30));
-        f[5] = b.log_taps;
+        f[0] = usable > 0 ? feature_asinh((mean - (double)c) / sg) : 0.0f;
+        f[1] = feature_asinh(gy * (double)b.ty / sg);
+        f[2] = feature_asinh(gx * 64.0 / sg);
+        f[3] = cells > 0.0 ? (float)(n / cells) : 0.0f;
+        f[4] = b.quantile;
+        f[5] = usable > 0 ? (float)(bright / (double)usable) : 0.0f;
         double *m = b.moments + (int64_t)tile * 3;
-        m[0] = mu;
-        m[1] = sigma;
+        m[0] = (double)c;
+        m[1] = sg;
         m[2] = n;
     }
 }

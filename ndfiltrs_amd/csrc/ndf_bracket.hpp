@@ -24,9 +24,8 @@ struct BracketArgs {
     int32_t ty, tiles_y, tiles_x;
     int32_t low_y, high_y, low_x, high_x;  // tap offset extents of the window
     float quantile;                         // target quantile of the statistic
-    float log_taps;                         // log2(taps) / 16
     float *features;                        // tiles x FEATURES
-    double *moments;                        // tiles x 3: mean, std, retained count
+    double *moments;                        // tiles x 3: level c, scale sigma, retained count
     void *lo_keys;                          // tiles keys (uint32 / uint64)
     void *hi_keys;
     float *predicted;                       // tiles x 2 standardised offsets

@@ -197,6 +197,9 @@ struct ndf_plan {
     bool bracket_ready = false;
     bool bracket_launched = false;  // the last launch took the bracket path
     int bracket_e = 16;
+    int bracket_ty = 0;  // tile rows of the bracket kernel (0: no bracket for this plan)
+    SlideArgs slide_fb{};  // plain kernel on the bracket tiles (fallback)
+    size_t slide_fb_lds = 0;
     std::vector<float> bracket_weights;
     SlideArgs slide_b{};
     size_t slide_b_lds = 0;
@@ -221,6 +224,24 @@ struct ndf_plan {
 };
 
 namespace {
+
+// Slide arguments of the same 2-D plan (one plane, kz = 1) with tiles of
+// `ty` rows.
+SlideArgs with_tile_rows(const SlideArgs &s, int ty)
+{
+    SlideArgs r = s;
+    r.ty = ty;
+    r.tw = ty / SLIDE_WAVES;
+    r.sy = ty + s.ky - 1;
+    r.zrows = r.sy;
+    r.rows = r.sy;
+    r.s = r.rows * s.sx;
+    int e = 4;
+    while (256 * e < r.s) e <<= 1;
+    r.sort_e = e;
+    r.tiles_y = (int32_t)((s.ny + ty - 1) / ty);
+    return r;
+}
 
 // Plan the slide engine; false when the geometry cannot be staged.
 bool plan_slide(ndf_plan *p)
@@ -316,7 +337,8 @@ bool plan_slide(ndf_plan *p)
         }
     };
     build(p->slide_mapz, dims[0] + kz - 1, dims[0], lo[0], kz, modes[0], in_sz);
-    build(p->slide_mapy, tiles_y * ty + ky - 1, dims[1], lo[1], ky, modes[1], in_sy);
+    // Long enough for any tile height up to 256 rows (the bracket tiles).
+    build(p->slide_mapy, std::max(tiles_y * ty, dims[1] + 256) + ky - 1, dims[1], lo[1], ky, modes[1], in_sy);
     build(p->slide_mapx, tiles_x * SLIDE_TILE_X + kx - 1, dims[2], lo[2], kx, modes[2], 1);
 
     p->slide_taps.clear();
@@ -367,6 +389,29 @@ bool plan_slide(ndf_plan *p)
     s_args.min_count = (int32_t)p->eff_min_count;
     s_args.rank_param = (int32_t)std::min(p->parameter, p->count);
     p->slide_lds = slide_lds_bytes(p->dbytes, s, sort_e, (int)hist_stride, mask_bytes);
+
+    // Bracket tiles (2-D rectangles with support rows of at most 128
+    // samples): the tallest tile of 24 to 32 rows among those that keep the
+    // most bracket workgroups resident per CU (at most 3 counted); shorter
+    // tiles only when none of those fits.
+    p->bracket_ty = 0;
+    if (rect && kz == 1 && dims[0] == 1 && kx <= 64 && p->rank >= 2) {
+        int best_per_cu = 0;
+        for (int cand = 32; cand >= SLIDE_WAVES && (best_per_cu == 0 || cand >= 24); cand -= SLIDE_WAVES) {
+            const SlideArgs v = with_tile_rows(s_args, cand);
+            if (v.rows > 254 || v.s > 16384 || v.sort_e > 64 ||
+                slide_lds_bytes(p->dbytes, v.s, v.sort_e, (int)hist_stride, 0) > (size_t)limit) {
+                continue;
+            }
+            const size_t need = bracket_lds_bytes(p->dbytes, v.s, v.rows, 16);
+            const size_t granule = 512;
+            const int per_cu = std::min(3, (int)((size_t)limit / ((need + 16 + granule - 1) / granule * granule)));
+            if (per_cu > best_per_cu) {
+                best_per_cu = per_cu;
+                p->bracket_ty = cand;
+            }
+        }
+    }
     return true;
 }
 
@@ -447,18 +492,19 @@ ndf_status prepare_engine_tables(ndf_plan *p, char *err, size_t err_len)
     return NDF_OK;
 }
 
-bool bracket_supported(const ndf_plan *p)
-{
-    return p->slide_ok && p->rank >= 2 && p->slide.nz == 1 && p->slide.kz == 1;
-}
+bool bracket_supported(const ndf_plan *p) { return p->slide_ok && p->bracket_ty > 0; }
 
-int64_t slide_tiles(const ndf_plan *p) { return (int64_t)p->slide.tiles_x * p->slide.tiles_y * p->slide.nz; }
+// Tiles of the bracket geometry (bracket_ty rows each).
+int64_t bracket_tiles(const ndf_plan *p)
+{
+    return (int64_t)p->slide.tiles_x * ((p->slide.ny + p->bracket_ty - 1) / p->bracket_ty);
+}
 
 // Allocates the bracket buffers and fills the bracket launch arguments.
 ndf_status prepare_bracket(ndf_plan *p, char *err, size_t err_len)
 {
     if (p->bracket_ready) return NDF_OK;
-    const int64_t tiles = slide_tiles(p);
+    const int64_t tiles = bracket_tiles(p);
     const int64_t by = (p->slide.ny + 7) / 8, bx = (p->slide.nx + 7) / 8;
     hipError_t e;
     if ((e = p->b_blocks.ensure((size_t)(by * bx * 5) * sizeof(double))) != hipSuccess ||
@@ -474,14 +520,18 @@ ndf_status prepare_bracket(ndf_plan *p, char *err, size_t err_len)
     if (!p->bracket_weights.empty() && (e = upload(p->b_weights, p->bracket_weights)) != hipSuccess) {
         return fail_hip(e, err, err_len);
     }
+    // Plain kernel on the bracket tiles (recomputes fallback tiles); it
+    // shares the plan's device tables.
+    p->slide_fb = with_tile_rows(p->slide, p->bracket_ty);
+    p->slide_fb_lds = slide_lds_bytes(p->dbytes, p->slide_fb.s, p->slide_fb.sort_e, p->slide_fb.hist_stride, 0);
     BracketArgs &b = p->bargs;
     b = BracketArgs{};
     b.ny = p->slide.ny;
     b.nx = p->slide.nx;
     b.bx = bx;
-    b.ty = p->slide.ty;
-    b.tiles_y = p->slide.tiles_y;
-    b.tiles_x = p->slide.tiles_x;
+    b.ty = p->slide_fb.ty;
+    b.tiles_y = p->slide_fb.tiles_y;
+    b.tiles_x = p->slide_fb.tiles_x;
     const size_t ay = p->rank - 2, ax = p->rank - 1;
     b.low_y = (int32_t)p->low[ay];
     b.high_y = (int32_t)p->high[ay];
@@ -490,20 +540,18 @@ ndf_status prepare_bracket(ndf_plan *p, char *err, size_t err_len)
     b.quantile = p->stat == STAT_RANK
                      ? (float)((double)std::min(p->parameter, p->count) / (double)(p->count > 1 ? p->count - 1 : 1))
                      : 0.5f;
-    b.log_taps = (float)(std::log2((double)p->count) / 16.0);
     b.features = static_cast<float *>(p->b_features.ptr);
     b.moments = static_cast<double *>(p->b_moments.ptr);
     b.lo_keys = p->b_lo.ptr;
     b.hi_keys = p->b_hi.ptr;
     b.predicted = static_cast<float *>(p->b_pred.ptr);
-    p->slide_b = p->slide;
+    p->slide_b = p->slide_fb;
     p->slide_b.sort_e = p->bracket_e;
     p->slide_b.bracket_lo = p->b_lo.ptr;
     p->slide_b.bracket_hi = p->b_hi.ptr;
     p->slide_b.fallback_tiles = static_cast<int32_t *>(p->b_fallback.ptr);
     p->slide_b.fallback_count = static_cast<int32_t *>(p->b_fcount.ptr);
-    p->slide_b_lds = slide_lds_bytes(p->dbytes, p->slide.s, p->bracket_e, p->slide.hist_stride,
-                                     (int)p->slide_mask.size());
+    p->slide_b_lds = bracket_lds_bytes(p->dbytes, p->slide_fb.s, p->slide_fb.rows, p->bracket_e);
     p->bracket_ready = true;
     return NDF_OK;
 }
@@ -511,7 +559,7 @@ ndf_status prepare_bracket(ndf_plan *p, char *err, size_t err_len)
 // Features and bracket keys of every tile for the input in d_in.
 hipError_t bracket_propose(ndf_plan *p, const void *d_in, hipStream_t stream)
 {
-    const int64_t tiles = slide_tiles(p);
+    const int64_t tiles = bracket_tiles(p);
     hipError_t e = launch_block_stats(p->dbytes, d_in, p->slide.ny, p->slide.nx, p->eff_ignore_inf,
                                       static_cast<double *>(p->b_blocks.ptr), stream);
     if (e == hipSuccess) e = launch_tile_features(p->bargs, static_cast<const double *>(p->b_blocks.ptr), tiles, stream);
@@ -531,7 +579,7 @@ ndf_status launch(ndf_plan *p, const void *d_in, void *d_out, hipStream_t stream
     } else if (p->engine == NDF_ENGINE_SLIDE && p->bracket_on) {
         // Learned bracket: propose, filter in-bracket samples only, then
         // recompute every fallback tile with the plain kernel.
-        const int tiles = (int)slide_tiles(p);
+        const int tiles = (int)bracket_tiles(p);
         p->bracket_launched = true;
         e = bracket_propose(p, d_in, stream);
         if (e == hipSuccess) e = launch_fill(4, p->b_fcount.ptr, 1, 0, stream);
@@ -539,15 +587,15 @@ ndf_status launch(ndf_plan *p, const void *d_in, void *d_out, hipStream_t stream
             SlideArgs args = p->slide_b;
             args.input = d_in;
             args.output = d_out;
-            e = launch_slide(p->dbytes, args, p->slide_b_lds, tiles, stream);
+            e = launch_bracket_slide(p->dbytes, args, p->slide_b_lds, tiles, stream);
         }
         if (e == hipSuccess) {
-            SlideArgs args = p->slide;
+            SlideArgs args = p->slide_fb;
             args.input = d_in;
             args.output = d_out;
             args.tile_list = static_cast<const int32_t *>(p->b_fallback.ptr);
             args.tile_count = static_cast<const int32_t *>(p->b_fcount.ptr);
-            e = launch_slide(p->dbytes, args, p->slide_lds, tiles, stream);
+            e = launch_slide(p->dbytes, args, p->slide_fb_lds, tiles, stream);
         }
     } else if (p->engine == NDF_ENGINE_SLIDE) {
         SlideArgs args = p->slide;
@@ -762,10 +810,10 @@ ndf_status ndf_plan_describe(const ndf_plan *plan, char *buffer, size_t buffer_l
     if (plan->engine == NDF_ENGINE_SLIDE) {
         snprintf(buffer, buffer_len,
                  "{\"engine\": \"%s\", \"count\": %zu, \"tile\": [%d, %d], \"support\": %d, \"sort_len\": %d, "
-                 "\"lds_bytes\": %zu, \"edge_taps\": %d, \"tiles\": %lld}",
+                 "\"lds_bytes\": %zu, \"edge_taps\": %d, \"tiles\": %lld, \"bracket_tile_rows\": %d}",
                  engine, plan->count, plan->slide.ty, SLIDE_TILE_X, plan->slide.s, 256 * plan->slide.sort_e, plan->slide_lds,
                  plan->slide.nedge,
-                 (long long)plan->slide.tiles_x * plan->slide.tiles_y * plan->slide.nz);
+                 (long long)plan->slide.tiles_x * plan->slide.tiles_y * plan->slide.nz, plan->bracket_ty);
     } else {
         snprintf(buffer, buffer_len, "{\"engine\": \"%s\", \"count\": %zu}", engine, plan->count);
     }
@@ -874,15 +922,17 @@ ndf_status ndf_plan_set_bracket(ndf_plan *plan, const float *weights, size_t n_w
     }
     if (n_weights != (size_t)PARAMS) return fail(NDF_ERR_INVALID, "bracket model must have 1346 parameters", err, err_len);
     if (!bracket_supported(plan)) {
-        return fail(NDF_ERR_INVALID, "the learned bracket needs a 2-D image on the slide engine", err, err_len);
+        return fail(NDF_ERR_INVALID,
+                    "the learned bracket needs a 2-D image, a rectangular window at most 64 wide and the slide engine",
+                    err, err_len);
     }
     int e = 4;
     while (256 * e < sort_len && e < 64) e <<= 1;
-    if (sort_len <= 0 || 256 * e != sort_len) {
-        return fail(NDF_ERR_INVALID, "bracket sort length must be 1024, 2048, 4096, 8192 or 16384", err, err_len);
+    if (sort_len <= 0 || 256 * e != sort_len || e > 32) {
+        return fail(NDF_ERR_INVALID, "bracket sort length must be 1024, 2048, 4096 or 8192", err, err_len);
     }
-    if (slide_lds_bytes(plan->dbytes, plan->slide.s, e, plan->slide.hist_stride, (int)plan->slide_mask.size()) >
-        (size_t)lds_limit()) {
+    const SlideArgs geometry = with_tile_rows(plan->slide, plan->bracket_ty);
+    if (bracket_lds_bytes(plan->dbytes, geometry.s, geometry.rows, e) > (size_t)lds_limit()) {
         return fail(NDF_ERR_INVALID, "bracket sort length does not fit LDS", err, err_len);
     }
     plan->bracket_weights.assign(weights, weights + n_weights);
@@ -895,7 +945,7 @@ ndf_status ndf_plan_set_bracket(ndf_plan *plan, const float *weights, size_t n_w
 ndf_status ndf_plan_bracket_report(ndf_plan *plan, int64_t *tiles, int64_t *fallback_tiles)
 {
     std::lock_guard<std::mutex> guard(plan->mutex);
-    *tiles = plan->slide_ok ? slide_tiles(plan) : 0;
+    *tiles = bracket_supported(plan) ? bracket_tiles(plan) : 0;
     *fallback_tiles = 0;
     if (!plan->bracket_launched) return NDF_OK;
     int32_t count = 0;
@@ -909,7 +959,7 @@ ndf_status ndf_plan_bracket_report(ndf_plan *plan, int64_t *tiles, int64_t *fall
 
 size_t ndf_plan_bracket_tiles(const ndf_plan *plan)
 {
-    return bracket_supported(plan) ? (size_t)slide_tiles(plan) : 0;
+    return bracket_supported(plan) ? (size_t)bracket_tiles(plan) : 0;
 }
 
 ndf_status ndf_plan_bracket_training_data(ndf_plan *plan, const void *d_input, const void *d_output, float *features,
@@ -917,7 +967,9 @@ ndf_status ndf_plan_bracket_training_data(ndf_plan *plan, const void *d_input, c
 {
     std::lock_guard<std::mutex> guard(plan->mutex);
     if (!bracket_supported(plan)) {
-        return fail(NDF_ERR_INVALID, "the learned bracket needs a 2-D image on the slide engine", err, err_len);
+        return fail(NDF_ERR_INVALID,
+                    "the learned bracket needs a 2-D image, a rectangular window at most 64 wide and the slide engine",
+                    err, err_len);
     }
     ndf_status st = prepare_tables(plan, err, err_len);
     if (st != NDF_OK) return st;
@@ -926,7 +978,7 @@ ndf_status ndf_plan_bracket_training_data(ndf_plan *plan, const void *d_input, c
     st = prepare_bracket(plan, err, err_len);
     plan->bracket_on = was_on;
     if (st != NDF_OK) return st;
-    const int64_t tiles = slide_tiles(plan);
+    const int64_t tiles = bracket_tiles(plan);
     DeviceBuffer d_targets, d_weight;
     hipError_t e;
     if ((e = d_targets.ensure((size_t)tiles * 2 * sizeof(float))) != hipSuccess ||

@@ -91,5 +91,11 @@ hipError_t launch_generic(int dtype_bytes, const GenericArgs &args, hipStream_t 
 hipError_t launch_slide(int dtype_bytes, const SlideArgs &args, size_t lds_bytes, int max_tiles, hipStream_t stream);
 hipError_t launch_fill(int dtype_bytes, void *output, int64_t len, uint64_t bits, hipStream_t stream);
 size_t slide_lds_bytes(int dtype_bytes, int s, int sort_e, int hist_stride, int mask_bytes);
+// Bracket kernel (2-D rectangles, support rows <= 128 samples): ranks only
+// the samples inside each tile's key bracket; tiles it cannot finish are
+// appended to args.fallback_tiles for the plain kernel.
+hipError_t launch_bracket_slide(int dtype_bytes, const SlideArgs &args, size_t lds_bytes, int max_tiles,
+                                hipStream_t stream);
+size_t bracket_lds_bytes(int dtype_bytes, int s, int rows, int sort_e);
 
 }  // namespace ndf

@@ -731,6 +731,287 @@ __global__ void __launch_bounds__(NT, 1) k_slide(SlideArgs a)
     }
 }
 
+// Bit masks of one support row of a bracket tile (bit x <-> column x of the
+// support; rows are at most 128 samples wide).
+struct RowMasks {
+    uint64_t kept[2];    // retained samples
+    uint64_t below[2];   // retained samples below the tile's bracket
+    uint64_t inside[2];  // samples inside the bracket (ranked)
+};
+
+// Bits [lane, lane + kx) of a 128-bit row mask, shifted down to bit 0.
+__device__ __forceinline__ uint64_t window_bits(const uint64_t w[2], int lane, uint64_t kxmask)
+{
+    const uint64_t low = w[0] >> lane;
+    const uint64_t high = lane ? (w[1] << (64 - lane)) : 0ull;
+    return (low | high) & kxmask;
+}
+
+// Bracket kernel for 2-D rectangular windows (support rows <= 128 samples).
+// One workgroup per tile, persistent over tiles. Support samples are
+// classified against the tile's key bracket [lo, hi]: only inside samples
+// are sorted and ranked; per support row, bit masks record the retained,
+// below and inside samples. Each lane keeps the retained / below / inside
+// counts of its window as popcounts of its window's bits, and a cursor
+// (m, c) over the inside ranks where c counts the window's below samples
+// plus its inside samples ranked under m. A row step therefore touches only
+// the inside samples of the leaving and entering rows. A wave starts at
+// m = 1 (c = below) and walks up. A window whose target rank is not inside
+// the bracket sends the tile to the fallback list, where the plain kernel
+// recomputes it.
+template <typename T, int E>
+__global__ void __launch_bounds__(NT, 2) k_bracket(SlideArgs a)
+{
+    using KT = KeyTraits<T>;
+    using Key = typename KT::Key;
+    constexpr int N = NT * E;
+    constexpr int R = 4;  // support rows a wave loads per batch
+    extern __shared__ __align__(16) unsigned char lds[];
+    Key *xk = reinterpret_cast<Key *>(lds);
+    uint16_t *xp = reinterpret_cast<uint16_t *>(xk + NT * (E + 1));
+    Key *keys = reinterpret_cast<Key *>(lds);
+    uint16_t *pos0 = reinterpret_cast<uint16_t *>(keys + N) + GUARD;
+    uint16_t *pos1 = pos0 - 1;
+    uint16_t *codes = reinterpret_cast<uint16_t *>(lds + exchange_bytes<Key, E>());
+    RowMasks *rowmasks = reinterpret_cast<RowMasks *>(codes + ((a.s + 7) & ~7));
+    __shared__ int count_shared;
+    __shared__ int failed_shared;
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int ntiles = a.tile_count ? *a.tile_count : a.tiles_x * a.tiles_y;
+    const Key *input = static_cast<const Key *>(a.input);
+    Key *output = static_cast<Key *>(a.output);
+    const Key cval_key = classify<T>((Key)a.cval_bits, a.ignore_inf);
+    const Geometry g{1, a.ky, a.kx, 8, 0xFFFFu, nullptr};
+    const uint64_t kxmask = a.kx >= 64 ? ~0ull : ((1ull << a.kx) - 1ull);
+    const int64_t mz = a.mapz[0];
+    const uint64_t lanemask_lt = (1ull << lane) - 1ull;
+
+    const int chunk = (ntiles + 7) / 8;
+    for (int slot = blockIdx.x; slot < 8 * chunk; slot += gridDim.x) {
+        const int t = (slot % 8) * chunk + slot / 8;
+        if (t >= ntiles) {
+            continue;
+        }
+        const int tile = a.tile_list ? a.tile_list[t] : t;
+        const int tyi = tile % a.tiles_y;
+        const int tx = tile / a.tiles_y;
+        const int64_t x0 = (int64_t)tx * SLIDE_TILE_X;
+        const int64_t y0 = (int64_t)tyi * a.ty;
+        const Key lo_key = static_cast<const Key *>(a.bracket_lo)[tile];
+        const Key hi_key = static_cast<const Key *>(a.bracket_hi)[tile];
+        if (tid == 0) {
+            count_shared = 0;
+            failed_shared = 0;
+        }
+        __syncthreads();
+
+        // ---- stage: R rows per wave per batch, loads first ----
+        const int64_t mx0 = a.mapx[x0 + lane];
+        const int64_t mx1 = lane + 64 < a.sx ? a.mapx[x0 + 64 + lane] : MAP_PAD;
+        for (int row0 = wave * R; row0 < a.rows; row0 += SLIDE_WAVES * R) {
+            Key kv[R][2];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int row = row0 + r;
+                const int64_t my = row < a.rows ? a.mapy[y0 + row] : MAP_PAD;
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int64_t mx = h ? mx1 : mx0;
+                    const bool pad = my == MAP_PAD || mx == MAP_PAD;
+                    const bool cval = !pad && (my < 0 || mx < 0);
+                    const Key raw = input[pad || cval ? 0 : mz + my + mx];
+                    kv[r][h] = pad ? KT::SENTINEL : (cval ? cval_key : classify<T>(raw, a.ignore_inf));
+                }
+            }
+            uint64_t in_bits[R][2];
+            int total = 0;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const Key key = kv[r][h];
+                    const bool kept = key != KT::SENTINEL;
+                    const uint64_t bk = __ballot(kept);
+                    const uint64_t bb = __ballot(kept && key < lo_key);
+                    const uint64_t bi = __ballot(kept && key >= lo_key && key <= hi_key);
+                    if (lane == 0 && row0 + r < a.rows) {
+                        rowmasks[row0 + r].kept[h] = bk;
+                        rowmasks[row0 + r].below[h] = bb;
+                        rowmasks[row0 + r].inside[h] = bi;
+                    }
+                    in_bits[r][h] = bi;
+                    total += (int)__popcll(bi);
+                }
+            }
+            int base = 0;
+            if (lane == 0 && total) {
+                base = atomicAdd(&count_shared, total);
+            }
+            base = __shfl(base, 0);
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const uint64_t bi = in_bits[r][h];
+                    if ((bi >> lane) & 1ull) {
+                        const int dst = base + (int)__popcll(bi & lanemask_lt);
+                        if (dst < N) {
+                            xk[dst] = kv[r][h];
+                            xp[dst] = (uint16_t)(((row0 + r) << 8) | (h * 64 + lane));
+                        }
+                    }
+                    base += (int)__popcll(bi);
+                }
+            }
+        }
+        __syncthreads();
+        const int retained = count_shared;
+        if (retained > N) {
+            if (tid == 0) {
+                a.fallback_tiles[atomicAdd(a.fallback_count, 1)] = tile;
+            }
+            __syncthreads();
+            continue;
+        }
+
+        // ---- sort and rank the inside samples ----
+        {
+            Chunk<Key, E> chunk;
+            static_for<0, E>([&](auto ic) {
+                constexpr int I = decltype(ic)::value;
+                const int gi = tid * E + I;
+                chunk.k[I] = gi < retained ? xk[gi] : KT::SENTINEL;
+                chunk.p[I] = gi < retained ? xp[gi] : POS_NONE;
+            });
+            __syncthreads();
+            block_sort<Key, E>(chunk, xk, xp, tid);
+            __syncthreads();
+            static_for<0, E>([&](auto ic) {
+                constexpr int I = decltype(ic)::value;
+                keys[tid * E + I] = chunk.k[I];
+                pos0[tid * E + I] = (uint16_t)chunk.p[I];
+                if (chunk.p[I] != POS_NONE) {
+                    codes[(chunk.p[I] >> 8) * a.sx + (chunk.p[I] & 0xFFu)] = (uint16_t)(tid * E + I + 1);
+                }
+            });
+            if (tid < GUARD) {
+                pos0[tid - GUARD] = POS_NONE;
+                pos0[N + tid] = POS_NONE;
+            }
+        }
+        __syncthreads();
+
+        // ---- per-lane slide ----
+        const int y_first = wave * a.tw;
+        if (y0 + y_first < a.ny) {
+            const bool col_valid = x0 + lane < a.nx;
+            int n = 0, below = 0, inside = 0;
+            for (int dy = 0; dy < a.ky; ++dy) {
+                const RowMasks &rm = rowmasks[y_first + dy];
+                n += (int)__popcll(window_bits(rm.kept, lane, kxmask));
+                below += (int)__popcll(window_bits(rm.below, lane, kxmask));
+                inside += (int)__popcll(window_bits(rm.inside, lane, kxmask));
+            }
+            int m = 1, c = below;
+            bool failed = false;
+            for (int y = y_first; y < y_first + a.tw && y0 + y < a.ny && !failed; ++y) {
+                if (y > y_first) {
+                    const RowMasks &leave = rowmasks[y - 1];
+                    const RowMasks &enter = rowmasks[y - 1 + a.ky];
+                    const int db = (int)__popcll(window_bits(enter.below, lane, kxmask)) -
+                                   (int)__popcll(window_bits(leave.below, lane, kxmask));
+                    n += (int)__popcll(window_bits(enter.kept, lane, kxmask)) -
+                         (int)__popcll(window_bits(leave.kept, lane, kxmask));
+                    below += db;
+                    c += db;
+                    uint64_t bits_out = window_bits(leave.inside, lane, kxmask);
+                    uint64_t bits_in = window_bits(enter.inside, lane, kxmask);
+                    inside += (int)__popcll(bits_in) - (int)__popcll(bits_out);
+                    const uint16_t *code_out = codes + (y - 1) * a.sx + lane;
+                    const uint16_t *code_in = codes + (y - 1 + a.ky) * a.sx + lane;
+                    while (bits_out) {
+                        const int i = (int)__builtin_ctzll(bits_out);
+                        bits_out &= bits_out - 1ull;
+                        c -= (int)code_out[i] < m;
+                    }
+                    while (bits_in) {
+                        const int i = (int)__builtin_ctzll(bits_in);
+                        bits_in &= bits_in - 1ull;
+                        c += (int)code_in[i] < m;
+                    }
+                }
+                Key out = KT::CANONICAL_NAN;
+                int k = -1;
+                if (n >= a.min_count && n > 0) {
+                    k = a.stat == STAT_RANK ? (a.rank_param < n ? a.rank_param : -1) : (n >> 1);
+                }
+                if (k >= 0) {
+                    const uint32_t base = ((uint32_t)y << 8) | (uint32_t)lane;
+                    if (k < below || k >= below + inside || !walk_to(m, c, k, pos1, retained, g, base)) {
+                        failed = true;
+                    } else if (a.stat == STAT_MEDIAN && (n & 1) == 0) {
+                        const int m2 = previous_member(m, pos1, g, base);
+                        if (m2 < 1) {
+                            failed = true;
+                        } else {
+                            out = median_pair_bits<T>(keys[m2 - 1], keys[m - 1]);
+                        }
+                    } else {
+                        out = KT::unkey(keys[m - 1]);
+                    }
+                }
+                if (col_valid && !failed) {
+                    output[(y0 + y) * a.nx + x0 + lane] = out;
+                }
+            }
+            if (failed) {
+                failed_shared = 1;
+            }
+        }
+        __syncthreads();
+        if (tid == 0 && failed_shared) {
+            a.fallback_tiles[atomicAdd(a.fallback_count, 1)] = tile;
+        }
+    }
+}
+
+template <typename T>
+hipError_t launch_bracket_typed(const SlideArgs &args, size_t lds_bytes, int max_tiles, hipStream_t stream)
+{
+    const dim3 block(NT);
+    const void *fn = nullptr;
+    switch (args.sort_e) {
+    case 4: fn = reinterpret_cast<const void *>(&k_bracket<T, 4>); break;
+    case 8: fn = reinterpret_cast<const void *>(&k_bracket<T, 8>); break;
+    case 16: fn = reinterpret_cast<const void *>(&k_bracket<T, 16>); break;
+    case 32: fn = reinterpret_cast<const void *>(&k_bracket<T, 32>); break;
+    default: return hipErrorInvalidValue;
+    }
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+    if (e != hipSuccess) {
+        return e;
+    }
+    int per_cu = 0, device = 0, cus = 0;
+    if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, NT, lds_bytes)) != hipSuccess ||
+        (e = hipGetDevice(&device)) != hipSuccess ||
+        (e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device)) != hipSuccess) {
+        return e;
+    }
+    int64_t blocks = (int64_t)(per_cu < 1 ? 1 : per_cu) * (cus < 1 ? 1 : cus);
+    blocks = blocks < max_tiles ? blocks : max_tiles;
+    const dim3 grid((unsigned)(blocks < 1 ? 1 : blocks));
+    switch (args.sort_e) {
+    case 4: hipLaunchKernelGGL((k_bracket<T, 4>), grid, block, lds_bytes, stream, args); break;
+    case 8: hipLaunchKernelGGL((k_bracket<T, 8>), grid, block, lds_bytes, stream, args); break;
+    case 16: hipLaunchKernelGGL((k_bracket<T, 16>), grid, block, lds_bytes, stream, args); break;
+    default: hipLaunchKernelGGL((k_bracket<T, 32>), grid, block, lds_bytes, stream, args); break;
+    }
+    return hipGetLastError();
+}
+
 template <typename T>
 hipError_t launch_typed(const SlideArgs &args, size_t lds_bytes, int max_tiles, hipStream_t stream)
 {
@@ -777,6 +1058,22 @@ size_t slide_lds_bytes(int dtype_bytes, int s, int sort_e, int hist_stride, int 
     const size_t exchange = (((size_t)NT * (size_t)(sort_e + 1) * (size_t)(dtype_bytes + 2)) + 15) & ~(size_t)15;
     return exchange + (size_t)((s + 7) & ~7) * 2 + (size_t)SLIDE_WAVES * (size_t)hist_stride * 2 +
            (size_t)mask_bytes;
+}
+
+size_t bracket_lds_bytes(int dtype_bytes, int s, int rows, int sort_e)
+{
+    const size_t exchange = (((size_t)NT * (size_t)(sort_e + 1) * (size_t)(dtype_bytes + 2)) + 15) & ~(size_t)15;
+    return exchange + (size_t)((s + 7) & ~7) * 2 + (size_t)rows * sizeof(RowMasks);
+}
+
+hipError_t launch_bracket_slide(int dtype_bytes, const SlideArgs &args, size_t lds_bytes, int max_tiles,
+                                hipStream_t stream)
+{
+    if (max_tiles <= 0) {
+        return hipSuccess;
+    }
+    return dtype_bytes == 4 ? launch_bracket_typed<float>(args, lds_bytes, max_tiles, stream)
+                            : launch_bracket_typed<double>(args, lds_bytes, max_tiles, stream);
 }
 
 hipError_t launch_slide(int dtype_bytes, const SlideArgs &args, size_t lds_bytes, int max_tiles, hipStream_t stream)

@@ -85,6 +85,26 @@ def sky_quadratic(shape=(8192, 8192), seed: int = BASE_SEED + 5, sources: int = 
     return image.astype(np.float32)
 
 
+def sky_quadratic_crop(shape=(2048, 2048), seed: int = BASE_SEED + 6, field=(8192, 8192)) -> np.ndarray:
+    """A crop of the 63x63 workload's law: the quadratic background of an
+    8192^2 field (up to 300 at its corners) seen through a window of `shape`
+    at a seeded random offset, with the field's source density
+    (2000 per 8192^2) and 2% NaN, float32. Per-pixel statistics (noise,
+    background slope, sources) match the full workload image."""
+    rng = np.random.default_rng(seed)
+    rows, cols = shape
+    oy = int(rng.integers(0, field[0] - rows + 1))
+    ox = int(rng.integers(0, field[1] - cols + 1))
+    image = rng.normal(1000.0, 10.0, size=shape)
+    y = np.linspace(-1.0, 1.0, field[0])[oy : oy + rows, None]
+    x = np.linspace(-1.0, 1.0, field[1])[None, ox : ox + cols]
+    image += 150.0 * (x**2 + y**2)
+    sources = int(rng.poisson(2000.0 * rows * cols / (field[0] * field[1])))
+    _add_sources(image, rng, sources)
+    image[rng.random(shape) < 0.02] = np.nan
+    return image.astype(np.float32)
+
+
 def rank_for_quantile(quantile: float, count: int) -> int:
     """Rank at a quantile over the full tap count: floor(q * (W - 1) + 0.5)."""
     return int(np.floor(quantile * (count - 1) + 0.5))

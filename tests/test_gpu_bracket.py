@@ -10,6 +10,7 @@ constant features the fitted outputs are the tau quantiles of the targets.
 
 from __future__ import annotations
 
+import json
 import os
 
 import numpy as np
@@ -88,16 +89,14 @@ def test_bracket_bits_equal_plain(case, lo, hi) -> None:
         assert fallback >= 0.9 * tiles, (fallback, tiles)
     if hi >= 1e30 and image.dtype == np.float32:
         # Unbounded bracket: only capacity (more samples than the sort) can fail.
-        support = plan_support(plan)
-        if support <= 4096:
+        if bracket_support(plan, size) <= 4096:
             assert fallback == 0
 
 
-def plan_support(plan) -> int:
-    import json
-
+def bracket_support(plan, size) -> int:
+    """Samples in the support of one bracket tile (64 columns x tile rows)."""
     info = json.loads(plan.describe())
-    return int(info.get("support", 1 << 30))
+    return (int(info["bracket_tile_rows"]) + size[0] - 1) * (64 + size[1] - 1)
 
 
 def test_bracket_matches_oracle_small() -> None:
@@ -130,16 +129,33 @@ def test_sort_lengths(sort_len) -> None:
     assert_same_bits(out, plain)
 
 
-def test_footprint_and_propagate() -> None:
-    rng = np.random.default_rng(24)
+def test_propagate_policy() -> None:
     image = synthetic.sky_small((180, 200), seed=24)
-    fp = rng.random((9, 13)) < 0.6
-    fp[4, 6] = True
-    for omit in (True, False):
-        plan = make_plan(image, "median", None, footprint=fp, omit=omit)
+    image[::7, ::5] = np.inf
+    for kind, parameter in (("median", 0), ("rank", 60)):
+        plan = make_plan(image, kind, (9, 13), parameter, omit=False)
         plain, _ = run(plan, image)
         out, _ = run(plan, image, constant_model(-0.7, 0.7))
         assert_same_bits(out, plain)
+
+
+def test_ignore_inf_and_mirror() -> None:
+    image = synthetic.sky_small((170, 230), seed=29)
+    image[::11, ::3] = np.inf
+    image[5::13, ::4] = -np.inf
+    plan = make_plan(image, "median", (17, 15), mode="mirror", ignore_inf=True)
+    plain, _ = run(plan, image)
+    out, _ = run(plan, image, constant_model(-0.9, 0.9))
+    assert_same_bits(out, plain)
+
+
+def test_footprint_is_rejected() -> None:
+    image = synthetic.sky_small((64, 64), seed=30)
+    fp = np.ones((5, 5), dtype=bool)
+    fp[0, 0] = False
+    plan = make_plan(image, "median", None, footprint=fp)
+    with pytest.raises(ValueError, match="rectangular window"):
+        plan.set_bracket(constant_model(-1.0, 1.0))
 
 
 def test_all_nan_tiles_and_min_count() -> None:
