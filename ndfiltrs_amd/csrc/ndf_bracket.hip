@@ -25,47 +25,44 @@ namespace {
 constexpr int BLOCK = 8;
 constexpr int MAX_TILE_BLOCKS = 1024;
 
-struct Stats {
-    double n, sum, sumsq, mn, mx;
-};
-
+// One thread per 8x8 block; consecutive threads take consecutive blocks of
+// a block row, so each image row segment is read by 64 lanes as 2 KB of
+// contiguous memory. Output per block: retained count, sum, sum of squares
+// (f64).
 template <typename T>
 __global__ void __launch_bounds__(256) k_block_stats(const T *image, int64_t ny, int64_t nx, int64_t by, int64_t bx,
                                                      int ignore_inf, double *out)
 {
     using KT = KeyTraits<T>;
-    const int64_t block = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int64_t block = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (block >= by * bx) {
         return;
     }
-    const int lane = threadIdx.x & 63;
-    const int64_t y = (block / bx) * BLOCK + (lane >> 3);
-    const int64_t x = (block % bx) * BLOCK + (lane & 7);
-    double v = 0.0;
-    bool keep = false;
-    if (y < ny && x < nx) {
-        const typename KT::Key bits = KT::bits(image[y * nx + x]);
-        keep = classify<T>(bits, ignore_inf) != KT::SENTINEL;
-        v = KT::to_double(bits);
+    const int64_t y0 = (block / bx) * BLOCK, x0 = (block % bx) * BLOCK;
+    double n = 0.0, s = 0.0, q = 0.0;
+    for (int dy = 0; dy < BLOCK; ++dy) {
+        const int64_t y = y0 + dy;
+        if (y >= ny) {
+            break;
+        }
+        const T *row = image + y * nx;
+#pragma unroll
+        for (int dx = 0; dx < BLOCK; ++dx) {
+            if (x0 + dx < nx) {
+                const typename KT::Key bits = KT::bits(row[x0 + dx]);
+                if (classify<T>(bits, ignore_inf) != KT::SENTINEL) {
+                    const double v = KT::to_double(bits);
+                    n += 1.0;
+                    s += v;
+                    q += v * v;
+                }
+            }
+        }
     }
-    double n = keep ? 1.0 : 0.0, s = keep ? v : 0.0, q = keep ? v * v : 0.0;
-    double mn = keep ? v : __builtin_huge_val(), mx = keep ? v : -__builtin_huge_val();
-    for (int off = 32; off > 0; off >>= 1) {
-        n += __shfl_xor(n, off);
-        s += __shfl_xor(s, off);
-        q += __shfl_xor(q, off);
-        const double a = __shfl_xor(mn, off), b = __shfl_xor(mx, off);
-        mn = a < mn ? a : mn;
-        mx = b > mx ? b : mx;
-    }
-    if (lane == 0) {
-        double *o = out + block * 5;
-        o[0] = n;
-        o[1] = s;
-        o[2] = q;
-        o[3] = mn;
-        o[4] = mx;
-    }
+    double *o = out + block * 3;
+    o[0] = n;
+    o[1] = s;
+    o[2] = q;
 }
 
 // asinh(v) = sign(v) * log(|v| + sqrt(v^2 + 1)); log(2|v|) once v^2 would overflow.
@@ -139,7 +136,7 @@ __global__ void __launch_bounds__(64) k_tile_features(BracketArgs b, const doubl
         float mean = __builtin_huge_valf(), sd = __builtin_huge_valf();
         if (i < count) {
             const int64_t br = br0 + (int64_t)(i / w_used) * step, bc = bc0 + (int64_t)(i % w_used) * step;
-            const double *o = blocks + (br * b.bx + bc) * 5;
+            const double *o = blocks + (br * b.bx + bc) * 3;
             n += o[0];
             s += o[1];
             cells += 64.0;
@@ -174,7 +171,7 @@ __global__ void __launch_bounds__(64) k_tile_features(BracketArgs b, const doubl
     double sw = 0, sr = 0, sq = 0, sm = 0, srr = 0, sqq = 0, srq = 0, srm = 0, sqm = 0, bright = 0;
     for (int i = lane; i < count; i += 64) {
         const int64_t br = br0 + (int64_t)(i / w_used) * step, bc = bc0 + (int64_t)(i % w_used) * step;
-        const double *o = blocks + (br * b.bx + bc) * 5;
+        const double *o = blocks + (br * b.bx + bc) * 3;
         if (o[0] >= 16.0) {
             const double mu = o[1] / o[0] - (double)c;
             bright += mu > 0.75 * (double)sigma ? 1.0 : 0.0;
@@ -451,7 +448,7 @@ hipError_t launch_block_stats(int dtype_bytes, const void *image, int64_t ny, in
                               double *blocks, hipStream_t stream)
 {
     const int64_t by = (ny + BLOCK - 1) / BLOCK, bx = (nx + BLOCK - 1) / BLOCK;
-    const dim3 grid((unsigned)((by * bx + 3) / 4));
+    const dim3 grid((unsigned)((by * bx + 255) / 256));
     if (dtype_bytes == 4) {
         hipLaunchKernelGGL(k_block_stats<float>, grid, dim3(256), 0, stream, static_cast<const float *>(image), ny, nx,
                            by, bx, ignore_inf, blocks);

@@ -507,7 +507,7 @@ ndf_status prepare_bracket(ndf_plan *p, char *err, size_t err_len)
     const int64_t tiles = bracket_tiles(p);
     const int64_t by = (p->slide.ny + 7) / 8, bx = (p->slide.nx + 7) / 8;
     hipError_t e;
-    if ((e = p->b_blocks.ensure((size_t)(by * bx * 5) * sizeof(double))) != hipSuccess ||
+    if ((e = p->b_blocks.ensure((size_t)(by * bx * 3) * sizeof(double))) != hipSuccess ||
         (e = p->b_features.ensure((size_t)tiles * FEATURES * sizeof(float))) != hipSuccess ||
         (e = p->b_moments.ensure((size_t)tiles * 3 * sizeof(double))) != hipSuccess ||
         (e = p->b_lo.ensure((size_t)tiles * (size_t)p->dbytes)) != hipSuccess ||

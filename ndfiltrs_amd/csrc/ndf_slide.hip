@@ -731,6 +731,41 @@ __global__ void __launch_bounds__(NT, 1) k_slide(SlideArgs a)
     }
 }
 
+// Sorts the first `retained` staged (key, position) pairs (xk/xp, flat) with
+// a network of NT * E entries padded with sentinels, then writes the sorted
+// keys and positions from the start of the region (keys[NT*E], guard,
+// pos[NT*E], guard) and the 1-based rank of every sorted sample into codes.
+template <typename Key, int E>
+__device__ __forceinline__ void sort_inside(Key *xk, uint16_t *xp, uint16_t *codes, int retained, int sx, int tid)
+{
+    using KT = KeyTraits<std::conditional_t<sizeof(Key) == 4, float, double>>;
+    constexpr int N = NT * E;
+    Key *keys = xk;
+    uint16_t *pos0 = reinterpret_cast<uint16_t *>(keys + N) + GUARD;
+    Chunk<Key, E> chunk;
+    static_for<0, E>([&](auto ic) {
+        constexpr int I = decltype(ic)::value;
+        const int gi = tid * E + I;
+        chunk.k[I] = gi < retained ? xk[gi] : KT::SENTINEL;
+        chunk.p[I] = gi < retained ? xp[gi] : POS_NONE;
+    });
+    __syncthreads();
+    block_sort<Key, E>(chunk, xk, xp, tid);
+    __syncthreads();
+    static_for<0, E>([&](auto ic) {
+        constexpr int I = decltype(ic)::value;
+        keys[tid * E + I] = chunk.k[I];
+        pos0[tid * E + I] = (uint16_t)chunk.p[I];
+        if (chunk.p[I] != POS_NONE) {
+            codes[(chunk.p[I] >> 8) * sx + (chunk.p[I] & 0xFFu)] = (uint16_t)(tid * E + I + 1);
+        }
+    });
+    if (tid < GUARD) {
+        pos0[tid - GUARD] = POS_NONE;
+        pos0[N + tid] = POS_NONE;
+    }
+}
+
 // Bit masks of one support row of a bracket tile (bit x <-> column x of the
 // support; rows are at most 128 samples wide).
 struct RowMasks {
@@ -760,7 +795,7 @@ __device__ __forceinline__ uint64_t window_bits(const uint64_t w[2], int lane, u
 // the bracket sends the tile to the fallback list, where the plain kernel
 // recomputes it.
 template <typename T, int E>
-__global__ void __launch_bounds__(NT, 2) k_bracket(SlideArgs a)
+__global__ void __launch_bounds__(NT, E >= 32 ? 1 : (sizeof(T) == 8 && E >= 16 ? 2 : 3)) k_bracket(SlideArgs a)
 {
     using KT = KeyTraits<T>;
     using Key = typename KT::Key;
@@ -770,8 +805,6 @@ __global__ void __launch_bounds__(NT, 2) k_bracket(SlideArgs a)
     Key *xk = reinterpret_cast<Key *>(lds);
     uint16_t *xp = reinterpret_cast<uint16_t *>(xk + NT * (E + 1));
     Key *keys = reinterpret_cast<Key *>(lds);
-    uint16_t *pos0 = reinterpret_cast<uint16_t *>(keys + N) + GUARD;
-    uint16_t *pos1 = pos0 - 1;
     uint16_t *codes = reinterpret_cast<uint16_t *>(lds + exchange_bytes<Key, E>());
     RowMasks *rowmasks = reinterpret_cast<RowMasks *>(codes + ((a.s + 7) & ~7));
     __shared__ int count_shared;
@@ -877,32 +910,33 @@ __global__ void __launch_bounds__(NT, 2) k_bracket(SlideArgs a)
             continue;
         }
 
-        // ---- sort and rank the inside samples ----
-        {
-            Chunk<Key, E> chunk;
-            static_for<0, E>([&](auto ic) {
-                constexpr int I = decltype(ic)::value;
-                const int gi = tid * E + I;
-                chunk.k[I] = gi < retained ? xk[gi] : KT::SENTINEL;
-                chunk.p[I] = gi < retained ? xp[gi] : POS_NONE;
-            });
-            __syncthreads();
-            block_sort<Key, E>(chunk, xk, xp, tid);
-            __syncthreads();
-            static_for<0, E>([&](auto ic) {
-                constexpr int I = decltype(ic)::value;
-                keys[tid * E + I] = chunk.k[I];
-                pos0[tid * E + I] = (uint16_t)chunk.p[I];
-                if (chunk.p[I] != POS_NONE) {
-                    codes[(chunk.p[I] >> 8) * a.sx + (chunk.p[I] & 0xFFu)] = (uint16_t)(tid * E + I + 1);
-                }
-            });
-            if (tid < GUARD) {
-                pos0[tid - GUARD] = POS_NONE;
-                pos0[N + tid] = POS_NONE;
+        // ---- sort and rank the inside samples (smallest network that holds them) ----
+        int nsort = N;
+        if constexpr (E >= 16) {
+            if (retained <= NT * (E / 4)) {
+                nsort = NT * (E / 4);
             }
         }
+        if constexpr (E >= 8) {
+            if (nsort == N && retained <= NT * (E / 2)) {
+                nsort = NT * (E / 2);
+            }
+        }
+        if constexpr (E >= 16) {
+            if (nsort == NT * (E / 4)) {
+                sort_inside<Key, E / 4>(xk, xp, codes, retained, a.sx, tid);
+            }
+        }
+        if constexpr (E >= 8) {
+            if (nsort == NT * (E / 2)) {
+                sort_inside<Key, E / 2>(xk, xp, codes, retained, a.sx, tid);
+            }
+        }
+        if (nsort == N) {
+            sort_inside<Key, E>(xk, xp, codes, retained, a.sx, tid);
+        }
         __syncthreads();
+        const uint16_t *pos1 = reinterpret_cast<uint16_t *>(keys + nsort) + GUARD - 1;  // 1-based ranks
 
         // ---- per-lane slide ----
         const int y_first = wave * a.tw;
@@ -915,7 +949,56 @@ __global__ void __launch_bounds__(NT, 2) k_bracket(SlideArgs a)
                 below += (int)__popcll(window_bits(rm.below, lane, kxmask));
                 inside += (int)__popcll(window_bits(rm.inside, lane, kxmask));
             }
+            // Start rank m0 (wave-uniform): the mean relative position of the
+            // lanes' targets inside their brackets, applied to the tile's
+            // ranks. The cursor's c for m0 comes from per-column counts of
+            // inside ranks below m0 over the window rows, summed over each
+            // lane's kx columns with a wave prefix scan.
             int m = 1, c = below;
+            {
+                float rel = 0.0f;
+                int have = 0;
+                if (n >= a.min_count && n > 0 && inside > 0) {
+                    const int k0 = a.stat == STAT_RANK ? a.rank_param : (n >> 1);
+                    if (k0 < n && k0 >= below && k0 < below + inside) {
+                        rel = ((float)(k0 - below) + 0.5f) / (float)inside;
+                        have = 1;
+                    }
+                }
+                for (int off = 32; off > 0; off >>= 1) {
+                    rel += __shfl_xor(rel, off);
+                    have += __shfl_xor(have, off);
+                }
+                if (have > 0) {
+                    int m0 = 1 + (int)(rel / (float)have * (float)retained);
+                    m0 = m0 < 1 ? 1 : (m0 > retained ? retained : m0);
+                    int col0 = 0, col1 = 0;
+                    const bool has1 = lane + 64 < a.sx;
+                    for (int dy = 0; dy < a.ky; ++dy) {
+                        const int row = y_first + dy;
+                        const RowMasks &rm = rowmasks[row];
+                        const uint16_t *crow = codes + row * a.sx;
+                        const int code0 = crow[lane];
+                        const int code1 = has1 ? crow[lane + 64] : 0;
+                        col0 += (int)((rm.inside[0] >> lane) & 1ull) & (int)(code0 < m0);
+                        col1 += (int)((rm.inside[1] >> lane) & 1ull) & (int)(code1 < m0);
+                    }
+                    // Inclusive prefix over columns 0..127.
+                    int p0 = col0, p1 = col1;
+                    for (int off = 1; off < 64; off <<= 1) {
+                        const int u0 = __shfl_up(p0, off), u1 = __shfl_up(p1, off);
+                        p0 += lane >= off ? u0 : 0;
+                        p1 += lane >= off ? u1 : 0;
+                    }
+                    const int total0 = __shfl(p0, 63);
+                    const int last = lane + a.kx - 1;  // last column of this lane's window
+                    const int hi0 = __shfl(p0, last & 63), hi1 = __shfl(p1, (last - 64) & 63);
+                    const int before = __shfl(p0, (lane + 63) & 63);
+                    const int upto = last < 64 ? hi0 : total0 + hi1;
+                    m = m0;
+                    c = below + upto - (lane ? before : 0);
+                }
+            }
             bool failed = false;
             for (int y = y_first; y < y_first + a.tw && y0 + y < a.ny && !failed; ++y) {
                 if (y > y_first) {
