@@ -152,12 +152,17 @@ ndf_status ndf_device_malloc(void **ptr, size_t bytes);
 ndf_status ndf_device_free(void *ptr);
 ndf_status ndf_memcpy_to_device(void *d_dst, const void *h_src, size_t bytes);
 ndf_status ndf_memcpy_to_host(void *h_dst, const void *d_src, size_t bytes);
+/* Device-to-device copy (a plain copy kernel; used to calibrate HBM counters). */
+ndf_status ndf_memcpy_device(void *d_dst, const void *d_src, size_t bytes);
 ndf_status ndf_device_synchronize(void);
 /* Times `iterations` launches of ndf_plan_apply_device with HIP events on
- * one stream; writes the mean milliseconds per launch. */
+ * one stream; writes the mean milliseconds per launch (every kernel of the
+ * launch) and, when ms_main_kernel is not NULL, the mean duration of the
+ * launch's main kernel (the slide, bracket or generic kernel), measured with
+ * events recorded around it on the same stream. */
 ndf_status ndf_plan_time_device(ndf_plan *plan, const void *d_input, void *d_output, size_t len,
-                                int32_t warmup, int32_t iterations, double *ms_per_launch, char *err,
-                                size_t err_len);
+                                int32_t warmup, int32_t iterations, double *ms_per_launch, double *ms_main_kernel,
+                                char *err, size_t err_len);
 
 #ifdef __cplusplus
 }

@@ -235,16 +235,20 @@ class Plan:
             _raise(status, err.value.decode())
         return features, targets, weight
 
-    def time_device(self, d_input: int, d_output: int, warmup: int, iterations: int) -> float:
-        """Mean milliseconds per launch, timed with HIP events on one stream."""
+    def time_device(self, d_input: int, d_output: int, warmup: int, iterations: int,
+                    main_kernel: bool = False):
+        """Mean milliseconds per launch (HIP events on one stream); with
+        ``main_kernel`` also the mean duration of the launch's main kernel."""
         err = ctypes.create_string_buffer(256)
         ms = ctypes.c_double(0.0)
+        main = ctypes.c_double(0.0)
         status = _lib.lib.ndf_plan_time_device(
-            self._handle, d_input, d_output, self.len, warmup, iterations, ctypes.byref(ms), err, len(err)
+            self._handle, d_input, d_output, self.len, warmup, iterations, ctypes.byref(ms), ctypes.byref(main),
+            err, len(err)
         )
         if status != _lib.OK:
             _raise(status, err.value.decode())
-        return ms.value
+        return (ms.value, main.value) if main_kernel else ms.value
 
 
 def window_filter(

@@ -101,6 +101,7 @@ def _declare(lib: ctypes.CDLL) -> None:
     lib.ndf_device_free.argtypes = [ctypes.c_void_p]
     lib.ndf_memcpy_to_device.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
     lib.ndf_memcpy_to_host.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
+    lib.ndf_memcpy_device.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
     lib.ndf_device_synchronize.argtypes = []
     lib.ndf_plan_time_device.argtypes = [
         plan_p,
@@ -109,6 +110,7 @@ def _declare(lib: ctypes.CDLL) -> None:
         ctypes.c_size_t,
         ctypes.c_int32,
         ctypes.c_int32,
+        ctypes.POINTER(ctypes.c_double),
         ctypes.POINTER(ctypes.c_double),
         *_err,
     ]
@@ -168,6 +170,7 @@ def _declare(lib: ctypes.CDLL) -> None:
         "ndf_device_free",
         "ndf_memcpy_to_device",
         "ndf_memcpy_to_host",
+        "ndf_memcpy_device",
         "ndf_device_synchronize",
         "ndf_plan_time_device",
     ):

@@ -1,14 +1,18 @@
 """Learned bracket proposal: training, model files and use with a plan.
 
 The model is a 6 -> 32 -> 32 -> 2 ReLU MLP (fp32, 1346 parameters) that maps
-box-sum features of a tile's support to a standardised value bracket for the
-tile's window results. Training labels are the smallest and largest result of
-each tile computed by the plain GPU kernel on seeded images; the HIP kernels
-``k_mlp_grad`` / ``k_mlp_adam`` fit pinball losses (quantile ``tau_lo`` of
-the lower end, ``tau_hi`` of the upper end). A bracket never changes results:
-tiles whose targets leave it are recomputed by the plain kernel.
+box-sum features of a bracket tile's support (robust level and noise scale
+from 8x8 block means and standard deviations, plane slope, retained and
+bright-block fractions, target quantile) to a standardised value bracket for
+the tile's window results. Training labels are the smallest and largest
+result of each tile computed by the plain GPU kernel on seeded images; the
+HIP kernels ``k_mlp_grad`` / ``k_mlp_adam`` fit pinball losses (quantile
+``tau_lo`` of the lower end, ``tau_hi`` of the upper end). A bracket never
+changes results: tiles whose targets leave it are recomputed by the plain
+kernel.
 
-    python -m ndfiltrs_amd.bracket train --out ndfiltrs_amd/models/bracket_sky_63x63.json
+    python -m ndfiltrs_amd.bracket train --workload sky63
+    python -m ndfiltrs_amd.bracket train --workload sky31_f64
 """
 
 from __future__ import annotations
@@ -28,7 +32,39 @@ PARAMS = 1346
 FEATURES = 6
 HIDDEN = 32
 HERE = os.path.dirname(os.path.abspath(__file__))
-DEFAULT_MODEL = os.path.join(HERE, "models", "bracket_sky_63x63.json")
+MODELS = os.path.join(HERE, "models")
+
+# Training workloads: the image law (seeded crops of the benchmark field),
+# window, boundary mode, dtype and the statistics labelled per image.
+WORKLOADS = {
+    "sky63": {
+        "model": "bracket_sky_63x63.json",
+        "dtype": np.float32,
+        "generator": synthetic.sky_quadratic_crop,
+        "side": 2048,
+        "size": 63,
+        "mode": "reflect",
+        "ranks": [397],
+        "law": "2048^2 crops (seeded offsets) of the 8192^2 field: sky N(1000,10) + quadratic background up to "
+               "300 + 2000 sources per 8192^2 + 2% NaN (float32)",
+    },
+    "sky31_f64": {
+        "model": "bracket_sky_31x31_f64.json",
+        "dtype": np.float64,
+        "generator": synthetic.sky_gradient_crop,
+        "side": 1024,
+        "size": 31,
+        "mode": "nearest",
+        "ranks": [154, 806],
+        "law": "1024^2 crops (seeded offsets) of the 4096^2 field: sky N(1000,10) + x-gradient 0..200 + 500 "
+               "sources per 4096^2 + 0.1% hot pixels U(1e4,1e5) + 2% NaN + the field's 3 NaN columns (float64)",
+    },
+}
+DEFAULT_MODEL = os.path.join(MODELS, WORKLOADS["sky63"]["model"])
+
+
+def model_path(workload: str) -> str:
+    return os.path.join(MODELS, WORKLOADS[workload]["model"])
 
 
 def init_weights(seed: int) -> np.ndarray:
@@ -47,11 +83,16 @@ def forward(weights: np.ndarray, features: np.ndarray) -> np.ndarray:
     """NumPy forward pass (evaluation only; inference on the device uses HIP)."""
     w = np.asarray(weights, dtype=np.float32)
     o = 0
-    w1 = w[o:o + FEATURES * HIDDEN].reshape(FEATURES, HIDDEN); o += FEATURES * HIDDEN
-    b1 = w[o:o + HIDDEN]; o += HIDDEN
-    w2 = w[o:o + HIDDEN * HIDDEN].reshape(HIDDEN, HIDDEN); o += HIDDEN * HIDDEN
-    b2 = w[o:o + HIDDEN]; o += HIDDEN
-    w3 = w[o:o + HIDDEN * 2].reshape(HIDDEN, 2); o += HIDDEN * 2
+    w1 = w[o:o + FEATURES * HIDDEN].reshape(FEATURES, HIDDEN)
+    o += FEATURES * HIDDEN
+    b1 = w[o:o + HIDDEN]
+    o += HIDDEN
+    w2 = w[o:o + HIDDEN * HIDDEN].reshape(HIDDEN, HIDDEN)
+    o += HIDDEN * HIDDEN
+    b2 = w[o:o + HIDDEN]
+    o += HIDDEN
+    w3 = w[o:o + HIDDEN * 2].reshape(HIDDEN, 2)
+    o += HIDDEN * 2
     b3 = w[o:o + 2]
     h1 = np.maximum(features @ w1 + b1, 0.0)
     h2 = np.maximum(h1 @ w2 + b2, 0.0)
@@ -59,13 +100,17 @@ def forward(weights: np.ndarray, features: np.ndarray) -> np.ndarray:
     return np.sort(out, axis=1)
 
 
-def training_image(seed: int, shape=(2048, 2048)) -> np.ndarray:
-    """A seeded crop of the 63x63 workload's law (see synthetic.sky_quadratic_crop)."""
-    return synthetic.sky_quadratic_crop(shape=shape, seed=seed)
+def training_image(seed: int, shape=(2048, 2048), workload: str = "sky63") -> np.ndarray:
+    """A seeded crop of a benchmark field's law."""
+    return WORKLOADS[workload]["generator"](shape=shape, seed=seed)
+
+
+def statistics_of(workload: str) -> list[tuple[str, int]]:
+    return [("median", 0)] + [("rank", r) for r in WORKLOADS[workload]["ranks"]]
 
 
 def collect(images, size: int, statistics, mode: str = "reflect"):
-    """Features/targets/weights of every tile for every (image, statistic)."""
+    """Features/targets/weights of every bracket tile for every (image, statistic)."""
     feats, targs, wts = [], [], []
     for image in images:
         d_in = _core.DeviceArray(image.nbytes)
@@ -117,7 +162,7 @@ def coverage(weights, features, targets, weight) -> dict:
 
 
 def save(path: str, weights: np.ndarray, meta: dict) -> None:
-    os.makedirs(os.path.dirname(path), exist_ok=True)
+    os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
     with open(path, "w") as handle:
         json.dump({"format": "ndfiltrs-amd-bracket-mlp/1", "meta": meta,
                    "weights": [float(v) for v in np.asarray(weights, dtype=np.float32)]}, handle)
@@ -135,45 +180,48 @@ def load(path: str = DEFAULT_MODEL) -> tuple[np.ndarray, dict]:
 
 def train_main(argv=None) -> int:
     parser = argparse.ArgumentParser(description="train the learned-bracket MLP")
-    parser.add_argument("--out", default=DEFAULT_MODEL)
+    parser.add_argument("--workload", choices=sorted(WORKLOADS), default="sky63")
+    parser.add_argument("--out", default=None)
     parser.add_argument("--images", type=int, default=64)
-    parser.add_argument("--side", type=int, default=2048)
-    parser.add_argument("--size", type=int, default=63)
-    parser.add_argument("--rank-quantile", type=float, default=0.10)
-    parser.add_argument("--steps", type=int, default=1500)
+    parser.add_argument("--side", type=int, default=None)
+    parser.add_argument("--steps", type=int, default=3000)
     parser.add_argument("--lr", type=float, default=3e-3)
-    parser.add_argument("--tau-lo", type=float, default=0.01)
-    parser.add_argument("--tau-hi", type=float, default=0.99)
+    parser.add_argument("--tau-lo", type=float, default=0.003)
+    parser.add_argument("--tau-hi", type=float, default=0.997)
     parser.add_argument("--seed", type=int, default=20261203)
     args = parser.parse_args(argv)
-    count = args.size * args.size
-    rank = synthetic.rank_for_quantile(args.rank_quantile, count)
-    statistics = [("median", 0), ("rank", rank)]
+    spec = WORKLOADS[args.workload]
+    side = args.side or spec["side"]
+    out = args.out or model_path(args.workload)
+    statistics = statistics_of(args.workload)
     start = time.time()
     seeds = [args.seed + 1000 + i for i in range(args.images)]
-    train_seeds, hold_seeds = seeds[: args.images - max(args.images // 8, 1)], seeds[args.images - max(args.images // 8, 1):]
+    held = max(args.images // 8, 1)
+    train_seeds, hold_seeds = seeds[: args.images - held], seeds[args.images - held:]
 
     def images(seed_list):
         for seed in seed_list:
-            yield training_image(seed, (args.side, args.side))
+            yield training_image(seed, (side, side), args.workload)
 
-    x, t, w = collect(images(train_seeds), args.size, statistics)
-    xh, th, wh = collect(images(hold_seeds), args.size, statistics)
+    x, t, w = collect(images(train_seeds), spec["size"], statistics, spec["mode"])
+    xh, th, wh = collect(images(hold_seeds), spec["size"], statistics, spec["mode"])
     print(f"collected {x.shape[0]} training and {xh.shape[0]} held-out tiles in {time.time() - start:.1f} s",
           file=sys.stderr, flush=True)
     weights = init_weights(args.seed)
     weights, losses = fit(weights, x, t, w, args.steps, args.lr, args.tau_lo, args.tau_hi)
+    probe = _core.Plan(spec["dtype"], (side, side), "median", (spec["size"], spec["size"]), None, (0, 0),
+                       (spec["mode"], spec["mode"]), 0.0, True, False, 1, 0)
     meta = {
+        "workload": args.workload,
         "seed": args.seed,
         "images": args.images,
-        "image_side": args.side,
-        "window": [args.size, args.size],
-        "statistics": [["median", 0], ["rank", rank]],
-        "law": "2048^2 crops (seeded offsets) of the 8192^2 field: sky N(1000,10) + quadratic background up to "
-               "300 + 2000 sources per 8192^2 + 2% NaN (float32)",
-        "bracket_tile": [int(json.loads(_core.Plan(np.float32, (args.side, args.side), "median", (args.size, args.size),
-                                                   None, (0, 0), ("reflect", "reflect"), 0.0, True, False, 1, 0)
-                                         .describe())["bracket_tile_rows"]), 64],
+        "held_out_images": held,
+        "image_side": side,
+        "window": [spec["size"], spec["size"]],
+        "mode": spec["mode"],
+        "statistics": [list(s) for s in statistics],
+        "law": spec["law"],
+        "bracket_tile": [int(json.loads(probe.describe())["bracket_tile_rows"]), 64],
         "steps": args.steps,
         "learning_rate": args.lr,
         "tau": [args.tau_lo, args.tau_hi],
@@ -182,7 +230,7 @@ def train_main(argv=None) -> int:
         "held_out": coverage(weights, xh, th, wh),
         "seconds": time.time() - start,
     }
-    save(args.out, weights, meta)
+    save(out, weights, meta)
     print(json.dumps(meta), flush=True)
     return 0
 

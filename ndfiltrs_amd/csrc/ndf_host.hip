@@ -206,6 +206,10 @@ struct ndf_plan {
     BracketArgs bargs{};
     DeviceBuffer b_weights, b_blocks, b_features, b_moments, b_lo, b_hi, b_pred, b_fallback, b_fcount;
 
+    // When set, launch() records these events around its main kernel (the
+    // slide, bracket or generic kernel) for ndf_plan_time_device.
+    hipEvent_t main_begin = nullptr, main_end = nullptr;
+
     // Device state.
     std::mutex mutex;
     bool tables_ready = false;
@@ -574,8 +578,13 @@ ndf_status launch(ndf_plan *p, const void *d_in, void *d_out, hipStream_t stream
     hipError_t e = hipSuccess;
     const uint64_t nan_bits = p->dbytes == 4 ? 0x7FC00000ull : 0x7FF8000000000000ull;
     p->bracket_launched = false;
+    auto mark = [&](hipEvent_t ev) {
+        if (ev && e == hipSuccess) e = hipEventRecord(ev, stream);
+    };
     if (p->all_nan) {
+        mark(p->main_begin);
         e = launch_fill(p->dbytes, d_out, (int64_t)p->len, nan_bits, stream);
+        mark(p->main_end);
     } else if (p->engine == NDF_ENGINE_SLIDE && p->bracket_on) {
         // Learned bracket: propose, filter in-bracket samples only, then
         // recompute every fallback tile with the plain kernel.
@@ -587,7 +596,9 @@ ndf_status launch(ndf_plan *p, const void *d_in, void *d_out, hipStream_t stream
             SlideArgs args = p->slide_b;
             args.input = d_in;
             args.output = d_out;
-            e = launch_bracket_slide(p->dbytes, args, p->slide_b_lds, tiles, stream);
+            mark(p->main_begin);
+            if (e == hipSuccess) e = launch_bracket_slide(p->dbytes, args, p->slide_b_lds, tiles, stream);
+            mark(p->main_end);
         }
         if (e == hipSuccess) {
             SlideArgs args = p->slide_fb;
@@ -601,12 +612,18 @@ ndf_status launch(ndf_plan *p, const void *d_in, void *d_out, hipStream_t stream
         SlideArgs args = p->slide;
         args.input = d_in;
         args.output = d_out;
-        e = launch_slide(p->dbytes, args, p->slide_lds, args.tiles_x * args.tiles_y * (int)args.nz, stream);
+        mark(p->main_begin);
+        if (e == hipSuccess) {
+            e = launch_slide(p->dbytes, args, p->slide_lds, args.tiles_x * args.tiles_y * (int)args.nz, stream);
+        }
+        mark(p->main_end);
     } else {
         GenericArgs args = p->generic;
         args.input = d_in;
         args.output = d_out;
-        e = launch_generic(p->dbytes, args, stream);
+        mark(p->main_begin);
+        if (e == hipSuccess) e = launch_generic(p->dbytes, args, stream);
+        mark(p->main_end);
     }
     return e == hipSuccess ? NDF_OK : fail_hip(e, err, err_len);
 }
@@ -1089,33 +1106,59 @@ ndf_status ndf_memcpy_to_host(void *h_dst, const void *d_src, size_t bytes)
     return hipMemcpy(h_dst, d_src, bytes, hipMemcpyDeviceToHost) == hipSuccess ? NDF_OK : NDF_ERR_DEVICE;
 }
 
+ndf_status ndf_memcpy_device(void *d_dst, const void *d_src, size_t bytes)
+{
+    return hipMemcpy(d_dst, d_src, bytes, hipMemcpyDeviceToDevice) == hipSuccess ? NDF_OK : NDF_ERR_DEVICE;
+}
+
 ndf_status ndf_device_synchronize(void) { return hipDeviceSynchronize() == hipSuccess ? NDF_OK : NDF_ERR_DEVICE; }
 
 ndf_status ndf_plan_time_device(ndf_plan *plan, const void *d_input, void *d_output, size_t len, int32_t warmup,
-                                int32_t iterations, double *ms_per_launch, char *err, size_t err_len)
+                                int32_t iterations, double *ms_per_launch, double *ms_main_kernel, char *err,
+                                size_t err_len)
 {
     if (len != plan->len) return fail_code(NDF_ERR_LENGTH_MISMATCH, err, err_len);
+    if (iterations < 0 || warmup < 0) return fail(NDF_ERR_INVALID, "iterations must be nonnegative", err, err_len);
     std::lock_guard<std::mutex> guard(plan->mutex);
     ndf_status st = prepare_tables(plan, err, err_len);
     if (st != NDF_OK) return st;
     hipStream_t stream = nullptr;
     hipEvent_t start = nullptr, stop = nullptr;
+    std::vector<hipEvent_t> marks((size_t)iterations * 2, nullptr);
     hipError_t e;
     if ((e = hipStreamCreate(&stream)) != hipSuccess) return fail_hip(e, err, err_len);
     if ((e = hipEventCreate(&start)) != hipSuccess || (e = hipEventCreate(&stop)) != hipSuccess) {
-        (void)hipStreamDestroy(stream);
-        return fail_hip(e, err, err_len);
+        st = fail_hip(e, err, err_len);
     }
-    if ((e = launch_fill(4, plan->d_error.ptr, 1, 0, stream)) != hipSuccess) st = fail_hip(e, err, err_len);
+    for (size_t i = 0; st == NDF_OK && i < marks.size(); ++i) {
+        if ((e = hipEventCreate(&marks[i])) != hipSuccess) st = fail_hip(e, err, err_len);
+    }
+    if (st == NDF_OK && (e = launch_fill(4, plan->d_error.ptr, 1, 0, stream)) != hipSuccess) {
+        st = fail_hip(e, err, err_len);
+    }
     for (int i = 0; st == NDF_OK && i < warmup; ++i) st = launch(plan, d_input, d_output, stream, err, err_len);
     if (st == NDF_OK && (e = hipEventRecord(start, stream)) != hipSuccess) st = fail_hip(e, err, err_len);
-    for (int i = 0; st == NDF_OK && i < iterations; ++i) st = launch(plan, d_input, d_output, stream, err, err_len);
+    for (int i = 0; st == NDF_OK && i < iterations; ++i) {
+        plan->main_begin = marks[(size_t)i * 2];
+        plan->main_end = marks[(size_t)i * 2 + 1];
+        st = launch(plan, d_input, d_output, stream, err, err_len);
+    }
+    plan->main_begin = plan->main_end = nullptr;
     if (st == NDF_OK && (e = hipEventRecord(stop, stream)) != hipSuccess) st = fail_hip(e, err, err_len);
     if (st == NDF_OK && (e = hipEventSynchronize(stop)) != hipSuccess) st = fail_hip(e, err, err_len);
     float ms = 0.0f;
     if (st == NDF_OK && (e = hipEventElapsedTime(&ms, start, stop)) != hipSuccess) st = fail_hip(e, err, err_len);
+    double main_ms = 0.0;
+    for (int i = 0; st == NDF_OK && i < iterations; ++i) {
+        float part = 0.0f;
+        if ((e = hipEventElapsedTime(&part, marks[(size_t)i * 2], marks[(size_t)i * 2 + 1])) != hipSuccess) {
+            st = fail_hip(e, err, err_len);
+        }
+        main_ms += part;
+    }
     if (st == NDF_OK) {
         *ms_per_launch = iterations > 0 ? (double)ms / iterations : 0.0;
+        if (ms_main_kernel) *ms_main_kernel = iterations > 0 ? main_ms / iterations : 0.0;
         int32_t flag = 0;
         if ((e = hipMemcpy(&flag, plan->d_error.ptr, sizeof(flag), hipMemcpyDeviceToHost)) != hipSuccess) {
             st = fail_hip(e, err, err_len);
@@ -1123,8 +1166,11 @@ ndf_status ndf_plan_time_device(ndf_plan *plan, const void *d_input, void *d_out
             st = fail(NDF_ERR_DEVICE, "internal error: rank walk left its bounds", err, err_len);
         }
     }
-    (void)hipEventDestroy(start);
-    (void)hipEventDestroy(stop);
+    for (hipEvent_t ev : marks) {
+        if (ev) (void)hipEventDestroy(ev);
+    }
+    if (start) (void)hipEventDestroy(start);
+    if (stop) (void)hipEventDestroy(stop);
     (void)hipStreamDestroy(stream);
     return st;
 }

@@ -50,6 +50,29 @@ def sky_gradient_f64(shape=(4096, 4096), seed: int = BASE_SEED + 2) -> np.ndarra
     return image
 
 
+def sky_gradient_crop(shape=(1024, 1024), seed: int = BASE_SEED + 7, field=(4096, 4096)) -> np.ndarray:
+    """A crop of the 31x31 workload's law: the x-gradient (0..200 across the
+    4096-column field) seen through a window of `shape` at a seeded random
+    offset, the field's source density (500 per 4096^2), 0.1% hot pixels
+    U(1e4, 1e5), 2% NaN and those of the field's 3 NaN columns that fall in
+    the crop, float64."""
+    rng = np.random.default_rng(seed)
+    rows, cols = shape
+    oy = int(rng.integers(0, field[0] - rows + 1))
+    ox = int(rng.integers(0, field[1] - cols + 1))
+    image = rng.normal(1000.0, 10.0, size=shape)
+    image += np.linspace(0.0, 200.0, field[1])[None, ox : ox + cols]
+    sources = int(rng.poisson(500.0 * rows * cols / (field[0] * field[1])))
+    _add_sources(image, rng, sources)
+    hot = rng.random(shape) < 0.001
+    image[hot] = rng.uniform(1e4, 1e5, size=int(hot.sum()))
+    image[rng.random(shape) < 0.02] = np.nan
+    for column in rng.choice(field[1], size=3, replace=False):
+        if ox <= column < ox + cols:
+            image[:, column - ox] = np.nan
+    return image
+
+
 def sky_annulus(shape=(8192, 8192), seed: int = BASE_SEED + 3) -> np.ndarray:
     """Annulus workload: Poisson(200) sky as float32, 5% of 16x16 blocks NaN,
     0.01% +inf."""

@@ -129,7 +129,14 @@ def check_kat(entry: dict, output: np.ndarray) -> None:
 # ---------------------------------------------------------------------------
 
 
+# Planted mistakes for the sensitivity test (tests/test_oracle.py): each name
+# switches one plausible error into the brute force below. None = correct.
+MUTATION: str | None = None
+
+
 def _map(index: int, length: int, mode: str) -> int | None:
+    if MUTATION == "reflect_as_mirror" and mode == "reflect":
+        mode = "mirror"
     if 0 <= index < length:
         return index
     if mode == "constant":
@@ -153,6 +160,10 @@ def _total_order(value: float) -> tuple:
 
 
 def _mid(lo: float, hi: float) -> float:
+    if MUTATION == "naive_midpoint":
+        return (lo + hi) / 2.0
+    if MUTATION == "signed_zero_lower" and lo == 0.0 and hi == 0.0:
+        return lo
     if lo == hi and math.copysign(1.0, lo) == math.copysign(1.0, hi):
         return lo
     if lo == 0.0 and hi == 0.0:
@@ -191,12 +202,15 @@ def brute_force(
         saw_nan = False
         for tap in taps:
             mapped = [_map(c + t, n, m) for c, t, n, m in zip(center, tap, shape, mode)]
+            if None in mapped and MUTATION == "cval_not_a_sample":
+                continue
             value = float(cval) if None in mapped else float(array[tuple(mapped)])
             if math.isnan(value):
                 saw_nan = True
                 continue
             if omit_nan and ignore_inf and math.isinf(value):
-                continue
+                if not (MUTATION == "keep_negative_inf" and value < 0):
+                    continue
             samples.append(value)
         samples.sort(key=_total_order)
         if (not omit_nan and saw_nan) or (omit_nan and len(samples) < min_count):
@@ -205,7 +219,11 @@ def brute_force(
             out[center] = samples[rank] if rank < len(samples) else nan
         elif len(samples) % 2:
             out[center] = samples[len(samples) // 2]
+        elif MUTATION == "upper_middle":
+            out[center] = samples[len(samples) // 2]
         else:
             m = _mid(samples[len(samples) // 2 - 1], samples[len(samples) // 2])
+            if MUTATION == "lower_middle":
+                m = samples[len(samples) // 2 - 1]
             out[center] = nan if math.isnan(m) else m
     return out

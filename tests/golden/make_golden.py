@@ -106,6 +106,14 @@ def cases() -> list[dict]:
                     [1.0, np.nan, np.inf], f64,
                     {"size": 3, "mode": "constant", "cval": np.nan, "ignore_inf": True, "min_count": 2},
                     {"at_bits": {1: F64_NAN}}))
+    # src/reduction.rs:11-14: ignore_inf drops both +inf and -inf (the 3-D
+    # case of tests/test_semantics.py:28-50 mixes -inf with ignore_inf); with
+    # ignore_inf off, -inf is an ordinary lowest sample.
+    out.append(case("reduction_ignore_inf_drops_negative", "src/reduction.rs:11-14", "nanmedian_filter",
+                    [1.0, -np.inf, 3.0], f64, {"size": 3, "mode": "nearest", "ignore_inf": True},
+                    {"at": {1: 2.0}}))
+    out.append(case("reduction_negative_inf_retained", "src/reduction.rs:11-14", "nanmedian_filter",
+                    [1.0, -np.inf, 3.0], f64, {"size": 3, "mode": "nearest"}, {"at": {1: 1.0}}))
     # src/reduction.rs:222-246: signed zeros, extremes, mixed infinities.
     out.append(case("reduction_signed_zero_median", "src/reduction.rs:223-230", "median_filter",
                     [0.0, -0.0, -0.0], f64, {"size": 3, "mode": "nearest"},

@@ -230,12 +230,57 @@ def test_result_is_thread_count_invariant() -> None:
         np.testing.assert_array_equal(_cases.bits_of(out), _cases.bits_of(outs[0]))
 
 
-def test_plain_mistakes_are_caught() -> None:
-    """A planted error (upper middle instead of the midpoint) fails the KATs."""
-    values = np.array([0, 4, 1, 7, 2, 9, 3], dtype=np.float64)
-    upper = _cases.oracle_call("rank_filter", values, {"footprint": [1, 1], "mode": "nearest"}, 1)
-    mid = _cases.oracle_call("median_filter", values, {"footprint": [1, 1], "mode": "nearest"})
-    assert not np.array_equal(upper, mid)
+def _kat_failures() -> list[str]:
+    """Names of the golden cases the brute force gets wrong."""
+    failures = []
+    for entry in KAT["cases"]:
+        values = _cases.kat_input(entry)
+        kwargs = _cases.kat_kwargs(entry)
+        ndim = values.ndim
+        if "footprint" in kwargs:
+            fp = np.asarray(kwargs["footprint"]) != 0
+            fp = fp.reshape(fp.shape if fp.ndim == ndim else (-1,) * 0 + fp.shape)
+            size = list(fp.shape)
+        else:
+            fp = None
+            size = kwargs.get("size", 3)
+            size = [int(size)] * ndim if np.ndim(size) == 0 else list(size)
+        origin = kwargs.get("origin", 0)
+        origin = [int(origin)] * ndim if np.ndim(origin) == 0 else list(origin)
+        mode = kwargs.get("mode", "reflect")
+        mode = [mode] * ndim if isinstance(mode, str) else list(mode)
+        omit = entry["function"] == "nanmedian_filter"
+        count = int(fp.sum()) if fp is not None else int(np.prod(size))
+        rank = entry.get("rank") if entry["function"] == "rank_filter" else None
+        out = _cases.brute_force(
+            values, size, footprint=fp, origin=origin, mode=mode,
+            cval=float(values.dtype.type(kwargs.get("cval", 0.0))), omit_nan=omit,
+            ignore_inf=bool(kwargs.get("ignore_inf", False)) if omit else False,
+            min_count=int(kwargs.get("min_count", 1)) if omit else count,
+            rank=None if rank is None else int(rank) % count,
+        )
+        try:
+            _cases.check_kat(entry, out)
+        except AssertionError:
+            failures.append(entry["name"])
+    return failures
+
+
+PLANTED = ["upper_middle", "naive_midpoint", "lower_middle", "signed_zero_lower", "reflect_as_mirror",
+           "keep_negative_inf", "cval_not_a_sample"]
+
+
+def test_brute_force_passes_every_golden_case() -> None:
+    assert _kat_failures() == []
+
+
+@pytest.mark.parametrize("mutation", PLANTED)
+def test_planted_mistakes_are_caught(mutation: str, monkeypatch) -> None:
+    """Each plausible mistake, planted in the CPU brute force, fails at least
+    one golden case: the golden set pins the midpoint contract, the signed
+    zero rule, the boundary tables, the ignore-inf policy and cval-as-sample."""
+    monkeypatch.setattr(_cases, "MUTATION", mutation)
+    assert _kat_failures(), f"no golden case detects {mutation}"
 
 
 def test_oracle_error_messages() -> None:
