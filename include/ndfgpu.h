@@ -142,6 +142,39 @@ ndf_status ndf_mlp_train(float *weights, size_t n_weights, const float *features
                          const float *weight, size_t count, int32_t steps, float learning_rate, float tau_lo,
                          float tau_hi, float *losses, char *err, size_t err_len);
 
+/* Batches of 2-D images (no reference counterpart: the reference filters one
+ * array per call, src/pyapi/filter.rs:265-314, so a caller with many cutouts
+ * makes one call each). Image i has shape shapes[2i..2i+1], window
+ * sizes[2i..2i+1] (rectangle, centred) and modes[2i..2i+1]; cval, kind and
+ * the policy are shared. Each image is validated exactly as
+ * ndf_plan_create would validate it (messages prefixed "batch image i: ").
+ * The images are stored back to back in one device buffer (image i starts
+ * at the sum of the earlier images' element counts; ndf_batch_length gives
+ * the total). One launch runs every tile of every image through a work
+ * queue, and is captured once per input/output pair in a hipGraph and
+ * replayed (use_graph = 0 launches directly). */
+typedef struct ndf_batch ndf_batch;
+ndf_status ndf_batch_create(ndf_batch **batch, int32_t dtype_bytes, size_t count, const size_t *shapes,
+                            const size_t *sizes, const int32_t *modes, double cval, int32_t kind, int32_t omit_nan,
+                            int32_t ignore_inf, size_t min_count, size_t parameter, char *err, size_t err_len);
+size_t ndf_batch_length(const ndf_batch *batch);
+/* Tile sort length (1024 .. 16384) and cap on tile rows (multiple of 4). */
+ndf_status ndf_batch_configure(ndf_batch *batch, int32_t sort_len, int32_t max_tile_rows, int32_t use_graph,
+                               char *err, size_t err_len);
+/* Times every (sort length, tile-row cap) candidate on these buffers
+ * (`repeats` launches each) and keeps the fastest. */
+ndf_status ndf_batch_autotune(ndf_batch *batch, const void *d_input, void *d_output, size_t len, int32_t repeats,
+                              char *err, size_t err_len);
+/* Asynchronous on `stream` (a hipStream_t, NULL = default stream). */
+ndf_status ndf_batch_apply_device(ndf_batch *batch, const void *d_input, void *d_output, size_t len, void *stream,
+                                  char *err, size_t err_len);
+/* Mean milliseconds per launch over `iterations` (HIP events). */
+ndf_status ndf_batch_time_device(ndf_batch *batch, const void *d_input, void *d_output, size_t len, int32_t warmup,
+                                 int32_t iterations, double *ms_per_launch, char *err, size_t err_len);
+/* JSON: configuration, work tiles, LDS and workgroups. */
+ndf_status ndf_batch_describe(ndf_batch *batch, char *buffer, size_t buffer_len);
+void ndf_batch_destroy(ndf_batch *batch);
+
 /* Replaces get_num_threads (src/pyapi/filter.rs:377-380): the host thread
  * count a CPU caller would use (RAYON_NUM_THREADS, else the CPU count). */
 size_t ndf_get_num_threads(void);

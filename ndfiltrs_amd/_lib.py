@@ -152,7 +152,41 @@ def _declare(lib: ctypes.CDLL) -> None:
         _f32_p,
         *_err,
     ]
+    batch_p = ctypes.c_void_p
+    lib.ndf_batch_create.argtypes = [
+        ctypes.POINTER(ctypes.c_void_p),
+        ctypes.c_int32,
+        ctypes.c_size_t,
+        _size_p,
+        _size_p,
+        _i32_p,
+        ctypes.c_double,
+        ctypes.c_int32,
+        ctypes.c_int32,
+        ctypes.c_int32,
+        ctypes.c_size_t,
+        ctypes.c_size_t,
+        *_err,
+    ]
+    lib.ndf_batch_length.argtypes = [batch_p]
+    lib.ndf_batch_length.restype = ctypes.c_size_t
+    lib.ndf_batch_configure.argtypes = [batch_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, *_err]
+    lib.ndf_batch_autotune.argtypes = [batch_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int32,
+                                       *_err]
+    lib.ndf_batch_apply_device.argtypes = [batch_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
+                                           ctypes.c_void_p, *_err]
+    lib.ndf_batch_time_device.argtypes = [batch_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
+                                          ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_double), *_err]
+    lib.ndf_batch_describe.argtypes = [batch_p, ctypes.c_char_p, ctypes.c_size_t]
+    lib.ndf_batch_destroy.argtypes = [batch_p]
+    lib.ndf_batch_destroy.restype = None
     for name in (
+        "ndf_batch_create",
+        "ndf_batch_configure",
+        "ndf_batch_autotune",
+        "ndf_batch_apply_device",
+        "ndf_batch_time_device",
+        "ndf_batch_describe",
         "ndf_plan_set_bracket",
         "ndf_plan_bracket_report",
         "ndf_plan_bracket_training_data",

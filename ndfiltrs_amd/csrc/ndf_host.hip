@@ -15,6 +15,7 @@
 #include <unistd.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <mutex>
 #include <new>
@@ -248,7 +249,9 @@ SlideArgs with_tile_rows(const SlideArgs &s, int ty)
 }
 
 // Plan the slide engine; false when the geometry cannot be staged.
-bool plan_slide(ndf_plan *p)
+// force_e > 0 fixes the sort (256 * force_e entries) and max_ty caps the tile
+// rows (the batch autotuner's knobs); 0 / 256 let the planner choose.
+bool plan_slide(ndf_plan *p, int force_e = 0, int max_ty = 256)
 {
     if (p->rank > 3 || p->count > 65534 || p->len == 0) return false;
     // Canonical 3-D view: leading singleton axes for rank 1 and 2.
@@ -300,14 +303,15 @@ bool plan_slide(ndf_plan *p)
     auto fits = [&](int64_t ty) {
         const int64_t s = support_of(ty);
         if (rows_of(ty) > 254 || s > 16384) return false;
-        const int e = sort_e_of(s);
+        if (force_e > 0 && (int64_t)256 * force_e < s) return false;
+        const int e = force_e > 0 ? force_e : sort_e_of(s);
         return e <= 64 && slide_lds_bytes(p->dbytes, (int)s, e, (int)hist_stride, mask_bytes) <= (size_t)limit;
     };
     const int64_t rows_needed = ((dims[1] + SLIDE_WAVES - 1) / SLIDE_WAVES) * SLIDE_WAVES;
     int64_t ty = 0;
-    for (int64_t cand = SLIDE_WAVES; cand <= 256; cand += SLIDE_WAVES) {
+    for (int64_t cand = SLIDE_WAVES; cand <= max_ty; cand += SLIDE_WAVES) {
         if (!fits(cand)) break;
-        if (cand > SLIDE_WAVES && support_of(cand) > 32 * (int64_t)p->count) break;
+        if (force_e == 0 && cand > SLIDE_WAVES && support_of(cand) > 32 * (int64_t)p->count) break;
         ty = cand;
         if (cand >= rows_needed) break;
     }
@@ -322,7 +326,7 @@ bool plan_slide(ndf_plan *p)
     }
     const int rows = (int)rows_of(ty);
     const int s = (int)support_of(ty);
-    const int sort_e = sort_e_of(s);
+    const int sort_e = force_e > 0 ? force_e : sort_e_of(s);
     const int64_t tiles_x = (dims[2] + SLIDE_TILE_X - 1) / SLIDE_TILE_X;
     const int64_t tiles_y = (dims[1] + ty - 1) / ty;
     if (tiles_x * tiles_y * dims[0] > 0x7FFFFFFF) return false;
@@ -399,7 +403,7 @@ bool plan_slide(ndf_plan *p)
     // most bracket workgroups resident per CU (at most 3 counted); shorter
     // tiles only when none of those fits.
     p->bracket_ty = 0;
-    if (rect && kz == 1 && dims[0] == 1 && kx <= 64 && p->rank >= 2) {
+    if (force_e == 0 && rect && kz == 1 && dims[0] == 1 && kx <= 64 && p->rank >= 2) {
         int best_per_cu = 0;
         for (int cand = 32; cand >= SLIDE_WAVES && (best_per_cu == 0 || cand >= 24); cand -= SLIDE_WAVES) {
             const SlideArgs v = with_tile_rows(s_args, cand);
@@ -1064,6 +1068,363 @@ ndf_status ndf_mlp_train(float *weights, size_t n_weights, const float *features
     }
     return e == hipSuccess ? NDF_OK : fail_hip(e, err, err_len);
 }
+
+// ---------------------------------------------------------------------------
+// Batches of 2-D images, each with its own window size and boundary modes
+// (no reference counterpart: the reference filters one array per call). One
+// persistent launch of the batched plain kernel covers every tile of every
+// image through a work queue; the launch is captured once in a hipGraph and
+// replayed. ndf_batch_autotune picks the sort length and the tile-row cap.
+// ---------------------------------------------------------------------------
+
+}  // extern "C"
+
+struct ndf_batch {
+    int32_t dbytes = 4;
+    std::vector<ndf_plan *> plans;
+    std::vector<int64_t> offsets;  // first element of each image
+    int64_t total = 0;
+    int sort_e = 8, max_ty = 16;
+    bool use_graph = true;
+    bool tuned = false;
+    // Device state for the current configuration.
+    bool ready = false;
+    std::vector<SlideArgs> images;
+    std::vector<int32_t> fill_images;  // images whose every output is NaN
+    DeviceBuffer d_images, d_offsets, d_work, d_counter, d_error, d_maps;
+    int nwork = 0;
+    size_t lds = 0;
+    int grid = 0;
+    // Captured launch (for one input/output pair).
+    hipStream_t capture_stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    const void *graph_in = nullptr;
+    void *graph_out = nullptr;
+    std::mutex mutex;
+
+    void drop_graph()
+    {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+        graph_in = nullptr;
+        graph_out = nullptr;
+    }
+    ~ndf_batch()
+    {
+        drop_graph();
+        if (capture_stream) (void)hipStreamDestroy(capture_stream);
+        for (ndf_plan *p : plans) ndf_plan_destroy(p);
+        for (DeviceBuffer *b : {&d_images, &d_offsets, &d_work, &d_counter, &d_error, &d_maps}) b->release();
+    }
+};
+
+namespace {
+
+// Plans every image for (sort_e, max_ty), pools the maps, builds the work
+// queue (largest estimated tile cost first) and uploads it all.
+ndf_status batch_prepare(ndf_batch *b, char *err, size_t err_len)
+{
+    if (b->ready) return NDF_OK;
+    b->drop_graph();
+    std::vector<int64_t> maps;
+    std::vector<std::array<int64_t, 3>> map_at;
+    b->images.assign(b->plans.size(), SlideArgs{});
+    b->fill_images.clear();
+    size_t lds = 0;
+    struct Work {
+        int64_t cost;
+        int32_t image, tile;
+    };
+    std::vector<Work> work;
+    for (size_t i = 0; i < b->plans.size(); ++i) {
+        ndf_plan *p = b->plans[i];
+        map_at.push_back({0, 0, 0});
+        if (p->len == 0) continue;
+        if (p->all_nan) {
+            b->fill_images.push_back((int32_t)i);
+            continue;
+        }
+        if (!plan_slide(p, b->sort_e, b->max_ty)) {
+            char msg[160];
+            snprintf(msg, sizeof msg, "batch image %zu: its window does not fit a %d-entry tile sort", i,
+                     256 * b->sort_e);
+            return fail(NDF_ERR_INVALID, msg, err, err_len);
+        }
+        map_at.back() = {(int64_t)maps.size(), (int64_t)(maps.size() + p->slide_mapz.size()),
+                         (int64_t)(maps.size() + p->slide_mapz.size() + p->slide_mapy.size())};
+        maps.insert(maps.end(), p->slide_mapz.begin(), p->slide_mapz.end());
+        maps.insert(maps.end(), p->slide_mapy.begin(), p->slide_mapy.end());
+        maps.insert(maps.end(), p->slide_mapx.begin(), p->slide_mapx.end());
+        b->images[i] = p->slide;
+        lds = std::max(lds, p->slide_lds);
+        const SlideArgs &a = p->slide;
+        const int tiles = a.tiles_x * a.tiles_y;
+        const int64_t cost = (int64_t)a.ty * SLIDE_TILE_X * (2 * a.kx + 8) + (int64_t)a.s * 12;
+        for (int t = 0; t < tiles; ++t) work.push_back({cost, (int32_t)i, (int32_t)t});
+    }
+    std::stable_sort(work.begin(), work.end(), [](const Work &x, const Work &y) { return x.cost > y.cost; });
+    std::vector<int32_t> pairs;
+    pairs.reserve(work.size() * 2);
+    for (const Work &w : work) {
+        pairs.push_back(w.image);
+        pairs.push_back(w.tile);
+    }
+    hipError_t e;
+    if ((e = upload(b->d_maps, maps)) != hipSuccess) return fail_hip(e, err, err_len);
+    const int64_t *base = static_cast<const int64_t *>(b->d_maps.ptr);
+    for (size_t i = 0; i < b->images.size(); ++i) {
+        SlideArgs &a = b->images[i];
+        if (b->plans[i]->len == 0 || b->plans[i]->all_nan) continue;
+        a.mapz = base + map_at[i][0];
+        a.mapy = base + map_at[i][1];
+        a.mapx = base + map_at[i][2];
+        a.taps = nullptr;
+        a.edge_remove = nullptr;
+        a.edge_add = nullptr;
+        a.mask = nullptr;
+        a.input = nullptr;
+        a.output = nullptr;
+    }
+    if ((e = upload(b->d_images, b->images)) != hipSuccess || (e = upload(b->d_offsets, b->offsets)) != hipSuccess ||
+        (e = upload(b->d_work, pairs)) != hipSuccess || (e = b->d_counter.ensure(sizeof(int32_t))) != hipSuccess ||
+        (e = b->d_error.ensure(sizeof(int32_t))) != hipSuccess) {
+        return fail_hip(e, err, err_len);
+    }
+    b->nwork = (int)work.size();
+    b->lds = lds;
+    b->grid = 1;
+    if (b->nwork > 0 && (e = slide_batch_grid(b->dbytes, b->sort_e, lds, &b->grid)) != hipSuccess) {
+        return fail_hip(e, err, err_len);
+    }
+    b->grid = std::max(1, std::min(b->grid, b->nwork));
+    b->ready = true;
+    return NDF_OK;
+}
+
+// Enqueues the batch on `stream`: NaN fills for all-NaN images, the queue
+// counter reset and the batched kernel.
+hipError_t batch_enqueue(ndf_batch *b, const void *d_in, void *d_out, hipStream_t stream)
+{
+    const uint64_t nan_bits = b->dbytes == 4 ? 0x7FC00000ull : 0x7FF8000000000000ull;
+    hipError_t e = hipSuccess;
+    for (int32_t i : b->fill_images) {
+        if (e != hipSuccess) break;
+        void *out = static_cast<char *>(d_out) + b->offsets[(size_t)i] * b->dbytes;
+        e = launch_fill(b->dbytes, out, (int64_t)b->plans[(size_t)i]->len, nan_bits, stream);
+    }
+    if (e == hipSuccess && b->nwork > 0) e = launch_fill(4, b->d_counter.ptr, 1, 0, stream);
+    if (e == hipSuccess && b->nwork > 0) {
+        e = launch_slide_batch(b->dbytes, b->sort_e, static_cast<const SlideArgs *>(b->d_images.ptr),
+                               static_cast<const int64_t *>(b->d_offsets.ptr),
+                               static_cast<const int32_t *>(b->d_work.ptr), b->nwork,
+                               static_cast<int32_t *>(b->d_counter.ptr), d_in, d_out,
+                               static_cast<int32_t *>(b->d_error.ptr), b->lds, b->grid, stream);
+    }
+    return e;
+}
+
+// Runs the batch once: replays the captured graph (capturing it first for a
+// new input/output pair) or enqueues the kernels directly.
+ndf_status batch_launch(ndf_batch *b, const void *d_in, void *d_out, hipStream_t stream, char *err, size_t err_len)
+{
+    ndf_status st = batch_prepare(b, err, err_len);
+    if (st != NDF_OK) return st;
+    hipError_t e = hipSuccess;
+    if (!b->use_graph) {
+        e = batch_enqueue(b, d_in, d_out, stream);
+        return e == hipSuccess ? NDF_OK : fail_hip(e, err, err_len);
+    }
+    if (!b->exec || b->graph_in != d_in || b->graph_out != d_out) {
+        b->drop_graph();
+        if (!b->capture_stream && (e = hipStreamCreate(&b->capture_stream)) != hipSuccess) {
+            return fail_hip(e, err, err_len);
+        }
+        if ((e = hipStreamBeginCapture(b->capture_stream, hipStreamCaptureModeThreadLocal)) != hipSuccess) {
+            return fail_hip(e, err, err_len);
+        }
+        const hipError_t enqueue = batch_enqueue(b, d_in, d_out, b->capture_stream);
+        e = hipStreamEndCapture(b->capture_stream, &b->graph);
+        if (enqueue != hipSuccess) e = enqueue;
+        if (e == hipSuccess) e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            b->drop_graph();
+            return fail_hip(e, err, err_len);
+        }
+        b->graph_in = d_in;
+        b->graph_out = d_out;
+    }
+    e = hipGraphLaunch(b->exec, stream);
+    return e == hipSuccess ? NDF_OK : fail_hip(e, err, err_len);
+}
+
+// Mean milliseconds per batch launch over `iterations` (HIP events).
+ndf_status batch_time(ndf_batch *b, const void *d_in, void *d_out, int warmup, int iterations, double *ms, char *err,
+                      size_t err_len)
+{
+    ndf_status st = batch_prepare(b, err, err_len);
+    if (st != NDF_OK) return st;
+    hipStream_t stream = nullptr;
+    hipEvent_t start = nullptr, stop = nullptr;
+    hipError_t e;
+    if ((e = hipStreamCreate(&stream)) != hipSuccess) return fail_hip(e, err, err_len);
+    if ((e = hipEventCreate(&start)) != hipSuccess || (e = hipEventCreate(&stop)) != hipSuccess) {
+        st = fail_hip(e, err, err_len);
+    }
+    if (st == NDF_OK && (e = launch_fill(4, b->d_error.ptr, 1, 0, stream)) != hipSuccess) st = fail_hip(e, err, err_len);
+    for (int i = 0; st == NDF_OK && i < warmup; ++i) st = batch_launch(b, d_in, d_out, stream, err, err_len);
+    if (st == NDF_OK && (e = hipEventRecord(start, stream)) != hipSuccess) st = fail_hip(e, err, err_len);
+    for (int i = 0; st == NDF_OK && i < iterations; ++i) st = batch_launch(b, d_in, d_out, stream, err, err_len);
+    if (st == NDF_OK && (e = hipEventRecord(stop, stream)) != hipSuccess) st = fail_hip(e, err, err_len);
+    if (st == NDF_OK && (e = hipEventSynchronize(stop)) != hipSuccess) st = fail_hip(e, err, err_len);
+    float elapsed = 0.0f;
+    if (st == NDF_OK && (e = hipEventElapsedTime(&elapsed, start, stop)) != hipSuccess) st = fail_hip(e, err, err_len);
+    if (st == NDF_OK) {
+        *ms = iterations > 0 ? (double)elapsed / iterations : 0.0;
+        int32_t flag = 0;
+        if ((e = hipMemcpy(&flag, b->d_error.ptr, sizeof(flag), hipMemcpyDeviceToHost)) != hipSuccess) {
+            st = fail_hip(e, err, err_len);
+        } else if (flag) {
+            st = fail(NDF_ERR_DEVICE, "internal error: rank walk left its bounds", err, err_len);
+        }
+    }
+    if (start) (void)hipEventDestroy(start);
+    if (stop) (void)hipEventDestroy(stop);
+    (void)hipStreamDestroy(stream);
+    return st;
+}
+
+}  // namespace
+
+extern "C" {
+
+ndf_status ndf_batch_create(ndf_batch **batch, int32_t dtype_bytes, size_t count, const size_t *shapes,
+                            const size_t *sizes, const int32_t *modes, double cval, int32_t kind, int32_t omit_nan,
+                            int32_t ignore_inf, size_t min_count, size_t parameter, char *err, size_t err_len)
+{
+    if (!batch) return fail(NDF_ERR_INVALID, "batch pointer is null", err, err_len);
+    *batch = nullptr;
+    if (count == 0) return fail(NDF_ERR_INVALID, "a batch needs at least one image", err, err_len);
+    ndf_batch *b = new (std::nothrow) ndf_batch();
+    if (!b) return fail_code(NDF_ERR_MEMORY, err, err_len);
+    b->dbytes = dtype_bytes;
+    for (size_t i = 0; i < count; ++i) {
+        ndf_plan *p = new (std::nothrow) ndf_plan();
+        if (!p) {
+            delete b;
+            return fail_code(NDF_ERR_MEMORY, err, err_len);
+        }
+        b->plans.push_back(p);
+        const ptrdiff_t origin[2] = {0, 0};
+        char inner[200] = {0};
+        const ndf_status st = build_plan(p, dtype_bytes, 2, shapes + 2 * i, sizes + 2 * i, nullptr, origin,
+                                         modes + 2 * i, cval, kind, omit_nan, ignore_inf, min_count, parameter,
+                                         inner, sizeof inner);
+        if (st != NDF_OK) {
+            char msg[260];
+            snprintf(msg, sizeof msg, "batch image %zu: %s", i, inner);
+            delete b;
+            return fail(st, msg, err, err_len);
+        }
+        b->offsets.push_back(b->total);
+        b->total += (int64_t)p->len;
+    }
+    *batch = b;
+    return NDF_OK;
+}
+
+size_t ndf_batch_length(const ndf_batch *batch) { return batch ? (size_t)batch->total : 0; }
+
+ndf_status ndf_batch_configure(ndf_batch *batch, int32_t sort_len, int32_t max_tile_rows, int32_t use_graph,
+                               char *err, size_t err_len)
+{
+    std::lock_guard<std::mutex> guard(batch->mutex);
+    int e = 4;
+    while (256 * e < sort_len && e < 64) e <<= 1;
+    if (sort_len <= 0 || 256 * e != sort_len) {
+        return fail(NDF_ERR_INVALID, "batch sort length must be 1024, 2048, 4096, 8192 or 16384", err, err_len);
+    }
+    if (max_tile_rows < SLIDE_WAVES || max_tile_rows > 256 || max_tile_rows % SLIDE_WAVES != 0) {
+        return fail(NDF_ERR_INVALID, "batch tile rows must be a multiple of 4 in [4, 256]", err, err_len);
+    }
+    if (e != batch->sort_e || max_tile_rows != batch->max_ty) batch->ready = false;
+    batch->sort_e = e;
+    batch->max_ty = max_tile_rows;
+    batch->use_graph = use_graph != 0;
+    if (!batch->use_graph) batch->drop_graph();
+    return NDF_OK;
+}
+
+ndf_status ndf_batch_apply_device(ndf_batch *batch, const void *d_input, void *d_output, size_t len, void *stream,
+                                  char *err, size_t err_len)
+{
+    if (len != (size_t)batch->total) return fail_code(NDF_ERR_LENGTH_MISMATCH, err, err_len);
+    if (len == 0) return NDF_OK;
+    std::lock_guard<std::mutex> guard(batch->mutex);
+    return batch_launch(batch, d_input, d_output, static_cast<hipStream_t>(stream), err, err_len);
+}
+
+ndf_status ndf_batch_time_device(ndf_batch *batch, const void *d_input, void *d_output, size_t len, int32_t warmup,
+                                 int32_t iterations, double *ms_per_launch, char *err, size_t err_len)
+{
+    if (len != (size_t)batch->total) return fail_code(NDF_ERR_LENGTH_MISMATCH, err, err_len);
+    std::lock_guard<std::mutex> guard(batch->mutex);
+    return batch_time(batch, d_input, d_output, warmup, iterations, ms_per_launch, err, err_len);
+}
+
+ndf_status ndf_batch_autotune(ndf_batch *batch, const void *d_input, void *d_output, size_t len, int32_t repeats,
+                              char *err, size_t err_len)
+{
+    if (len != (size_t)batch->total) return fail_code(NDF_ERR_LENGTH_MISMATCH, err, err_len);
+    std::lock_guard<std::mutex> guard(batch->mutex);
+    const int sort_es[] = {4, 8, 16};
+    const int rows[] = {4, 8, 16, 32, 64};
+    double best = -1.0;
+    int best_e = batch->sort_e, best_rows = batch->max_ty;
+    char inner[200];
+    for (int e : sort_es) {
+        for (int r : rows) {
+            batch->sort_e = e;
+            batch->max_ty = r;
+            batch->ready = false;
+            double ms = 0.0;
+            // A configuration some image cannot use is skipped.
+            if (batch_time(batch, d_input, d_output, 1, repeats > 0 ? repeats : 3, &ms, inner, sizeof inner) !=
+                NDF_OK) {
+                continue;
+            }
+            if (best < 0.0 || ms < best) {
+                best = ms;
+                best_e = e;
+                best_rows = r;
+            }
+        }
+    }
+    batch->sort_e = best_e;
+    batch->max_ty = best_rows;
+    batch->ready = false;
+    if (best < 0.0) return fail(NDF_ERR_INVALID, "no batch configuration fits every image", err, err_len);
+    batch->tuned = true;
+    return batch_prepare(batch, err, err_len);
+}
+
+ndf_status ndf_batch_describe(ndf_batch *batch, char *buffer, size_t buffer_len)
+{
+    std::lock_guard<std::mutex> guard(batch->mutex);
+    char err[200];
+    const ndf_status st = batch_prepare(batch, err, sizeof err);
+    snprintf(buffer, buffer_len,
+             "{\"images\": %zu, \"elements\": %lld, \"sort_len\": %d, \"max_tile_rows\": %d, \"graph\": %s, "
+             "\"tuned\": %s, \"work_tiles\": %d, \"lds_bytes\": %zu, \"workgroups\": %d, \"ok\": %s}",
+             batch->plans.size(), (long long)batch->total, 256 * batch->sort_e, batch->max_ty,
+             batch->use_graph ? "true" : "false", batch->tuned ? "true" : "false", batch->nwork, batch->lds,
+             batch->grid, st == NDF_OK ? "true" : "false");
+    return st;
+}
+
+void ndf_batch_destroy(ndf_batch *batch) { delete batch; }
 
 size_t ndf_get_num_threads(void)
 {

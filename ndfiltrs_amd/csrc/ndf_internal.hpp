@@ -97,5 +97,13 @@ size_t slide_lds_bytes(int dtype_bytes, int s, int sort_e, int hist_stride, int 
 hipError_t launch_bracket_slide(int dtype_bytes, const SlideArgs &args, size_t lds_bytes, int max_tiles,
                                 hipStream_t stream);
 size_t bracket_lds_bytes(int dtype_bytes, int s, int rows, int sort_e);
+// Batched plain kernel (2-D rectangles): `images` holds one SlideArgs per
+// image (input/output fields unused), `offsets` its first element in the
+// batch buffers, `work` nwork (image, tile) pairs taken in order through
+// `counter` (zero before the launch). `grid` from slide_batch_grid.
+hipError_t slide_batch_grid(int dtype_bytes, int sort_e, size_t lds_bytes, int *grid);
+hipError_t launch_slide_batch(int dtype_bytes, int sort_e, const SlideArgs *images, const int64_t *offsets,
+                              const int32_t *work, int nwork, int32_t *counter, const void *input, void *output,
+                              int32_t *error_flag, size_t lds_bytes, int grid, hipStream_t stream);
 
 }  // namespace ndf

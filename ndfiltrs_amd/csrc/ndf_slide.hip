@@ -406,18 +406,17 @@ constexpr size_t exchange_bytes()
     return ((size_t)NT * (E + 1) * (sizeof(Key) + 2) + 15) & ~(size_t)15;
 }
 
-// One workgroup per tile, persistent over tiles. Plain mode (no bracket):
-// every retained sample is ranked. Bracket mode: only samples whose keys lie
-// in the tile's [lo, hi] are ranked; the others count as below or above, and
-// a window whose target rank leaves the bracket sends the tile to the
-// fallback list (the plain kernel then recomputes it).
+// One tile of the plain kernel: stage, sort, rank and slide. Every retained
+// sample of the support is ranked; returns false when a rank walk left its
+// bounds (never for a consistent tile; reported through the error flag).
+// `mask` is the footprint in LDS (nullptr for a rectangle).
 template <typename T, int E>
-__global__ void __launch_bounds__(NT, 1) k_slide(SlideArgs a)
+__device__ __forceinline__ bool slide_tile(const SlideArgs &a, int tile, unsigned char *lds, const uint8_t *mask,
+                                           int &count_shared, int &failed_shared)
 {
     using KT = KeyTraits<T>;
     using Key = typename KT::Key;
     constexpr int N = NT * E;
-    extern __shared__ __align__(16) unsigned char lds[];
     // Layout: [exchange rows NT x (E + 1) keys + u16 positions; after the
     // sort: keys[N] | guard | pos[N] | guard] | codes [s] | hist | mask.
     Key *xk = reinterpret_cast<Key *>(lds);
@@ -427,39 +426,18 @@ __global__ void __launch_bounds__(NT, 1) k_slide(SlideArgs a)
     uint16_t *pos1 = pos0 - 1;  // 1-based ranks
     uint16_t *codes = reinterpret_cast<uint16_t *>(lds + exchange_bytes<Key, E>());
     uint16_t *hist_all = codes + ((a.s + 7) & ~7);
-    uint8_t *mask = a.mask ? reinterpret_cast<uint8_t *>(hist_all + SLIDE_WAVES * a.hist_stride) : nullptr;
-    __shared__ int count_shared;
-    __shared__ int failed_shared;
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
-    const bool bracket = a.bracket_lo != nullptr;
-
-    if (mask) {
-        const int mbytes = a.kz * a.ky * a.kx;
-        for (int q = tid; q < mbytes; q += NT) {
-            mask[q] = a.mask[q];
-        }
-    }
-    const int ntiles = a.tile_count ? *a.tile_count : a.tiles_x * a.tiles_y * (int)a.nz;
     const Key *input = static_cast<const Key *>(a.input);
     Key *output = static_cast<Key *>(a.output);
     const Key cval_key = classify<T>((Key)a.cval_bits, a.ignore_inf);
     const Geometry g{a.kz, a.ky, a.kx, a.zshift, ((((uint32_t)1 << a.zshift) - 1u) << 8) | 0xFFu, mask};
     uint16_t *hist = hist_all + wave * a.hist_stride;
     const bool wide = a.kz * a.ky > 255 || (a.mask != nullptr && a.count > 255);
-
-    // XCD-aware order: block b runs on XCD b % 8, so slot L = b + i * grid
-    // maps to tile (L % 8) * chunk + L / 8 and each XCD walks a contiguous
-    // run of tiles (vertical neighbours share their halo rows in its L2).
-    const int chunk = (ntiles + 7) / 8;
-    for (int slot = blockIdx.x; slot < 8 * chunk; slot += gridDim.x) {
-        const int t = (slot % 8) * chunk + slot / 8;
-        if (t >= ntiles) {
-            continue;
-        }
-        const int tile = a.tile_list ? a.tile_list[t] : t;
+    bool ok = true;
+    {
         int rest = tile;
         const int tyi = rest % a.tiles_y;  // tiles run down a column first
         rest /= a.tiles_y;
@@ -467,18 +445,13 @@ __global__ void __launch_bounds__(NT, 1) k_slide(SlideArgs a)
         const int64_t z0 = rest / a.tiles_x;
         const int64_t x0 = (int64_t)tx * SLIDE_TILE_X;
         const int64_t y0 = (int64_t)tyi * a.ty;
-        Key lo_key = 0, hi_key = KT::SENTINEL;
-        if (bracket) {
-            lo_key = static_cast<const Key *>(a.bracket_lo)[tile];
-            hi_key = static_cast<const Key *>(a.bracket_hi)[tile];
-        }
         if (tid == 0) {
             count_shared = 0;
             failed_shared = 0;
         }
         __syncthreads();
 
-        // ---- stage: classify the support, compact in-bracket samples ----
+        // ---- stage: classify the support, compact the retained samples ----
         for (int row = wave; row < a.rows; row += SLIDE_WAVES) {
             const int z = row >> a.zshift;
             const int y = row & ((1 << a.zshift) - 1);
@@ -499,17 +472,9 @@ __global__ void __launch_bounds__(NT, 1) k_slide(SlideArgs a)
                     } else {
                         key = classify<T>(input[mz + my + mx], a.ignore_inf);
                     }
-                    uint16_t code = CODE_PENDING;
-                    if (key == KT::SENTINEL) {
-                        code = CODE_OMIT;
-                    } else if (key < lo_key) {
-                        code = CODE_BELOW;
-                    } else if (key > hi_key) {
-                        code = CODE_ABOVE;
-                    }
-                    codes[row * a.sx + x] = code;
+                    codes[row * a.sx + x] = key == KT::SENTINEL ? CODE_OMIT : CODE_PENDING;
                 }
-                const bool keep = key != KT::SENTINEL && key >= lo_key && key <= hi_key;
+                const bool keep = key != KT::SENTINEL;
                 const uint64_t ballot = __ballot(keep);
                 int base = 0;
                 if (lane == 0 && ballot) {
@@ -528,20 +493,12 @@ __global__ void __launch_bounds__(NT, 1) k_slide(SlideArgs a)
         __syncthreads();
         const int retained = count_shared;
         if (retained > N) {
-            // More in-bracket samples than the sort holds (bracket mode only;
-            // the plain sort holds the whole support).
-            if (tid == 0) {
-                if (bracket) {
-                    a.fallback_tiles[atomicAdd(a.fallback_count, 1)] = tile;
-                } else {
-                    atomicOr(a.error_flag, 1);
-                }
-            }
+            // Cannot happen: the planner sizes the sort for the whole support.
             __syncthreads();
-            continue;
+            return false;
         }
 
-        // ---- sort the in-bracket samples (padded with sentinels to N) ----
+        // ---- sort the retained samples (padded with sentinels to N) ----
         {
             Chunk<Key, E> chunk;
             static_for<0, E>([&](auto ic) {
@@ -721,12 +678,80 @@ __global__ void __launch_bounds__(NT, 1) k_slide(SlideArgs a)
             }
         }
         __syncthreads();
-        if (tid == 0 && failed_shared) {
-            if (bracket) {
-                a.fallback_tiles[atomicAdd(a.fallback_count, 1)] = tile;
-            } else {
-                atomicOr(a.error_flag, 1);
-            }
+        // Only thread 0 reads the flag: it is the one that resets it for the
+        // next tile, so no other thread can race with that reset.
+        ok = tid != 0 || failed_shared == 0;
+    }
+    return ok;
+}
+
+// Plain kernel on one image: one workgroup per tile, persistent over tiles
+// (all tiles, or tile_list[0 .. *tile_count) for fallback tiles).
+template <typename T, int E>
+__global__ void __launch_bounds__(NT, 1) k_slide(SlideArgs a)
+{
+    extern __shared__ __align__(16) unsigned char lds[];
+    __shared__ int count_shared;
+    __shared__ int failed_shared;
+    using Key = typename KeyTraits<T>::Key;
+    const int tid = threadIdx.x;
+    uint8_t *mask = nullptr;
+    if (a.mask) {
+        uint16_t *hist_all = reinterpret_cast<uint16_t *>(lds + exchange_bytes<Key, E>()) + ((a.s + 7) & ~7);
+        mask = reinterpret_cast<uint8_t *>(hist_all + SLIDE_WAVES * a.hist_stride);
+        const int mbytes = a.kz * a.ky * a.kx;
+        for (int q = tid; q < mbytes; q += NT) {
+            mask[q] = a.mask[q];
+        }
+    }
+    const int ntiles = a.tile_count ? *a.tile_count : a.tiles_x * a.tiles_y * (int)a.nz;
+    // XCD-aware order: block b runs on XCD b % 8, so slot L = b + i * grid
+    // maps to tile (L % 8) * chunk + L / 8 and each XCD walks a contiguous
+    // run of tiles (vertical neighbours share their halo rows in its L2).
+    const int chunk = (ntiles + 7) / 8;
+    for (int slot = blockIdx.x; slot < 8 * chunk; slot += gridDim.x) {
+        const int t = (slot % 8) * chunk + slot / 8;
+        if (t >= ntiles) {
+            continue;
+        }
+        const int tile = a.tile_list ? a.tile_list[t] : t;
+        if (!slide_tile<T, E>(a, tile, lds, mask, count_shared, failed_shared) && tid == 0) {
+            atomicOr(a.error_flag, 1);
+        }
+    }
+}
+
+// Plain kernel on a batch of 2-D images (rectangular windows): tiles come
+// from a work queue of (image, tile) pairs ordered by decreasing cost; each
+// image has its own geometry, maps and window. Inputs and outputs of image i
+// start at offsets[i] elements from the batch buffers.
+template <typename T, int E>
+__global__ void __launch_bounds__(NT, 1) k_slide_batch(const SlideArgs *images, const int64_t *offsets,
+                                                       const int32_t *work, int nwork, int32_t *counter,
+                                                       const void *input, void *output, int32_t *error_flag)
+{
+    extern __shared__ __align__(16) unsigned char lds[];
+    __shared__ int count_shared;
+    __shared__ int failed_shared;
+    __shared__ int next_shared;
+    using Key = typename KeyTraits<T>::Key;
+    const int tid = threadIdx.x;
+    for (;;) {
+        if (tid == 0) {
+            next_shared = atomicAdd(counter, 1);
+        }
+        __syncthreads();
+        const int w = next_shared;
+        __syncthreads();
+        if (w >= nwork) {
+            break;
+        }
+        const int image = work[2 * w];
+        SlideArgs a = images[image];
+        a.input = static_cast<const Key *>(input) + offsets[image];
+        a.output = static_cast<Key *>(output) + offsets[image];
+        if (!slide_tile<T, E>(a, work[2 * w + 1], lds, nullptr, count_shared, failed_shared) && tid == 0) {
+            atomicOr(error_flag, 1);
         }
     }
 }
@@ -1132,7 +1157,73 @@ hipError_t launch_typed(const SlideArgs &args, size_t lds_bytes, int max_tiles, 
     return hipGetLastError();
 }
 
+template <typename T, int E>
+hipError_t launch_batch_typed(const SlideArgs *images, const int64_t *offsets, const int32_t *work, int nwork,
+                              int32_t *counter, const void *input, void *output, int32_t *error_flag,
+                              size_t lds_bytes, int grid, hipStream_t stream)
+{
+    hipLaunchKernelGGL((k_slide_batch<T, E>), dim3((unsigned)grid), dim3(NT), lds_bytes, stream, images, offsets,
+                       work, nwork, counter, input, output, error_flag);
+    return hipGetLastError();
+}
+
+template <typename T, int E>
+hipError_t batch_grid_typed(size_t lds_bytes, int *grid)
+{
+    const void *fn = reinterpret_cast<const void *>(&k_slide_batch<T, E>);
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+    if (e != hipSuccess) {
+        return e;
+    }
+    int per_cu = 0, device = 0, cus = 0;
+    if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, NT, lds_bytes)) != hipSuccess ||
+        (e = hipGetDevice(&device)) != hipSuccess ||
+        (e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device)) != hipSuccess) {
+        return e;
+    }
+    *grid = (per_cu < 1 ? 1 : per_cu) * (cus < 1 ? 1 : cus);
+    return hipSuccess;
+}
+
 }  // namespace
+
+hipError_t slide_batch_grid(int dtype_bytes, int sort_e, size_t lds_bytes, int *grid)
+{
+#define NDF_BATCH_GRID(E)                                                                          \
+    case E:                                                                                        \
+        return dtype_bytes == 4 ? batch_grid_typed<float, E>(lds_bytes, grid)                      \
+                                : batch_grid_typed<double, E>(lds_bytes, grid);
+    switch (sort_e) {
+        NDF_BATCH_GRID(4)
+        NDF_BATCH_GRID(8)
+        NDF_BATCH_GRID(16)
+        NDF_BATCH_GRID(32)
+        NDF_BATCH_GRID(64)
+    default: return hipErrorInvalidValue;
+    }
+#undef NDF_BATCH_GRID
+}
+
+hipError_t launch_slide_batch(int dtype_bytes, int sort_e, const SlideArgs *images, const int64_t *offsets,
+                              const int32_t *work, int nwork, int32_t *counter, const void *input, void *output,
+                              int32_t *error_flag, size_t lds_bytes, int grid, hipStream_t stream)
+{
+#define NDF_BATCH_LAUNCH(E)                                                                            \
+    case E:                                                                                            \
+        return dtype_bytes == 4 ? launch_batch_typed<float, E>(images, offsets, work, nwork, counter, input, \
+                                                               output, error_flag, lds_bytes, grid, stream) \
+                                : launch_batch_typed<double, E>(images, offsets, work, nwork, counter, input, \
+                                                                output, error_flag, lds_bytes, grid, stream);
+    switch (sort_e) {
+        NDF_BATCH_LAUNCH(4)
+        NDF_BATCH_LAUNCH(8)
+        NDF_BATCH_LAUNCH(16)
+        NDF_BATCH_LAUNCH(32)
+        NDF_BATCH_LAUNCH(64)
+    default: return hipErrorInvalidValue;
+    }
+#undef NDF_BATCH_LAUNCH
+}
 
 size_t slide_lds_bytes(int dtype_bytes, int s, int sort_e, int hist_stride, int mask_bytes)
 {

@@ -131,3 +131,21 @@ def sky_quadratic_crop(shape=(2048, 2048), seed: int = BASE_SEED + 6, field=(819
 def rank_for_quantile(quantile: float, count: int) -> int:
     """Rank at a quantile over the full tap count: floor(q * (W - 1) + 0.5)."""
     return int(np.floor(quantile * (count - 1) + 0.5))
+
+
+def cutouts(count: int = 512, seed: int = BASE_SEED + 8):
+    """Batched-cutout workload: `count` float32 images with independent
+    log-uniform side lengths in [64, 1024], each with its own odd window size
+    in {3, 5, ..., 15} and boundary mode, sky N(100, 5) plus 1% NaN.
+    Returns (images, sizes, modes)."""
+    rng = np.random.default_rng(seed)
+    modes_all = ("reflect", "constant", "nearest", "mirror", "wrap")
+    images, sizes, modes = [], [], []
+    for _ in range(count):
+        rows, cols = (int(v) for v in np.exp(rng.uniform(np.log(64.0), np.log(1024.0), size=2)))
+        image = rng.normal(100.0, 5.0, size=(rows, cols))
+        image[rng.random((rows, cols)) < 0.01] = np.nan
+        images.append(image.astype(np.float32))
+        sizes.append(int(rng.choice(np.arange(3, 16, 2))))
+        modes.append(modes_all[int(rng.integers(0, 5))])
+    return images, sizes, modes

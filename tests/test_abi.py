@@ -107,3 +107,23 @@ def test_bracket_validation_without_a_device() -> None:
                       False, 1, 0)
     with pytest.raises(ValueError, match="2-D image"):
         cube.set_bracket(weights)
+
+
+def test_batch_validation_without_a_device() -> None:
+    from ndfiltrs_amd import batch
+
+    b = batch.Batch(np.float32, [(10, 12), (0, 4), (30, 7)], [3, [5, 1], 15], ["reflect", "wrap", ["mirror", "nearest"]])
+    assert b.len == 10 * 12 + 30 * 7
+    assert list(b.offsets) == [0, 120, 120, 330]
+    with pytest.raises(ValueError, match="batch image 1: window sizes must be positive"):
+        batch.Batch(np.float32, [(10, 12), (5, 5)], [3, 0], ["reflect", "reflect"])
+    with pytest.raises(ValueError, match="unsupported boundary mode"):
+        batch.Batch(np.float32, [(10, 12)], [3], ["bogus"])
+    with pytest.raises(ValueError, match="batch images must be 2-D"):
+        batch.Batch(np.float32, [(10,)], [3], ["reflect"])
+    with pytest.raises(ValueError, match="sort length"):
+        b.configure(3000, 16)
+    with pytest.raises(ValueError, match="tile rows"):
+        b.configure(2048, 6)
+    with pytest.raises(ValueError, match="input length"):
+        b.pack([np.zeros((10, 12), np.float32), np.zeros((0, 4), np.float32), np.zeros((30, 8), np.float32)])
