@@ -30,7 +30,7 @@ KINDS = {
     "rank": 3,
     "rank_histogram": 4,
 }
-ENGINES = {"auto": 0, "generic": 1, "slide": 2}
+ENGINES = {"auto": 0, "generic": 1, "slide": 2, "network": 3}
 
 _size_p = ctypes.POINTER(ctypes.c_size_t)
 _ssize_p = ctypes.POINTER(ctypes.c_ssize_t)

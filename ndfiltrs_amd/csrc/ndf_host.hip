@@ -186,6 +186,13 @@ struct ndf_plan {
     int32_t engine = NDF_ENGINE_GENERIC;
     bool slide_ok = false;
 
+    // Network geometry (host side).
+    bool network_ok = false;
+    NetworkArgs network{};
+    size_t network_lds = 0;
+    std::vector<int64_t> net_mapz, net_mapy, net_mapx;
+    std::vector<uint32_t> net_taps;
+
     // Slide geometry (host side).
     SlideArgs slide{};
     size_t slide_lds = 0;
@@ -216,10 +223,14 @@ struct ndf_plan {
     bool tables_ready = false;
     bool uploaded = false;
     DeviceBuffer g_maps, g_taps, s_mapz, s_mapy, s_mapx, s_taps, s_rm, s_add, s_mask, d_in, d_out, d_error;
+    DeviceBuffer n_mapz, n_mapy, n_mapx, n_taps;
     GenericArgs generic{};
 
     ~ndf_plan()
     {
+        for (DeviceBuffer *b : {&n_mapz, &n_mapy, &n_mapx, &n_taps}) {
+            b->release();
+        }
         for (DeviceBuffer *b : {&g_maps, &g_taps, &s_mapz, &s_mapy, &s_mapx, &s_taps, &s_rm, &s_add, &s_mask, &d_in,
                                 &d_out, &d_error, &b_weights, &b_blocks, &b_features, &b_moments, &b_lo, &b_hi,
                                 &b_pred, &b_fallback, &b_fcount}) {
@@ -246,6 +257,75 @@ SlideArgs with_tile_rows(const SlideArgs &s, int ty)
     r.sort_e = e;
     r.tiles_y = (int32_t)((s.ny + ty - 1) / ty);
     return r;
+}
+
+// Plan the network engine (small windows); false when it does not apply.
+bool plan_network(ndf_plan *p)
+{
+    if (p->rank > 3 || p->len == 0) return false;
+    const int live = network_live(p->dbytes, (int)p->count);
+    if (live == 0) return false;
+    int64_t dims[3] = {1, 1, 1}, lo[3] = {0, 0, 0}, box[3] = {1, 1, 1};
+    int32_t modes[3] = {NDF_MODE_NEAREST, NDF_MODE_NEAREST, NDF_MODE_NEAREST};
+    const size_t pad = 3 - p->rank;
+    for (size_t a = 0; a < p->rank; ++a) {
+        dims[pad + a] = (int64_t)p->shape[a];
+        lo[pad + a] = p->low[a];
+        box[pad + a] = p->high[a] - p->low[a] + 1;
+        modes[pad + a] = p->modes[a];
+    }
+    const int64_t kz = box[0], ky = box[1], kx = box[2];
+    const int64_t sx = 64 + kx - 1;
+    // Tile rows: 32 when the staged support stays within 32 KB, else 16, 8, 4.
+    int64_t ty = 32;
+    while (ty > 4 && kz * (ty + ky - 1) * sx * p->dbytes > 32768) ty >>= 1;
+    const int64_t sy = ty + ky - 1;
+    const int64_t rows = kz * sy;
+    const int64_t s = rows * sx;
+    const size_t lds = network_lds_bytes(p->dbytes, (int)s, (int)p->count);
+    if (lds > (size_t)lds_limit() || s > (1 << 24)) return false;
+    const int64_t tiles_x = (dims[2] + 63) / 64;
+    const int64_t tiles_y = (dims[1] + ty - 1) / ty;
+    if (tiles_x * tiles_y * dims[0] > 0x7FFFFFFF) return false;
+    const int64_t in_sy = dims[2], in_sz = dims[1] * dims[2];
+    auto build = [](std::vector<int64_t> &map, int64_t length, int64_t n, int64_t low, int64_t k, int32_t mode,
+                    int64_t stride) {
+        map.assign((size_t)length, MAP_PAD);
+        for (int64_t i = 0; i < length && i < n + k - 1; ++i) {
+            const int64_t m = map_coordinate(i + low, n, mode);
+            map[(size_t)i] = m < 0 ? MAP_CVAL : m * stride;
+        }
+    };
+    build(p->net_mapz, dims[0] + kz - 1, dims[0], lo[0], kz, modes[0], in_sz);
+    build(p->net_mapy, tiles_y * ty + ky - 1, dims[1], lo[1], ky, modes[1], in_sy);
+    build(p->net_mapx, tiles_x * 64 + kx - 1, dims[2], lo[2], kx, modes[2], 1);
+    p->net_taps.clear();
+    for (size_t t = 0; t < p->count; ++t) {
+        int64_t d[3] = {0, 0, 0};
+        for (size_t a = 0; a < p->rank; ++a) d[pad + a] = p->taps[t * p->rank + a] - p->low[a];
+        p->net_taps.push_back((uint32_t)((d[0] * sy + d[1]) * sx + d[2]));
+    }
+    NetworkArgs &n = p->network;
+    n = NetworkArgs{};
+    n.nz = dims[0];
+    n.ny = dims[1];
+    n.nx = dims[2];
+    n.sx = (int32_t)sx;
+    n.sy = (int32_t)sy;
+    n.rows = (int32_t)rows;
+    n.s = (int32_t)s;
+    n.ty = (int32_t)ty;
+    n.tiles_x = (int32_t)tiles_x;
+    n.tiles_y = (int32_t)tiles_y;
+    n.count = (int32_t)p->count;
+    n.live = live;
+    n.cval_bits = p->cval_bits;
+    n.ignore_inf = p->eff_ignore_inf;
+    n.stat = p->stat;
+    n.min_count = (int32_t)p->eff_min_count;
+    n.rank_param = (int32_t)std::min(p->parameter, p->count);
+    p->network_lds = lds;
+    return true;
 }
 
 // Plan the slide engine; false when the geometry cannot be staged.
@@ -477,6 +557,16 @@ ndf_status prepare_engine_tables(ndf_plan *p, char *err, size_t err_len)
         g.stat = p->stat;
         g.min_count = p->eff_min_count;
         g.rank_param = (int64_t)std::min(p->parameter, p->count);
+    } else if (p->engine == NDF_ENGINE_NETWORK) {
+        if ((e = upload(p->n_mapz, p->net_mapz)) != hipSuccess || (e = upload(p->n_mapy, p->net_mapy)) != hipSuccess ||
+            (e = upload(p->n_mapx, p->net_mapx)) != hipSuccess || (e = upload(p->n_taps, p->net_taps)) != hipSuccess) {
+            return fail_hip(e, err, err_len);
+        }
+        NetworkArgs &n = p->network;
+        n.mapz = static_cast<const int64_t *>(p->n_mapz.ptr);
+        n.mapy = static_cast<const int64_t *>(p->n_mapy.ptr);
+        n.mapx = static_cast<const int64_t *>(p->n_mapx.ptr);
+        n.taps = static_cast<const uint32_t *>(p->n_taps.ptr);
     } else if (p->engine == NDF_ENGINE_SLIDE) {
         if ((e = upload(p->s_mapz, p->slide_mapz)) != hipSuccess ||
             (e = upload(p->s_mapy, p->slide_mapy)) != hipSuccess ||
@@ -612,6 +702,13 @@ ndf_status launch(ndf_plan *p, const void *d_in, void *d_out, hipStream_t stream
             args.tile_count = static_cast<const int32_t *>(p->b_fcount.ptr);
             e = launch_slide(p->dbytes, args, p->slide_fb_lds, tiles, stream);
         }
+    } else if (p->engine == NDF_ENGINE_NETWORK) {
+        NetworkArgs args = p->network;
+        args.input = d_in;
+        args.output = d_out;
+        mark(p->main_begin);
+        if (e == hipSuccess) e = launch_network(p->dbytes, args, p->network_lds, stream);
+        mark(p->main_end);
     } else if (p->engine == NDF_ENGINE_SLIDE) {
         SlideArgs args = p->slide;
         args.input = d_in;
@@ -775,7 +872,8 @@ ndf_status build_plan(ndf_plan *p, int32_t dtype_bytes, size_t rank, const size_
     p->eff_min_count = p->omit ? (int64_t)min_count : (int64_t)count;
     p->all_nan = p->stat == STAT_RANK && parameter >= count;
     p->slide_ok = plan_slide(p);
-    p->engine = p->slide_ok ? NDF_ENGINE_SLIDE : NDF_ENGINE_GENERIC;
+    p->network_ok = plan_network(p);
+    p->engine = p->network_ok ? NDF_ENGINE_NETWORK : (p->slide_ok ? NDF_ENGINE_SLIDE : NDF_ENGINE_GENERIC);
     return NDF_OK;
 }
 
@@ -809,12 +907,18 @@ ndf_status ndf_plan_set_engine(ndf_plan *plan, int32_t engine, char *err, size_t
     std::lock_guard<std::mutex> guard(plan->mutex);
     int32_t chosen;
     if (engine == NDF_ENGINE_AUTO) {
-        chosen = plan->slide_ok ? NDF_ENGINE_SLIDE : NDF_ENGINE_GENERIC;
+        chosen = plan->network_ok ? NDF_ENGINE_NETWORK : (plan->slide_ok ? NDF_ENGINE_SLIDE : NDF_ENGINE_GENERIC);
     } else if (engine == NDF_ENGINE_GENERIC) {
         chosen = NDF_ENGINE_GENERIC;
     } else if (engine == NDF_ENGINE_SLIDE) {
         if (!plan->slide_ok) return fail(NDF_ERR_INVALID, "the slide engine cannot stage this geometry", err, err_len);
         chosen = NDF_ENGINE_SLIDE;
+    } else if (engine == NDF_ENGINE_NETWORK) {
+        if (!plan->network_ok) {
+            return fail(NDF_ERR_INVALID, "the network engine needs a window of at most 96 taps (64 for float64)",
+                        err, err_len);
+        }
+        chosen = NDF_ENGINE_NETWORK;
     } else {
         return fail(NDF_ERR_INVALID, "unknown engine", err, err_len);
     }
@@ -827,8 +931,17 @@ ndf_status ndf_plan_set_engine(ndf_plan *plan, int32_t engine, char *err, size_t
 
 ndf_status ndf_plan_describe(const ndf_plan *plan, char *buffer, size_t buffer_len)
 {
-    const char *engine = plan->all_nan ? "fill" : (plan->engine == NDF_ENGINE_SLIDE ? "slide" : "generic");
-    if (plan->engine == NDF_ENGINE_SLIDE) {
+    const char *engine = plan->all_nan ? "fill"
+                         : plan->engine == NDF_ENGINE_SLIDE ? "slide"
+                         : plan->engine == NDF_ENGINE_NETWORK ? "network"
+                                                              : "generic";
+    if (plan->engine == NDF_ENGINE_NETWORK) {
+        snprintf(buffer, buffer_len,
+                 "{\"engine\": \"%s\", \"count\": %zu, \"tile\": [%d, 64], \"network_wires\": %d, "
+                 "\"lds_bytes\": %zu, \"tiles\": %lld, \"bracket_tile_rows\": %d}",
+                 engine, plan->count, plan->network.ty, plan->network.live, plan->network_lds,
+                 (long long)plan->network.tiles_x * plan->network.tiles_y * plan->network.nz, plan->bracket_ty);
+    } else if (plan->engine == NDF_ENGINE_SLIDE) {
         snprintf(buffer, buffer_len,
                  "{\"engine\": \"%s\", \"count\": %zu, \"tile\": [%d, %d], \"support\": %d, \"sort_len\": %d, "
                  "\"lds_bytes\": %zu, \"edge_taps\": %d, \"tiles\": %lld, \"bracket_tile_rows\": %d}",
@@ -960,6 +1073,12 @@ ndf_status ndf_plan_set_bracket(ndf_plan *plan, const float *weights, size_t n_w
     plan->bracket_e = e;
     plan->bracket_on = true;
     plan->bracket_ready = false;
+    // The bracket is a mode of the slide engine (its fallback is the plain
+    // slide kernel on the same tiles).
+    if (plan->engine != NDF_ENGINE_SLIDE) {
+        plan->engine = NDF_ENGINE_SLIDE;
+        plan->tables_ready = false;
+    }
     return NDF_OK;
 }
 

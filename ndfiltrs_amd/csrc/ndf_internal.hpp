@@ -83,6 +83,32 @@ struct SlideArgs {
     int32_t *fallback_count;
 };
 
+// Engine 3: small windows (count <= 96 for f32, 64 for f64). A tile of 64
+// columns x ty rows of one plane; its support (kz planes of sy = ty + ky - 1
+// rows of sx = 64 + kx - 1 samples) is staged in LDS as classified keys; each
+// thread sorts its window in registers with a sorting network of `live`
+// wires and reads the statistic. taps[t] = (dz * sy + dy) * sx + dx.
+struct NetworkArgs {
+    const void *input;
+    void *output;
+    int64_t nz, ny, nx;
+    const int64_t *mapz, *mapy, *mapx;
+    int32_t sx, sy, rows, s;
+    int32_t ty, tiles_x, tiles_y;
+    int32_t count, live;
+    const uint32_t *taps;
+    uint64_t cval_bits;
+    int32_t ignore_inf;
+    int32_t stat;
+    int32_t min_count;
+    int32_t rank_param;
+};
+
+// Network size (live wires) for a window of `count` taps, 0 if too large.
+int network_live(int dtype_bytes, int count);
+size_t network_lds_bytes(int dtype_bytes, int s, int count);
+hipError_t launch_network(int dtype_bytes, const NetworkArgs &args, size_t lds_bytes, hipStream_t stream);
+
 constexpr int SLIDE_WAVES = 4;
 constexpr int SLIDE_TILE_X = 64;
 constexpr uint32_t SLIDE_NO_POS = 0xFFFFFFFFu;

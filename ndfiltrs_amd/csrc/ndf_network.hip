@@ -1,0 +1,249 @@
+// Network engine: small windows (up to 96 taps), one output per thread, the
+// whole window sorted in registers by a compile-time sorting network.
+//
+// The reference's median_network (src/median_network.rs:65-209) applies a
+// Batcher network column-wise over many outputs on the CPU. Here the same
+// idea is MI355X-shaped:
+//   * a workgroup (4 waves) stages the support of a 64-column x TILE_ROWS
+//     output tile (all window planes) into LDS through the boundary maps,
+//     classifying each sample once (omitted samples become the sentinel key,
+//     constant-mode samples the cval key);
+//   * each thread gathers its window's keys from LDS into registers (taps
+//     beyond the window are sentinels) and sorts them with a Batcher
+//     odd-even merge network generated at compile time and pruned to the
+//     live wires (comparators into sentinel-only wires are no-ops);
+//   * sentinels sort last, so the retained count n is the number of
+//     non-sentinel keys and the statistic is read at a data-dependent index
+//     with a select chain.
+// The result is the exact sample (or the f64 midpoint of two), bit-identical
+// to the slide and generic engines.
+#include <array>
+#include <utility>
+
+#include "ndf_device.hpp"
+#include "ndf_internal.hpp"
+
+namespace ndf {
+namespace {
+
+constexpr int NETWORK_THREADS = 256;
+
+// Batcher odd-even merge sort on WIRES (a power of two) wires, keeping only
+// the comparators whose upper wire is below LIVE.
+struct Comparator {
+    int lo, hi;
+};
+
+template <int WIRES, int LIVE>
+constexpr int batcher_count()
+{
+    int c = 0;
+    for (int p = 1; p < WIRES; p <<= 1) {
+        for (int k = p; k >= 1; k >>= 1) {
+            for (int j = k % p; j + k < WIRES; j += 2 * k) {
+                for (int i = 0; i < k && i < WIRES - j - k; ++i) {
+                    if ((i + j) / (2 * p) == (i + j + k) / (2 * p) && i + j + k < LIVE) {
+                        ++c;
+                    }
+                }
+            }
+        }
+    }
+    return c;
+}
+
+template <int WIRES, int LIVE>
+constexpr std::array<Comparator, batcher_count<WIRES, LIVE>()> batcher_pairs()
+{
+    std::array<Comparator, batcher_count<WIRES, LIVE>()> out{};
+    int c = 0;
+    for (int p = 1; p < WIRES; p <<= 1) {
+        for (int k = p; k >= 1; k >>= 1) {
+            for (int j = k % p; j + k < WIRES; j += 2 * k) {
+                for (int i = 0; i < k && i < WIRES - j - k; ++i) {
+                    if ((i + j) / (2 * p) == (i + j + k) / (2 * p) && i + j + k < LIVE) {
+                        out[c] = Comparator{i + j, i + j + k};
+                        ++c;
+                    }
+                }
+            }
+        }
+    }
+    return out;
+}
+
+constexpr int wires_for(int live)
+{
+    int w = 1;
+    while (w < live) {
+        w <<= 1;
+    }
+    return w;
+}
+
+template <int LIVE>
+constexpr auto NETWORK = batcher_pairs<wires_for(LIVE), LIVE>();
+
+template <typename Key, int LIVE, size_t... I>
+__device__ __forceinline__ void sort_keys(Key (&k)[LIVE], std::index_sequence<I...>)
+{
+    (
+        [&] {
+            constexpr int a = NETWORK<LIVE>[I].lo;
+            constexpr int b = NETWORK<LIVE>[I].hi;
+            const Key x = k[a], y = k[b];
+            k[a] = x < y ? x : y;
+            k[b] = x < y ? y : x;
+        }(),
+        ...);
+}
+
+template <typename T, int LIVE>
+__global__ void __launch_bounds__(NETWORK_THREADS) k_network(NetworkArgs a)
+{
+    using KT = KeyTraits<T>;
+    using Key = typename KT::Key;
+    extern __shared__ __align__(16) unsigned char lds[];
+    Key *support = reinterpret_cast<Key *>(lds);
+    uint32_t *taps = reinterpret_cast<uint32_t *>(support + a.s);
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    for (int t = tid; t < a.count; t += NETWORK_THREADS) {
+        taps[t] = a.taps[t];
+    }
+    const Key *input = static_cast<const Key *>(a.input);
+    Key *output = static_cast<Key *>(a.output);
+    const Key cval_key = classify<T>((Key)a.cval_bits, a.ignore_inf);
+    const int tiles = a.tiles_x * a.tiles_y * (int)a.nz;
+    for (int tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+        int rest = tile;
+        const int tx = rest % a.tiles_x;
+        rest /= a.tiles_x;
+        const int tyi = rest % a.tiles_y;
+        const int64_t z0 = rest / a.tiles_y;
+        const int64_t x0 = (int64_t)tx * 64;
+        const int64_t y0 = (int64_t)tyi * a.ty;
+        __syncthreads();  // the previous tile's reads of `support` are done
+        // Stage: support row r = z * sy + y, column x; all loads of a batch
+        // of rows first, then the classification.
+        for (int row = wave; row < a.rows; row += 4) {
+            const int z = row / a.sy;
+            const int y = row - z * a.sy;
+            const int64_t mz = a.mapz[z0 + z];
+            const int64_t my = a.mapy[y0 + y];
+            for (int x = lane; x < a.sx; x += 64) {
+                const int64_t mx = a.mapx[x0 + x];
+                const bool pad = mz == MAP_PAD || my == MAP_PAD || mx == MAP_PAD;
+                const bool cval = !pad && (mz < 0 || my < 0 || mx < 0);
+                const Key raw = input[pad || cval ? 0 : mz + my + mx];
+                support[row * a.sx + x] = pad ? KT::SENTINEL : (cval ? cval_key : classify<T>(raw, a.ignore_inf));
+            }
+        }
+        __syncthreads();
+        const int64_t x = x0 + lane;
+        for (int r = wave; r < a.ty; r += 4) {
+            const int64_t y = y0 + r;
+            if (y >= a.ny || x >= a.nx) {
+                continue;
+            }
+            Key k[LIVE];
+            const Key *window = support + r * a.sx + lane;
+#pragma unroll
+            for (int i = 0; i < LIVE; ++i) {
+                k[i] = i < a.count ? window[taps[i]] : KT::SENTINEL;
+            }
+            sort_keys<Key, LIVE>(k, std::make_index_sequence<NETWORK<LIVE>.size()>{});
+            int n = 0;
+#pragma unroll
+            for (int i = 0; i < LIVE; ++i) {
+                n += k[i] != KT::SENTINEL;
+            }
+            Key out = KT::CANONICAL_NAN;
+            if (n >= a.min_count && n > 0) {
+                int lo_index, hi_index;
+                if (a.stat == STAT_RANK) {
+                    lo_index = hi_index = a.rank_param;
+                } else {
+                    hi_index = n >> 1;
+                    lo_index = (n & 1) ? hi_index : hi_index - 1;
+                }
+                if (hi_index < n) {
+                    Key lo = 0, hi = 0;
+#pragma unroll
+                    for (int i = 0; i < LIVE; ++i) {
+                        lo = i == lo_index ? k[i] : lo;
+                        hi = i == hi_index ? k[i] : hi;
+                    }
+                    out = lo_index == hi_index ? KT::unkey(hi) : median_pair_bits<T>(lo, hi);
+                }
+            }
+            output[(z0 * a.ny + y) * a.nx + x] = out;
+        }
+    }
+}
+
+template <typename T, int LIVE>
+hipError_t launch_live(const NetworkArgs &args, size_t lds_bytes, hipStream_t stream)
+{
+    const void *fn = reinterpret_cast<const void *>(&k_network<T, LIVE>);
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+    if (e != hipSuccess) {
+        return e;
+    }
+    int per_cu = 0, device = 0, cus = 0;
+    if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, NETWORK_THREADS, lds_bytes)) != hipSuccess ||
+        (e = hipGetDevice(&device)) != hipSuccess ||
+        (e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device)) != hipSuccess) {
+        return e;
+    }
+    const int64_t tiles = (int64_t)args.tiles_x * args.tiles_y * args.nz;
+    int64_t blocks = (int64_t)(per_cu < 1 ? 1 : per_cu) * (cus < 1 ? 1 : cus) * 4;
+    blocks = blocks < tiles ? blocks : tiles;
+    hipLaunchKernelGGL((k_network<T, LIVE>), dim3((unsigned)(blocks < 1 ? 1 : blocks)), dim3(NETWORK_THREADS),
+                       lds_bytes, stream, args);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+int network_live(int dtype_bytes, int count)
+{
+    const int limit = dtype_bytes == 4 ? 96 : 64;
+    if (count > limit) {
+        return 0;
+    }
+    for (int live : {9, 16, 25, 32, 49, 64, 96}) {
+        if (count <= live) {
+            return live;
+        }
+    }
+    return 0;
+}
+
+size_t network_lds_bytes(int dtype_bytes, int s, int count)
+{
+    return (size_t)s * (size_t)dtype_bytes + (size_t)count * 4;
+}
+
+hipError_t launch_network(int dtype_bytes, const NetworkArgs &args, size_t lds_bytes, hipStream_t stream)
+{
+#define NDF_NETWORK_CASE(LIVE)                                                                     \
+    case LIVE:                                                                                     \
+        return dtype_bytes == 4 ? launch_live<float, LIVE>(args, lds_bytes, stream)                \
+                                : launch_live<double, LIVE>(args, lds_bytes, stream);
+    switch (args.live) {
+        NDF_NETWORK_CASE(9)
+        NDF_NETWORK_CASE(16)
+        NDF_NETWORK_CASE(25)
+        NDF_NETWORK_CASE(32)
+        NDF_NETWORK_CASE(49)
+        NDF_NETWORK_CASE(64)
+    case 96:
+        return dtype_bytes == 4 ? launch_live<float, 96>(args, lds_bytes, stream) : hipErrorInvalidValue;
+    default: return hipErrorInvalidValue;
+    }
+#undef NDF_NETWORK_CASE
+}
+
+}  // namespace ndf

@@ -29,13 +29,11 @@
 namespace ndf {
 namespace {
 
-// Codes of support samples: 1 + tile rank for sorted (in-bracket) samples,
-// CODE_BELOW / CODE_ABOVE for retained samples outside the tile's bracket,
-// CODE_OMIT for omitted samples; CODE_PENDING marks in-bracket samples
-// between staging and ranking. Every rank code is below CODE_PENDING.
-constexpr uint16_t CODE_BELOW = 0;
+// Codes of support samples in the plain kernel: 1 + tile rank for retained
+// samples, CODE_OMIT for omitted samples; CODE_PENDING marks retained
+// samples between staging and ranking. Every rank code is below
+// CODE_PENDING. (The bracket kernel keeps codes only for inside samples.)
 constexpr uint16_t CODE_PENDING = 0xFFFD;
-constexpr uint16_t CODE_ABOVE = 0xFFFE;
 constexpr uint16_t CODE_OMIT = 0xFFFF;
 constexpr uint16_t POS_NONE = 0xFFFF;
 constexpr int NT = SLIDE_WAVES * 64;

@@ -64,7 +64,7 @@ def test_engine_choice_for_the_benchmark_shapes() -> None:
     yy, xx = np.mgrid[-6:7, -6:7]
     ring = (((yy**2 + xx**2) >= 16) & ((yy**2 + xx**2) <= 36)).ravel().tolist()
     c3 = _plan(np.float32, (8192, 8192), (13, 13), footprint=ring, origin=[1, 0], mode=["wrap"] * 2)
-    assert c3["engine"] == "slide" and c3["count"] == 68
+    assert c3["engine"] == "network" and c3["count"] == 68 and c3["network_wires"] == 96
     cube = _plan(np.float32, (128, 1024, 1024), (3, 7, 7), mode=["wrap"] * 3)
     assert cube["engine"] == "slide" and cube["count"] == 147
     five = _plan(np.float64, (2, 2, 2, 2, 3), (3, 3, 3, 3, 5))
@@ -82,12 +82,24 @@ def test_plan_rejects_length_mismatch_without_a_device() -> None:
     assert err.value.decode() == "input length does not match the configured shape"
 
 
-@pytest.mark.parametrize("engine", ["generic", "slide", "auto"])
+@pytest.mark.parametrize("engine", ["generic", "slide", "network", "auto"])
 def test_engines_can_be_forced(engine: str) -> None:
     plan = _core.Plan(np.float32, (40, 40), "median", (5, 5), None, (0, 0), ("reflect",) * 2, 0.0, False, False, 1, 0)
     plan.set_engine(engine)
     described = json.loads(plan.describe())["engine"]
-    assert described == ("slide" if engine == "auto" else engine)
+    assert described == ("network" if engine == "auto" else engine)
+
+
+def test_network_engine_limits() -> None:
+    big = _core.Plan(np.float32, (40, 40), "median", (11, 11), None, (0, 0), ("reflect",) * 2, 0.0, True, False, 1,
+                     0)
+    assert json.loads(big.describe())["engine"] == "slide"
+    with pytest.raises(ValueError, match="network engine needs"):
+        big.set_engine("network")
+    f64 = _core.Plan(np.float64, (40, 40), "median", (9, 9), None, (0, 0), ("reflect",) * 2, 0.0, True, False, 1, 0)
+    assert json.loads(f64.describe())["engine"] == "slide"  # 81 taps > 64 for float64
+    f32 = _core.Plan(np.float32, (40, 40), "median", (9, 9), None, (0, 0), ("reflect",) * 2, 0.0, True, False, 1, 0)
+    assert json.loads(f32.describe())["network_wires"] == 96
 
 
 def test_bracket_validation_without_a_device() -> None:
