@@ -105,7 +105,7 @@ class Batch:
         return ms.value
 
     def describe(self) -> dict:
-        buffer = ctypes.create_string_buffer(512)
+        buffer = ctypes.create_string_buffer(4096)
         _lib.lib.ndf_batch_describe(self._handle, buffer, len(buffer))
         return json.loads(buffer.value.decode())
 

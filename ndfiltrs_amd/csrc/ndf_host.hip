@@ -1210,10 +1210,23 @@ struct ndf_batch {
     bool ready = false;
     std::vector<SlideArgs> images;
     std::vector<int32_t> fill_images;  // images whose every output is NaN
-    DeviceBuffer d_images, d_offsets, d_work, d_counter, d_error, d_maps;
+    DeviceBuffer d_images, d_offsets, d_work, d_counter, d_error, d_maps, d_taps;
     int nwork = 0;
     size_t lds = 0;
     int grid = 0;
+    // Images with small windows run on the batched network kernel, one
+    // group (and launch) per network size.
+    struct NetworkGroup {
+        int live = 0;
+        std::vector<NetworkArgs> args;
+        std::vector<int64_t> offsets;
+        std::vector<int32_t> pairs;
+        DeviceBuffer d_args, d_offsets, d_work;
+        size_t lds = 0;
+        int grid = 0;
+        int nwork = 0;
+    };
+    std::vector<NetworkGroup> groups;
     // Captured launch (for one input/output pair).
     hipStream_t capture_stream = nullptr;
     hipGraph_t graph = nullptr;
@@ -1236,7 +1249,10 @@ struct ndf_batch {
         drop_graph();
         if (capture_stream) (void)hipStreamDestroy(capture_stream);
         for (ndf_plan *p : plans) ndf_plan_destroy(p);
-        for (DeviceBuffer *b : {&d_images, &d_offsets, &d_work, &d_counter, &d_error, &d_maps}) b->release();
+        for (DeviceBuffer *b : {&d_images, &d_offsets, &d_work, &d_counter, &d_error, &d_maps, &d_taps}) b->release();
+        for (NetworkGroup &g : groups) {
+            for (DeviceBuffer *b : {&g.d_args, &g.d_offsets, &g.d_work}) b->release();
+        }
     }
 };
 
@@ -1258,12 +1274,44 @@ ndf_status batch_prepare(ndf_batch *b, char *err, size_t err_len)
         int32_t image, tile;
     };
     std::vector<Work> work;
+    for (ndf_batch::NetworkGroup &g : b->groups) {
+        for (DeviceBuffer *buffer : {&g.d_args, &g.d_offsets, &g.d_work}) buffer->release();
+    }
+    b->groups.clear();
+    std::vector<uint32_t> taps;
+    std::vector<std::array<int64_t, 5>> net_at;  // image, maps offset, taps offset, group, index in group
     for (size_t i = 0; i < b->plans.size(); ++i) {
         ndf_plan *p = b->plans[i];
         map_at.push_back({0, 0, 0});
         if (p->len == 0) continue;
         if (p->all_nan) {
             b->fill_images.push_back((int32_t)i);
+            continue;
+        }
+        if (p->network_ok) {
+            const int live = p->network.live;
+            size_t gi = 0;
+            while (gi < b->groups.size() && b->groups[gi].live != live) ++gi;
+            if (gi == b->groups.size()) {
+                b->groups.emplace_back();
+                b->groups.back().live = live;
+            }
+            ndf_batch::NetworkGroup &g = b->groups[gi];
+            net_at.push_back({(int64_t)i, (int64_t)maps.size(), (int64_t)taps.size(), (int64_t)gi,
+                              (int64_t)g.args.size()});
+            maps.insert(maps.end(), p->net_mapz.begin(), p->net_mapz.end());
+            maps.insert(maps.end(), p->net_mapy.begin(), p->net_mapy.end());
+            maps.insert(maps.end(), p->net_mapx.begin(), p->net_mapx.end());
+            taps.insert(taps.end(), p->net_taps.begin(), p->net_taps.end());
+            const int index = (int)g.args.size();
+            g.args.push_back(p->network);
+            g.offsets.push_back(b->offsets[i]);
+            g.lds = std::max(g.lds, p->network_lds);
+            const int tiles = p->network.tiles_x * p->network.tiles_y * (int)p->network.nz;
+            for (int t = 0; t < tiles; ++t) {
+                g.pairs.push_back(index);
+                g.pairs.push_back(t);
+            }
             continue;
         }
         if (!plan_slide(p, b->sort_e, b->max_ty)) {
@@ -1292,8 +1340,29 @@ ndf_status batch_prepare(ndf_batch *b, char *err, size_t err_len)
         pairs.push_back(w.tile);
     }
     hipError_t e;
-    if ((e = upload(b->d_maps, maps)) != hipSuccess) return fail_hip(e, err, err_len);
+    if ((e = upload(b->d_maps, maps)) != hipSuccess || (e = upload(b->d_taps, taps)) != hipSuccess) {
+        return fail_hip(e, err, err_len);
+    }
     const int64_t *base = static_cast<const int64_t *>(b->d_maps.ptr);
+    for (const std::array<int64_t, 5> &at : net_at) {
+        const ndf_plan *p = b->plans[(size_t)at[0]];
+        NetworkArgs &n = b->groups[(size_t)at[3]].args[(size_t)at[4]];
+        n.mapz = base + at[1];
+        n.mapy = n.mapz + p->net_mapz.size();
+        n.mapx = n.mapy + p->net_mapy.size();
+        n.taps = static_cast<const uint32_t *>(b->d_taps.ptr) + at[2];
+        n.input = nullptr;
+        n.output = nullptr;
+    }
+    for (ndf_batch::NetworkGroup &g : b->groups) {
+        g.nwork = (int)(g.pairs.size() / 2);
+        if ((e = upload(g.d_args, g.args)) != hipSuccess || (e = upload(g.d_offsets, g.offsets)) != hipSuccess ||
+            (e = upload(g.d_work, g.pairs)) != hipSuccess ||
+            (e = network_batch_grid(b->dbytes, g.live, g.lds, &g.grid)) != hipSuccess) {
+            return fail_hip(e, err, err_len);
+        }
+        g.grid = std::max(1, std::min(g.grid, g.nwork));
+    }
     for (size_t i = 0; i < b->images.size(); ++i) {
         SlideArgs &a = b->images[i];
         if (b->plans[i]->len == 0 || b->plans[i]->all_nan) continue;
@@ -1308,7 +1377,8 @@ ndf_status batch_prepare(ndf_batch *b, char *err, size_t err_len)
         a.output = nullptr;
     }
     if ((e = upload(b->d_images, b->images)) != hipSuccess || (e = upload(b->d_offsets, b->offsets)) != hipSuccess ||
-        (e = upload(b->d_work, pairs)) != hipSuccess || (e = b->d_counter.ensure(sizeof(int32_t))) != hipSuccess ||
+        (e = upload(b->d_work, pairs)) != hipSuccess ||
+        (e = b->d_counter.ensure(sizeof(int32_t) * (1 + b->groups.size()))) != hipSuccess ||
         (e = b->d_error.ensure(sizeof(int32_t))) != hipSuccess) {
         return fail_hip(e, err, err_len);
     }
@@ -1334,7 +1404,16 @@ hipError_t batch_enqueue(ndf_batch *b, const void *d_in, void *d_out, hipStream_
         void *out = static_cast<char *>(d_out) + b->offsets[(size_t)i] * b->dbytes;
         e = launch_fill(b->dbytes, out, (int64_t)b->plans[(size_t)i]->len, nan_bits, stream);
     }
-    if (e == hipSuccess && b->nwork > 0) e = launch_fill(4, b->d_counter.ptr, 1, 0, stream);
+    if (e == hipSuccess) e = launch_fill(4, b->d_counter.ptr, (int64_t)(1 + b->groups.size()), 0, stream);
+    for (size_t gi = 0; gi < b->groups.size() && e == hipSuccess; ++gi) {
+        const ndf_batch::NetworkGroup &g = b->groups[gi];
+        if (g.nwork == 0) continue;
+        e = launch_network_batch(b->dbytes, g.live, static_cast<const NetworkArgs *>(g.d_args.ptr),
+                                 static_cast<const int64_t *>(g.d_offsets.ptr),
+                                 static_cast<const int32_t *>(g.d_work.ptr), g.nwork,
+                                 static_cast<int32_t *>(b->d_counter.ptr) + 1 + gi, d_in, d_out, g.lds, g.grid,
+                                 stream);
+    }
     if (e == hipSuccess && b->nwork > 0) {
         e = launch_slide_batch(b->dbytes, b->sort_e, static_cast<const SlideArgs *>(b->d_images.ptr),
                                static_cast<const int64_t *>(b->d_offsets.ptr),
@@ -1536,10 +1615,19 @@ ndf_status ndf_batch_describe(ndf_batch *batch, char *buffer, size_t buffer_len)
     const ndf_status st = batch_prepare(batch, err, sizeof err);
     snprintf(buffer, buffer_len,
              "{\"images\": %zu, \"elements\": %lld, \"sort_len\": %d, \"max_tile_rows\": %d, \"graph\": %s, "
-             "\"tuned\": %s, \"work_tiles\": %d, \"lds_bytes\": %zu, \"workgroups\": %d, \"ok\": %s}",
+             "\"tuned\": %s, \"work_tiles\": %d, \"lds_bytes\": %zu, \"workgroups\": %d, \"ok\": %s, "
+             "\"network_groups\": [",
              batch->plans.size(), (long long)batch->total, 256 * batch->sort_e, batch->max_ty,
              batch->use_graph ? "true" : "false", batch->tuned ? "true" : "false", batch->nwork, batch->lds,
              batch->grid, st == NDF_OK ? "true" : "false");
+    for (size_t gi = 0; gi < batch->groups.size(); ++gi) {
+        const size_t used = strlen(buffer);
+        if (used + 96 >= buffer_len) break;
+        snprintf(buffer + used, buffer_len - used, "%s{\"wires\": %d, \"images\": %zu, \"tiles\": %d}",
+                 gi ? ", " : "", batch->groups[gi].live, batch->groups[gi].args.size(), batch->groups[gi].nwork);
+    }
+    const size_t used = strlen(buffer);
+    if (used + 3 < buffer_len) snprintf(buffer + used, buffer_len - used, "]}");
     return st;
 }
 

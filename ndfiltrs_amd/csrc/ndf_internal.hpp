@@ -108,6 +108,13 @@ struct NetworkArgs {
 int network_live(int dtype_bytes, int count);
 size_t network_lds_bytes(int dtype_bytes, int s, int count);
 hipError_t launch_network(int dtype_bytes, const NetworkArgs &args, size_t lds_bytes, hipStream_t stream);
+// Batched network kernel for images whose windows have at most `live` taps:
+// `images` one NetworkArgs per image (input/output unused; taps in global
+// memory), `work` nwork (image, tile) pairs taken through `counter`.
+hipError_t network_batch_grid(int dtype_bytes, int live, size_t lds_bytes, int *grid);
+hipError_t launch_network_batch(int dtype_bytes, int live, const NetworkArgs *images, const int64_t *offsets,
+                                const int32_t *work, int nwork, int32_t *counter, const void *input, void *output,
+                                size_t lds_bytes, int grid, hipStream_t stream);
 
 constexpr int SLIDE_WAVES = 4;
 constexpr int SLIDE_TILE_X = 64;

@@ -98,89 +98,156 @@ __device__ __forceinline__ void sort_keys(Key (&k)[LIVE], std::index_sequence<I.
         ...);
 }
 
+// One tile: stage the support (classified keys) into LDS, then one output
+// per thread and row step. `taps` holds the window's support offsets.
 template <typename T, int LIVE>
-__global__ void __launch_bounds__(NETWORK_THREADS) k_network(NetworkArgs a)
+__device__ __forceinline__ void network_tile(const NetworkArgs &a, int tile,
+                                             typename KeyTraits<T>::Key *support, const uint32_t *taps)
 {
     using KT = KeyTraits<T>;
     using Key = typename KT::Key;
-    extern __shared__ __align__(16) unsigned char lds[];
-    Key *support = reinterpret_cast<Key *>(lds);
-    uint32_t *taps = reinterpret_cast<uint32_t *>(support + a.s);
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
-    for (int t = tid; t < a.count; t += NETWORK_THREADS) {
-        taps[t] = a.taps[t];
-    }
     const Key *input = static_cast<const Key *>(a.input);
     Key *output = static_cast<Key *>(a.output);
     const Key cval_key = classify<T>((Key)a.cval_bits, a.ignore_inf);
-    const int tiles = a.tiles_x * a.tiles_y * (int)a.nz;
-    for (int tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
-        int rest = tile;
-        const int tx = rest % a.tiles_x;
-        rest /= a.tiles_x;
-        const int tyi = rest % a.tiles_y;
-        const int64_t z0 = rest / a.tiles_y;
-        const int64_t x0 = (int64_t)tx * 64;
-        const int64_t y0 = (int64_t)tyi * a.ty;
-        __syncthreads();  // the previous tile's reads of `support` are done
-        // Stage: support row r = z * sy + y, column x; all loads of a batch
-        // of rows first, then the classification.
-        for (int row = wave; row < a.rows; row += 4) {
-            const int z = row / a.sy;
-            const int y = row - z * a.sy;
-            const int64_t mz = a.mapz[z0 + z];
-            const int64_t my = a.mapy[y0 + y];
-            for (int x = lane; x < a.sx; x += 64) {
-                const int64_t mx = a.mapx[x0 + x];
-                const bool pad = mz == MAP_PAD || my == MAP_PAD || mx == MAP_PAD;
-                const bool cval = !pad && (mz < 0 || my < 0 || mx < 0);
-                const Key raw = input[pad || cval ? 0 : mz + my + mx];
-                support[row * a.sx + x] = pad ? KT::SENTINEL : (cval ? cval_key : classify<T>(raw, a.ignore_inf));
-            }
-        }
-        __syncthreads();
-        const int64_t x = x0 + lane;
-        for (int r = wave; r < a.ty; r += 4) {
-            const int64_t y = y0 + r;
-            if (y >= a.ny || x >= a.nx) {
-                continue;
-            }
-            Key k[LIVE];
-            const Key *window = support + r * a.sx + lane;
-#pragma unroll
-            for (int i = 0; i < LIVE; ++i) {
-                k[i] = i < a.count ? window[taps[i]] : KT::SENTINEL;
-            }
-            sort_keys<Key, LIVE>(k, std::make_index_sequence<NETWORK<LIVE>.size()>{});
-            int n = 0;
-#pragma unroll
-            for (int i = 0; i < LIVE; ++i) {
-                n += k[i] != KT::SENTINEL;
-            }
-            Key out = KT::CANONICAL_NAN;
-            if (n >= a.min_count && n > 0) {
-                int lo_index, hi_index;
-                if (a.stat == STAT_RANK) {
-                    lo_index = hi_index = a.rank_param;
-                } else {
-                    hi_index = n >> 1;
-                    lo_index = (n & 1) ? hi_index : hi_index - 1;
-                }
-                if (hi_index < n) {
-                    Key lo = 0, hi = 0;
-#pragma unroll
-                    for (int i = 0; i < LIVE; ++i) {
-                        lo = i == lo_index ? k[i] : lo;
-                        hi = i == hi_index ? k[i] : hi;
-                    }
-                    out = lo_index == hi_index ? KT::unkey(hi) : median_pair_bits<T>(lo, hi);
-                }
-            }
-            output[(z0 * a.ny + y) * a.nx + x] = out;
+    int rest = tile;
+    const int tx = rest % a.tiles_x;
+    rest /= a.tiles_x;
+    const int tyi = rest % a.tiles_y;
+    const int64_t z0 = rest / a.tiles_y;
+    const int64_t x0 = (int64_t)tx * 64;
+    const int64_t y0 = (int64_t)tyi * a.ty;
+    __syncthreads();  // the previous tile's reads of `support` are done
+    for (int row = wave; row < a.rows; row += 4) {
+        const int z = row / a.sy;
+        const int y = row - z * a.sy;
+        const int64_t mz = a.mapz[z0 + z];
+        const int64_t my = a.mapy[y0 + y];
+        for (int x = lane; x < a.sx; x += 64) {
+            const int64_t mx = a.mapx[x0 + x];
+            const bool pad = mz == MAP_PAD || my == MAP_PAD || mx == MAP_PAD;
+            const bool cval = !pad && (mz < 0 || my < 0 || mx < 0);
+            const Key raw = input[pad || cval ? 0 : mz + my + mx];
+            support[row * a.sx + x] = pad ? KT::SENTINEL : (cval ? cval_key : classify<T>(raw, a.ignore_inf));
         }
     }
+    __syncthreads();
+    const int64_t x = x0 + lane;
+    for (int r = wave; r < a.ty; r += 4) {
+        const int64_t y = y0 + r;
+        if (y >= a.ny || x >= a.nx) {
+            continue;
+        }
+        Key k[LIVE];
+        const Key *window = support + r * a.sx + lane;
+#pragma unroll
+        for (int i = 0; i < LIVE; ++i) {
+            k[i] = i < a.count ? window[taps[i]] : KT::SENTINEL;
+        }
+        sort_keys<Key, LIVE>(k, std::make_index_sequence<NETWORK<LIVE>.size()>{});
+        int n = 0;
+#pragma unroll
+        for (int i = 0; i < LIVE; ++i) {
+            n += k[i] != KT::SENTINEL;
+        }
+        Key out = KT::CANONICAL_NAN;
+        if (n >= a.min_count && n > 0) {
+            int lo_index, hi_index;
+            if (a.stat == STAT_RANK) {
+                lo_index = hi_index = a.rank_param;
+            } else {
+                hi_index = n >> 1;
+                lo_index = (n & 1) ? hi_index : hi_index - 1;
+            }
+            if (hi_index < n) {
+                Key lo = 0, hi = 0;
+#pragma unroll
+                for (int i = 0; i < LIVE; ++i) {
+                    lo = i == lo_index ? k[i] : lo;
+                    hi = i == hi_index ? k[i] : hi;
+                }
+                out = lo_index == hi_index ? KT::unkey(hi) : median_pair_bits<T>(lo, hi);
+            }
+        }
+        output[(z0 * a.ny + y) * a.nx + x] = out;
+    }
+}
+
+// One image: workgroups stride over the tiles.
+template <typename T, int LIVE>
+__global__ void __launch_bounds__(NETWORK_THREADS) k_network(NetworkArgs a)
+{
+    using Key = typename KeyTraits<T>::Key;
+    extern __shared__ __align__(16) unsigned char lds[];
+    Key *support = reinterpret_cast<Key *>(lds);
+    uint32_t *taps = reinterpret_cast<uint32_t *>(support + a.s);
+    for (int t = threadIdx.x; t < a.count; t += NETWORK_THREADS) {
+        taps[t] = a.taps[t];
+    }
+    const int tiles = a.tiles_x * a.tiles_y * (int)a.nz;
+    for (int tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+        network_tile<T, LIVE>(a, tile, support, taps);
+    }
+}
+
+// A batch of images with windows of at most LIVE taps: (image, tile) pairs
+// from a work queue; taps are read from global memory (uniform loads).
+template <typename T, int LIVE>
+__global__ void __launch_bounds__(NETWORK_THREADS) k_network_batch(const NetworkArgs *images, const int64_t *offsets,
+                                                                   const int32_t *work, int nwork, int32_t *counter,
+                                                                   const void *input, void *output)
+{
+    using Key = typename KeyTraits<T>::Key;
+    extern __shared__ __align__(16) unsigned char lds[];
+    __shared__ int next_shared;
+    Key *support = reinterpret_cast<Key *>(lds);
+    for (;;) {
+        if (threadIdx.x == 0) {
+            next_shared = atomicAdd(counter, 1);
+        }
+        __syncthreads();
+        // Uniform across the workgroup: keeps the descriptor loads scalar.
+        const int w = __builtin_amdgcn_readfirstlane(next_shared);
+        __syncthreads();
+        if (w >= nwork) {
+            break;
+        }
+        const int image = work[2 * w];
+        NetworkArgs a = images[image];
+        a.input = static_cast<const Key *>(input) + offsets[image];
+        a.output = static_cast<Key *>(output) + offsets[image];
+        network_tile<T, LIVE>(a, work[2 * w + 1], support, a.taps);
+    }
+}
+
+template <typename T, int LIVE>
+hipError_t launch_batch_live(const NetworkArgs *images, const int64_t *offsets, const int32_t *work, int nwork,
+                             int32_t *counter, const void *input, void *output, size_t lds_bytes, int grid,
+                             hipStream_t stream)
+{
+    hipLaunchKernelGGL((k_network_batch<T, LIVE>), dim3((unsigned)grid), dim3(NETWORK_THREADS), lds_bytes, stream,
+                       images, offsets, work, nwork, counter, input, output);
+    return hipGetLastError();
+}
+
+template <typename T, int LIVE>
+hipError_t batch_grid_live(size_t lds_bytes, int *grid)
+{
+    const void *fn = reinterpret_cast<const void *>(&k_network_batch<T, LIVE>);
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+    if (e != hipSuccess) {
+        return e;
+    }
+    int per_cu = 0, device = 0, cus = 0;
+    if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, NETWORK_THREADS, lds_bytes)) != hipSuccess ||
+        (e = hipGetDevice(&device)) != hipSuccess ||
+        (e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device)) != hipSuccess) {
+        return e;
+    }
+    *grid = (per_cu < 1 ? 1 : per_cu) * (cus < 1 ? 1 : cus);
+    return hipSuccess;
 }
 
 template <typename T, int LIVE>
@@ -224,6 +291,52 @@ int network_live(int dtype_bytes, int count)
 size_t network_lds_bytes(int dtype_bytes, int s, int count)
 {
     return (size_t)s * (size_t)dtype_bytes + (size_t)count * 4;
+}
+
+hipError_t network_batch_grid(int dtype_bytes, int live, size_t lds_bytes, int *grid)
+{
+#define NDF_NETWORK_GRID(LIVE)                                                                     \
+    case LIVE:                                                                                     \
+        return dtype_bytes == 4 ? batch_grid_live<float, LIVE>(lds_bytes, grid)                    \
+                                : batch_grid_live<double, LIVE>(lds_bytes, grid);
+    switch (live) {
+        NDF_NETWORK_GRID(9)
+        NDF_NETWORK_GRID(16)
+        NDF_NETWORK_GRID(25)
+        NDF_NETWORK_GRID(32)
+        NDF_NETWORK_GRID(49)
+        NDF_NETWORK_GRID(64)
+    case 96:
+        return dtype_bytes == 4 ? batch_grid_live<float, 96>(lds_bytes, grid) : hipErrorInvalidValue;
+    default: return hipErrorInvalidValue;
+    }
+#undef NDF_NETWORK_GRID
+}
+
+hipError_t launch_network_batch(int dtype_bytes, int live, const NetworkArgs *images, const int64_t *offsets,
+                                const int32_t *work, int nwork, int32_t *counter, const void *input, void *output,
+                                size_t lds_bytes, int grid, hipStream_t stream)
+{
+#define NDF_NETWORK_BATCH(LIVE)                                                                                  \
+    case LIVE:                                                                                                   \
+        return dtype_bytes == 4 ? launch_batch_live<float, LIVE>(images, offsets, work, nwork, counter, input, \
+                                                                 output, lds_bytes, grid, stream)               \
+                                : launch_batch_live<double, LIVE>(images, offsets, work, nwork, counter, input, \
+                                                                  output, lds_bytes, grid, stream);
+    switch (live) {
+        NDF_NETWORK_BATCH(9)
+        NDF_NETWORK_BATCH(16)
+        NDF_NETWORK_BATCH(25)
+        NDF_NETWORK_BATCH(32)
+        NDF_NETWORK_BATCH(49)
+        NDF_NETWORK_BATCH(64)
+    case 96:
+        return dtype_bytes == 4 ? launch_batch_live<float, 96>(images, offsets, work, nwork, counter, input, output,
+                                                               lds_bytes, grid, stream)
+                                : hipErrorInvalidValue;
+    default: return hipErrorInvalidValue;
+    }
+#undef NDF_NETWORK_BATCH
 }
 
 hipError_t launch_network(int dtype_bytes, const NetworkArgs &args, size_t lds_bytes, hipStream_t stream)

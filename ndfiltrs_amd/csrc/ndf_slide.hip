@@ -739,7 +739,8 @@ __global__ void __launch_bounds__(NT, 1) k_slide_batch(const SlideArgs *images, 
             next_shared = atomicAdd(counter, 1);
         }
         __syncthreads();
-        const int w = next_shared;
+        // Uniform across the workgroup: keeps the descriptor loads scalar.
+        const int w = __builtin_amdgcn_readfirstlane(next_shared);
         __syncthreads();
         if (w >= nwork) {
             break;
