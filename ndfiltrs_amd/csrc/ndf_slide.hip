@@ -362,14 +362,35 @@ struct Chunk {
     }
 };
 
+// Exchange layout of the bitonic sort's cross-thread stages. For E >= 8 a
+// thread's chunk is stored as 16-byte blocks (keys: 16 / sizeof(Key) per
+// block, positions: 8 u16 per block) at row t, with the block index XOR-
+// swizzled by the thread index so that the 8 lanes of one 128-byte LDS pass
+// hit 8 distinct bank groups; every access is an aligned b128. Small chunks
+// (E = 4) use rows of E + 1 scalar entries.
+template <typename Key, int E>
+struct SortLayout {
+    static constexpr bool VEC = E >= 8;
+    static constexpr int KPB = 16 / (int)sizeof(Key);  // keys per block
+    static constexpr int KB = E / KPB;                  // key blocks per row
+    static constexpr int PB = E / 8;                    // position blocks per row
+    static constexpr int log2c(int v) { return v <= 1 ? 0 : 1 + log2c(v / 2); }
+    static constexpr int KSHIFT = KB >= 8 ? 0 : log2c(8 / KB);
+    static constexpr int KMASK = (KB >= 8 ? 8 : KB) - 1;
+    static constexpr int PSHIFT = PB >= 8 ? 0 : log2c(8 / (PB > 0 ? PB : 1));
+    static constexpr int PMASK = (PB >= 8 ? 8 : (PB > 0 ? PB : 1)) - 1;
+    __device__ static int key_block(int t, int b) { return t * KB + (b ^ ((t >> KSHIFT) & KMASK)); }
+    __device__ static int pos_block(int t, int b) { return t * PB + (b ^ ((t >> PSHIFT) & PMASK)); }
+};
+
 // Bitonic sort of NT * E (key, pos) pairs; thread t holds elements
 // [t*E, t*E + E) in registers. Strides below E run in registers; larger
-// strides exchange whole chunks with the partner thread through LDS
-// (xk/xp, rows of E + 1 entries so a chunk-wide access is bank-conflict
-// free).
+// strides exchange whole chunks with the partner thread through LDS (xk:
+// NT * E keys, xp: NT * E u16 positions, both 16-byte aligned).
 template <typename Key, int E>
 __device__ void block_sort(Chunk<Key, E> &c, Key *xk, uint16_t *xp, int t)
 {
+    using L = SortLayout<Key, E>;
     constexpr int N = NT * E;
     const int tbase = t * E;
     c.template local_sort<2>(tbase);
@@ -378,30 +399,99 @@ __device__ void block_sort(Chunk<Key, E> &c, Key *xk, uint16_t *xp, int t)
         for (int stride = size >> 1; stride >= E; stride >>= 1) {
             const int partner = t ^ (stride / E);
             const bool keep_min = ((t & (stride / E)) == 0) == asc;
-            static_for<0, E>([&](auto ic) {
-                constexpr int I = decltype(ic)::value;
-                xk[t * (E + 1) + I] = c.k[I];
-                xp[t * (E + 1) + I] = (uint16_t)c.p[I];
-            });
-            __syncthreads();
-            static_for<0, E>([&](auto ic) {
-                constexpr int I = decltype(ic)::value;
-                const Key other = xk[partner * (E + 1) + I];
-                const uint32_t op = xp[partner * (E + 1) + I];
-                const bool take = keep_min ? (other < c.k[I]) : (c.k[I] < other);
-                c.k[I] = take ? other : c.k[I];
-                c.p[I] = take ? op : c.p[I];
-            });
+            if constexpr (L::VEC) {
+                uint4 *kb = reinterpret_cast<uint4 *>(xk);
+                uint4 *pb = reinterpret_cast<uint4 *>(xp);
+                static_for<0, L::KB>([&](auto bc) {
+                    constexpr int B = decltype(bc)::value;
+                    uint4 v;
+                    if constexpr (sizeof(Key) == 4) {
+                        v = make_uint4((uint32_t)c.k[4 * B], (uint32_t)c.k[4 * B + 1], (uint32_t)c.k[4 * B + 2],
+                                       (uint32_t)c.k[4 * B + 3]);
+                    } else {
+                        v = make_uint4((uint32_t)c.k[2 * B], (uint32_t)((uint64_t)c.k[2 * B] >> 32),
+                                       (uint32_t)c.k[2 * B + 1], (uint32_t)((uint64_t)c.k[2 * B + 1] >> 32));
+                    }
+                    kb[L::key_block(t, B)] = v;
+                });
+                static_for<0, L::PB>([&](auto bc) {
+                    constexpr int B = decltype(bc)::value;
+                    pb[L::pos_block(t, B)] =
+                        make_uint4(c.p[8 * B] | (c.p[8 * B + 1] << 16), c.p[8 * B + 2] | (c.p[8 * B + 3] << 16),
+                                   c.p[8 * B + 4] | (c.p[8 * B + 5] << 16), c.p[8 * B + 6] | (c.p[8 * B + 7] << 16));
+                });
+                __syncthreads();
+                // Eight elements at a time: one position block and the key
+                // blocks holding the same elements, then compare-and-take.
+                static_for<0, L::PB>([&](auto bc) {
+                    constexpr int B = decltype(bc)::value;
+                    const uint4 pv = pb[L::pos_block(partner, B)];
+                    const uint32_t w[4] = {pv.x, pv.y, pv.z, pv.w};
+                    Key other[8];
+                    static_for<0, 8 / L::KPB>([&](auto qc) {
+                        constexpr int Q = decltype(qc)::value;
+                        const uint4 v = kb[L::key_block(partner, B * (8 / L::KPB) + Q)];
+                        if constexpr (sizeof(Key) == 4) {
+                            other[4 * Q] = v.x;
+                            other[4 * Q + 1] = v.y;
+                            other[4 * Q + 2] = v.z;
+                            other[4 * Q + 3] = v.w;
+                        } else {
+                            other[2 * Q] = (Key)v.x | ((Key)v.y << 32);
+                            other[2 * Q + 1] = (Key)v.z | ((Key)v.w << 32);
+                        }
+                    });
+                    static_for<0, 8>([&](auto ic) {
+                        constexpr int J = decltype(ic)::value;
+                        constexpr int I = 8 * B + J;
+                        const uint32_t op = (J & 1) ? (w[J / 2] >> 16) : (w[J / 2] & 0xFFFFu);
+                        const bool take = keep_min ? (other[J] < c.k[I]) : (c.k[I] < other[J]);
+                        c.k[I] = take ? other[J] : c.k[I];
+                        c.p[I] = take ? op : c.p[I];
+                    });
+                });
+            } else {
+                static_for<0, E>([&](auto ic) {
+                    constexpr int I = decltype(ic)::value;
+                    xk[t * (E + 1) + I] = c.k[I];
+                    xp[t * (E + 1) + I] = (uint16_t)c.p[I];
+                });
+                __syncthreads();
+                static_for<0, E>([&](auto ic) {
+                    constexpr int I = decltype(ic)::value;
+                    const Key other = xk[partner * (E + 1) + I];
+                    const uint32_t op = xp[partner * (E + 1) + I];
+                    const bool take = keep_min ? (other < c.k[I]) : (c.k[I] < other);
+                    c.k[I] = take ? other : c.k[I];
+                    c.p[I] = take ? op : c.p[I];
+                });
+            }
             __syncthreads();
         }
         c.merge_local(asc);
     }
 }
 
+// Bytes of the exchange region: the sort's exchange (xk then xp) and, after
+// the sort, keys[N] | guard | pos[N] | guard. 16-byte aligned.
 template <typename Key, int E>
 constexpr size_t exchange_bytes()
 {
-    return ((size_t)NT * (E + 1) * (sizeof(Key) + 2) + 15) & ~(size_t)15;
+    const size_t rows = SortLayout<Key, E>::VEC ? (size_t)NT * E * (sizeof(Key) + 2)
+                                                : (size_t)NT * (E + 1) * (sizeof(Key) + 2);
+    const size_t sorted = (size_t)NT * E * (sizeof(Key) + 2) + 4 * GUARD;
+    return ((rows > sorted ? rows : sorted) + 15) & ~(size_t)15;
+}
+
+static_assert(exchange_bytes<uint32_t, 64>() == (size_t)NT * 64 * 6 + 4 * GUARD, "exchange layout");
+static_assert(exchange_bytes<uint64_t, 4>() == (((size_t)NT * 5 * 10 + 15) & ~(size_t)15), "exchange layout");
+
+// Start of the position half of the exchange (16-byte aligned).
+template <typename Key, int E>
+__device__ __forceinline__ uint16_t *exchange_positions(Key *xk)
+{
+    return SortLayout<Key, E>::VEC ? reinterpret_cast<uint16_t *>(xk + NT * E)
+                                   : reinterpret_cast<uint16_t *>(xk + NT * (E + 1));
 }
 
 // One tile of the plain kernel: stage, sort, rank and slide. Every retained
@@ -418,7 +508,7 @@ __device__ __forceinline__ bool slide_tile(const SlideArgs &a, int tile, unsigne
     // Layout: [exchange rows NT x (E + 1) keys + u16 positions; after the
     // sort: keys[N] | guard | pos[N] | guard] | codes [s] | hist | mask.
     Key *xk = reinterpret_cast<Key *>(lds);
-    uint16_t *xp = reinterpret_cast<uint16_t *>(xk + NT * (E + 1));
+    uint16_t *xp = exchange_positions<Key, E>(xk);
     Key *keys = reinterpret_cast<Key *>(lds);
     uint16_t *pos0 = reinterpret_cast<uint16_t *>(keys + N) + GUARD;
     uint16_t *pos1 = pos0 - 1;  // 1-based ranks
@@ -827,7 +917,7 @@ __global__ void __launch_bounds__(NT, E >= 32 ? 1 : (sizeof(T) == 8 && E >= 16 ?
     constexpr int R = 4;  // support rows a wave loads per batch
     extern __shared__ __align__(16) unsigned char lds[];
     Key *xk = reinterpret_cast<Key *>(lds);
-    uint16_t *xp = reinterpret_cast<uint16_t *>(xk + NT * (E + 1));
+    uint16_t *xp = exchange_positions<Key, E>(xk);
     Key *keys = reinterpret_cast<Key *>(lds);
     uint16_t *codes = reinterpret_cast<uint16_t *>(lds + exchange_bytes<Key, E>());
     RowMasks *rowmasks = reinterpret_cast<RowMasks *>(codes + ((a.s + 7) & ~7));
@@ -1224,19 +1314,25 @@ hipError_t launch_slide_batch(int dtype_bytes, int sort_e, const SlideArgs *imag
 #undef NDF_BATCH_LAUNCH
 }
 
+// Host-side twin of exchange_bytes<Key, E>().
+static size_t exchange_region(int dtype_bytes, int sort_e)
+{
+    const size_t per = (size_t)dtype_bytes + 2;
+    const size_t rows = sort_e >= 8 ? (size_t)NT * sort_e * per : (size_t)NT * (sort_e + 1) * per;
+    const size_t sorted = (size_t)NT * sort_e * per + 4 * GUARD;
+    return ((rows > sorted ? rows : sorted) + 15) & ~(size_t)15;
+}
+
 size_t slide_lds_bytes(int dtype_bytes, int s, int sort_e, int hist_stride, int mask_bytes)
 {
-    // Exchange rows (NT x (E + 1) keys and u16 positions, 16-byte aligned);
-    // after the sort the same region holds keys[N], pos[N] and the guards.
-    const size_t exchange = (((size_t)NT * (size_t)(sort_e + 1) * (size_t)(dtype_bytes + 2)) + 15) & ~(size_t)15;
-    return exchange + (size_t)((s + 7) & ~7) * 2 + (size_t)SLIDE_WAVES * (size_t)hist_stride * 2 +
-           (size_t)mask_bytes;
+    // Exchange region, then codes[s], the per-wave histograms and the mask.
+    return exchange_region(dtype_bytes, sort_e) + (size_t)((s + 7) & ~7) * 2 +
+           (size_t)SLIDE_WAVES * (size_t)hist_stride * 2 + (size_t)mask_bytes;
 }
 
 size_t bracket_lds_bytes(int dtype_bytes, int s, int rows, int sort_e)
 {
-    const size_t exchange = (((size_t)NT * (size_t)(sort_e + 1) * (size_t)(dtype_bytes + 2)) + 15) & ~(size_t)15;
-    return exchange + (size_t)((s + 7) & ~7) * 2 + (size_t)rows * sizeof(RowMasks);
+    return exchange_region(dtype_bytes, sort_e) + (size_t)((s + 7) & ~7) * 2 + (size_t)rows * sizeof(RowMasks);
 }
 
 hipError_t launch_bracket_slide(int dtype_bytes, const SlideArgs &args, size_t lds_bytes, int max_tiles,
