@@ -287,6 +287,27 @@ __device__ __forceinline__ void static_for(F &&f)
     }
 }
 
+// Value of lane ^ MASK (MASK < 64) within the wave: ds_swizzle in bit mode
+// for masks below 32, a lane permute for 32.
+template <int MASK>
+__device__ __forceinline__ uint32_t xor_lane(uint32_t v)
+{
+    if constexpr (MASK < 32) {
+        // Swizzle offset: and_mask = 0x1F, or_mask = 0, xor_mask = MASK.
+        return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, (MASK << 10) | 0x1F);
+    } else {
+        return (uint32_t)__builtin_amdgcn_ds_bpermute((int)((threadIdx.x ^ MASK) << 2), (int)v);
+    }
+}
+
+template <int MASK>
+__device__ __forceinline__ uint64_t xor_lane(uint64_t v)
+{
+    const uint32_t lo = xor_lane<MASK>((uint32_t)v);
+    const uint32_t hi = xor_lane<MASK>((uint32_t)(v >> 32));
+    return (uint64_t)lo | ((uint64_t)hi << 32);
+}
+
 template <typename Key, int E>
 struct Chunk {
     Key k[E];
@@ -340,6 +361,28 @@ struct Chunk {
         }
     }
 
+    // Compare-exchange of the whole chunk with the chunk of lane ^ MASK (same
+    // wave): keep_min keeps the smaller of each pair of elements.
+    template <int MASK>
+    __device__ __forceinline__ void exchange_xor(bool keep_min)
+    {
+        static_for<0, E / 2>([&](auto ic) {
+            constexpr int I = 2 * decltype(ic)::value;
+            const uint32_t packed = xor_lane<MASK>((uint32_t)(p[I] | (p[I + 1] << 16)));
+            const Key o0 = xor_lane<MASK>(k[I]);
+            const Key o1 = xor_lane<MASK>(k[I + 1]);
+            const bool t0 = keep_min ? (o0 < k[I]) : (k[I] < o0);
+            const bool t1 = keep_min ? (o1 < k[I + 1]) : (k[I + 1] < o1);
+            k[I] = t0 ? o0 : k[I];
+            p[I] = t0 ? (packed & 0xFFFFu) : p[I];
+            k[I + 1] = t1 ? o1 : k[I + 1];
+            p[I + 1] = t1 ? (packed >> 16) : p[I + 1];
+            if constexpr (I % 8 == 6) {
+                __builtin_amdgcn_sched_barrier(0);  // bounds the swizzles in flight
+            }
+        });
+    }
+
     // Strides E/2 .. 1 of a merge level larger than the chunk: one direction
     // for the whole chunk.
     __device__ __forceinline__ void merge_local(bool asc)
@@ -388,7 +431,7 @@ struct SortLayout {
 // strides exchange whole chunks with the partner thread through LDS (xk:
 // NT * E keys, xp: NT * E u16 positions, both 16-byte aligned).
 template <typename Key, int E>
-__device__ void block_sort(Chunk<Key, E> &c, Key *xk, uint16_t *xp, int t)
+__device__ __forceinline__ void block_sort(Chunk<Key, E> &c, Key *xk, uint16_t *xp, int t)
 {
     using L = SortLayout<Key, E>;
     constexpr int N = NT * E;
@@ -399,6 +442,19 @@ __device__ void block_sort(Chunk<Key, E> &c, Key *xk, uint16_t *xp, int t)
         for (int stride = size >> 1; stride >= E; stride >>= 1) {
             const int partner = t ^ (stride / E);
             const bool keep_min = ((t & (stride / E)) == 0) == asc;
+            if (stride / E < 64) {
+                // Partner in the same wave: exchange through cross-lane
+                // swizzles, no LDS round trip and no workgroup barrier.
+                switch (stride / E) {
+                case 1: c.template exchange_xor<1>(keep_min); break;
+                case 2: c.template exchange_xor<2>(keep_min); break;
+                case 4: c.template exchange_xor<4>(keep_min); break;
+                case 8: c.template exchange_xor<8>(keep_min); break;
+                case 16: c.template exchange_xor<16>(keep_min); break;
+                default: c.template exchange_xor<32>(keep_min); break;
+                }
+                continue;
+            }
             if constexpr (L::VEC) {
                 uint4 *kb = reinterpret_cast<uint4 *>(xk);
                 uint4 *pb = reinterpret_cast<uint4 *>(xp);
@@ -540,40 +596,70 @@ __device__ __forceinline__ bool slide_tile(const SlideArgs &a, int tile, unsigne
         __syncthreads();
 
         // ---- stage: classify the support, compact the retained samples ----
-        for (int row = wave; row < a.rows; row += SLIDE_WAVES) {
-            const int z = row >> a.zshift;
-            const int y = row & ((1 << a.zshift) - 1);
-            if (y >= a.sy) {
-                continue;  // padding rows between z planes
+        // Each wave takes R support rows per batch and issues all their loads
+        // before classifying; one LDS atomic reserves the batch's slots.
+        {
+            constexpr int R = 4;
+            constexpr int H = 4;  // 64-column chunks of a support row (sx <= 254)
+            int64_t mx[H];
+#pragma unroll
+            for (int h = 0; h < H; ++h) {
+                mx[h] = h * 64 + lane < a.sx ? a.mapx[x0 + h * 64 + lane] : MAP_PAD;
             }
-            const int64_t mz = a.mapz[z0 + z];
-            const int64_t my = a.mapy[y0 + y];
-            for (int xb = 0; xb < a.sx; xb += 64) {
-                const int x = xb + lane;
-                Key key = KT::SENTINEL;
-                if (x < a.sx) {
-                    const int64_t mx = a.mapx[x0 + x];
-                    if (mz == MAP_PAD || my == MAP_PAD || mx == MAP_PAD) {
-                        key = KT::SENTINEL;
-                    } else if (mz < 0 || my < 0 || mx < 0) {
-                        key = cval_key;
-                    } else {
-                        key = classify<T>(input[mz + my + mx], a.ignore_inf);
+            const int chunks = (a.sx + 63) >> 6;
+            for (int row0 = wave * R; row0 < a.rows; row0 += SLIDE_WAVES * R) {
+                Key kv[R][H];
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const int row = row0 + r;
+                    const int z = row >> a.zshift;
+                    const int y = row & ((1 << a.zshift) - 1);
+                    const bool live = row < a.rows && y < a.sy;  // padding rows between z planes are skipped
+                    const int64_t mz = live ? a.mapz[z0 + z] : MAP_PAD;
+                    const int64_t my = live ? a.mapy[y0 + y] : MAP_PAD;
+#pragma unroll
+                    for (int h = 0; h < H; ++h) {
+                        const bool pad = h >= chunks || mz == MAP_PAD || my == MAP_PAD || mx[h] == MAP_PAD;
+                        const bool cval = !pad && (mz < 0 || my < 0 || mx[h] < 0);
+                        const Key raw = input[pad || cval ? 0 : mz + my + mx[h]];
+                        kv[r][h] = pad ? KT::SENTINEL : (cval ? cval_key : classify<T>(raw, a.ignore_inf));
                     }
-                    codes[row * a.sx + x] = key == KT::SENTINEL ? CODE_OMIT : CODE_PENDING;
                 }
-                const bool keep = key != KT::SENTINEL;
-                const uint64_t ballot = __ballot(keep);
+                uint64_t kept[R][H];
+                int total = 0;
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const int row = row0 + r;
+                    const int y = row & ((1 << a.zshift) - 1);
+                    const bool live = row < a.rows && y < a.sy;
+#pragma unroll
+                    for (int h = 0; h < H; ++h) {
+                        const int x = h * 64 + lane;
+                        if (live && h < chunks && x < a.sx) {
+                            codes[row * a.sx + x] = kv[r][h] == KT::SENTINEL ? CODE_OMIT : CODE_PENDING;
+                        }
+                        kept[r][h] = __ballot(kv[r][h] != KT::SENTINEL);
+                        total += (int)__popcll(kept[r][h]);
+                    }
+                }
                 int base = 0;
-                if (lane == 0 && ballot) {
-                    base = atomicAdd(&count_shared, __popcll(ballot));
+                if (lane == 0 && total) {
+                    base = atomicAdd(&count_shared, total);
                 }
                 base = __shfl(base, 0);
-                if (keep) {
-                    const int dst = base + (int)__popcll(ballot & ((1ull << lane) - 1ull));
-                    if (dst < N) {
-                        xk[dst] = key;
-                        xp[dst] = (uint16_t)((row << 8) | x);
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+#pragma unroll
+                    for (int h = 0; h < H; ++h) {
+                        const uint64_t bits = kept[r][h];
+                        if ((bits >> lane) & 1ull) {
+                            const int dst = base + (int)__popcll(bits & ((1ull << lane) - 1ull));
+                            if (dst < N) {
+                                xk[dst] = kv[r][h];
+                                xp[dst] = (uint16_t)(((row0 + r) << 8) | (h * 64 + lane));
+                            }
+                        }
+                        base += (int)__popcll(bits);
                     }
                 }
             }
