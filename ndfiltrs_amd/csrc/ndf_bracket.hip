@@ -98,10 +98,12 @@ __device__ void wave_sort(float *v, int n, int lane)
 //   sigma = median of the block std devs    (robust noise scale)
 //   (gy, gx) = least-squares plane slope of the block means over the blocks
 //           within 0.75 sigma of c (sources excluded), per pixel
-// Features (all scale free): asinh((mean - c) / sigma), asinh(gy * ty / sigma),
+// Features (all scale free): asinh of the standardised excess of extended
+// bright blocks per block (mean above c + sigma / 2, spread below 4 x the
+// shift: sources), asinh(gy * ty / sigma),
 // asinh(gx * 64 / sigma), retained fraction, target quantile, fraction of
-// blocks brighter than c + 0.75 sigma. Targets are standardised as
-// (value - c) / sigma.
+// spiky bright blocks (spread at least 4 x the shift: isolated hot pixels).
+// Targets are standardised as (value - c) / sigma.
 __global__ void __launch_bounds__(64) k_tile_features(BracketArgs b, const double *blocks)
 {
     __shared__ float means[MAX_TILE_BLOCKS];
@@ -130,7 +132,7 @@ __global__ void __launch_bounds__(64) k_tile_features(BracketArgs b, const doubl
     while (n_sort < count) {
         n_sort <<= 1;
     }
-    double n = 0.0, s = 0.0, cells = 0.0;
+    double n = 0.0, cells = 0.0;
     int usable = 0;
     for (int i = lane; i < n_sort; i += 64) {
         float mean = __builtin_huge_valf(), sd = __builtin_huge_valf();
@@ -138,7 +140,6 @@ __global__ void __launch_bounds__(64) k_tile_features(BracketArgs b, const doubl
             const int64_t br = br0 + (int64_t)(i / w_used) * step, bc = bc0 + (int64_t)(i % w_used) * step;
             const double *o = blocks + (br * b.bx + bc) * 3;
             n += o[0];
-            s += o[1];
             cells += 64.0;
             if (o[0] >= 16.0) {
                 const double mu = o[1] / o[0];
@@ -153,7 +154,6 @@ __global__ void __launch_bounds__(64) k_tile_features(BracketArgs b, const doubl
     }
     for (int off = 32; off > 0; off >>= 1) {
         n += __shfl_xor(n, off);
-        s += __shfl_xor(s, off);
         cells += __shfl_xor(cells, off);
         usable += __shfl_xor(usable, off);
     }
@@ -168,13 +168,25 @@ __global__ void __launch_bounds__(64) k_tile_features(BracketArgs b, const doubl
     const float floor_sigma = 1e-6f * (__builtin_fabsf(c) + 1e-30f);
     sigma = sigma > floor_sigma ? sigma : floor_sigma;
     // Plane fit over unsorted block means (re-read from the block table).
-    double sw = 0, sr = 0, sq = 0, sm = 0, srr = 0, sqq = 0, srq = 0, srm = 0, sqm = 0, bright = 0;
+    double sw = 0, sr = 0, sq = 0, sm = 0, srr = 0, sqq = 0, srq = 0, srm = 0, sqm = 0, extended = 0, spiky = 0;
     for (int i = lane; i < count; i += 64) {
         const int64_t br = br0 + (int64_t)(i / w_used) * step, bc = bc0 + (int64_t)(i % w_used) * step;
         const double *o = blocks + (br * b.bx + bc) * 3;
         if (o[0] >= 16.0) {
             const double mu = o[1] / o[0] - (double)c;
-            bright += mu > 0.75 * (double)sigma ? 1.0 : 0.0;
+            // A bright block is "spiky" when its spread dwarfs its mean
+            // shift (one hot pixel: sd ~ 8 x shift) and "extended" otherwise
+            // (a source covering the block); only extended brightness moves
+            // window ranks much.
+            if (mu > 0.5 * (double)sigma) {
+                const double var = o[2] / o[0] - (o[1] / o[0]) * (o[1] / o[0]);
+                const double sd = var > 0.0 ? __builtin_sqrt(var) : 0.0;
+                if (sd < 4.0 * mu) {
+                    extended += mu / (double)sigma;
+                } else {
+                    spiky += 1.0;
+                }
+            }
             if (__builtin_fabs(mu) <= 0.75 * (double)sigma) {
                 const double r = (double)((i / w_used) * step * BLOCK), q = (double)((i % w_used) * step * BLOCK);
                 sw += 1.0;
@@ -199,7 +211,8 @@ __global__ void __launch_bounds__(64) k_tile_features(BracketArgs b, const doubl
         srq += __shfl_xor(srq, off);
         srm += __shfl_xor(srm, off);
         sqm += __shfl_xor(sqm, off);
-        bright += __shfl_xor(bright, off);
+        extended += __shfl_xor(extended, off);
+        spiky += __shfl_xor(spiky, off);
     }
     if (lane == 0) {
         double gy = 0.0, gx = 0.0;
@@ -217,14 +230,13 @@ __global__ void __launch_bounds__(64) k_tile_features(BracketArgs b, const doubl
             }
         }
         const double sg = (double)sigma;
-        const double mean = n > 0.0 ? s / n : (double)c;
         float *f = b.features + (int64_t)tile * FEATURES;
-        f[0] = usable > 0 ? feature_asinh((mean - (double)c) / sg) : 0.0f;
+        f[0] = usable > 0 ? feature_asinh(extended / (double)usable) : 0.0f;
         f[1] = feature_asinh(gy * (double)b.ty / sg);
         f[2] = feature_asinh(gx * 64.0 / sg);
         f[3] = cells > 0.0 ? (float)(n / cells) : 0.0f;
         f[4] = b.quantile;
-        f[5] = usable > 0 ? (float)(bright / (double)usable) : 0.0f;
+        f[5] = usable > 0 ? (float)(spiky / (double)usable) : 0.0f;
         double *m = b.moments + (int64_t)tile * 3;
         m[0] = (double)c;
         m[1] = sg;
