@@ -113,38 +113,25 @@ __device__ inline bool walk_to(int &m, int &c, int k, const uint16_t *pos, int r
                                uint32_t base)
 {
     const int limit = retained / BATCH + 4;
-    if (c > k) {
-        // The (c - k)-th in-window rank below m, counting downward.
-        for (int step = 0; step <= limit; ++step) {
-            const uint32_t mask = batch_mask(pos, m - BATCH, g, base);
-            const int count = __builtin_popcount(mask);
-            const int need = c - k;
-            if (count >= need) {
-                m = m - BATCH + select_bit16(mask, count - need);
-                c = k;
-                return true;
-            }
-            c -= count;
-            m -= BATCH;
-            if (m <= 1) {
-                return false;
-            }
-        }
-        return false;
-    }
-    // The (k - c)-th in-window rank at or above m, counting upward.
+    // Upward (c <= k): the (k - c)-th (0-based) in-window rank at or above m.
+    // Downward (c > k): the (c - k)-th in-window rank below m, counting down.
+    // One loop serves both directions, so a wave whose lanes walk both ways
+    // runs its longest walk, not the sum of the longest walk each way.
+    const bool up = c <= k;
+    int need = up ? k - c : c - k;
+    int first = up ? m : m - BATCH;
+    const int advance = up ? BATCH : -BATCH;
     for (int step = 0; step <= limit; ++step) {
-        const uint32_t mask = batch_mask(pos, m, g, base);
+        const uint32_t mask = batch_mask(pos, first, g, base);
         const int count = __builtin_popcount(mask);
-        const int need = k - c;
-        if (count > need) {
-            m += select_bit16(mask, need);
+        if (up ? count > need : count >= need) {
+            m = first + select_bit16(mask, up ? need : count - need);
             c = k;
             return true;
         }
-        c += count;
-        m += BATCH;
-        if (m > retained) {
+        need -= count;
+        first += advance;
+        if (up ? first > retained : first + BATCH <= 1) {
             return false;
         }
     }
