@@ -88,6 +88,23 @@ __device__ __forceinline__ int select_bit16(uint32_t mask, int need)
 // Membership bits of ranks [first, first + BATCH) (bit i <-> rank first + i).
 // `pos` has BATCH guard entries (POS_NONE) on both sides, and ranks in
 // [retained, N) hold POS_NONE, so first in [-BATCH, N) needs no clamping.
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+
+// Rectangle membership of two packed positions at once with 16-bit lane
+// arithmetic: for each half, dy = row - y and dx = x - lane (both wrap when
+// negative), and d < K holds exactly when min(d, K) - K wraps below zero, so
+// the result's sign bits (15 and 31) are the two membership bits.
+__device__ __forceinline__ uint32_t contains2(u16x2 v, u16x2 vmask, u16x2 y, u16x2 lane, u16x2 ky, u16x2 kx)
+{
+    v &= vmask;
+    const u16x2 dy = (u16x2)(v >> (u16x2)8) - y;
+    const u16x2 dx = (u16x2)(v & (u16x2)0xFF) - lane;
+    const u16x2 iny = __builtin_elementwise_min(dy, ky) - ky;
+    const u16x2 inx = __builtin_elementwise_min(dx, kx) - kx;
+    const u16x2 both = iny & inx;
+    return __builtin_bit_cast(uint32_t, both) & 0x80008000u;
+}
+
 __device__ __forceinline__ uint32_t batch_mask(const uint16_t *pos, int first, const Geometry &g, uint32_t base)
 {
     uint32_t v[BATCH];
@@ -96,9 +113,26 @@ __device__ __forceinline__ uint32_t batch_mask(const uint16_t *pos, int first, c
         v[i] = pos[first + i];
     }
     uint32_t mask = 0;
+    if (g.mask == nullptr) {
+        // Rectangles: two positions per 16-bit-pair operation. Bit 15 of a
+        // pair result lands on bit 2j of `acc`, bit 31 on bit 16 + 2j.
+        const u16x2 vmask = (u16x2)(unsigned short)g.vmask;
+        const u16x2 y = (u16x2)(unsigned short)(base >> 8);
+        const u16x2 lane = (u16x2)(unsigned short)(base & 0xFFu);
+        const u16x2 ky = (u16x2)(unsigned short)g.ky;
+        const u16x2 kx = (u16x2)(unsigned short)g.kx;
+        uint32_t acc = 0;
 #pragma unroll
-    for (int i = 0; i < BATCH; ++i) {
-        mask |= g.contains(v[i], base) << i;
+        for (int j = 0; j < BATCH / 2; ++j) {
+            const u16x2 pair = {(unsigned short)v[2 * j], (unsigned short)v[2 * j + 1]};
+            acc |= contains2(pair, vmask, y, lane, ky, kx) >> (15 - 2 * j);
+        }
+        mask = (acc & 0x5555u) | ((acc >> 15) & 0xAAAAu);
+    } else {
+#pragma unroll
+        for (int i = 0; i < BATCH; ++i) {
+            mask |= g.contains(v[i], base) << i;
+        }
     }
     return mask;
 }
