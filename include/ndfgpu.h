@@ -50,7 +50,7 @@ enum {
     NDF_KIND_RANK_HISTOGRAM = 4
 };
 
-/* Device engines. AUTO picks by geometry (NETWORK for windows of at most 96
+/* Device engines. AUTO picks by geometry (NETWORK for windows of at most 160
  * taps in float32 or 64 in float64, else SLIDE where the tile can be staged,
  * else GENERIC); the others force one engine and exist so tests can check
  * engines against each other. */

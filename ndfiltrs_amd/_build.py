@@ -5,6 +5,7 @@ from __future__ import annotations
 import os
 import shutil
 import subprocess
+import tempfile
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
@@ -18,7 +19,6 @@ FLAGS = [
     "-O3",
     "-std=c++17",
     "-fPIC",
-    "-shared",
     "-ffp-contract=off",
     "-fno-fast-math",
     "-fno-gpu-flush-denormals-to-zero",
@@ -37,20 +37,41 @@ def hipcc() -> str:
     return path
 
 
-def build(verbose: bool = False) -> str:
-    """Compile every HIP source into ``ndfiltrs_amd/libndfgpu.so``."""
-    tmp = LIBRARY + ".tmp"
-    cmd = [hipcc(), *FLAGS, "-o", tmp, *(os.path.join(CSRC, s) for s in SOURCES)]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    result = subprocess.run(cmd, capture_output=True, text=True)
+def _report(result: subprocess.CompletedProcess) -> None:
     # The host pass does not know the packed-fp32 feature; that note is noise.
     noise = "is not a recognized feature for this target"
     for line in (result.stdout + result.stderr).splitlines():
         if line.strip() and noise not in line:
             print(line, flush=True)
-    if result.returncode != 0:
-        raise RuntimeError(f"hipcc failed with exit code {result.returncode}")
+
+
+def build(verbose: bool = False) -> str:
+    """Compile every HIP source into ``ndfiltrs_amd/libndfgpu.so``: one hipcc
+    per source in parallel, then one link."""
+    tmp = LIBRARY + ".tmp"
+    with tempfile.TemporaryDirectory(prefix="ndfgpu_build_") as objdir:
+        jobs = []
+        for source in SOURCES:
+            obj = os.path.join(objdir, os.path.splitext(source)[0] + ".o")
+            cmd = [hipcc(), *FLAGS, "-c", "-o", obj, os.path.join(CSRC, source)]
+            if verbose:
+                print(" ".join(cmd), flush=True)
+            jobs.append((obj, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)))
+        failed = []
+        for obj, proc in jobs:
+            out, err = proc.communicate()
+            _report(subprocess.CompletedProcess(proc.args, proc.returncode, out, err))
+            if proc.returncode != 0:
+                failed.append((os.path.basename(obj), proc.returncode))
+        if failed:
+            raise RuntimeError(f"hipcc failed: {failed}")
+        cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *(obj for obj, _ in jobs)]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        result = subprocess.run(cmd, capture_output=True, text=True)
+        _report(result)
+        if result.returncode != 0:
+            raise RuntimeError(f"hipcc link failed with exit code {result.returncode}")
     os.replace(tmp, LIBRARY)
     return LIBRARY
 

@@ -919,7 +919,7 @@ ndf_status ndf_plan_set_engine(ndf_plan *plan, int32_t engine, char *err, size_t
         chosen = NDF_ENGINE_SLIDE;
     } else if (engine == NDF_ENGINE_NETWORK) {
         if (!plan->network_ok) {
-            return fail(NDF_ERR_INVALID, "the network engine needs a window of at most 96 taps (64 for float64)",
+            return fail(NDF_ERR_INVALID, "the network engine needs a window of at most 160 taps (64 for float64)",
                         err, err_len);
         }
         chosen = NDF_ENGINE_NETWORK;

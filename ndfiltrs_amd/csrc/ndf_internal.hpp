@@ -83,7 +83,7 @@ struct SlideArgs {
     int32_t *fallback_count;
 };
 
-// Engine 3: small windows (count <= 96 for f32, 64 for f64). A tile of 64
+// Engine 3: small windows (count <= 160 for f32, 64 for f64). A tile of 64
 // columns x ty rows of one plane; its support (kz planes of sy = ty + ky - 1
 // rows of sx = 64 + kx - 1 samples) is staged in LDS as classified keys; each
 // thread sorts its window in registers with a sorting network of `live`

@@ -1,4 +1,4 @@
-// Network engine: small windows (up to 96 taps), one output per thread, the
+// Network engine: small windows (up to 160 taps), one output per thread, the
 // whole window sorted in registers by a compile-time sorting network.
 //
 // The reference's median_network (src/median_network.rs:65-209) applies a
@@ -84,18 +84,33 @@ constexpr int wires_for(int live)
 template <int LIVE>
 constexpr auto NETWORK = batcher_pairs<wires_for(LIVE), LIVE>();
 
-template <typename Key, int LIVE, size_t... I>
-__device__ __forceinline__ void sort_keys(Key (&k)[LIVE], std::index_sequence<I...>)
+// The comparators are applied in chunks of at most 1024 so that no single
+// fold expression exceeds the compiler's nesting limit.
+constexpr size_t NETWORK_CHUNK = 1024;
+
+template <typename Key, int LIVE, size_t BASE, size_t... I>
+__device__ __forceinline__ void sort_chunk(Key (&k)[LIVE], std::index_sequence<I...>)
 {
     (
         [&] {
-            constexpr int a = NETWORK<LIVE>[I].lo;
-            constexpr int b = NETWORK<LIVE>[I].hi;
+            constexpr int a = NETWORK<LIVE>[BASE + I].lo;
+            constexpr int b = NETWORK<LIVE>[BASE + I].hi;
             const Key x = k[a], y = k[b];
             k[a] = x < y ? x : y;
             k[b] = x < y ? y : x;
         }(),
         ...);
+}
+
+template <typename Key, int LIVE, size_t BASE = 0>
+__device__ __forceinline__ void sort_keys(Key (&k)[LIVE])
+{
+    constexpr size_t total = NETWORK<LIVE>.size();
+    if constexpr (BASE < total) {
+        constexpr size_t len = total - BASE < NETWORK_CHUNK ? total - BASE : NETWORK_CHUNK;
+        sort_chunk<Key, LIVE, BASE>(k, std::make_index_sequence<len>{});
+        sort_keys<Key, LIVE, BASE + NETWORK_CHUNK>(k);
+    }
 }
 
 // One tile: stage the support (classified keys) into LDS, then one output
@@ -146,7 +161,7 @@ __device__ __forceinline__ void network_tile(const NetworkArgs &a, int tile,
         for (int i = 0; i < LIVE; ++i) {
             k[i] = i < a.count ? window[taps[i]] : KT::SENTINEL;
         }
-        sort_keys<Key, LIVE>(k, std::make_index_sequence<NETWORK<LIVE>.size()>{});
+        sort_keys<Key, LIVE>(k);
         int n = 0;
 #pragma unroll
         for (int i = 0; i < LIVE; ++i) {
@@ -276,11 +291,11 @@ hipError_t launch_live(const NetworkArgs &args, size_t lds_bytes, hipStream_t st
 
 int network_live(int dtype_bytes, int count)
 {
-    const int limit = dtype_bytes == 4 ? 96 : 64;
+    const int limit = dtype_bytes == 4 ? 160 : 64;
     if (count > limit) {
         return 0;
     }
-    for (int live : {9, 16, 25, 32, 49, 64, 96}) {
+    for (int live : {9, 16, 25, 32, 49, 64, 96, 128, 160}) {
         if (count <= live) {
             return live;
         }
@@ -308,6 +323,10 @@ hipError_t network_batch_grid(int dtype_bytes, int live, size_t lds_bytes, int *
         NDF_NETWORK_GRID(64)
     case 96:
         return dtype_bytes == 4 ? batch_grid_live<float, 96>(lds_bytes, grid) : hipErrorInvalidValue;
+    case 128:
+        return dtype_bytes == 4 ? batch_grid_live<float, 128>(lds_bytes, grid) : hipErrorInvalidValue;
+    case 160:
+        return dtype_bytes == 4 ? batch_grid_live<float, 160>(lds_bytes, grid) : hipErrorInvalidValue;
     default: return hipErrorInvalidValue;
     }
 #undef NDF_NETWORK_GRID
@@ -334,6 +353,14 @@ hipError_t launch_network_batch(int dtype_bytes, int live, const NetworkArgs *im
         return dtype_bytes == 4 ? launch_batch_live<float, 96>(images, offsets, work, nwork, counter, input, output,
                                                                lds_bytes, grid, stream)
                                 : hipErrorInvalidValue;
+    case 128:
+        return dtype_bytes == 4 ? launch_batch_live<float, 128>(images, offsets, work, nwork, counter, input, output,
+                                                                lds_bytes, grid, stream)
+                                : hipErrorInvalidValue;
+    case 160:
+        return dtype_bytes == 4 ? launch_batch_live<float, 160>(images, offsets, work, nwork, counter, input, output,
+                                                                lds_bytes, grid, stream)
+                                : hipErrorInvalidValue;
     default: return hipErrorInvalidValue;
     }
 #undef NDF_NETWORK_BATCH
@@ -354,6 +381,10 @@ hipError_t launch_network(int dtype_bytes, const NetworkArgs &args, size_t lds_b
         NDF_NETWORK_CASE(64)
     case 96:
         return dtype_bytes == 4 ? launch_live<float, 96>(args, lds_bytes, stream) : hipErrorInvalidValue;
+    case 128:
+        return dtype_bytes == 4 ? launch_live<float, 128>(args, lds_bytes, stream) : hipErrorInvalidValue;
+    case 160:
+        return dtype_bytes == 4 ? launch_live<float, 160>(args, lds_bytes, stream) : hipErrorInvalidValue;
     default: return hipErrorInvalidValue;
     }
 #undef NDF_NETWORK_CASE

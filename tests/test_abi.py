@@ -66,7 +66,7 @@ def test_engine_choice_for_the_benchmark_shapes() -> None:
     c3 = _plan(np.float32, (8192, 8192), (13, 13), footprint=ring, origin=[1, 0], mode=["wrap"] * 2)
     assert c3["engine"] == "network" and c3["count"] == 68 and c3["network_wires"] == 96
     cube = _plan(np.float32, (128, 1024, 1024), (3, 7, 7), mode=["wrap"] * 3)
-    assert cube["engine"] == "slide" and cube["count"] == 147
+    assert cube["engine"] == "network" and cube["count"] == 147 and cube["network_wires"] == 160
     five = _plan(np.float64, (2, 2, 2, 2, 3), (3, 3, 3, 3, 5))
     assert five["engine"] == "generic"
     fill = _plan(np.float32, (10, 10), (3, 3), kind="rank", parameter=9)
@@ -91,7 +91,7 @@ def test_engines_can_be_forced(engine: str) -> None:
 
 
 def test_network_engine_limits() -> None:
-    big = _core.Plan(np.float32, (40, 40), "median", (11, 11), None, (0, 0), ("reflect",) * 2, 0.0, True, False, 1,
+    big = _core.Plan(np.float32, (40, 40), "median", (13, 13), None, (0, 0), ("reflect",) * 2, 0.0, True, False, 1,
                      0)
     assert json.loads(big.describe())["engine"] == "slide"
     with pytest.raises(ValueError, match="network engine needs"):
@@ -100,6 +100,9 @@ def test_network_engine_limits() -> None:
     assert json.loads(f64.describe())["engine"] == "slide"  # 81 taps > 64 for float64
     f32 = _core.Plan(np.float32, (40, 40), "median", (9, 9), None, (0, 0), ("reflect",) * 2, 0.0, True, False, 1, 0)
     assert json.loads(f32.describe())["network_wires"] == 96
+    for size, wires in (((11, 11), 128), ((12, 12), 160)):
+        plan = _core.Plan(np.float32, (40, 40), "median", size, None, (0, 0), ("reflect",) * 2, 0.0, True, False, 1, 0)
+        assert json.loads(plan.describe())["network_wires"] == wires
 
 
 def test_bracket_validation_without_a_device() -> None:

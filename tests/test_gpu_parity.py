@@ -115,6 +115,8 @@ GEOMETRIES = [
     ((1, 90), (4, 7), (1, 0)),
     ((9, 40, 37), (3, 2, 5), (-1, 0, 2)),
     ((9, 40, 37), (12, 1, 4), (5, 0, -2)),
+    ((70, 66), (11, 11), (2, -3)),
+    ((9, 40, 37), (3, 7, 7), (1, 0, -2)),
 ]
 
 
