@@ -335,17 +335,33 @@ struct Chunk {
     uint32_t p[E];
 
     // Compare-exchange of local elements I < J: ascending puts the smaller
-    // key at I. Equal keys are never swapped.
+    // key at I. A descending pair of equal keys is swapped, which only
+    // permutes two equal elements of this thread. The swap condition is one
+    // compare and a lane-mask xor (no boolean select, which the compiler
+    // would materialise in vector registers).
     template <int I, int J>
     __device__ __forceinline__ void ce(bool asc)
     {
-        const bool swap = asc ? (k[J] < k[I]) : (k[I] < k[J]);
         const Key ki = k[I], kj = k[J];
+        const bool swap = (kj < ki) ^ !asc;
         const uint32_t pi = p[I], pj = p[J];
         k[I] = swap ? kj : ki;
         k[J] = swap ? ki : kj;
         p[I] = swap ? pj : pi;
         p[J] = swap ? pi : pj;
+    }
+
+    // Element I against the partner's element `o` (position `op`): the lane
+    // that keeps the minimum takes `o` only when it is smaller, the other
+    // lane only when it is larger, so equal keys stay where they are on both
+    // sides and no position is duplicated.
+    __device__ __forceinline__ static void take_partner(Key &k_i, uint32_t &p_i, Key o, uint32_t op, bool keep_min)
+    {
+        const Key lo = o < k_i ? o : k_i;
+        const Key hi = o < k_i ? k_i : o;
+        const Key kept = keep_min ? lo : hi;
+        p_i = kept != k_i ? op : p_i;
+        k_i = kept;
     }
 
     // One in-register bitonic stage of stride STRIDE; the direction of the
@@ -392,12 +408,8 @@ struct Chunk {
             const uint32_t packed = xor_lane<MASK>((uint32_t)(p[I] | (p[I + 1] << 16)));
             const Key o0 = xor_lane<MASK>(k[I]);
             const Key o1 = xor_lane<MASK>(k[I + 1]);
-            const bool t0 = keep_min ? (o0 < k[I]) : (k[I] < o0);
-            const bool t1 = keep_min ? (o1 < k[I + 1]) : (k[I + 1] < o1);
-            k[I] = t0 ? o0 : k[I];
-            p[I] = t0 ? (packed & 0xFFFFu) : p[I];
-            k[I + 1] = t1 ? o1 : k[I + 1];
-            p[I + 1] = t1 ? (packed >> 16) : p[I + 1];
+            take_partner(k[I], p[I], o0, packed & 0xFFFFu, keep_min);
+            take_partner(k[I + 1], p[I + 1], o1, packed >> 16, keep_min);
             if constexpr (I % 8 == 6) {
                 __builtin_amdgcn_sched_barrier(0);  // bounds the swizzles in flight
             }
@@ -522,9 +534,7 @@ __device__ __forceinline__ void block_sort(Chunk<Key, E> &c, Key *xk, uint16_t *
                         constexpr int J = decltype(ic)::value;
                         constexpr int I = 8 * B + J;
                         const uint32_t op = (J & 1) ? (w[J / 2] >> 16) : (w[J / 2] & 0xFFFFu);
-                        const bool take = keep_min ? (other[J] < c.k[I]) : (c.k[I] < other[J]);
-                        c.k[I] = take ? other[J] : c.k[I];
-                        c.p[I] = take ? op : c.p[I];
+                        Chunk<Key, E>::take_partner(c.k[I], c.p[I], other[J], op, keep_min);
                     });
                 });
             } else {
@@ -538,9 +548,7 @@ __device__ __forceinline__ void block_sort(Chunk<Key, E> &c, Key *xk, uint16_t *
                     constexpr int I = decltype(ic)::value;
                     const Key other = xk[partner * (E + 1) + I];
                     const uint32_t op = xp[partner * (E + 1) + I];
-                    const bool take = keep_min ? (other < c.k[I]) : (c.k[I] < other);
-                    c.k[I] = take ? other : c.k[I];
-                    c.p[I] = take ? op : c.p[I];
+                    Chunk<Key, E>::take_partner(c.k[I], c.p[I], other, op, keep_min);
                 });
             }
             __syncthreads();
