@@ -271,6 +271,42 @@ def cpu_baseline(image: np.ndarray, size: int, mode: str) -> dict:
     }
 
 
+def cpu_sample(image: np.ndarray, rows: int, method: str, **kwargs) -> dict:
+    """The oracle's CPU restatement on the first `rows` rows (axis 0) of a
+    workload, all host threads: a bounded sample of the reference's CPU path
+    for that configuration, reported beside the GPU line."""
+    from oracle import oracle
+
+    threads = oracle.default_threads()
+    sample = np.ascontiguousarray(image[:rows])
+    start = time.perf_counter()
+    oracle.window_filter(sample, method=method, threads=threads, **kwargs)
+    seconds = time.perf_counter() - start
+    return {
+        "mpix_s": sample.size / seconds / 1e6,
+        "cores": threads,
+        "kind": "port",
+        "method": method,
+        "sample": f"first {rows} of {image.shape[0]} axis-0 rows ({sample.size} outputs), {seconds:.2f} s",
+    }
+
+
+def end_to_end(image: np.ndarray, repeats: int = 3, **kwargs) -> dict:
+    """The public call from host memory (validation, H2D, kernel, D2H,
+    owned output): what a caller of the drop-in API sees."""
+    import ndfiltrs_amd as nf
+
+    nf.nanmedian_filter(image, **kwargs)
+    times = []
+    for _ in range(repeats):
+        start = time.perf_counter()
+        nf.nanmedian_filter(image, **kwargs)
+        times.append(time.perf_counter() - start)
+    ms = float(np.median(times)) * 1e3
+    return {"ms": ms, "mpix_s": image.size / (ms * 1e-3) / 1e6, "repeats": repeats,
+            "what": "nanmedian_filter on a host array: validation + H2D + kernel + D2H, median"}
+
+
 def load_traffic() -> dict:
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(path):
@@ -331,11 +367,17 @@ def main() -> int:
             log(f"31x31 float64 {kind} {parameter}: plain and learned bracket")
             extra[f"31x31_f64_{kind}{parameter if kind == 'rank' else ''}"] = bracket_pair(
                 c2, kind, 31, "nearest", parameter, w31, args.steps, args.warmup, check_rows=64)
+        if not args.no_cpu:
+            extra["31x31_f64_median"]["cpu"] = cpu_sample(c2.image, 1024, "histogram", size=(31, 31),
+                                                          mode="nearest", omit_nan=True)
         c2.free()
         log("5x5 on 2048^2 float32")
         c1 = Workload(synthetic.sky_small())
         extra["5x5_f32"] = plain_config("nanmedian size=5 reflect 2048x2048 float32", c1, args.steps, args.warmup,
                                         64, size=(5, 5))
+        if not args.no_cpu:
+            extra["5x5_f32"]["cpu"] = cpu_sample(c1.image, 2048, "direct", size=(5, 5), mode="reflect",
+                                                 omit_nan=True)
         c1.free()
         log("annulus footprint on 8192^2 float32")
         yy, xx = np.mgrid[-6:7, -6:7]
@@ -344,12 +386,19 @@ def main() -> int:
         extra["annulus_f32"] = plain_config("nanmedian annulus r 4..6 origin (1,0) wrap ignore_inf 8192x8192 float32",
                                             c3a, args.steps, args.warmup, 64, mode="wrap", footprint=ring,
                                             origin=(1, 0), ignore_inf=True)
+        if not args.no_cpu:
+            extra["annulus_f32"]["cpu"] = cpu_sample(c3a.image, 1024, "direct", size=(13, 13),
+                                                     footprint=ring, origin=(1, 0), mode="wrap", omit_nan=True,
+                                                     ignore_inf=True)
         c3a.free()
         log("3x7x7 on the 128x1024x1024 cube")
         c3b = Workload(synthetic.cube())
         extra["cube_3x7x7_f32"] = plain_config("nanmedian size (3,7,7) wrap ignore_inf 128x1024x1024 float32", c3b,
                                                args.steps, args.warmup, 16, size=(3, 7, 7), mode="wrap",
                                                ignore_inf=True)
+        if not args.no_cpu:
+            extra["cube_3x7x7_f32"]["cpu"] = cpu_sample(c3b.image, 8, "direct", size=(3, 7, 7), mode="wrap",
+                                                        omit_nan=True, ignore_inf=True)
         c3b.free()
         log("512 cutouts: batched graph launch vs separate calls")
         extra["cutouts_512"] = cutout_batch(args.steps, args.warmup)
@@ -359,6 +408,9 @@ def main() -> int:
     if not args.no_cpu:
         log("timing the CPU baseline")
         baseline = cpu_baseline(c5_image, 63, "reflect")
+    if not args.quick:
+        log("end-to-end public call on the 8192x8192 image")
+        median63["end_to_end"] = end_to_end(c5_image, size=63, mode="reflect")
 
     pixels = c5_image.size
     taps = 63 * 63

@@ -39,6 +39,7 @@ constexpr uint16_t POS_NONE = 0xFFFF;
 constexpr int NT = SLIDE_WAVES * 64;
 constexpr int BINS = 16;
 constexpr int BATCH = 16;
+constexpr int START = 8;  // start-cursor candidates per wave in the bracket kernel
 constexpr int GUARD = BATCH;
 
 // Support positions are packed as (row << 8) | x with row = z * zrows + y
@@ -1178,54 +1179,113 @@ __global__ void __launch_bounds__(NT, E >= 32 ? 1 : (sizeof(T) == 8 && E >= 16 ?
                 below += (int)__popcll(window_bits(rm.below, lane, kxmask));
                 inside += (int)__popcll(window_bits(rm.inside, lane, kxmask));
             }
-            // Start rank m0 (wave-uniform): the mean relative position of the
-            // lanes' targets inside their brackets, applied to the tile's
-            // ranks. The cursor's c for m0 comes from per-column counts of
-            // inside ranks below m0 over the window rows, summed over each
-            // lane's kx columns with a wave prefix scan.
+            // Start cursor: START candidate ranks spread over the lanes'
+            // estimated targets (each lane's target position relative to its
+            // bracket, applied to the tile's ranks). For every candidate m_j,
+            // per-column counts of inside ranks below m_j over the window rows
+            // (two candidates per 16-bit pair), summed over each lane's kx
+            // columns with a wave prefix scan, give the exact cursor count
+            // c_j; each lane starts from the candidate nearest its target.
             int m = 1, c = below;
             {
-                float rel = 0.0f;
-                int have = 0;
+                int est = 0;
+                bool have = false;
                 if (n >= a.min_count && n > 0 && inside > 0) {
                     const int k0 = a.stat == STAT_RANK ? a.rank_param : (n >> 1);
                     if (k0 < n && k0 >= below && k0 < below + inside) {
-                        rel = ((float)(k0 - below) + 0.5f) / (float)inside;
-                        have = 1;
+                        const float rel = ((float)(k0 - below) + 0.5f) / (float)inside;
+                        est = 1 + (int)(rel * (float)retained);
+                        est = est < 1 ? 1 : (est > retained ? retained : est);
+                        have = true;
                     }
                 }
+                int lo_est = have ? est : retained, hi_est = have ? est : 1;
                 for (int off = 32; off > 0; off >>= 1) {
-                    rel += __shfl_xor(rel, off);
-                    have += __shfl_xor(have, off);
+                    const int l = __shfl_xor(lo_est, off), h = __shfl_xor(hi_est, off);
+                    lo_est = l < lo_est ? l : lo_est;
+                    hi_est = h > hi_est ? h : hi_est;
                 }
-                if (have > 0) {
-                    int m0 = 1 + (int)(rel / (float)have * (float)retained);
-                    m0 = m0 < 1 ? 1 : (m0 > retained ? retained : m0);
-                    int col0 = 0, col1 = 0;
+                if (lo_est <= hi_est) {
+                    // Candidates: midpoints of START equal parts of [lo_est, hi_est].
+                    u16x2 thr[START / 2];
+#pragma unroll
+                    for (int q = 0; q < START / 2; ++q) {
+                        const int span = hi_est - lo_est;
+                        const int t0 = lo_est + (span * (4 * q + 1)) / (2 * START);
+                        const int t1 = lo_est + (span * (4 * q + 3)) / (2 * START);
+                        thr[q] = u16x2{(unsigned short)t0, (unsigned short)t1};
+                    }
+                    u16x2 col0[START / 2], col1[START / 2];
+#pragma unroll
+                    for (int q = 0; q < START / 2; ++q) {
+                        col0[q] = (u16x2)(unsigned short)0;
+                        col1[q] = (u16x2)(unsigned short)0;
+                    }
                     const bool has1 = lane + 64 < a.sx;
                     for (int dy = 0; dy < a.ky; ++dy) {
                         const int row = y_first + dy;
                         const RowMasks &rm = rowmasks[row];
                         const uint16_t *crow = codes + row * a.sx;
-                        const int code0 = crow[lane];
-                        const int code1 = has1 ? crow[lane + 64] : 0;
-                        col0 += (int)((rm.inside[0] >> lane) & 1ull) & (int)(code0 < m0);
-                        col1 += (int)((rm.inside[1] >> lane) & 1ull) & (int)(code1 < m0);
+                        // Samples that are not inside get 0x7FFF, above every
+                        // rank, so a 16-bit difference's sign is `code < m_j`.
+                        const unsigned short e0 =
+                            ((rm.inside[0] >> lane) & 1ull) ? crow[lane] : (unsigned short)0x7FFF;
+                        const unsigned short e1 =
+                            (has1 && ((rm.inside[1] >> lane) & 1ull)) ? crow[lane + 64] : (unsigned short)0x7FFF;
+                        const u16x2 v0 = (u16x2)e0, v1 = (u16x2)e1;
+#pragma unroll
+                        for (int q = 0; q < START / 2; ++q) {
+                            col0[q] += (u16x2)(v0 - thr[q]) >> (u16x2)15;
+                            col1[q] += (u16x2)(v1 - thr[q]) >> (u16x2)15;
+                        }
                     }
-                    // Inclusive prefix over columns 0..127.
-                    int p0 = col0, p1 = col1;
-                    for (int off = 1; off < 64; off <<= 1) {
-                        const int u0 = __shfl_up(p0, off), u1 = __shfl_up(p1, off);
-                        p0 += lane >= off ? u0 : 0;
-                        p1 += lane >= off ? u1 : 0;
-                    }
-                    const int total0 = __shfl(p0, 63);
+                    // Inclusive prefix over columns 0..127 (two candidates per
+                    // register; counts stay below 2^16).
                     const int last = lane + a.kx - 1;  // last column of this lane's window
-                    const int hi0 = __shfl(p0, last & 63), hi1 = __shfl(p1, (last - 64) & 63);
-                    const int before = __shfl(p0, (lane + 63) & 63);
-                    const int upto = last < 64 ? hi0 : total0 + hi1;
-                    m = m0;
-                    c = below + upto - (lane ? before : 0);
+                    int cj[START];
+#pragma unroll
+                    for (int q = 0; q < START / 2; ++q) {
+                        uint32_t p0 = __builtin_bit_cast(uint32_t, col0[q]);
+                        uint32_t p1 = __builtin_bit_cast(uint32_t, col1[q]);
+                        for (int off = 1; off < 64; off <<= 1) {
+                            const uint32_t u0 = (uint32_t)__shfl_up((int)p0, off);
+                            const uint32_t u1 = (uint32_t)__shfl_up((int)p1, off);
+                            p0 += lane >= off ? u0 : 0u;  // no carry: both halves stay below 2^16
+                            p1 += lane >= off ? u1 : 0u;
+                        }
+                        const uint32_t total0 = (uint32_t)__shfl((int)p0, 63);
+                        const uint32_t hi0 = (uint32_t)__shfl((int)p0, last & 63);
+                        const uint32_t hi1 = (uint32_t)__shfl((int)p1, (last - 64) & 63);
+                        const uint32_t before = (uint32_t)__shfl((int)p0, (lane + 63) & 63);
+                        const uint32_t upto = last < 64 ? hi0 : total0 + hi1;
+                        const uint32_t win = upto - (lane ? before : 0u);
+                        cj[2 * q] = (int)(win & 0xFFFFu);
+                        cj[2 * q + 1] = (int)(win >> 16);
+                    }
+                    // Nearest candidate by window count: the last one at or
+                    // below the target, or the next one if that is closer.
+                    const int k0 = have ? (a.stat == STAT_RANK ? a.rank_param : (n >> 1)) : 0;
+                    int pick = 0;
+#pragma unroll
+                    for (int j = 1; j < START; ++j) {
+                        pick = below + cj[j] <= k0 ? j : pick;
+                    }
+                    int best_c = cj[0], best_j = 0;
+#pragma unroll
+                    for (int j = 0; j < START; ++j) {
+                        const bool take = j == pick || (j == pick + 1 && below + cj[j] - k0 < k0 - below - cj[pick]);
+                        best_c = take ? cj[j] : best_c;
+                        best_j = take ? j : best_j;
+                    }
+                    const u16x2 tq = thr[0];
+                    int mj = tq.x;
+#pragma unroll
+                    for (int q = 0; q < START / 2; ++q) {
+                        mj = best_j == 2 * q ? (int)thr[q].x : mj;
+                        mj = best_j == 2 * q + 1 ? (int)thr[q].y : mj;
+                    }
+                    m = mj;
+                    c = below + best_c;
                 }
             }
             bool failed = false;
