@@ -49,6 +49,9 @@ struct Geometry {
     int zshift;
     uint32_t vmask;  // keeps (y within its plane, x) of a packed position
     const uint8_t *mask;
+    // Byte-wise membership applies: kx, ky < 128 and a row difference
+    // y - row (at most ty - 1) never wraps into [256 - ky, 256).
+    bool bytes_ok;
 
     // Window membership of the sample at packed position v for the window
     // whose top-left support sample is `base` = (y << 8) | lane. Branch-free:
@@ -106,8 +109,37 @@ __device__ __forceinline__ uint32_t contains2(u16x2 v, u16x2 vmask, u16x2 y, u16
     return __builtin_bit_cast(uint32_t, both) & 0x80008000u;
 }
 
+// Byte-wise rectangle membership of the two packed positions in `w` (one per
+// 16-bit half) for windows with kx, ky < 128: u = w - base per half gives the
+// bytes (dx, dy) of each position (dx >= 193 when x < lane, dy >= 128 when
+// row < y); a byte b is inside its bound K exactly when b < 128 and
+// (b | 128) - K < 128, which one 32-bit subtraction tests for all four bytes
+// without borrows between them. Returns bit 7 (position 0) and bit 23
+// (position 1); bit 15 (dy of position 0 with dx of position 1) is cleared.
+__device__ __forceinline__ uint32_t contains2_bytes(uint32_t w, uint32_t base2, uint32_t bounds)
+{
+    const uint32_t u = __builtin_bit_cast(uint32_t, __builtin_bit_cast(u16x2, w) - __builtin_bit_cast(u16x2, base2));
+    const uint32_t s = (u | 0x80808080u) - bounds;
+    const uint32_t ok = ~(s | u) & 0x80808080u;
+    return ok & (ok >> 8) & 0x00800080u;
+}
+
 __device__ __forceinline__ uint32_t batch_mask(const uint16_t *pos, int first, const Geometry &g, uint32_t base)
 {
+    if (g.mask == nullptr && g.bytes_ok) {
+        uint32_t w[BATCH / 2];
+        __builtin_memcpy(w, pos + first, sizeof(w));
+        const uint32_t base2 = base * 0x10001u;
+        const uint32_t vmask2 = g.vmask * 0x10001u;
+        const uint32_t bounds = ((uint32_t)g.kx | ((uint32_t)g.ky << 8)) * 0x10001u;
+        uint32_t acc = 0;
+#pragma unroll
+        for (int j = 0; j < BATCH / 2; ++j) {
+            const uint32_t both = contains2_bytes(w[j] & vmask2, base2, bounds);
+            acc |= 2 * j >= 7 ? both << (2 * j - 7) : both >> (7 - 2 * j);
+        }
+        return (acc & 0x5555u) | ((acc >> 15) & 0xAAAAu);
+    }
     uint32_t v[BATCH];
 #pragma unroll
     for (int i = 0; i < BATCH; ++i) {
@@ -607,7 +639,8 @@ __device__ __forceinline__ bool slide_tile(const SlideArgs &a, int tile, unsigne
     const Key *input = static_cast<const Key *>(a.input);
     Key *output = static_cast<Key *>(a.output);
     const Key cval_key = classify<T>((Key)a.cval_bits, a.ignore_inf);
-    const Geometry g{a.kz, a.ky, a.kx, a.zshift, ((((uint32_t)1 << a.zshift) - 1u) << 8) | 0xFFu, mask};
+    const Geometry g{a.kz, a.ky, a.kx, a.zshift, ((((uint32_t)1 << a.zshift) - 1u) << 8) | 0xFFu, mask,
+                     a.ky < 128 && a.kx < 128 && a.ty + a.ky <= 257};
     uint16_t *hist = hist_all + wave * a.hist_stride;
     const bool wide = a.kz * a.ky > 255 || (a.mask != nullptr && a.count > 255);
     bool ok = true;
@@ -1047,7 +1080,7 @@ __global__ void __launch_bounds__(NT, E >= 32 ? 1 : (sizeof(T) == 8 && E >= 16 ?
     const Key *input = static_cast<const Key *>(a.input);
     Key *output = static_cast<Key *>(a.output);
     const Key cval_key = classify<T>((Key)a.cval_bits, a.ignore_inf);
-    const Geometry g{1, a.ky, a.kx, 8, 0xFFFFu, nullptr};
+    const Geometry g{1, a.ky, a.kx, 8, 0xFFFFu, nullptr, a.ky < 128 && a.kx < 128 && a.ty + a.ky <= 257};
     const uint64_t kxmask = a.kx >= 64 ? ~0ull : ((1ull << a.kx) - 1ull);
     const int64_t mz = a.mapz[0];
     const uint64_t lanemask_lt = (1ull << lane) - 1ull;
