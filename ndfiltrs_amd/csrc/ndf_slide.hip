@@ -341,16 +341,36 @@ __device__ __forceinline__ void static_for(F &&f)
     }
 }
 
-// Value of lane ^ MASK (MASK < 64) within the wave: ds_swizzle in bit mode
-// for masks below 32, a lane permute for 32.
+// Value of lane ^ MASK (MASK < 64) within the wave, through VALU lane
+// movement only (no LDS crossbar round trip):
+//   1, 2: DPP quad_perm [1,0,3,2] / [2,3,0,1];
+//   4:    DPP row_half_mirror (lane ^ 7 within 8) then quad_perm [3,2,1,0]
+//         (lane ^ 3);
+//   8:    DPP row_ror:8 (a half rotation of a 16-lane row is lane ^ 8);
+//   16:   v_permlane16_swap of v with itself: the first result holds rows
+//         (0, 0, 2, 2), the second rows (1, 1, 3, 3);
+//   32:   v_permlane32_swap of v with itself: the first result holds halves
+//         (0, 0), the second (1, 1).
 template <int MASK>
 __device__ __forceinline__ uint32_t xor_lane(uint32_t v)
 {
-    if constexpr (MASK < 32) {
-        // Swizzle offset: and_mask = 0x1F, or_mask = 0, xor_mask = MASK.
-        return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, (MASK << 10) | 0x1F);
+    constexpr int ALL = 0xF;
+    if constexpr (MASK == 1) {
+        return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, ALL, ALL, true);
+    } else if constexpr (MASK == 2) {
+        return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, ALL, ALL, true);
+    } else if constexpr (MASK == 4) {
+        const int mirrored = __builtin_amdgcn_update_dpp(0, (int)v, 0x141, ALL, ALL, true);
+        return (uint32_t)__builtin_amdgcn_update_dpp(0, mirrored, 0x1B, ALL, ALL, true);
+    } else if constexpr (MASK == 8) {
+        return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x128, ALL, ALL, true);
+    } else if constexpr (MASK == 16) {
+        const auto r = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+        return (threadIdx.x & 16) ? r[0] : r[1];
     } else {
-        return (uint32_t)__builtin_amdgcn_ds_bpermute((int)((threadIdx.x ^ MASK) << 2), (int)v);
+        static_assert(MASK == 32, "lane masks are powers of two below 64");
+        const auto r = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+        return (threadIdx.x & 32) ? r[0] : r[1];
     }
 }
 
