@@ -456,6 +456,18 @@ struct Chunk {
     template <int MASK>
     __device__ __forceinline__ void exchange_xor(bool keep_min)
     {
+        if constexpr (MASK <= 8 && sizeof(Key) == 4) {
+            // DPP lane movement folds into the min / max / select that
+            // consume it, so keys and positions move one by one.
+            static_for<0, E>([&](auto ic) {
+                constexpr int I = decltype(ic)::value;
+                take_partner(k[I], p[I], xor_lane<MASK>(k[I]), xor_lane<MASK>(p[I]), keep_min);
+                if constexpr (I % 8 == 7) {
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+            });
+            return;
+        }
         static_for<0, E / 2>([&](auto ic) {
             constexpr int I = 2 * decltype(ic)::value;
             const uint32_t packed = xor_lane<MASK>((uint32_t)(p[I] | (p[I + 1] << 16)));
@@ -607,6 +619,67 @@ __device__ __forceinline__ void block_sort(Chunk<Key, E> &c, Key *xk, uint16_t *
             __syncthreads();
         }
         c.merge_local(asc);
+    }
+}
+
+// Stable merge sort of NT * E (key, pos) pairs; thread t ends with the
+// sorted elements [t*E, t*E + E) in registers. Each thread first sorts its
+// own E elements (a register bitonic network with compile-time directions);
+// then log2(NT) merge levels double the sorted run length. At each level a
+// thread produces the E outputs [t*E, t*E + E) of its pair of runs (A, B):
+// a binary search on the merge path finds how many of them come from A, and
+// a sequential merge reads one key per output from LDS (ties take A first).
+// Outputs carry their source index, and positions are gathered afterwards.
+// xk / xp hold N keys and N positions (flat).
+template <typename Key, int E>
+__device__ __forceinline__ void merge_sort(Chunk<Key, E> &c, Key *xk, uint16_t *xp, int t)
+{
+    constexpr int N = NT * E;
+    c.template local_sort<2>(0);
+    const int first = t * E;
+    for (int run = E; run < N; run <<= 1) {
+        static_for<0, E>([&](auto ic) {
+            constexpr int I = decltype(ic)::value;
+            xk[first + I] = c.k[I];
+            xp[first + I] = (uint16_t)c.p[I];
+        });
+        __syncthreads();
+        const int a0 = first & ~(2 * run - 1);
+        const int b0 = a0 + run;
+        const int diag = first - a0;
+        int lo = diag > run ? diag - run : 0;
+        int hi = diag < run ? diag : run;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (xk[a0 + mid] <= xk[b0 + diag - mid - 1]) {
+                lo = mid + 1;
+            } else {
+                hi = mid;
+            }
+        }
+        int ia = a0 + lo, ib = b0 + diag - lo;
+        const int b_end = b0 + run;
+        Key ka = xk[ia < b0 ? ia : b0 - 1];
+        Key kb = xk[ib < b_end ? ib : b_end - 1];
+        static_for<0, E>([&](auto ic) {
+            constexpr int I = decltype(ic)::value;
+            const bool take_b = ib < b_end && (ia >= b0 || kb < ka);
+            c.k[I] = take_b ? kb : ka;
+            c.p[I] = (uint32_t)(take_b ? ib : ia);
+            if constexpr (I + 1 < E) {
+                const int next = (take_b ? ib : ia) + 1;
+                const Key loaded = xk[next < N ? next : N - 1];
+                ia += take_b ? 0 : 1;
+                ib += take_b ? 1 : 0;
+                ka = take_b ? ka : loaded;
+                kb = take_b ? loaded : kb;
+            }
+        });
+        static_for<0, E>([&](auto ic) {
+            constexpr int I = decltype(ic)::value;
+            c.p[I] = xp[c.p[I]];
+        });
+        __syncthreads();
     }
 }
 
@@ -765,7 +838,7 @@ __device__ __forceinline__ bool slide_tile(const SlideArgs &a, int tile, unsigne
                 chunk.p[I] = gi < retained ? xp[gi] : POS_NONE;
             });
             __syncthreads();
-            block_sort<Key, E>(chunk, xk, xp, tid);
+            merge_sort<Key, E>(chunk, xk, xp, tid);
             __syncthreads();
             static_for<0, E>([&](auto ic) {
                 constexpr int I = decltype(ic)::value;
@@ -1033,7 +1106,7 @@ __device__ __forceinline__ void sort_inside(Key *xk, uint16_t *xp, uint16_t *cod
         chunk.p[I] = gi < retained ? xp[gi] : POS_NONE;
     });
     __syncthreads();
-    block_sort<Key, E>(chunk, xk, xp, tid);
+    merge_sort<Key, E>(chunk, xk, xp, tid);
     __syncthreads();
     static_for<0, E>([&](auto ic) {
         constexpr int I = decltype(ic)::value;
