@@ -66,10 +66,11 @@ template <typename T>
 __device__ __host__ inline typename KeyTraits<T>::Key classify(typename KeyTraits<T>::Key bits, int ignore_inf)
 {
     using KT = KeyTraits<T>;
-    if (KT::is_nan(bits) || (ignore_inf && KT::is_inf(bits))) {
-        return KT::SENTINEL;
-    }
-    return KT::key(bits);
+    // Both arms are computed and selected (no branch), so a caller's loads
+    // feeding this are not sunk into a conditional block.
+    const typename KT::Key key = KT::key(bits);
+    const bool omit = KT::is_nan(bits) || (ignore_inf && KT::is_inf(bits));
+    return omit ? KT::SENTINEL : key;
 }
 
 // Even-count midpoint in f64 (docs/semantics.md:74-85,

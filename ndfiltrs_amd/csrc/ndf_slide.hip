@@ -220,23 +220,21 @@ __device__ inline int previous_member(int m, const uint16_t *pos, const Geometry
 
 // Packed 16-bin counters: 8 bits per bin while counts stay below 256,
 // otherwise 16 bits per bin.
+// add(b) counts bin b when b < 16 and nothing otherwise, without branches.
 struct Bins8 {
     uint64_t lo = 0, hi = 0;
-    __device__ inline void add(int b)
+    __device__ __forceinline__ void add(uint32_t b)
     {
         const uint64_t one = 1ull << ((b & 7) * 8);
-        if (b < 8) {
-            lo += one;
-        } else {
-            hi += one;
-        }
+        lo += b < 8 ? one : 0ull;
+        hi += b - 8u < 8u ? one : 0ull;
     }
     __device__ inline int get(int b) const { return (int)(((b < 8 ? lo : hi) >> ((b & 7) * 8)) & 0xFF); }
 };
 
 struct Bins16 {
     uint64_t q[4] = {0, 0, 0, 0};
-    __device__ inline void add(int b)
+    __device__ __forceinline__ void add(uint32_t b)
     {
         const uint64_t one = 1ull << ((b & 3) * 16);
 #pragma unroll
@@ -252,7 +250,7 @@ struct Bins16 {
 // or over the range. Rectangles use per-column counts shared through LDS
 // (hist, BINS + 2 rows of sx + 1 entries); other footprints scan their taps.
 template <typename Bins>
-__device__ void window_bins(int counts[BINS], int &below, int &above, int lo, int shift, const uint16_t *codes,
+__device__ __forceinline__ void window_bins(int counts[BINS], int &below, int &above, int lo, int shift, const uint16_t *codes,
                             uint16_t *hist, const SlideArgs &a, int lane, int y)
 {
     const int hi = lo + (BINS << shift);
@@ -261,15 +259,28 @@ __device__ void window_bins(int counts[BINS], int &below, int &above, int lo, in
         for (int x = lane; x < a.sx; x += 64) {
             Bins bins;
             int under = 0, over = 0;
+            auto count = [&](int code) {
+                under += code < lo;
+                over += code >= hi && code != CODE_OMIT;
+                bins.add((uint32_t)(code - lo) >> shift);  // no bin below lo or at hi and above
+            };
             for (int z = 0; z < a.kz; ++z) {
                 const uint16_t *column = codes + (z * a.zrows + y) * a.sx + x;
-                for (int dy = 0; dy < a.ky; ++dy) {
-                    const int code = column[dy * a.sx];
-                    under += code < lo;
-                    over += code >= hi && code != CODE_OMIT;
-                    if (code >= lo && code < hi) {
-                        bins.add((code - lo) >> shift);
+                int dy = 0;
+                // Eight reads in flight per step.
+                for (; dy + 8 <= a.ky; dy += 8) {
+                    int code[8];
+#pragma unroll
+                    for (int i = 0; i < 8; ++i) {
+                        code[i] = column[(dy + i) * a.sx];
                     }
+#pragma unroll
+                    for (int i = 0; i < 8; ++i) {
+                        count(code[i]);
+                    }
+                }
+                for (; dy < a.ky; ++dy) {
+                    count(column[dy * a.sx]);
                 }
             }
 #pragma unroll
@@ -646,6 +657,7 @@ __device__ __forceinline__ void merge_sort(Chunk<Key, E> &c, Key *xk, uint16_t *
         __syncthreads();
         const int a0 = first & ~(2 * run - 1);
         const int b0 = a0 + run;
+        const int b_end = b0 + run;
         const int diag = first - a0;
         int lo = diag > run ? diag - run : 0;
         int hi = diag < run ? diag : run;
@@ -658,7 +670,6 @@ __device__ __forceinline__ void merge_sort(Chunk<Key, E> &c, Key *xk, uint16_t *
             }
         }
         int ia = a0 + lo, ib = b0 + diag - lo;
-        const int b_end = b0 + run;
         Key ka = xk[ia < b0 ? ia : b0 - 1];
         Key kb = xk[ib < b_end ? ib : b_end - 1];
         static_for<0, E>([&](auto ic) {
@@ -703,6 +714,97 @@ __device__ __forceinline__ uint16_t *exchange_positions(Key *xk)
 {
     return SortLayout<Key, E>::VEC ? reinterpret_cast<uint16_t *>(xk + NT * E)
                                    : reinterpret_cast<uint16_t *>(xk + NT * (E + 1));
+}
+
+// Stage of the plain kernel: loads the support of the tile at (x0, y0, z0)
+// through the boundary maps, writes every support sample's code (CODE_OMIT or
+// CODE_PENDING) and compacts the retained samples' keys and packed positions
+// into xk / xp (count in count_shared; at most n_sort are stored). Each wave takes R support rows per
+// batch (H 64-column chunks each): the rows' map entries are read first
+// (wave-uniform, scalar loads), then every lane issues all R * H sample
+// loads before the first wait, and one LDS atomic reserves the batch's
+// slots.
+template <typename T, int R, int H>
+__device__ __forceinline__ void stage_support(const SlideArgs &a, typename KeyTraits<T>::Key *xk, uint16_t *xp,
+                                              uint16_t *codes, int &count_shared, int64_t x0, int64_t y0, int64_t z0,
+                                              typename KeyTraits<T>::Key cval_key, int n_sort)
+{
+    using KT = KeyTraits<T>;
+    using Key = typename KT::Key;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const Key *input = static_cast<const Key *>(a.input);
+    int64_t mx[H];
+#pragma unroll
+    for (int h = 0; h < H; ++h) {
+        mx[h] = h * 64 + lane < a.sx ? a.mapx[x0 + h * 64 + lane] : MAP_PAD;
+    }
+    for (int row0 = wave * R; row0 < a.rows; row0 += SLIDE_WAVES * R) {
+        // Row offsets (mz + my) and row kinds: 0 loaded, 1 constant, 2 padding.
+        int64_t roff[R];
+        int rkind[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int row = row0 + r;
+            const int z = row >> a.zshift;
+            const int y = row & ((1 << a.zshift) - 1);
+            const bool live = row < a.rows && y < a.sy;  // padding rows between z planes are skipped
+            const int64_t mz = live ? a.mapz[z0 + z] : MAP_PAD;
+            const int64_t my = live ? a.mapy[y0 + y] : MAP_PAD;
+            const bool pad = mz == MAP_PAD || my == MAP_PAD;
+            rkind[r] = pad ? 2 : (mz < 0 || my < 0 ? 1 : 0);
+            roff[r] = pad ? 0 : mz + my;
+        }
+        Key kv[R][H];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+#pragma unroll
+            for (int h = 0; h < H; ++h) {
+                // Every lane loads (element 0 stands in for padding and
+                // constant samples).
+                const bool load = rkind[r] == 0 && mx[h] >= 0;
+                kv[r][h] = input[load ? roff[r] + mx[h] : 0];
+            }
+        }
+        int total = 0;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int row = row0 + r;
+            const int y = row & ((1 << a.zshift) - 1);
+            const bool live = row < a.rows && y < a.sy;
+#pragma unroll
+            for (int h = 0; h < H; ++h) {
+                const bool pad = rkind[r] == 2 || mx[h] == MAP_PAD;
+                const bool cval = !pad && (rkind[r] == 1 || mx[h] < 0);
+                kv[r][h] = pad ? KT::SENTINEL : (cval ? cval_key : classify<T>(kv[r][h], a.ignore_inf));
+                const int x = h * 64 + lane;
+                if (live && x < a.sx) {
+                    codes[row * a.sx + x] = kv[r][h] == KT::SENTINEL ? CODE_OMIT : CODE_PENDING;
+                }
+                total += (int)__popcll(__ballot(kv[r][h] != KT::SENTINEL));
+            }
+        }
+        int base = 0;
+        if (lane == 0 && total) {
+            base = atomicAdd(&count_shared, total);
+        }
+        base = __shfl(base, 0);
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+#pragma unroll
+            for (int h = 0; h < H; ++h) {
+                const uint64_t bits = __ballot(kv[r][h] != KT::SENTINEL);
+                if ((bits >> lane) & 1ull) {
+                    const int dst = base + (int)__popcll(bits & ((1ull << lane) - 1ull));
+                    if (dst < n_sort) {
+                        xk[dst] = kv[r][h];
+                        xp[dst] = (uint16_t)(((row0 + r) << 8) | (h * 64 + lane));
+                    }
+                }
+                base += (int)__popcll(bits);
+            }
+        }
+    }
 }
 
 // One tile of the plain kernel: stage, sort, rank and slide. Every retained
@@ -752,73 +854,10 @@ __device__ __forceinline__ bool slide_tile(const SlideArgs &a, int tile, unsigne
         __syncthreads();
 
         // ---- stage: classify the support, compact the retained samples ----
-        // Each wave takes R support rows per batch and issues all their loads
-        // before classifying; one LDS atomic reserves the batch's slots.
-        {
-            constexpr int R = 4;
-            constexpr int H = 4;  // 64-column chunks of a support row (sx <= 254)
-            int64_t mx[H];
-#pragma unroll
-            for (int h = 0; h < H; ++h) {
-                mx[h] = h * 64 + lane < a.sx ? a.mapx[x0 + h * 64 + lane] : MAP_PAD;
-            }
-            const int chunks = (a.sx + 63) >> 6;
-            for (int row0 = wave * R; row0 < a.rows; row0 += SLIDE_WAVES * R) {
-                Key kv[R][H];
-#pragma unroll
-                for (int r = 0; r < R; ++r) {
-                    const int row = row0 + r;
-                    const int z = row >> a.zshift;
-                    const int y = row & ((1 << a.zshift) - 1);
-                    const bool live = row < a.rows && y < a.sy;  // padding rows between z planes are skipped
-                    const int64_t mz = live ? a.mapz[z0 + z] : MAP_PAD;
-                    const int64_t my = live ? a.mapy[y0 + y] : MAP_PAD;
-#pragma unroll
-                    for (int h = 0; h < H; ++h) {
-                        const bool pad = h >= chunks || mz == MAP_PAD || my == MAP_PAD || mx[h] == MAP_PAD;
-                        const bool cval = !pad && (mz < 0 || my < 0 || mx[h] < 0);
-                        const Key raw = input[pad || cval ? 0 : mz + my + mx[h]];
-                        kv[r][h] = pad ? KT::SENTINEL : (cval ? cval_key : classify<T>(raw, a.ignore_inf));
-                    }
-                }
-                uint64_t kept[R][H];
-                int total = 0;
-#pragma unroll
-                for (int r = 0; r < R; ++r) {
-                    const int row = row0 + r;
-                    const int y = row & ((1 << a.zshift) - 1);
-                    const bool live = row < a.rows && y < a.sy;
-#pragma unroll
-                    for (int h = 0; h < H; ++h) {
-                        const int x = h * 64 + lane;
-                        if (live && h < chunks && x < a.sx) {
-                            codes[row * a.sx + x] = kv[r][h] == KT::SENTINEL ? CODE_OMIT : CODE_PENDING;
-                        }
-                        kept[r][h] = __ballot(kv[r][h] != KT::SENTINEL);
-                        total += (int)__popcll(kept[r][h]);
-                    }
-                }
-                int base = 0;
-                if (lane == 0 && total) {
-                    base = atomicAdd(&count_shared, total);
-                }
-                base = __shfl(base, 0);
-#pragma unroll
-                for (int r = 0; r < R; ++r) {
-#pragma unroll
-                    for (int h = 0; h < H; ++h) {
-                        const uint64_t bits = kept[r][h];
-                        if ((bits >> lane) & 1ull) {
-                            const int dst = base + (int)__popcll(bits & ((1ull << lane) - 1ull));
-                            if (dst < N) {
-                                xk[dst] = kv[r][h];
-                                xp[dst] = (uint16_t)(((row0 + r) << 8) | (h * 64 + lane));
-                            }
-                        }
-                        base += (int)__popcll(bits);
-                    }
-                }
-            }
+        if (a.sx <= 128) {
+            stage_support<T, 16, 2>(a, xk, xp, codes, count_shared, x0, y0, z0, cval_key, N);
+        } else {
+            stage_support<T, 4, 4>(a, xk, xp, codes, count_shared, x0, y0, z0, cval_key, N);
         }
         __syncthreads();
         const int retained = count_shared;
