@@ -633,6 +633,26 @@ __device__ __forceinline__ void block_sort(Chunk<Key, E> &c, Key *xk, uint16_t *
     }
 }
 
+// Layout of the merge sort's exchange for chunks of E >= 32 elements: each
+// thread's chunk of keys is followed by 16 bytes of padding, and so is each
+// chunk of positions. Thread t's chunk then starts 16 * t bytes further into
+// the bank rotation than without padding, so the 16-byte chunk stores and
+// the key and position reads of neighbouring threads (which fall near the
+// same offset of their own chunks) spread over the LDS banks instead of
+// landing on one bank group. Smaller chunks keep the flat layout.
+template <typename Key, int E>
+struct MergeLayout {
+    static constexpr int KPAD = E >= 32 ? 16 / (int)sizeof(Key) : 0;  // keys of padding per chunk
+    static constexpr int PPAD = E >= 32 ? 8 : 0;                       // u16 positions of padding per chunk
+    static constexpr int log2c(int v) { return v <= 1 ? 0 : 1 + log2c(v / 2); }
+    static constexpr int SHIFT = log2c(E);
+    __device__ static int key(int i) { return i + KPAD * (i >> SHIFT); }
+    __device__ static int pos(int i) { return i + PPAD * (i >> SHIFT); }
+    // Start of the position array (16-byte aligned) and the bytes of both.
+    static constexpr size_t key_bytes() { return (size_t)NT * (E + KPAD) * sizeof(Key); }
+    static constexpr size_t bytes() { return key_bytes() + (size_t)NT * (E + PPAD) * 2; }
+};
+
 // Stable merge sort of NT * E (key, pos) pairs; thread t ends with the
 // sorted elements [t*E, t*E + E) in registers. Each thread first sorts its
 // own E elements (a register bitonic network with compile-time directions);
@@ -641,18 +661,26 @@ __device__ __forceinline__ void block_sort(Chunk<Key, E> &c, Key *xk, uint16_t *
 // a binary search on the merge path finds how many of them come from A, and
 // a sequential merge reads one key per output from LDS (ties take A first).
 // Outputs carry their source index, and positions are gathered afterwards.
-// xk / xp hold N keys and N positions (flat).
+// The levels exchange through `region` in the MergeLayout (the caller's
+// chunk has been read into registers before the call).
 template <typename Key, int E>
-__device__ __forceinline__ void merge_sort(Chunk<Key, E> &c, Key *xk, uint16_t *xp, int t)
+__device__ __forceinline__ void merge_sort(Chunk<Key, E> &c, unsigned char *region, int t)
 {
+    using ML = MergeLayout<Key, E>;
     constexpr int N = NT * E;
+    Key *mk = reinterpret_cast<Key *>(region);
+    uint16_t *mp = reinterpret_cast<uint16_t *>(region + ML::key_bytes());
+    // Logical element i of the exchange.
+    auto xk = [&](int i) -> Key { return mk[ML::key(i)]; };
     c.template local_sort<2>(0);
     const int first = t * E;
+    Key *own_k = mk + ML::key(first);
+    uint16_t *own_p = mp + ML::pos(first);
     for (int run = E; run < N; run <<= 1) {
         static_for<0, E>([&](auto ic) {
             constexpr int I = decltype(ic)::value;
-            xk[first + I] = c.k[I];
-            xp[first + I] = (uint16_t)c.p[I];
+            own_k[I] = c.k[I];
+            own_p[I] = (uint16_t)c.p[I];
         });
         __syncthreads();
         const int a0 = first & ~(2 * run - 1);
@@ -663,15 +691,15 @@ __device__ __forceinline__ void merge_sort(Chunk<Key, E> &c, Key *xk, uint16_t *
         int hi = diag < run ? diag : run;
         while (lo < hi) {
             const int mid = (lo + hi) >> 1;
-            if (xk[a0 + mid] <= xk[b0 + diag - mid - 1]) {
+            if (xk(a0 + mid) <= xk(b0 + diag - mid - 1)) {
                 lo = mid + 1;
             } else {
                 hi = mid;
             }
         }
         int ia = a0 + lo, ib = b0 + diag - lo;
-        Key ka = xk[ia < b0 ? ia : b0 - 1];
-        Key kb = xk[ib < b_end ? ib : b_end - 1];
+        Key ka = xk(ia < b0 ? ia : b0 - 1);
+        Key kb = xk(ib < b_end ? ib : b_end - 1);
         static_for<0, E>([&](auto ic) {
             constexpr int I = decltype(ic)::value;
             const bool take_b = ib < b_end && (ia >= b0 || kb < ka);
@@ -679,7 +707,7 @@ __device__ __forceinline__ void merge_sort(Chunk<Key, E> &c, Key *xk, uint16_t *
             c.p[I] = (uint32_t)(take_b ? ib : ia);
             if constexpr (I + 1 < E) {
                 const int next = (take_b ? ib : ia) + 1;
-                const Key loaded = xk[next < N ? next : N - 1];
+                const Key loaded = xk(next < N ? next : N - 1);
                 ia += take_b ? 0 : 1;
                 ib += take_b ? 1 : 0;
                 ka = take_b ? ka : loaded;
@@ -688,7 +716,7 @@ __device__ __forceinline__ void merge_sort(Chunk<Key, E> &c, Key *xk, uint16_t *
         });
         static_for<0, E>([&](auto ic) {
             constexpr int I = decltype(ic)::value;
-            c.p[I] = xp[c.p[I]];
+            c.p[I] = mp[ML::pos((int)c.p[I])];
         });
         __syncthreads();
     }
@@ -702,10 +730,13 @@ constexpr size_t exchange_bytes()
     const size_t rows = SortLayout<Key, E>::VEC ? (size_t)NT * E * (sizeof(Key) + 2)
                                                 : (size_t)NT * (E + 1) * (sizeof(Key) + 2);
     const size_t sorted = (size_t)NT * E * (sizeof(Key) + 2) + 4 * GUARD;
-    return ((rows > sorted ? rows : sorted) + 15) & ~(size_t)15;
+    const size_t merge = MergeLayout<Key, E>::bytes();
+    const size_t most = rows > sorted ? rows : sorted;
+    return ((most > merge ? most : merge) + 15) & ~(size_t)15;
 }
 
-static_assert(exchange_bytes<uint32_t, 64>() == (size_t)NT * 64 * 6 + 4 * GUARD, "exchange layout");
+static_assert(exchange_bytes<uint32_t, 64>() == (size_t)NT * 64 * 6 + (size_t)NT * 32, "exchange layout");
+static_assert(exchange_bytes<uint32_t, 16>() == (size_t)NT * 16 * 6 + 4 * GUARD, "exchange layout");
 static_assert(exchange_bytes<uint64_t, 4>() == (((size_t)NT * 5 * 10 + 15) & ~(size_t)15), "exchange layout");
 
 // Start of the position half of the exchange (16-byte aligned).
@@ -831,7 +862,6 @@ __device__ __forceinline__ bool slide_tile(const SlideArgs &a, int tile, unsigne
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
-    const Key *input = static_cast<const Key *>(a.input);
     Key *output = static_cast<Key *>(a.output);
     const Key cval_key = classify<T>((Key)a.cval_bits, a.ignore_inf);
     const Geometry g{a.kz, a.ky, a.kx, a.zshift, ((((uint32_t)1 << a.zshift) - 1u) << 8) | 0xFFu, mask,
@@ -877,7 +907,7 @@ __device__ __forceinline__ bool slide_tile(const SlideArgs &a, int tile, unsigne
                 chunk.p[I] = gi < retained ? xp[gi] : POS_NONE;
             });
             __syncthreads();
-            merge_sort<Key, E>(chunk, xk, xp, tid);
+            merge_sort<Key, E>(chunk, reinterpret_cast<unsigned char *>(xk), tid);
             __syncthreads();
             static_for<0, E>([&](auto ic) {
                 constexpr int I = decltype(ic)::value;
@@ -1145,7 +1175,7 @@ __device__ __forceinline__ void sort_inside(Key *xk, uint16_t *xp, uint16_t *cod
         chunk.p[I] = gi < retained ? xp[gi] : POS_NONE;
     });
     __syncthreads();
-    merge_sort<Key, E>(chunk, xk, xp, tid);
+    merge_sort<Key, E>(chunk, reinterpret_cast<unsigned char *>(xk), tid);
     __syncthreads();
     static_for<0, E>([&](auto ic) {
         constexpr int I = decltype(ic)::value;
@@ -1195,7 +1225,7 @@ __global__ void __launch_bounds__(NT, E >= 32 ? 1 : (sizeof(T) == 8 && E >= 16 ?
     using KT = KeyTraits<T>;
     using Key = typename KT::Key;
     constexpr int N = NT * E;
-    constexpr int R = 4;  // support rows a wave loads per batch
+    constexpr int R = 8;  // support rows a wave loads per batch
     extern __shared__ __align__(16) unsigned char lds[];
     Key *xk = reinterpret_cast<Key *>(lds);
     uint16_t *xp = exchange_positions<Key, E>(xk);
@@ -1208,6 +1238,7 @@ __global__ void __launch_bounds__(NT, E >= 32 ? 1 : (sizeof(T) == 8 && E >= 16 ?
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
+    const int wave_u = __builtin_amdgcn_readfirstlane(wave);
     const int ntiles = a.tile_count ? *a.tile_count : a.tiles_x * a.tiles_y;
     const Key *input = static_cast<const Key *>(a.input);
     Key *output = static_cast<Key *>(a.output);
@@ -1239,19 +1270,32 @@ __global__ void __launch_bounds__(NT, E >= 32 ? 1 : (sizeof(T) == 8 && E >= 16 ?
         // ---- stage: R rows per wave per batch, loads first ----
         const int64_t mx0 = a.mapx[x0 + lane];
         const int64_t mx1 = lane + 64 < a.sx ? a.mapx[x0 + 64 + lane] : MAP_PAD;
-        for (int row0 = wave * R; row0 < a.rows; row0 += SLIDE_WAVES * R) {
+        for (int row0 = wave_u * R; row0 < a.rows; row0 += SLIDE_WAVES * R) {
+            // Row map entries first (wave-uniform, scalar loads), then all
+            // R * 2 sample loads of the batch before the first wait.
+            int64_t my[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                my[r] = row0 + r < a.rows ? a.mapy[y0 + row0 + r] : MAP_PAD;
+            }
             Key kv[R][2];
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                const int row = row0 + r;
-                const int64_t my = row < a.rows ? a.mapy[y0 + row] : MAP_PAD;
 #pragma unroll
                 for (int h = 0; h < 2; ++h) {
                     const int64_t mx = h ? mx1 : mx0;
-                    const bool pad = my == MAP_PAD || mx == MAP_PAD;
-                    const bool cval = !pad && (my < 0 || mx < 0);
-                    const Key raw = input[pad || cval ? 0 : mz + my + mx];
-                    kv[r][h] = pad ? KT::SENTINEL : (cval ? cval_key : classify<T>(raw, a.ignore_inf));
+                    const bool load = my[r] >= 0 && mx >= 0;  // every lane loads; element 0 stands in
+                    kv[r][h] = input[load ? mz + my[r] + mx : 0];
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int64_t mx = h ? mx1 : mx0;
+                    const bool pad = my[r] == MAP_PAD || mx == MAP_PAD;
+                    const bool cval = !pad && (my[r] < 0 || mx < 0);
+                    kv[r][h] = pad ? KT::SENTINEL : (cval ? cval_key : classify<T>(kv[r][h], a.ignore_inf));
                 }
             }
             uint64_t in_bits[R][2];
@@ -1660,7 +1704,9 @@ static size_t exchange_region(int dtype_bytes, int sort_e)
     const size_t per = (size_t)dtype_bytes + 2;
     const size_t rows = sort_e >= 8 ? (size_t)NT * sort_e * per : (size_t)NT * (sort_e + 1) * per;
     const size_t sorted = (size_t)NT * sort_e * per + 4 * GUARD;
-    return ((rows > sorted ? rows : sorted) + 15) & ~(size_t)15;
+    const size_t merge = (size_t)NT * sort_e * per + (sort_e >= 32 ? (size_t)NT * 32 : 0);  // MergeLayout padding
+    const size_t most = rows > sorted ? rows : sorted;
+    return ((most > merge ? most : merge) + 15) & ~(size_t)15;
 }
 
 size_t slide_lds_bytes(int dtype_bytes, int s, int sort_e, int hist_stride, int mask_bytes)
