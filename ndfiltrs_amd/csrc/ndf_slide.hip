@@ -633,13 +633,14 @@ __device__ __forceinline__ void block_sort(Chunk<Key, E> &c, Key *xk, uint16_t *
     }
 }
 
-// Layout of the merge sort's exchange for chunks of E >= 32 elements: each
-// thread's chunk of keys is followed by 16 bytes of padding, and so is each
-// chunk of positions. Thread t's chunk then starts 16 * t bytes further into
-// the bank rotation than without padding, so the 16-byte chunk stores and
-// the key and position reads of neighbouring threads (which fall near the
-// same offset of their own chunks) spread over the LDS banks instead of
-// landing on one bank group. Smaller chunks keep the flat layout.
+// Layout of the merge sort's exchange. Neighbouring threads access their
+// own chunks (E elements) at nearby offsets, which without a remedy land on
+// one LDS bank group. Chunks of E >= 32 elements are each followed by 16
+// bytes of padding (keys and positions), so thread t's chunk starts 16 * t
+// bytes further into the bank rotation and the 16-byte chunk stores stay
+// aligned. Smaller chunks (the bracket kernel's sorts, whose LDS budget has
+// no room for padding) keep the flat layout; an XOR swizzle there measured
+// slower (address arithmetic under that kernel's register cap).
 template <typename Key, int E>
 struct MergeLayout {
     static constexpr int KPAD = E >= 32 ? 16 / (int)sizeof(Key) : 0;  // keys of padding per chunk
@@ -674,13 +675,11 @@ __device__ __forceinline__ void merge_sort(Chunk<Key, E> &c, unsigned char *regi
     auto xk = [&](int i) -> Key { return mk[ML::key(i)]; };
     c.template local_sort<2>(0);
     const int first = t * E;
-    Key *own_k = mk + ML::key(first);
-    uint16_t *own_p = mp + ML::pos(first);
     for (int run = E; run < N; run <<= 1) {
         static_for<0, E>([&](auto ic) {
             constexpr int I = decltype(ic)::value;
-            own_k[I] = c.k[I];
-            own_p[I] = (uint16_t)c.p[I];
+            mk[ML::key(first + I)] = c.k[I];
+            mp[ML::pos(first + I)] = (uint16_t)c.p[I];
         });
         __syncthreads();
         const int a0 = first & ~(2 * run - 1);
@@ -900,9 +899,12 @@ __device__ __forceinline__ bool slide_tile(const SlideArgs &a, int tile, unsigne
         // ---- sort the retained samples (padded with sentinels to N) ----
         {
             Chunk<Key, E> chunk;
+            // Thread tid starts with elements I * NT + tid: lanes read
+            // consecutive entries (no LDS bank conflicts), and the sorted
+            // result does not depend on the initial assignment.
             static_for<0, E>([&](auto ic) {
                 constexpr int I = decltype(ic)::value;
-                const int gi = tid * E + I;
+                const int gi = I * NT + tid;
                 chunk.k[I] = gi < retained ? xk[gi] : KT::SENTINEL;
                 chunk.p[I] = gi < retained ? xp[gi] : POS_NONE;
             });
@@ -996,21 +998,28 @@ __device__ __forceinline__ bool slide_tile(const SlideArgs &a, int tile, unsigne
                         window_bins<Bins8>(counts2, below2, above2, lo2, shift2, codes, hist, a, lane, y_first);
                     }
                     if (bin >= 0) {
-                        int run = below2, bin2 = -1, base2 = below2;
+                        int run = below2, bin2 = -1, base2 = below2, in2 = 0;
 #pragma unroll
                         for (int b = 0; b < BINS; ++b) {
                             if (bin2 < 0 && k < run + counts2[b]) {
                                 bin2 = b;
                                 base2 = run;
+                                in2 = counts2[b];
                             }
                             run += counts2[b];
                         }
                         if (bin2 < 0) {
                             bin2 = 0;
                             base2 = below2;
+                            in2 = 0;
                         }
-                        m = lo2 + (bin2 << shift2);
-                        c = base2;
+                        // Start from the bin's end when the target is nearer to
+                        // it (c there counts the whole bin); the end must stay
+                        // within the ranked samples plus one.
+                        const int m_end = lo2 + ((bin2 + 1) << shift2);
+                        const bool from_end = in2 > 0 && m_end <= retained + 1 && base2 + in2 - k < k - base2;
+                        m = from_end ? m_end : lo2 + (bin2 << shift2);
+                        c = from_end ? base2 + in2 : base2;
                     }
                 } else if (bin >= 0) {
                     m = 1 + (bin << shift1);
@@ -1168,9 +1177,11 @@ __device__ __forceinline__ void sort_inside(Key *xk, uint16_t *xp, uint16_t *cod
     Key *keys = xk;
     uint16_t *pos0 = reinterpret_cast<uint16_t *>(keys + N) + GUARD;
     Chunk<Key, E> chunk;
+    // Thread tid starts with elements I * NT + tid (consecutive entries per
+    // lane; the sorted result does not depend on the initial assignment).
     static_for<0, E>([&](auto ic) {
         constexpr int I = decltype(ic)::value;
-        const int gi = tid * E + I;
+        const int gi = I * NT + tid;
         chunk.k[I] = gi < retained ? xk[gi] : KT::SENTINEL;
         chunk.p[I] = gi < retained ? xp[gi] : POS_NONE;
     });
@@ -1431,22 +1442,42 @@ __global__ void __launch_bounds__(NT, E >= 32 ? 1 : (sizeof(T) == 8 && E >= 16 ?
                         col1[q] = (u16x2)(unsigned short)0;
                     }
                     const bool has1 = lane + 64 < a.sx;
-                    for (int dy = 0; dy < a.ky; ++dy) {
-                        const int row = y_first + dy;
-                        const RowMasks &rm = rowmasks[row];
-                        const uint16_t *crow = codes + row * a.sx;
-                        // Samples that are not inside get 0x7FFF, above every
-                        // rank, so a 16-bit difference's sign is `code < m_j`.
-                        const unsigned short e0 =
-                            ((rm.inside[0] >> lane) & 1ull) ? crow[lane] : (unsigned short)0x7FFF;
-                        const unsigned short e1 =
-                            (has1 && ((rm.inside[1] >> lane) & 1ull)) ? crow[lane + 64] : (unsigned short)0x7FFF;
+                    // Samples that are not inside get 0x7FFF, above every rank,
+                    // so a 16-bit difference's sign is `code < m_j`. The masks
+                    // and codes of four rows are read before any is used (the
+                    // code reads are unconditional; codes of samples that are
+                    // not inside are stale and replaced).
+                    auto add_row = [&](uint64_t in0, uint64_t in1, unsigned short c0, unsigned short c1) {
+                        const unsigned short e0 = ((in0 >> lane) & 1ull) ? c0 : (unsigned short)0x7FFF;
+                        const unsigned short e1 = (has1 && ((in1 >> lane) & 1ull)) ? c1 : (unsigned short)0x7FFF;
                         const u16x2 v0 = (u16x2)e0, v1 = (u16x2)e1;
 #pragma unroll
                         for (int q = 0; q < START / 2; ++q) {
                             col0[q] += (u16x2)(v0 - thr[q]) >> (u16x2)15;
                             col1[q] += (u16x2)(v1 - thr[q]) >> (u16x2)15;
                         }
+                    };
+                    int dy = 0;
+                    for (; dy + 4 <= a.ky; dy += 4) {
+                        uint64_t in0[4], in1[4];
+                        unsigned short c0[4], c1[4];
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) {
+                            const int row = y_first + dy + r;
+                            in0[r] = rowmasks[row].inside[0];
+                            in1[r] = rowmasks[row].inside[1];
+                            c0[r] = codes[row * a.sx + lane];
+                            c1[r] = codes[row * a.sx + lane + 64];
+                        }
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) {
+                            add_row(in0[r], in1[r], c0[r], c1[r]);
+                        }
+                    }
+                    for (; dy < a.ky; ++dy) {
+                        const int row = y_first + dy;
+                        add_row(rowmasks[row].inside[0], rowmasks[row].inside[1], codes[row * a.sx + lane],
+                                codes[row * a.sx + lane + 64]);
                     }
                     // Inclusive prefix over columns 0..127 (two candidates per
                     // register; counts stay below 2^16).
