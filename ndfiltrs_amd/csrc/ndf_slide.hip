@@ -124,21 +124,28 @@ __device__ __forceinline__ uint32_t contains2_bytes(uint32_t w, uint32_t base2, 
     return ok & (ok >> 8) & 0x00800080u;
 }
 
+// Byte-wise membership bits of the BATCH packed positions in w (rectangles
+// with g.bytes_ok).
+__device__ __forceinline__ uint32_t bytes_mask(const uint32_t w[BATCH / 2], const Geometry &g, uint32_t base)
+{
+    const uint32_t base2 = base * 0x10001u;
+    const uint32_t vmask2 = g.vmask * 0x10001u;
+    const uint32_t bounds = ((uint32_t)g.kx | ((uint32_t)g.ky << 8)) * 0x10001u;
+    uint32_t acc = 0;
+#pragma unroll
+    for (int j = 0; j < BATCH / 2; ++j) {
+        const uint32_t both = contains2_bytes(w[j] & vmask2, base2, bounds);
+        acc |= 2 * j >= 7 ? both << (2 * j - 7) : both >> (7 - 2 * j);
+    }
+    return (acc & 0x5555u) | ((acc >> 15) & 0xAAAAu);
+}
+
 __device__ __forceinline__ uint32_t batch_mask(const uint16_t *pos, int first, const Geometry &g, uint32_t base)
 {
     if (g.mask == nullptr && g.bytes_ok) {
         uint32_t w[BATCH / 2];
         __builtin_memcpy(w, pos + first, sizeof(w));
-        const uint32_t base2 = base * 0x10001u;
-        const uint32_t vmask2 = g.vmask * 0x10001u;
-        const uint32_t bounds = ((uint32_t)g.kx | ((uint32_t)g.ky << 8)) * 0x10001u;
-        uint32_t acc = 0;
-#pragma unroll
-        for (int j = 0; j < BATCH / 2; ++j) {
-            const uint32_t both = contains2_bytes(w[j] & vmask2, base2, bounds);
-            acc |= 2 * j >= 7 ? both << (2 * j - 7) : both >> (7 - 2 * j);
-        }
-        return (acc & 0x5555u) | ((acc >> 15) & 0xAAAAu);
+        return bytes_mask(w, g, base);
     }
     uint32_t v[BATCH];
 #pragma unroll
@@ -188,6 +195,34 @@ __device__ inline bool walk_to(int &m, int &c, int k, const uint16_t *pos, int r
     int need = up ? k - c : c - k;
     int first = up ? m : m - BATCH;
     const int advance = up ? BATCH : -BATCH;
+    if (g.mask == nullptr && g.bytes_ok) {
+        // Rectangles: the next batch's positions are read while this one is
+        // tested. A read past the last batch stays inside LDS (keys lie below
+        // the positions, the codes above) and is never used.
+        uint32_t cur[BATCH / 2];
+        __builtin_memcpy(cur, pos + first, sizeof(cur));
+        for (int step = 0; step <= limit; ++step) {
+            uint32_t next[BATCH / 2];
+            __builtin_memcpy(next, pos + first + advance, sizeof(next));
+            const uint32_t mask = bytes_mask(cur, g, base);
+            const int count = __builtin_popcount(mask);
+            if (up ? count > need : count >= need) {
+                m = first + select_bit16(mask, up ? need : count - need);
+                c = k;
+                return true;
+            }
+            need -= count;
+            first += advance;
+            if (up ? first > retained : first + BATCH <= 1) {
+                return false;
+            }
+#pragma unroll
+            for (int j = 0; j < BATCH / 2; ++j) {
+                cur[j] = next[j];
+            }
+        }
+        return false;
+    }
     for (int step = 0; step <= limit; ++step) {
         const uint32_t mask = batch_mask(pos, first, g, base);
         const int count = __builtin_popcount(mask);
