@@ -307,6 +307,46 @@ def end_to_end(image: np.ndarray, repeats: int = 3, **kwargs) -> dict:
             "what": "nanmedian_filter on a host array: validation + H2D + kernel + D2H, median"}
 
 
+def repair_bench(image: np.ndarray, steps: int, warmup: int, check: bool) -> dict:
+    """fixpix(method="median") on device buffers: 1 % of the pixels masked
+    (seeded), windows 5x5 and 31x31, reflect, both device paths."""
+    from ndfiltrs_amd import _core
+
+    rng = np.random.default_rng(20261209)
+    mask = rng.random(image.shape) < 0.01
+    positions = np.flatnonzero(mask).astype(np.int64)
+    d_in, d_out, d_snap = (_core.DeviceArray(image.nbytes) for _ in range(3))
+    d_pos = _core.DeviceArray(positions.nbytes)
+    d_in.upload(image)
+    d_pos.upload(positions)
+    result = {"workload": f"fixpix median, {positions.size} masked pixels (1 %) of a {image.shape[0]}x"
+                          f"{image.shape[1]} float32 sky, reflect, cval NaN", "masked": int(positions.size)}
+    for size in (5, 31):
+        plan = _core.Plan(image.dtype, image.shape, "median", [size, size], None, [0, 0], ["reflect"] * 2, np.nan,
+                          True, False, 1, 0)
+        entry = {}
+        outputs = {}
+        for path in ("sparse", "dense"):
+            ms = plan.time_repair_device(d_in.address, d_out.address, d_snap.address, d_pos.address,
+                                         positions.size, path, warmup, steps)
+            plan.repair_device(d_in.address, d_out.address, d_snap.address, d_pos.address, positions.size, path)
+            outputs[path] = d_out.download(np.empty_like(image))
+            entry[path] = {"ms": ms, "masked_mpix_s": positions.size / (ms * 1e-3) / 1e6,
+                           "image_mpix_s": image.size / (ms * 1e-3) / 1e6}
+        entry["paths_bit_equal"] = bool(np.array_equal(outputs["sparse"].view(np.uint32),
+                                                       outputs["dense"].view(np.uint32)))
+        if check:
+            from oracle import oracle
+
+            want = oracle.fixpix_median(image, mask, (size, size), mode="reflect")
+            entry["bit_exact_vs_oracle"] = bool(np.array_equal(outputs["sparse"].view(np.uint32),
+                                                               want.view(np.uint32)))
+        result[f"{size}x{size}"] = entry
+    for buffer in (d_in, d_out, d_snap, d_pos):
+        buffer.free()
+    return result
+
+
 def load_traffic() -> dict:
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(path):
@@ -400,6 +440,9 @@ def main() -> int:
             extra["cube_3x7x7_f32"]["cpu"] = cpu_sample(c3b.image, 8, "direct", size=(3, 7, 7), mode="wrap",
                                                         omit_nan=True, ignore_inf=True)
         c3b.free()
+        log("fixpix median on the 8192x8192 sky: sparse and dense repair paths")
+        extra["fixpix_median_1pct"] = repair_bench(c5_image, args.steps, args.warmup, check=not args.no_cpu)
+        log(json.dumps(extra["fixpix_median_1pct"]))
         log("512 cutouts: batched graph launch vs separate calls")
         extra["cutouts_512"] = cutout_batch(args.steps, args.warmup)
         log(json.dumps(extra["cutouts_512"]))

@@ -115,6 +115,43 @@ ndf_status ndf_plan_describe(const ndf_plan *plan, char *buffer, size_t buffer_l
 
 void ndf_plan_destroy(ndf_plan *plan);
 
+/* Mask-directed median repair (fixpix(method="median")).
+ * Replaces _core.fixpix_median(array, mask, size, footprint, origin, mode,
+ * cval, ignore_inf, min_count) (src/pyapi/repair.rs:125-183) and
+ * fixpix_median (src/repair.rs:107-126) with its repair_median
+ * (src/filter_nd.rs:174-215): every output whose mask byte is nonzero is the
+ * NaN-omitting median (Omit{ignore_inf, min_count}) of its window over a
+ * snapshot of `input` in which every masked element is NaN; every other
+ * output equals the input. `mask` has mask_len bytes (LENGTH_MISMATCH unless
+ * mask_len == prod(shape)); the other arguments are those of
+ * ndf_window_filter. Blocking; host buffers. */
+ndf_status ndf_fixpix_median(int32_t dtype_bytes, const void *input, void *output, const uint8_t *mask,
+                             size_t mask_len, size_t rank, const size_t *shape, const size_t *size,
+                             const uint8_t *footprint, const ptrdiff_t *origin, const int32_t *modes, double cval,
+                             int32_t ignore_inf, size_t min_count, char *err, size_t err_len);
+
+/* Repair paths: SPARSE evaluates only the listed positions (one generic-
+ * engine thread each), DENSE runs the plan's engine over the whole snapshot
+ * and keeps the listed outputs, AUTO picks by the fraction of listed
+ * positions. All give identical bits. */
+enum { NDF_REPAIR_AUTO = 0, NDF_REPAIR_SPARSE = 1, NDF_REPAIR_DENSE = 2 };
+
+/* Device-resident form of the repair for any plan (fixpix uses a median
+ * plan with omit_nan = 1): d_output = d_input with the plan's statistic over
+ * the snapshot at the `npos` flat indices in d_positions (int64). d_snapshot
+ * is caller scratch of `len` elements. Positions may come in any order;
+ * one outside [0, len) gives LENGTH_MISMATCH. Blocking on `stream`. No
+ * reference counterpart (the reference has no device). */
+ndf_status ndf_plan_repair_device(ndf_plan *plan, const void *d_input, void *d_output, void *d_snapshot,
+                                  const int64_t *d_positions, size_t npos, size_t len, int32_t path, void *stream,
+                                  char *err, size_t err_len);
+/* Times `iterations` repairs on one stream with HIP events (mean ms per
+ * repair: snapshot, estimates and the result copy). */
+ndf_status ndf_plan_time_repair_device(ndf_plan *plan, const void *d_input, void *d_output, void *d_snapshot,
+                                       const int64_t *d_positions, size_t npos, size_t len, int32_t path,
+                                       int32_t warmup, int32_t iterations, double *ms_per_call, char *err,
+                                       size_t err_len);
+
 /* Learned bracket (2-D images on the slide engine; no reference
  * counterpart). A tiny MLP (6 -> 32 -> 32 -> 2, ReLU, fp32; NDF_BRACKET_PARAMS
  * floats: W1[6][32], b1[32], W2[32][32], b2[32], W3[32][2], b3[2]) proposes

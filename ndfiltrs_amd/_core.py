@@ -235,6 +235,31 @@ class Plan:
             _raise(status, err.value.decode())
         return features, targets, weight
 
+    def repair_device(self, d_input: int, d_output: int, d_snapshot: int, d_positions: int, npos: int,
+                      path: str = "auto", stream: int | None = None) -> None:
+        """Mask-directed repair of device buffers (see ``ndf_plan_repair_device``);
+        ``d_positions`` holds ``npos`` int64 flat indices."""
+        err = ctypes.create_string_buffer(256)
+        status = _lib.lib.ndf_plan_repair_device(
+            self._handle, d_input, d_output, d_snapshot, d_positions, npos, self.len, _lib.REPAIR_PATHS[path],
+            stream, err, len(err)
+        )
+        if status != _lib.OK:
+            _raise(status, err.value.decode())
+
+    def time_repair_device(self, d_input: int, d_output: int, d_snapshot: int, d_positions: int, npos: int,
+                           path: str, warmup: int, iterations: int) -> float:
+        """Mean milliseconds per repair (HIP events on one stream)."""
+        err = ctypes.create_string_buffer(256)
+        ms = ctypes.c_double(0.0)
+        status = _lib.lib.ndf_plan_time_repair_device(
+            self._handle, d_input, d_output, d_snapshot, d_positions, npos, self.len, _lib.REPAIR_PATHS[path],
+            warmup, iterations, ctypes.byref(ms), err, len(err)
+        )
+        if status != _lib.OK:
+            _raise(status, err.value.decode())
+        return ms.value
+
     def time_device(self, d_input: int, d_output: int, warmup: int, iterations: int,
                     main_kernel: bool = False):
         """Mean milliseconds per launch (HIP events on one stream); with
@@ -290,6 +315,79 @@ def window_filter(
         parameter,
     )
     return plan.apply(typed)
+
+
+def fixpix_median(
+    array: np.ndarray,
+    mask: np.ndarray,
+    size: Sequence[int],
+    footprint: Sequence[bool] | None,
+    origin: Sequence[int],
+    mode: Sequence[str],
+    cval: float,
+    ignore_inf: bool,
+    min_count: int,
+) -> np.ndarray:
+    """Median repair of the masked pixels (``_core.fixpix_median``,
+    ``src/pyapi/repair.rs:125-183``) on the GPU.
+
+    ``mask`` is uint8 with one entry per element (nonzero = bad). The data
+    and mask are snapshotted while the GIL is held; the GIL is released for
+    the device work. Returns a new owned C-contiguous array.
+    """
+    typed = _typed_array(array)
+    bad = np.asarray(mask)
+    if bad.dtype != np.uint8:
+        raise TypeError("argument 'mask': expected a uint8 numpy.ndarray")
+    ndim = typed.ndim
+    size = _as_sizes(size, "size")
+    origin = [int(value) for value in origin]
+    min_count = _as_sizes([min_count], "min_count")[0]
+    modes = []
+    for value in mode:
+        if value not in _lib.MODES:
+            raise ValueError("unsupported boundary mode")
+        modes.append(_lib.MODES.index(value))
+    if ndim == 0:
+        raise ValueError(_MSG_RANK0)
+    if len(size) != ndim or len(origin) != ndim or len(modes) != ndim:
+        raise ValueError(_MSG_EQUAL_RANK)
+    fp = None
+    if footprint is not None:
+        fp = np.asarray([bool(value) for value in footprint], dtype=np.uint8)
+        volume = 1
+        for value in size:
+            volume *= value
+        if 0 in size:
+            raise ValueError(_MSG_SIZE0)
+        if len(fp) != volume:
+            raise ValueError(_MSG_LENGTH)
+    # Snapshot (GIL held): private C-ordered copies of the data and the mask.
+    data = np.array(typed, dtype=typed.dtype, order="C", copy=True)
+    flags = np.array(bad, dtype=np.uint8, order="C", copy=True)
+    output = np.zeros(data.shape, dtype=data.dtype)
+    err = ctypes.create_string_buffer(256)
+    status = _lib.lib.ndf_fixpix_median(
+        data.dtype.itemsize,
+        data.ctypes.data,
+        output.ctypes.data,
+        flags.ctypes.data,
+        flags.size,
+        ndim,
+        (ctypes.c_size_t * ndim)(*data.shape),
+        (ctypes.c_size_t * ndim)(*size),
+        None if fp is None else fp.ctypes.data,
+        (ctypes.c_ssize_t * ndim)(*origin),
+        (ctypes.c_int32 * ndim)(*modes),
+        float(cval),
+        int(bool(ignore_inf)),
+        min_count,
+        err,
+        len(err),
+    )
+    if status != _lib.OK:
+        _raise(status, err.value.decode())
+    return output
 
 
 def get_num_threads() -> int:

@@ -31,6 +31,7 @@ KINDS = {
     "rank_histogram": 4,
 }
 ENGINES = {"auto": 0, "generic": 1, "slide": 2, "network": 3}
+REPAIR_PATHS = {"auto": 0, "sparse": 1, "dense": 2}
 
 _size_p = ctypes.POINTER(ctypes.c_size_t)
 _ssize_p = ctypes.POINTER(ctypes.c_ssize_t)
@@ -180,7 +181,22 @@ def _declare(lib: ctypes.CDLL) -> None:
     lib.ndf_batch_describe.argtypes = [batch_p, ctypes.c_char_p, ctypes.c_size_t]
     lib.ndf_batch_destroy.argtypes = [batch_p]
     lib.ndf_batch_destroy.restype = None
+    lib.ndf_fixpix_median.argtypes = [
+        ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_size_t,
+        _size_p, _size_p, ctypes.c_void_p, _ssize_p, _i32_p, ctypes.c_double, ctypes.c_int32, ctypes.c_size_t, *_err,
+    ]
+    lib.ndf_plan_repair_device.argtypes = [
+        plan_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
+        ctypes.c_size_t, ctypes.c_int32, ctypes.c_void_p, *_err,
+    ]
+    lib.ndf_plan_time_repair_device.argtypes = [
+        plan_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
+        ctypes.c_size_t, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_double), *_err,
+    ]
     for name in (
+        "ndf_fixpix_median",
+        "ndf_plan_repair_device",
+        "ndf_plan_time_repair_device",
         "ndf_batch_create",
         "ndf_batch_configure",
         "ndf_batch_autotune",

@@ -67,7 +67,15 @@ __global__ void __launch_bounds__(256) k_generic(GenericArgs a)
     using KT = KeyTraits<T>;
     using Key = typename KT::Key;
     const int64_t step = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t index = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; index < a.len; index += step) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.len; i += step) {
+        int64_t index = i;
+        if (a.positions) {
+            index = a.positions[i];
+            if (index < 0 || index >= a.extent) {
+                atomicOr(a.error_flag, 1);
+                continue;
+            }
+        }
         Gather<T> g(a, index);
         int64_t n = 0;
         for (int t = 0; t < a.count; ++t) {
@@ -91,12 +99,142 @@ __global__ void __launch_bounds__(256) k_generic(GenericArgs a)
     }
 }
 
+// Sparse repair, one wave per listed position (windows of at most
+// REPAIR_WAVE_MAX_TAPS taps). The wave gathers the window once, 64 taps per
+// step, and compacts the retained keys into its own LDS region (ballot +
+// prefix count). Selection is the same bitwise search as k_generic, but
+// each pass counts the keys below the candidate across the wave: every lane
+// scans n/64 keys and the counts are summed across lanes. An even median
+// takes the (n/2 - 1)-th key and then the smallest key above it, unless that
+// key repeats.
+constexpr int REPAIR_WAVES = 4;
+
+__device__ inline int64_t wave_sum(int64_t v)
+{
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d);
+    return v;
+}
+
+template <typename Key>
+__device__ inline Key wave_min(Key v)
+{
+    for (int d = 32; d >= 1; d >>= 1) {
+        const Key o = __shfl_xor(v, d);
+        v = o < v ? o : v;
+    }
+    return v;
+}
+
+__device__ inline void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// k-th smallest (0-based) of the n keys in `keys`; requires k < n.
+template <typename Key>
+__device__ Key wave_select(const Key *keys, int n, int64_t k, int lane)
+{
+    constexpr int BITS = sizeof(Key) * 8;
+    Key answer = 0;
+    for (int b = BITS - 1; b >= 0; --b) {
+        const Key candidate = answer | (Key(1) << b);
+        int64_t below = 0;
+        for (int j = lane; j < n; j += 64) below += keys[j] < candidate;
+        if (wave_sum(below) <= k) answer = candidate;
+    }
+    return answer;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(64 * REPAIR_WAVES) k_repair_wave(GenericArgs a)
+{
+    using KT = KeyTraits<T>;
+    using Key = typename KT::Key;
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    Key *keys = reinterpret_cast<Key *>(lds_raw) + (size_t)wave * a.count;
+    const int64_t step = (int64_t)gridDim.x * REPAIR_WAVES;
+    for (int64_t i = (int64_t)blockIdx.x * REPAIR_WAVES + wave; i < a.len; i += step) {
+        const int64_t index = a.positions[i];
+        if (index < 0 || index >= a.extent) {
+            if (lane == 0) atomicOr(a.error_flag, 1);
+            continue;
+        }
+        Gather<T> g(a, index);
+        int n = 0;
+        for (int t0 = 0; t0 < a.count; t0 += 64) {
+            const int t = t0 + lane;
+            const Key key = t < a.count ? g.key(t) : KT::SENTINEL;
+            const bool keep = key != KT::SENTINEL;
+            const uint64_t ballot = __ballot(keep);
+            if (keep) keys[n + __popcll(ballot & ((1ull << lane) - 1))] = key;
+            n += __popcll(ballot);
+        }
+        wave_lds_sync();
+        Key out = KT::CANONICAL_NAN;
+        if (n >= a.min_count && n > 0) {
+            if (a.stat == STAT_RANK) {
+                if (a.rank_param < n) out = KT::unkey(wave_select(keys, n, a.rank_param, lane));
+            } else if (n & 1) {
+                out = KT::unkey(wave_select(keys, n, n / 2, lane));
+            } else {
+                const Key lower = wave_select(keys, n, n / 2 - 1, lane);
+                int64_t at_most = 0;
+                Key above = KT::SENTINEL;
+                for (int j = lane; j < n; j += 64) {
+                    const Key v = keys[j];
+                    at_most += v <= lower;
+                    if (v > lower && v < above) above = v;
+                }
+                at_most = wave_sum(at_most);
+                above = wave_min(above);
+                out = median_pair_bits<T>(lower, at_most > n / 2 ? lower : above);
+            }
+        }
+        if (lane == 0) static_cast<Key *>(a.output)[index] = out;
+        wave_lds_sync();
+    }
+}
+
 template <typename Key>
 __global__ void __launch_bounds__(256) k_fill(Key *out, int64_t len, Key bits)
 {
     const int64_t step = (int64_t)gridDim.x * blockDim.x;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < len; i += step) {
         out[i] = bits;
+    }
+}
+
+template <typename Key>
+__global__ void __launch_bounds__(256)
+    k_scatter_fill(Key *out, const int64_t *positions, int64_t n, int64_t extent, Key bits, int32_t *error_flag)
+{
+    const int64_t step = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += step) {
+        const int64_t p = positions[i];
+        if (p < 0 || p >= extent) {
+            atomicOr(error_flag, 1);
+        } else {
+            out[p] = bits;
+        }
+    }
+}
+
+template <typename Key>
+__global__ void __launch_bounds__(256) k_scatter_copy(Key *dst, const Key *src, const int64_t *positions, int64_t n,
+                                                      int64_t extent, int32_t *error_flag)
+{
+    const int64_t step = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += step) {
+        const int64_t p = positions[i];
+        if (p < 0 || p >= extent) {
+            atomicOr(error_flag, 1);
+        } else {
+            dst[p] = src[p];
+        }
     }
 }
 
@@ -119,6 +257,26 @@ hipError_t launch_generic(int dtype_bytes, const GenericArgs &args, hipStream_t 
     return hipGetLastError();
 }
 
+size_t repair_wave_lds_bytes(int dtype_bytes, int count) { return (size_t)REPAIR_WAVES * count * dtype_bytes; }
+
+hipError_t launch_repair_wave(int dtype_bytes, const GenericArgs &args, hipStream_t stream)
+{
+    int64_t blocks = (args.len + REPAIR_WAVES - 1) / REPAIR_WAVES;
+    if (blocks > 8192) blocks = 8192;
+    if (blocks < 1) blocks = 1;
+    const size_t lds = repair_wave_lds_bytes(dtype_bytes, args.count);
+    const void *fn = dtype_bytes == 4 ? reinterpret_cast<const void *>(&k_repair_wave<float>)
+                                      : reinterpret_cast<const void *>(&k_repair_wave<double>);
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    if (dtype_bytes == 4) {
+        hipLaunchKernelGGL(k_repair_wave<float>, dim3((unsigned)blocks), dim3(64 * REPAIR_WAVES), lds, stream, args);
+    } else {
+        hipLaunchKernelGGL(k_repair_wave<double>, dim3((unsigned)blocks), dim3(64 * REPAIR_WAVES), lds, stream, args);
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_fill(int dtype_bytes, void *output, int64_t len, uint64_t bits, hipStream_t stream)
 {
     const dim3 grid((unsigned)grid_for(len));
@@ -128,6 +286,34 @@ hipError_t launch_fill(int dtype_bytes, void *output, int64_t len, uint64_t bits
     } else {
         hipLaunchKernelGGL(k_fill<uint64_t>, grid, dim3(256), 0, stream, static_cast<uint64_t *>(output), len,
                            bits);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_scatter_fill(int dtype_bytes, void *output, const int64_t *positions, int64_t n, int64_t extent,
+                               uint64_t bits, int32_t *error_flag, hipStream_t stream)
+{
+    const dim3 grid((unsigned)grid_for(n));
+    if (dtype_bytes == 4) {
+        hipLaunchKernelGGL(k_scatter_fill<uint32_t>, grid, dim3(256), 0, stream, static_cast<uint32_t *>(output),
+                           positions, n, extent, (uint32_t)bits, error_flag);
+    } else {
+        hipLaunchKernelGGL(k_scatter_fill<uint64_t>, grid, dim3(256), 0, stream, static_cast<uint64_t *>(output),
+                           positions, n, extent, bits, error_flag);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_scatter_copy(int dtype_bytes, void *dst, const void *src, const int64_t *positions, int64_t n,
+                               int64_t extent, int32_t *error_flag, hipStream_t stream)
+{
+    const dim3 grid((unsigned)grid_for(n));
+    if (dtype_bytes == 4) {
+        hipLaunchKernelGGL(k_scatter_copy<uint32_t>, grid, dim3(256), 0, stream, static_cast<uint32_t *>(dst),
+                           static_cast<const uint32_t *>(src), positions, n, extent, error_flag);
+    } else {
+        hipLaunchKernelGGL(k_scatter_copy<uint64_t>, grid, dim3(256), 0, stream, static_cast<uint64_t *>(dst),
+                           static_cast<const uint64_t *>(src), positions, n, extent, error_flag);
     }
     return hipGetLastError();
 }

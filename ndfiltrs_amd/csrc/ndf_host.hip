@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <memory>
 #include <mutex>
 #include <new>
 #include <string>
@@ -221,8 +222,10 @@ struct ndf_plan {
     // Device state.
     std::mutex mutex;
     bool tables_ready = false;
+    bool generic_ready = false;
     bool uploaded = false;
     DeviceBuffer g_maps, g_taps, s_mapz, s_mapy, s_mapx, s_taps, s_rm, s_add, s_mask, d_in, d_out, d_error;
+    DeviceBuffer r_flag, r_snap, r_pos;  // mask-directed repair: position flag, snapshot, positions
     DeviceBuffer n_mapz, n_mapy, n_mapx, n_taps;
     GenericArgs generic{};
 
@@ -232,7 +235,7 @@ struct ndf_plan {
             b->release();
         }
         for (DeviceBuffer *b : {&g_maps, &g_taps, &s_mapz, &s_mapy, &s_mapx, &s_taps, &s_rm, &s_add, &s_mask, &d_in,
-                                &d_out, &d_error, &b_weights, &b_blocks, &b_features, &b_moments, &b_lo, &b_hi,
+                                &d_out, &d_error, &r_flag, &r_snap, &r_pos, &b_weights, &b_blocks, &b_features, &b_moments, &b_lo, &b_hi,
                                 &b_pred, &b_fallback, &b_fcount}) {
             b->release();
         }
@@ -519,48 +522,60 @@ ndf_status prepare_tables(ndf_plan *p, char *err, size_t err_len)
     return st;
 }
 
+// Generic-engine tables (per-axis maps and tap offsets). Also used by the
+// sparse repair path whatever engine the plan runs for whole images.
+ndf_status prepare_generic_tables(ndf_plan *p, char *err, size_t err_len)
+{
+    if (p->generic_ready) return NDF_OK;
+    hipError_t e;
+    std::vector<int64_t> maps;
+    GenericArgs &g = p->generic;
+    g = GenericArgs{};
+    g.rank = (int32_t)p->rank;
+    g.count = (int32_t)p->count;
+    g.len = (int64_t)p->len;
+    int64_t stride = 1;
+    for (size_t a = p->rank; a-- > 0;) {
+        g.stride[a] = stride;
+        stride *= (int64_t)p->shape[a];
+    }
+    for (size_t a = 0; a < p->rank; ++a) {
+        g.map_base[a] = (int64_t)maps.size();
+        const int64_t n = (int64_t)p->shape[a];
+        const int64_t length = n + p->high[a] - p->low[a];
+        for (int64_t i = 0; i < length; ++i) {
+            const int64_t m = map_coordinate(i + p->low[a], n, p->modes[a]);
+            maps.push_back(m < 0 ? MAP_CVAL : m * g.stride[a]);
+        }
+    }
+    std::vector<int32_t> taps(p->count * p->rank);
+    for (size_t t = 0; t < p->count; ++t) {
+        for (size_t a = 0; a < p->rank; ++a) {
+            taps[t * p->rank + a] = (int32_t)(p->taps[t * p->rank + a] - p->low[a]);
+        }
+    }
+    if ((e = upload(p->g_maps, maps)) != hipSuccess || (e = upload(p->g_taps, taps)) != hipSuccess) {
+        return fail_hip(e, err, err_len);
+    }
+    g.maps = static_cast<const int64_t *>(p->g_maps.ptr);
+    g.taps = static_cast<const int32_t *>(p->g_taps.ptr);
+    g.cval_bits = p->cval_bits;
+    g.ignore_inf = p->eff_ignore_inf;
+    g.stat = p->stat;
+    g.min_count = p->eff_min_count;
+    g.rank_param = (int64_t)std::min(p->parameter, p->count);
+    p->generic_ready = true;
+    return NDF_OK;
+}
+
 ndf_status prepare_engine_tables(ndf_plan *p, char *err, size_t err_len)
 {
     if (p->tables_ready) return NDF_OK;
     hipError_t e = p->d_error.ensure(sizeof(int32_t));
     if (e != hipSuccess) return fail_hip(e, err, err_len);
     if (p->engine == NDF_ENGINE_GENERIC) {
-        std::vector<int64_t> maps;
-        GenericArgs &g = p->generic;
-        g = GenericArgs{};
-        g.rank = (int32_t)p->rank;
-        g.count = (int32_t)p->count;
-        g.len = (int64_t)p->len;
-        int64_t stride = 1;
-        for (size_t a = p->rank; a-- > 0;) {
-            g.stride[a] = stride;
-            stride *= (int64_t)p->shape[a];
-        }
-        for (size_t a = 0; a < p->rank; ++a) {
-            g.map_base[a] = (int64_t)maps.size();
-            const int64_t n = (int64_t)p->shape[a];
-            const int64_t length = n + p->high[a] - p->low[a];
-            for (int64_t i = 0; i < length; ++i) {
-                const int64_t m = map_coordinate(i + p->low[a], n, p->modes[a]);
-                maps.push_back(m < 0 ? MAP_CVAL : m * g.stride[a]);
-            }
-        }
-        std::vector<int32_t> taps(p->count * p->rank);
-        for (size_t t = 0; t < p->count; ++t) {
-            for (size_t a = 0; a < p->rank; ++a) {
-                taps[t * p->rank + a] = (int32_t)(p->taps[t * p->rank + a] - p->low[a]);
-            }
-        }
-        if ((e = upload(p->g_maps, maps)) != hipSuccess || (e = upload(p->g_taps, taps)) != hipSuccess) {
-            return fail_hip(e, err, err_len);
-        }
-        g.maps = static_cast<const int64_t *>(p->g_maps.ptr);
-        g.taps = static_cast<const int32_t *>(p->g_taps.ptr);
-        g.cval_bits = p->cval_bits;
-        g.ignore_inf = p->eff_ignore_inf;
-        g.stat = p->stat;
-        g.min_count = p->eff_min_count;
-        g.rank_param = (int64_t)std::min(p->parameter, p->count);
+        const ndf_status st = prepare_generic_tables(p, err, err_len);
+        if (st != NDF_OK) return st;
     } else if (p->engine == NDF_ENGINE_NETWORK) {
         if ((e = upload(p->n_mapz, p->net_mapz)) != hipSuccess || (e = upload(p->n_mapy, p->net_mapy)) != hipSuccess ||
             (e = upload(p->n_mapx, p->net_mapx)) != hipSuccess || (e = upload(p->n_taps, p->net_taps)) != hipSuccess) {
@@ -1745,6 +1760,219 @@ ndf_status ndf_plan_time_device(ndf_plan *plan, const void *d_input, void *d_out
     if (stop) (void)hipEventDestroy(stop);
     (void)hipStreamDestroy(stream);
     return st;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Mask-directed repair of one array (the reference's repair_median,
+// src/filter_nd.rs:174-215): the snapshot is the input with every listed
+// position set to NaN; each listed output is the plan's statistic over the
+// snapshot, every other output is the input. All estimates read the
+// snapshot only, so their order (and any duplicate position) cannot change
+// a result.
+//
+// Two device forms give identical bits:
+//   sparse  one wave per position (k_repair_wave: the window staged in LDS,
+//           a wave-wide bitwise select), or one generic-engine thread per
+//           position for windows above REPAIR_WAVE_MAX_TAPS taps
+//           (cost ~ positions);
+//   dense   the plan's engine over the whole snapshot, then the estimates
+//           are scattered back (cost ~ array).
+// Auto picks sparse when the plan has no tiled engine, or when the wave
+// kernel applies and at most 1/REPAIR_SPARSE_RATIO of the elements are
+// listed.
+constexpr double REPAIR_SPARSE_RATIO = 16.0;
+
+bool repair_sparse(const ndf_plan *p, size_t npos, int32_t path)
+{
+    if (path == NDF_REPAIR_SPARSE) return true;
+    if (path == NDF_REPAIR_DENSE) return false;
+    if (!p->slide_ok && !p->network_ok) return true;
+    return p->count <= (size_t)REPAIR_WAVE_MAX_TAPS && (double)npos * REPAIR_SPARSE_RATIO <= (double)p->len;
+}
+
+// Enqueues the repair on `stream`; positions outside the array set
+// p->r_flag. The caller holds p->mutex.
+ndf_status repair_launch(ndf_plan *p, const void *d_in, void *d_out, void *d_snap, const int64_t *d_pos,
+                         size_t npos, int32_t path, hipStream_t stream, char *err, size_t err_len)
+{
+    const size_t bytes = p->len * (size_t)p->dbytes;
+    const uint64_t nan_bits = p->dbytes == 4 ? 0x7FC00000ull : 0x7FF8000000000000ull;
+    int32_t *flag = static_cast<int32_t *>(p->r_flag.ptr);
+    const int64_t extent = (int64_t)p->len;
+    hipError_t e = launch_fill(4, flag, 1, 0, stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_snap, d_in, bytes, hipMemcpyDeviceToDevice, stream);
+    if (e == hipSuccess) e = launch_scatter_fill(p->dbytes, d_snap, d_pos, (int64_t)npos, extent, nan_bits, flag, stream);
+    if (e != hipSuccess) return fail_hip(e, err, err_len);
+    if (repair_sparse(p, npos, path)) {
+        ndf_status st = prepare_generic_tables(p, err, err_len);
+        if (st != NDF_OK) return st;
+        e = hipMemcpyAsync(d_out, d_in, bytes, hipMemcpyDeviceToDevice, stream);
+        if (e == hipSuccess && p->all_nan) {
+            e = launch_scatter_fill(p->dbytes, d_out, d_pos, (int64_t)npos, extent, nan_bits, flag, stream);
+        } else if (e == hipSuccess) {
+            GenericArgs args = p->generic;
+            args.input = d_snap;
+            args.output = d_out;
+            args.len = (int64_t)npos;
+            args.positions = d_pos;
+            args.extent = extent;
+            args.error_flag = flag;
+            e = p->count <= (size_t)REPAIR_WAVE_MAX_TAPS ? launch_repair_wave(p->dbytes, args, stream)
+                                                        : launch_generic(p->dbytes, args, stream);
+        }
+        return e == hipSuccess ? NDF_OK : fail_hip(e, err, err_len);
+    }
+    ndf_status st = prepare_tables(p, err, err_len);
+    if (st != NDF_OK) return st;
+    // Whole-image statistic of the snapshot into d_out, estimates back into
+    // the snapshot (whose unlisted entries are the input), then the result.
+    st = launch(p, d_snap, d_out, stream, err, err_len);
+    if (st != NDF_OK) return st;
+    e = launch_scatter_copy(p->dbytes, d_snap, d_out, d_pos, (int64_t)npos, extent, flag, stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_out, d_snap, bytes, hipMemcpyDeviceToDevice, stream);
+    return e == hipSuccess ? NDF_OK : fail_hip(e, err, err_len);
+}
+
+// Waits for the stream and reports a position outside the array or a rank
+// walk that left its bounds.
+ndf_status repair_finish(ndf_plan *p, hipStream_t stream, char *err, size_t err_len)
+{
+    hipError_t e = hipStreamSynchronize(stream);
+    int32_t flags[2] = {0, 0};
+    if (e == hipSuccess) e = hipMemcpy(&flags[0], p->r_flag.ptr, sizeof(int32_t), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(&flags[1], p->d_error.ptr, sizeof(int32_t), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return fail_hip(e, err, err_len);
+    if (flags[0]) return fail_code(NDF_ERR_LENGTH_MISMATCH, err, err_len);
+    if (flags[1]) return fail(NDF_ERR_DEVICE, "internal error: rank walk left its bounds", err, err_len);
+    return NDF_OK;
+}
+
+ndf_status repair_buffers(ndf_plan *p, char *err, size_t err_len)
+{
+    hipError_t e;
+    if ((e = p->r_flag.ensure(sizeof(int32_t))) != hipSuccess || (e = p->d_error.ensure(sizeof(int32_t))) != hipSuccess) {
+        return fail_hip(e, err, err_len);
+    }
+    if ((e = launch_fill(4, p->d_error.ptr, 1, 0, nullptr)) != hipSuccess) return fail_hip(e, err, err_len);
+    return NDF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+ndf_status ndf_plan_repair_device(ndf_plan *plan, const void *d_input, void *d_output, void *d_snapshot,
+                                  const int64_t *d_positions, size_t npos, size_t len, int32_t path, void *stream,
+                                  char *err, size_t err_len)
+{
+    if (len != plan->len) return fail_code(NDF_ERR_LENGTH_MISMATCH, err, err_len);
+    if (path < NDF_REPAIR_AUTO || path > NDF_REPAIR_DENSE) return fail(NDF_ERR_INVALID, "unknown repair path", err, err_len);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (len == 0) return NDF_OK;
+    std::lock_guard<std::mutex> guard(plan->mutex);
+    if (npos == 0) {
+        const hipError_t e = hipMemcpyAsync(d_output, d_input, len * (size_t)plan->dbytes, hipMemcpyDeviceToDevice, s);
+        if (e == hipSuccess) return NDF_OK;
+        return fail_hip(e, err, err_len);
+    }
+    ndf_status st = repair_buffers(plan, err, err_len);
+    if (st == NDF_OK) st = repair_launch(plan, d_input, d_output, d_snapshot, d_positions, npos, path, s, err, err_len);
+    if (st == NDF_OK) st = repair_finish(plan, s, err, err_len);
+    return st;
+}
+
+ndf_status ndf_plan_time_repair_device(ndf_plan *plan, const void *d_input, void *d_output, void *d_snapshot,
+                                       const int64_t *d_positions, size_t npos, size_t len, int32_t path,
+                                       int32_t warmup, int32_t iterations, double *ms_per_call, char *err,
+                                       size_t err_len)
+{
+    if (len != plan->len || npos == 0) return fail_code(NDF_ERR_LENGTH_MISMATCH, err, err_len);
+    if (path < NDF_REPAIR_AUTO || path > NDF_REPAIR_DENSE) return fail(NDF_ERR_INVALID, "unknown repair path", err, err_len);
+    std::lock_guard<std::mutex> guard(plan->mutex);
+    ndf_status st = repair_buffers(plan, err, err_len);
+    if (st != NDF_OK) return st;
+    hipStream_t stream = nullptr;
+    hipEvent_t start = nullptr, stop = nullptr;
+    hipError_t e = hipStreamCreateWithFlags(&stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&start);
+    if (e == hipSuccess) e = hipEventCreate(&stop);
+    if (e != hipSuccess) st = fail_hip(e, err, err_len);
+    for (int i = 0; st == NDF_OK && i < warmup; ++i) {
+        st = repair_launch(plan, d_input, d_output, d_snapshot, d_positions, npos, path, stream, err, err_len);
+    }
+    if (st == NDF_OK && (e = hipEventRecord(start, stream)) != hipSuccess) st = fail_hip(e, err, err_len);
+    for (int i = 0; st == NDF_OK && i < iterations; ++i) {
+        st = repair_launch(plan, d_input, d_output, d_snapshot, d_positions, npos, path, stream, err, err_len);
+    }
+    if (st == NDF_OK && (e = hipEventRecord(stop, stream)) != hipSuccess) st = fail_hip(e, err, err_len);
+    if (st == NDF_OK) st = repair_finish(plan, stream, err, err_len);
+    if (st == NDF_OK) {
+        float ms = 0.0f;
+        if ((e = hipEventElapsedTime(&ms, start, stop)) != hipSuccess) {
+            st = fail_hip(e, err, err_len);
+        } else {
+            *ms_per_call = iterations > 0 ? (double)ms / iterations : 0.0;
+        }
+    }
+    if (start) (void)hipEventDestroy(start);
+    if (stop) (void)hipEventDestroy(stop);
+    if (stream) (void)hipStreamDestroy(stream);
+    return st;
+}
+
+ndf_status ndf_fixpix_median(int32_t dtype_bytes, const void *input, void *output, const uint8_t *mask,
+                             size_t mask_len, size_t rank, const size_t *shape, const size_t *size,
+                             const uint8_t *footprint, const ptrdiff_t *origin, const int32_t *modes, double cval,
+                             int32_t ignore_inf, size_t min_count, char *err, size_t err_len)
+{
+    ndf_plan *plan = nullptr;
+    ndf_status st = ndf_plan_create(&plan, dtype_bytes, rank, shape, size, footprint, origin, modes, cval,
+                                    NDF_KIND_MEDIAN, 1, ignore_inf, min_count, 0, err, err_len);
+    if (st != NDF_OK) return st;
+    std::unique_ptr<ndf_plan> owner(plan);
+    // positions() in src/repair.rs:14-29: the mask must cover the array.
+    if (mask_len != plan->len) return fail_code(NDF_ERR_LENGTH_MISMATCH, err, err_len);
+    const size_t bytes = plan->len * (size_t)plan->dbytes;
+    std::vector<int64_t> positions;
+    size_t npos = 0;
+    for (size_t i = 0; i < plan->len; ++i) npos += mask[i] != 0;
+    try {
+        positions.reserve(npos);
+    } catch (const std::bad_alloc &) {
+        return fail_code(NDF_ERR_MEMORY, err, err_len);
+    }
+    for (size_t i = 0; i < plan->len; ++i) {
+        if (mask[i]) positions.push_back((int64_t)i);
+    }
+    if (npos == 0) {
+        if (bytes) memcpy(output, input, bytes);
+        return NDF_OK;
+    }
+    std::lock_guard<std::mutex> guard(plan->mutex);
+    hipError_t e;
+    if ((e = plan->d_in.ensure(bytes)) != hipSuccess || (e = plan->d_out.ensure(bytes)) != hipSuccess ||
+        (e = plan->r_snap.ensure(bytes)) != hipSuccess) {
+        return fail_hip(e, err, err_len);
+    }
+    if ((e = upload(plan->r_pos, positions)) != hipSuccess ||
+        (e = hipMemcpy(plan->d_in.ptr, input, bytes, hipMemcpyHostToDevice)) != hipSuccess) {
+        return fail_hip(e, err, err_len);
+    }
+    st = repair_buffers(plan, err, err_len);
+    if (st == NDF_OK) {
+        st = repair_launch(plan, plan->d_in.ptr, plan->d_out.ptr, plan->r_snap.ptr,
+                           static_cast<const int64_t *>(plan->r_pos.ptr), npos, NDF_REPAIR_AUTO, nullptr, err,
+                           err_len);
+    }
+    if (st == NDF_OK) st = repair_finish(plan, nullptr, err, err_len);
+    if (st != NDF_OK) return st;
+    if ((e = hipMemcpy(output, plan->d_out.ptr, bytes, hipMemcpyDeviceToHost)) != hipSuccess) {
+        return fail_hip(e, err, err_len);
+    }
+    return NDF_OK;
 }
 
 }  // extern "C"

@@ -38,6 +38,13 @@ struct GenericArgs {
     int32_t stat;
     int64_t min_count;
     int64_t rank_param;
+    // Sparse mode (mask-directed repair): when `positions` is set, `len`
+    // counts positions and thread i computes the output at flat index
+    // positions[i] of an array of `extent` elements. A position outside
+    // [0, extent) is skipped and sets *error_flag.
+    const int64_t *positions;
+    int64_t extent;
+    int32_t *error_flag;
 };
 
 // Engine 2: LDS-staged tile, block sort of the support, per-lane column slide
@@ -121,8 +128,20 @@ constexpr int SLIDE_TILE_X = 64;
 constexpr uint32_t SLIDE_NO_POS = 0xFFFFFFFFu;
 
 hipError_t launch_generic(int dtype_bytes, const GenericArgs &args, hipStream_t stream);
+// Sparse repair with one wave per position (args.positions set): the window's
+// retained keys are staged in LDS, so args.count <= REPAIR_WAVE_MAX_TAPS.
+constexpr int REPAIR_WAVE_MAX_TAPS = 4096;
+size_t repair_wave_lds_bytes(int dtype_bytes, int count);
+hipError_t launch_repair_wave(int dtype_bytes, const GenericArgs &args, hipStream_t stream);
 hipError_t launch_slide(int dtype_bytes, const SlideArgs &args, size_t lds_bytes, int max_tiles, hipStream_t stream);
 hipError_t launch_fill(int dtype_bytes, void *output, int64_t len, uint64_t bits, hipStream_t stream);
+// out[positions[i]] = bits for i < n; positions outside [0, extent) are
+// skipped and set *error_flag.
+hipError_t launch_scatter_fill(int dtype_bytes, void *output, const int64_t *positions, int64_t n, int64_t extent,
+                               uint64_t bits, int32_t *error_flag, hipStream_t stream);
+// dst[positions[i]] = src[positions[i]] for i < n (same bounds rule).
+hipError_t launch_scatter_copy(int dtype_bytes, void *dst, const void *src, const int64_t *positions, int64_t n,
+                               int64_t extent, int32_t *error_flag, hipStream_t stream);
 size_t slide_lds_bytes(int dtype_bytes, int s, int sort_e, int hist_stride, int mask_bytes);
 // Bracket kernel (2-D rectangles, support rows <= 128 samples): ranks only
 // the samples inside each tile's key bracket; tiles it cannot finish are

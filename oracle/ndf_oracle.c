@@ -409,6 +409,7 @@ typedef struct {
     int omit;
     int ignore_inf;
     size_t min_count;
+    const size_t *positions; /* NULL: every output; else outputs positions[begin..end) only */
 } direct_job_t;
 
 typedef struct {
@@ -453,7 +454,8 @@ static void *direct_worker(void *arg)
         return NULL;
     }
     int64_t coord[64];
-    for (size_t center = part->begin; center < part->end; ++center) {
+    for (size_t item = part->begin; item < part->end; ++item) {
+        const size_t center = job->positions ? job->positions[item] : item;
         size_t rem = center;
         int interior = 1;
         for (size_t a = 0; a < rank; ++a) {
@@ -1004,7 +1006,7 @@ int ndfo_filter(int dtype_bytes, const void *input, void *output, size_t rank, c
         return NDFO_OK;
     }
     direct_job_t job = {dtype_bytes, input, output, &w, cval_to_bits(cval, dtype_bytes), statistic, parameter,
-                        omit_nan != 0, ignore_inf != 0, min_count};
+                        omit_nan != 0, ignore_inf != 0, min_count, NULL};
     if (nthreads < 1) nthreads = 1;
     if (nthreads > 256) nthreads = 256;
     if ((size_t)nthreads > w.len) nthreads = (int)w.len;
@@ -1021,6 +1023,81 @@ int ndfo_filter(int dtype_bytes, const void *input, void *output, size_t rank, c
     for (int t = 0; t < nthreads; ++t) {
         if (parts[t].status != NDFO_OK) st = parts[t].status;
     }
+    window_free(&w);
+    return st == NDFO_OK ? NDFO_OK : fail_code(st, err, err_len);
+}
+
+/* fixpix(method="median"): src/repair.rs:107-126 (fixpix_median) with
+ * positions() at src/repair.rs:14-29 and repair_median at
+ * src/filter_nd.rs:174-215. The output starts as a copy of the input; every
+ * masked element of a snapshot becomes NaN; each masked output is the
+ * direct-method Omit median of its window over that snapshot (all estimates
+ * are computed before any is stored, so repaired values never feed another
+ * window). */
+int ndfo_fixpix_median(int dtype_bytes, const void *input, void *output, const uint8_t *mask, size_t mask_len,
+                       size_t rank, const size_t *shape, const size_t *size, const uint8_t *footprint,
+                       const ptrdiff_t *origin, const int32_t *modes, double cval, int32_t ignore_inf,
+                       size_t min_count, int32_t nthreads, char *err, size_t err_len)
+{
+    if (dtype_bytes != 4 && dtype_bytes != 8) return fail(NDFO_INVALID, "dtype must be float32 or float64", err, err_len);
+    for (size_t a = 0; a < rank; ++a) {
+        if (modes[a] < 0 || modes[a] > 4) return fail(NDFO_INVALID, "unsupported boundary mode", err, err_len);
+    }
+    window_t w;
+    int st = window_build(&w, rank, shape, size, footprint, origin, modes, err, err_len);
+    if (st != NDFO_OK) return st;
+    if (min_count == 0 || min_count > w.count) {
+        window_free(&w);
+        return fail(NDFO_INVALID, MSG_MIN_COUNT, err, err_len);
+    }
+    if (mask_len != w.len) {
+        window_free(&w);
+        return fail_code(NDFO_LENGTH_MISMATCH, err, err_len);
+    }
+    const size_t bytes = w.len * (size_t)dtype_bytes;
+    if (bytes) memcpy(output, input, bytes);
+    size_t count = 0;
+    for (size_t i = 0; i < w.len; ++i) count += mask[i] != 0;
+    if (count == 0) {
+        window_free(&w);
+        return NDFO_OK;
+    }
+    size_t *positions = (size_t *)malloc(sizeof(size_t) * count);
+    void *snapshot = malloc(bytes);
+    if (!positions || !snapshot) {
+        free(positions);
+        free(snapshot);
+        window_free(&w);
+        return fail_code(NDFO_MEMORY, err, err_len);
+    }
+    memcpy(snapshot, input, bytes);
+    count = 0;
+    for (size_t i = 0; i < w.len; ++i) {
+        if (mask[i]) {
+            positions[count++] = i;
+            store_bits(snapshot, dtype_bytes, i, canonical_nan(dtype_bytes));
+        }
+    }
+    direct_job_t job = {dtype_bytes, snapshot, output, &w, cval_to_bits(cval, dtype_bytes), NDFO_STAT_MEDIAN, 0,
+                        1, ignore_inf != 0, min_count, positions};
+    if (nthreads < 1) nthreads = 1;
+    if (nthreads > 256) nthreads = 256;
+    if ((size_t)nthreads > count) nthreads = (int)count;
+    direct_part_t parts[256];
+    const size_t chunk = (count + (size_t)nthreads - 1) / (size_t)nthreads;
+    for (int t = 0; t < nthreads; ++t) {
+        parts[t].job = &job;
+        parts[t].begin = chunk * (size_t)t < count ? chunk * (size_t)t : count;
+        parts[t].end = parts[t].begin + chunk < count ? parts[t].begin + chunk : count;
+        parts[t].status = NDFO_OK;
+    }
+    run_parts(direct_worker, parts, sizeof(direct_part_t), nthreads);
+    st = NDFO_OK;
+    for (int t = 0; t < nthreads; ++t) {
+        if (parts[t].status != NDFO_OK) st = parts[t].status;
+    }
+    free(positions);
+    free(snapshot);
     window_free(&w);
     return st == NDFO_OK ? NDFO_OK : fail_code(st, err, err_len);
 }

@@ -10,6 +10,8 @@
  *   - NaN/inf policy + selection  src/reduction.rs:78-130
  *   - total-order keys            src/median_network.rs:9-37
  *   - even-count midpoint         docs/semantics.md:74-85, tests/_reference.py:36-47
+ *   - mask-directed median repair src/repair.rs:14-29, 107-126,
+ *                                 src/filter_nd.rs:174-215
  *   - block rank histogram        src/histogram.rs:30-631 (used as the fast
  *                                 multi-threaded CPU baseline and as a second,
  *                                 independent algorithm for cross-checking)
@@ -64,6 +66,14 @@ int ndfo_filter(int dtype_bytes, const void *input, void *output, size_t rank,
                 int32_t statistic, size_t parameter, int32_t omit_nan, int32_t ignore_inf,
                 size_t min_count, int32_t method, int32_t nthreads, char *err,
                 size_t err_len);
+
+/* Mask-directed median repair (fixpix(method="median")): outputs whose
+ * mask byte is nonzero become the Omit median of their window over the input
+ * with every masked element set to NaN; the rest copy the input. */
+int ndfo_fixpix_median(int dtype_bytes, const void *input, void *output, const uint8_t *mask, size_t mask_len,
+                       size_t rank, const size_t *shape, const size_t *size, const uint8_t *footprint,
+                       const ptrdiff_t *origin, const int32_t *modes, double cval, int32_t ignore_inf,
+                       size_t min_count, int32_t nthreads, char *err, size_t err_len);
 
 #ifdef __cplusplus
 }

@@ -187,3 +187,56 @@ def window_filter(
 
 def median(array: np.ndarray, size, **kwargs) -> np.ndarray:
     return window_filter(array, size, statistic="median", **kwargs)
+
+
+def fixpix_median(
+    array: np.ndarray,
+    mask: np.ndarray,
+    size: Sequence[int],
+    *,
+    footprint: Sequence[bool] | np.ndarray | None = None,
+    origin: Sequence[int] | None = None,
+    mode: Sequence[str] | str = "reflect",
+    cval: float = float("nan"),
+    ignore_inf: bool = False,
+    min_count: int = 1,
+    threads: int | None = None,
+) -> np.ndarray:
+    """Mask-directed median repair (``ndfo_fixpix_median``) of a C-contiguous array."""
+    array = np.ascontiguousarray(array)
+    if array.dtype not in (np.float32, np.float64):
+        raise TypeError("oracle supports float32 and float64")
+    bad = np.ascontiguousarray(np.asarray(mask) != 0, dtype=np.uint8)
+    ndim = array.ndim
+    if isinstance(mode, str):
+        mode = (mode,) * ndim
+    if origin is None:
+        origin = (0,) * ndim
+    out = np.empty_like(array)
+    lib = _load()
+    if not hasattr(lib, "_fixpix_declared"):
+        lib.ndfo_fixpix_median.argtypes = [
+            ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_size_t,
+            ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t), ctypes.c_void_p,
+            ctypes.POINTER(ctypes.c_ssize_t), ctypes.POINTER(ctypes.c_int32), ctypes.c_double, ctypes.c_int32,
+            ctypes.c_size_t, ctypes.c_int32, ctypes.c_char_p, ctypes.c_size_t,
+        ]
+        lib.ndfo_fixpix_median.restype = ctypes.c_int
+        lib._fixpix_declared = True
+    shape = (ctypes.c_size_t * max(ndim, 1))(*array.shape)
+    sizes = (ctypes.c_size_t * max(ndim, 1))(*size)
+    origins = (ctypes.c_ssize_t * max(ndim, 1))(*origin)
+    modes = (ctypes.c_int32 * max(ndim, 1))(*(MODES.index(m) for m in mode))
+    fp = None
+    if footprint is not None:
+        fp_arr = np.ascontiguousarray(np.asarray(footprint).ravel() != 0, dtype=np.uint8)
+        fp = fp_arr.ctypes.data
+    err = ctypes.create_string_buffer(256)
+    status = lib.ndfo_fixpix_median(
+        array.dtype.itemsize, array.ctypes.data, out.ctypes.data, bad.ctypes.data, bad.size, ndim, shape, sizes,
+        fp, origins, modes, float(cval), int(bool(ignore_inf)), int(min_count),
+        default_threads() if threads is None else int(threads), err, len(err),
+    )
+    if status != 0:
+        raise OracleError(status, err.value.decode())
+    return out

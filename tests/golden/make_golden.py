@@ -199,6 +199,48 @@ def cases() -> list[dict]:
     return out
 
 
+def fixpix_case(name, source, values, mask, dtype, kwargs, expected):
+    array = np.asarray(values, dtype=dtype)
+    return {
+        "name": name,
+        "source": source,
+        "dtype": np.dtype(dtype).name,
+        "shape": list(array.shape),
+        "input": bits(array, dtype),
+        "mask": [int(v) for v in np.asarray(mask, dtype=bool).ravel()],
+        "kwargs": kwargs,
+        "expected_all": bits(np.asarray(expected, dtype=dtype), dtype),
+    }
+
+
+def fixpix_cases() -> list[dict]:
+    """Literal answers of fixpix(method="median") in the reference's doc example and tests."""
+    out = []
+    nan, inf = np.nan, np.inf
+    for dt in (np.float32, np.float64):
+        tag = np.dtype(dt).name
+        out.append(fixpix_case(f"doc_example_{tag}", "src/repair.rs:99-104", [1.0, 9.0, 99.0, 3.0, 5.0],
+                               [0, 0, 1, 0, 0], dt, {"size": 3, "cval": 0.0}, [1.0, 9.0, 6.0, 3.0, 5.0]))
+        out.append(fixpix_case(f"never_uses_repaired_{tag}", "tests/test_fixpix_methods.py:146-168",
+                               [0.0, 100.0, 200.0, 10.0], [0, 1, 1, 0], dt, {"size": 3, "mode": "reflect"},
+                               [0.0, 0.0, 10.0, 10.0]))
+        out.append(fixpix_case(f"edge_cval_{tag}", "tests/test_fixpix_methods.py:171-196", [99.0, 5.0], [1, 0], dt,
+                               {"size": 3, "mode": "constant", "cval": 11.0}, [8.0, 5.0]))
+        out.append(fixpix_case(f"inf_retained_{tag}", "tests/test_fixpix_methods.py:198-223", [inf, 99.0, 2.0],
+                               [0, 1, 0], dt, {"size": 3, "mode": "reflect"}, [inf, inf, 2.0]))
+        out.append(fixpix_case(f"inf_omitted_{tag}", "tests/test_fixpix_methods.py:198-225", [inf, 99.0, 2.0],
+                               [0, 1, 0], dt, {"size": 3, "mode": "reflect", "ignore_inf": True}, [inf, 2.0, 2.0]))
+        out.append(fixpix_case(f"min_count_enough_{tag}", "tests/test_fixpix_methods.py:228-252", [99.0, 7.0],
+                               [1, 0], dt, {"size": 5, "mode": "nearest", "min_count": 2}, [7.0, 7.0]))
+        out.append(fixpix_case(f"min_count_short_{tag}", "tests/test_fixpix_methods.py:228-252", [99.0, 7.0],
+                               [1, 0], dt, {"size": 5, "mode": "nearest", "min_count": 3}, [nan, 7.0]))
+        for size in (1, 3, 5):
+            out.append(fixpix_case(f"fully_masked_size{size}_{tag}", "tests/test_fixpix_methods.py:433-439",
+                                   np.arange(6.0).reshape(2, 3), np.ones((2, 3)), dt, {"size": size},
+                                   np.full((2, 3), nan)))
+    return out
+
+
 BOUNDARY_TABLES = {
     "source": "src/boundary.rs:77-187",
     "length4": {
@@ -229,6 +271,7 @@ def main() -> None:
         "about": "Known answers restated from the ndfiltrs reference tests (see each 'source').",
         "boundary": BOUNDARY_TABLES,
         "cases": cases(),
+        "fixpix_median": fixpix_cases(),
     }
     with open(OUT, "w") as handle:
         json.dump(data, handle, indent=1)
