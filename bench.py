@@ -362,6 +362,7 @@ def main() -> int:
     parser.add_argument("--warmup", type=int, default=2)
     parser.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     parser.add_argument("--quick", action="store_true", help="only the 63x63 lines")
+    parser.add_argument("--fixpix", action="store_true", help="only the fixpix median repair lines")
     parser.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                         help="only copy MIB MiB device-to-device 3 times (HBM counter calibration) and exit")
     args = parser.parse_args()
@@ -386,6 +387,11 @@ def main() -> int:
         print(json.dumps({"copy_calibration_bytes": nbytes, "copies": 3}), flush=True)
         return 0
 
+    if args.fixpix:
+        log("fixpix median on the 8192x8192 sky: sparse and dense repair paths")
+        print(json.dumps(repair_bench(synthetic.sky_quadratic(), args.steps, args.warmup, check=not args.no_cpu)),
+              flush=True)
+        return 0
     log("generating the 8192x8192 float32 sky (63x63 workload)")
     c5 = Workload(synthetic.sky_quadratic())
     w63, meta63 = bracket.load(bracket.model_path("sky63"))

@@ -11,16 +11,23 @@
 namespace ndf {
 namespace {
 
-template <typename T>
+// R > 0: the array rank is the compile-time R, so the loops unroll and the
+// coordinates stay in registers; R = 0: any rank up to MAX_RANK (the
+// coordinate array is indexed at run time and lives in scratch memory).
+template <typename T, int R = 0>
 struct Gather {
     using KT = KeyTraits<T>;
     using Key = typename KT::Key;
+    static constexpr int SLOTS = R > 0 ? R : MAX_RANK;
     const GenericArgs &a;
-    int64_t coord[MAX_RANK];
+    int64_t coord[SLOTS];
+
+    __device__ int rank() const { return R > 0 ? R : a.rank; }
 
     __device__ Gather(const GenericArgs &args, int64_t index) : a(args)
     {
-        for (int ax = 0; ax < a.rank; ++ax) {
+#pragma unroll
+        for (int ax = 0; ax < rank(); ++ax) {
             coord[ax] = index / a.stride[ax];
             index -= coord[ax] * a.stride[ax];
         }
@@ -29,8 +36,9 @@ struct Gather {
     __device__ Key key(int t) const
     {
         int64_t offset = 0;
-        const int32_t *tap = a.taps + (int64_t)t * a.rank;
-        for (int ax = 0; ax < a.rank; ++ax) {
+        const int32_t *tap = a.taps + (int64_t)t * rank();
+#pragma unroll
+        for (int ax = 0; ax < rank(); ++ax) {
             const int64_t m = a.maps[a.map_base[ax] + coord[ax] + tap[ax]];
             if (m < 0) {
                 return classify<T>((Key)a.cval_bits, a.ignore_inf);
@@ -42,8 +50,8 @@ struct Gather {
 };
 
 // k-th smallest (0-based) retained key; requires k < retained count.
-template <typename T>
-__device__ typename KeyTraits<T>::Key select_kth(const Gather<T> &g, int count, int64_t k)
+template <typename T, int R>
+__device__ typename KeyTraits<T>::Key select_kth(const Gather<T, R> &g, int count, int64_t k)
 {
     using Key = typename KeyTraits<T>::Key;
     constexpr int BITS = sizeof(Key) * 8;
@@ -61,7 +69,7 @@ __device__ typename KeyTraits<T>::Key select_kth(const Gather<T> &g, int count, 
     return answer;
 }
 
-template <typename T>
+template <typename T, int R>
 __global__ void __launch_bounds__(256) k_generic(GenericArgs a)
 {
     using KT = KeyTraits<T>;
@@ -76,7 +84,7 @@ __global__ void __launch_bounds__(256) k_generic(GenericArgs a)
                 continue;
             }
         }
-        Gather<T> g(a, index);
+        Gather<T, R> g(a, index);
         int64_t n = 0;
         for (int t = 0; t < a.count; ++t) {
             n += g.key(t) != KT::SENTINEL;
@@ -85,13 +93,13 @@ __global__ void __launch_bounds__(256) k_generic(GenericArgs a)
         if (n >= a.min_count && n > 0) {
             if (a.stat == STAT_RANK) {
                 if (a.rank_param < n) {
-                    out = KT::unkey(select_kth<T>(g, a.count, a.rank_param));
+                    out = KT::unkey(select_kth(g, a.count, a.rank_param));
                 }
             } else if (n & 1) {
-                out = KT::unkey(select_kth<T>(g, a.count, n / 2));
+                out = KT::unkey(select_kth(g, a.count, n / 2));
             } else {
-                const Key lower = select_kth<T>(g, a.count, n / 2 - 1);
-                const Key upper = select_kth<T>(g, a.count, n / 2);
+                const Key lower = select_kth(g, a.count, n / 2 - 1);
+                const Key upper = select_kth(g, a.count, n / 2);
                 out = median_pair_bits<T>(lower, upper);
             }
         }
@@ -108,11 +116,22 @@ __global__ void __launch_bounds__(256) k_generic(GenericArgs a)
 // takes the (n/2 - 1)-th key and then the smallest key above it, unless that
 // key repeats.
 constexpr int REPAIR_WAVES = 4;
+constexpr int REPAIR_GATHER = 8;
 
-__device__ inline int64_t wave_sum(int64_t v)
+// Sum over the 64 lanes, returned wave-uniform. DPP steps add lane ^ 1 and
+// lane ^ 2 (quad_perm [1,0,3,2], [2,3,0,1]), then the other quad of each 8
+// (row_half_mirror) and the other half of each 16-lane row (row_mirror);
+// the four row sums are read out with v_readlane. VALU lane movement only:
+// the reduction sits on every select pass's dependency chain.
+__device__ __forceinline__ int wave_sum(int v)
 {
-    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d);
-    return v;
+    constexpr int ALL = 0xF;
+    v += __builtin_amdgcn_update_dpp(0, v, 0xB1, ALL, ALL, true);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x4E, ALL, ALL, true);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x141, ALL, ALL, true);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x140, ALL, ALL, true);
+    return __builtin_amdgcn_readlane(v, 0) + __builtin_amdgcn_readlane(v, 16) + __builtin_amdgcn_readlane(v, 32) +
+           __builtin_amdgcn_readlane(v, 48);
 }
 
 template <typename Key>
@@ -140,14 +159,14 @@ __device__ Key wave_select(const Key *keys, int n, int64_t k, int lane)
     Key answer = 0;
     for (int b = BITS - 1; b >= 0; --b) {
         const Key candidate = answer | (Key(1) << b);
-        int64_t below = 0;
+        int below = 0;
         for (int j = lane; j < n; j += 64) below += keys[j] < candidate;
         if (wave_sum(below) <= k) answer = candidate;
     }
     return answer;
 }
 
-template <typename T>
+template <typename T, int R>
 __global__ void __launch_bounds__(64 * REPAIR_WAVES) k_repair_wave(GenericArgs a)
 {
     using KT = KeyTraits<T>;
@@ -163,15 +182,24 @@ __global__ void __launch_bounds__(64 * REPAIR_WAVES) k_repair_wave(GenericArgs a
             if (lane == 0) atomicOr(a.error_flag, 1);
             continue;
         }
-        Gather<T> g(a, index);
+        Gather<T, R> g(a, index);
         int n = 0;
-        for (int t0 = 0; t0 < a.count; t0 += 64) {
-            const int t = t0 + lane;
-            const Key key = t < a.count ? g.key(t) : KT::SENTINEL;
-            const bool keep = key != KT::SENTINEL;
-            const uint64_t ballot = __ballot(keep);
-            if (keep) keys[n + __popcll(ballot & ((1ull << lane) - 1))] = key;
-            n += __popcll(ballot);
+        // REPAIR_GATHER taps per lane in flight (each a tap -> map -> input
+        // chain of dependent loads) before they are compacted.
+        for (int t0 = 0; t0 < a.count; t0 += 64 * REPAIR_GATHER) {
+            Key gathered[REPAIR_GATHER];
+#pragma unroll
+            for (int u = 0; u < REPAIR_GATHER; ++u) {
+                const int t = t0 + u * 64 + lane;
+                gathered[u] = t < a.count ? g.key(t) : KT::SENTINEL;
+            }
+#pragma unroll
+            for (int u = 0; u < REPAIR_GATHER; ++u) {
+                const bool keep = gathered[u] != KT::SENTINEL;
+                const uint64_t ballot = __ballot(keep);
+                if (keep) keys[n + __popcll(ballot & ((1ull << lane) - 1))] = gathered[u];
+                n += __popcll(ballot);
+            }
         }
         wave_lds_sync();
         Key out = KT::CANONICAL_NAN;
@@ -182,7 +210,7 @@ __global__ void __launch_bounds__(64 * REPAIR_WAVES) k_repair_wave(GenericArgs a
                 out = KT::unkey(wave_select(keys, n, n / 2, lane));
             } else {
                 const Key lower = wave_select(keys, n, n / 2 - 1, lane);
-                int64_t at_most = 0;
+                int at_most = 0;
                 Key above = KT::SENTINEL;
                 for (int j = lane; j < n; j += 64) {
                     const Key v = keys[j];
@@ -249,11 +277,16 @@ int64_t grid_for(int64_t len)
 hipError_t launch_generic(int dtype_bytes, const GenericArgs &args, hipStream_t stream)
 {
     const dim3 grid((unsigned)grid_for(args.len));
+    const int r = args.rank;
+    void (*fn)(GenericArgs) = nullptr;
     if (dtype_bytes == 4) {
-        hipLaunchKernelGGL(k_generic<float>, grid, dim3(256), 0, stream, args);
+        fn = r == 1 ? k_generic<float, 1> : r == 2 ? k_generic<float, 2> : r == 3 ? k_generic<float, 3>
+                                                                          : k_generic<float, 0>;
     } else {
-        hipLaunchKernelGGL(k_generic<double>, grid, dim3(256), 0, stream, args);
+        fn = r == 1 ? k_generic<double, 1> : r == 2 ? k_generic<double, 2> : r == 3 ? k_generic<double, 3>
+                                                                           : k_generic<double, 0>;
     }
+    hipLaunchKernelGGL(fn, grid, dim3(256), 0, stream, args);
     return hipGetLastError();
 }
 
@@ -265,15 +298,19 @@ hipError_t launch_repair_wave(int dtype_bytes, const GenericArgs &args, hipStrea
     if (blocks > 8192) blocks = 8192;
     if (blocks < 1) blocks = 1;
     const size_t lds = repair_wave_lds_bytes(dtype_bytes, args.count);
-    const void *fn = dtype_bytes == 4 ? reinterpret_cast<const void *>(&k_repair_wave<float>)
-                                      : reinterpret_cast<const void *>(&k_repair_wave<double>);
-    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    if (e != hipSuccess) return e;
+    void (*fn)(GenericArgs) = nullptr;
+    const int r = args.rank;
     if (dtype_bytes == 4) {
-        hipLaunchKernelGGL(k_repair_wave<float>, dim3((unsigned)blocks), dim3(64 * REPAIR_WAVES), lds, stream, args);
+        fn = r == 1 ? k_repair_wave<float, 1> : r == 2 ? k_repair_wave<float, 2>
+             : r == 3 ? k_repair_wave<float, 3> : k_repair_wave<float, 0>;
     } else {
-        hipLaunchKernelGGL(k_repair_wave<double>, dim3((unsigned)blocks), dim3(64 * REPAIR_WAVES), lds, stream, args);
+        fn = r == 1 ? k_repair_wave<double, 1> : r == 2 ? k_repair_wave<double, 2>
+             : r == 3 ? k_repair_wave<double, 3> : k_repair_wave<double, 0>;
     }
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(fn), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(fn, dim3((unsigned)blocks), dim3(64 * REPAIR_WAVES), lds, stream, args);
     return hipGetLastError();
 }
 
