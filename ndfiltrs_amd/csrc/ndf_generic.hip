@@ -11,8 +11,8 @@
 namespace ndf {
 namespace {
 
-// R > 0: the array rank is the compile-time R, so the loops unroll and the
-// coordinates stay in registers; R = 0: any rank up to MAX_RANK (the
+// R > 0: the array rank is the compile-time R, so the loops (1 to 3 trips)
+// unroll fully and the coordinates stay in registers; R = 0: any rank up to MAX_RANK (the
 // coordinate array is indexed at run time and lives in scratch memory).
 template <typename T, int R = 0>
 struct Gather {
@@ -26,7 +26,6 @@ struct Gather {
 
     __device__ Gather(const GenericArgs &args, int64_t index) : a(args)
     {
-#pragma unroll
         for (int ax = 0; ax < rank(); ++ax) {
             coord[ax] = index / a.stride[ax];
             index -= coord[ax] * a.stride[ax];
@@ -37,7 +36,6 @@ struct Gather {
     {
         int64_t offset = 0;
         const int32_t *tap = a.taps + (int64_t)t * rank();
-#pragma unroll
         for (int ax = 0; ax < rank(); ++ax) {
             const int64_t m = a.maps[a.map_base[ax] + coord[ax] + tap[ax]];
             if (m < 0) {
@@ -151,22 +149,56 @@ __device__ inline void wave_lds_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// k-th smallest (0-based) of the n keys in `keys`; requires k < n.
-template <typename Key>
-__device__ Key wave_select(const Key *keys, int n, int64_t k, int lane)
+// The n retained keys of one wave. KPL > 0: lane l holds keys l + 64 q
+// (q < KPL) in registers, with the all-ones sentinel past n (it is above
+// every retained key, so it never counts below a candidate); KPL = 0: the
+// keys stay in the wave's LDS region and each pass reads them there.
+template <typename Key, int KPL>
+struct WaveKeys {
+    static constexpr Key SENTINEL = static_cast<Key>(~Key(0));
+    Key r[KPL > 0 ? KPL : 1];
+    const Key *lds;
+    int n;
+
+    __device__ WaveKeys(const Key *keys, int count, int lane) : lds(keys), n(count)
+    {
+        if constexpr (KPL > 0) {
+#pragma unroll
+            for (int q = 0; q < KPL; ++q) {
+                const int j = lane + 64 * q;
+                r[q] = j < n ? keys[j] : SENTINEL;
+            }
+        }
+    }
+
+    template <typename F>
+    __device__ void each(int lane, F &&f) const
+    {
+        if constexpr (KPL > 0) {
+#pragma unroll
+            for (int q = 0; q < KPL; ++q) f(r[q]);
+        } else {
+            for (int j = lane; j < n; j += 64) f(lds[j]);
+        }
+    }
+};
+
+// k-th smallest (0-based) of the wave's keys; requires k < n.
+template <typename Key, int KPL>
+__device__ Key wave_select(const WaveKeys<Key, KPL> &keys, int64_t k, int lane)
 {
     constexpr int BITS = sizeof(Key) * 8;
     Key answer = 0;
     for (int b = BITS - 1; b >= 0; --b) {
         const Key candidate = answer | (Key(1) << b);
         int below = 0;
-        for (int j = lane; j < n; j += 64) below += keys[j] < candidate;
+        keys.each(lane, [&](Key v) { below += v < candidate; });
         if (wave_sum(below) <= k) answer = candidate;
     }
     return answer;
 }
 
-template <typename T, int R>
+template <typename T, int R, int KPL>
 __global__ void __launch_bounds__(64 * REPAIR_WAVES) k_repair_wave(GenericArgs a)
 {
     using KT = KeyTraits<T>;
@@ -204,19 +236,19 @@ __global__ void __launch_bounds__(64 * REPAIR_WAVES) k_repair_wave(GenericArgs a
         wave_lds_sync();
         Key out = KT::CANONICAL_NAN;
         if (n >= a.min_count && n > 0) {
+            const WaveKeys<Key, KPL> held(keys, n, lane);
             if (a.stat == STAT_RANK) {
-                if (a.rank_param < n) out = KT::unkey(wave_select(keys, n, a.rank_param, lane));
+                if (a.rank_param < n) out = KT::unkey(wave_select(held, a.rank_param, lane));
             } else if (n & 1) {
-                out = KT::unkey(wave_select(keys, n, n / 2, lane));
+                out = KT::unkey(wave_select(held, n / 2, lane));
             } else {
-                const Key lower = wave_select(keys, n, n / 2 - 1, lane);
+                const Key lower = wave_select(held, n / 2 - 1, lane);
                 int at_most = 0;
                 Key above = KT::SENTINEL;
-                for (int j = lane; j < n; j += 64) {
-                    const Key v = keys[j];
+                held.each(lane, [&](Key v) {
                     at_most += v <= lower;
                     if (v > lower && v < above) above = v;
-                }
+                });
                 at_most = wave_sum(at_most);
                 above = wave_min(above);
                 out = median_pair_bits<T>(lower, at_most > n / 2 ? lower : above);
@@ -290,6 +322,20 @@ hipError_t launch_generic(int dtype_bytes, const GenericArgs &args, hipStream_t 
     return hipGetLastError();
 }
 
+template <typename T, int R>
+void (*repair_kernel_r(int kpl))(GenericArgs)
+{
+    return kpl == 1 ? k_repair_wave<T, R, 1> : kpl == 4 ? k_repair_wave<T, R, 4>
+           : kpl == 16 ? k_repair_wave<T, R, 16> : k_repair_wave<T, R, 0>;
+}
+
+template <typename T>
+void (*repair_kernel(int rank, int kpl))(GenericArgs)
+{
+    return rank == 1 ? repair_kernel_r<T, 1>(kpl) : rank == 2 ? repair_kernel_r<T, 2>(kpl)
+           : rank == 3 ? repair_kernel_r<T, 3>(kpl) : repair_kernel_r<T, 0>(kpl);
+}
+
 size_t repair_wave_lds_bytes(int dtype_bytes, int count) { return (size_t)REPAIR_WAVES * count * dtype_bytes; }
 
 hipError_t launch_repair_wave(int dtype_bytes, const GenericArgs &args, hipStream_t stream)
@@ -298,15 +344,11 @@ hipError_t launch_repair_wave(int dtype_bytes, const GenericArgs &args, hipStrea
     if (blocks > 8192) blocks = 8192;
     if (blocks < 1) blocks = 1;
     const size_t lds = repair_wave_lds_bytes(dtype_bytes, args.count);
-    void (*fn)(GenericArgs) = nullptr;
-    const int r = args.rank;
-    if (dtype_bytes == 4) {
-        fn = r == 1 ? k_repair_wave<float, 1> : r == 2 ? k_repair_wave<float, 2>
-             : r == 3 ? k_repair_wave<float, 3> : k_repair_wave<float, 0>;
-    } else {
-        fn = r == 1 ? k_repair_wave<double, 1> : r == 2 ? k_repair_wave<double, 2>
-             : r == 3 ? k_repair_wave<double, 3> : k_repair_wave<double, 0>;
-    }
+    // Keys per lane in registers: 1 (<= 64 taps), 4 (<= 256), 16 (<= 1024),
+    // else 0 (keys read from LDS on every pass).
+    const int kpl = args.count <= 64 ? 1 : args.count <= 256 ? 4 : args.count <= 1024 ? 16 : 0;
+    void (*fn)(GenericArgs) = dtype_bytes == 4 ? repair_kernel<float>(args.rank, kpl)
+                                               : repair_kernel<double>(args.rank, kpl);
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(fn), hipFuncAttributeMaxDynamicSharedMemorySize,
                                        (int)lds);
     if (e != hipSuccess) return e;

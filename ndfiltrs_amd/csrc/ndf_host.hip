@@ -1781,16 +1781,16 @@ namespace {
 //   dense   the plan's engine over the whole snapshot, then the estimates
 //           are scattered back (cost ~ array).
 // Auto picks sparse when the plan has no tiled engine, or when the wave
-// kernel applies and at most 1/64 (network-engine windows) or 1/32 (slide-
+// kernel applies and at most 1/64 (network-engine windows) or 1/24 (slide-
 // engine windows) of the elements are listed. Measured on 8192^2 float32
-// with 1 % listed: 5x5 sparse 1.08 ms vs dense 1.52 ms; 31x31 sparse
-// 4.43 ms vs dense 13.7 ms.
+// with 1 % listed: 5x5 sparse 0.87 ms vs dense 1.50 ms; 31x31 sparse
+// 2.99 ms vs dense 13.7 ms.
 bool repair_sparse(const ndf_plan *p, size_t npos, int32_t path)
 {
     if (path == NDF_REPAIR_SPARSE) return true;
     if (path == NDF_REPAIR_DENSE) return false;
     if (!p->slide_ok && !p->network_ok) return true;
-    const double ratio = p->engine == NDF_ENGINE_NETWORK ? 64.0 : 32.0;
+    const double ratio = p->engine == NDF_ENGINE_NETWORK ? 64.0 : 24.0;
     return p->count <= (size_t)REPAIR_WAVE_MAX_TAPS && (double)npos * ratio <= (double)p->len;
 }
 
