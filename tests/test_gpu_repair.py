@@ -169,3 +169,27 @@ def test_rank_plan_repair_matches_mask_exclusion(dtype) -> None:
     for path in ("sparse", "dense"):
         print(f"repair rank-3 {np.dtype(dtype).name} path={path}", flush=True)
         assert np.array_equal(bits(_device_repair(plan, values, positions, path)), bits(want)), path
+
+
+@pytest.mark.parametrize("dtype", (np.float32, np.float64))
+def test_rank_four_and_five_filters_and_repair(dtype) -> None:
+    """Arrays of rank 4 and 5 run the any-rank forms of the generic engine and
+    of the sparse repair kernel (run-time coordinate arrays)."""
+    rng = np.random.default_rng(45)
+    for shape, size, origin, mode in (((4, 5, 6, 7), (3, 1, 3, 2), (0, 0, 1, 0), ("reflect", "wrap", "nearest",
+                                                                                   "mirror")),
+                                      ((3, 4, 3, 5, 4), (2, 3, 1, 3, 3), (0, -1, 0, 1, 0), "constant")):
+        values = _cases.special_sample(shape, dtype, seed=len(shape))
+        print(f"rank-{len(shape)} {np.dtype(dtype).name} nanmedian / rank / fixpix", flush=True)
+        got = nf.nanmedian_filter(values, size, origin=origin, mode=mode, cval=1.5)
+        want = oracle.window_filter(values, size, origin=origin, mode=mode, cval=1.5, omit_nan=True)
+        assert np.array_equal(bits(got), bits(want))
+        count = int(np.prod(size))
+        got = nf.rank_filter(values, count // 3, size, origin=origin, mode=mode, cval=1.5)
+        want = oracle.window_filter(values, size, origin=origin, mode=mode, cval=1.5, statistic="rank",
+                                    rank=count // 3)
+        assert np.array_equal(bits(got), bits(want))
+        mask = rng.random(shape) < 0.2
+        got = nf.fixpix(values, mask, method="median", size=size, origin=origin, mode=mode, cval=1.5)
+        want = oracle.fixpix_median(values, mask, size, origin=origin, mode=mode, cval=1.5)
+        assert np.array_equal(bits(got), bits(want))
