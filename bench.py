@@ -309,7 +309,7 @@ def end_to_end(image: np.ndarray, repeats: int = 3, **kwargs) -> dict:
 
 def repair_bench(image: np.ndarray, steps: int, warmup: int, check: bool) -> dict:
     """fixpix(method="median") on device buffers: 1 % of the pixels masked
-    (seeded), windows 5x5 and 31x31, reflect, both device paths."""
+    (seeded), windows 5x5, 31x31 and 63x63, reflect, both device paths."""
     from ndfiltrs_amd import _core
 
     rng = np.random.default_rng(20261209)
@@ -321,7 +321,7 @@ def repair_bench(image: np.ndarray, steps: int, warmup: int, check: bool) -> dic
     d_pos.upload(positions)
     result = {"workload": f"fixpix median, {positions.size} masked pixels (1 %) of a {image.shape[0]}x"
                           f"{image.shape[1]} float32 sky, reflect, cval NaN", "masked": int(positions.size)}
-    for size in (5, 31):
+    for size in (5, 31, 63):
         plan = _core.Plan(image.dtype, image.shape, "median", [size, size], None, [0, 0], ["reflect"] * 2, np.nan,
                           True, False, 1, 0)
         entry = {}

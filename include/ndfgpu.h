@@ -130,16 +130,18 @@ ndf_status ndf_fixpix_median(int32_t dtype_bytes, const void *input, void *outpu
                              const uint8_t *footprint, const ptrdiff_t *origin, const int32_t *modes, double cval,
                              int32_t ignore_inf, size_t min_count, char *err, size_t err_len);
 
-/* Repair paths: SPARSE evaluates only the listed positions (one generic-
- * engine thread each), DENSE runs the plan's engine over the whole snapshot
- * and keeps the listed outputs, AUTO picks by the fraction of listed
+/* Repair paths: SPARSE evaluates only the listed positions (one wave each
+ * for windows of up to 4096 taps, else one thread each), DENSE runs the
+ * plan's engine over the whole snapshot and keeps the listed outputs, AUTO
+ * picks the one with the lower estimated cost for this window and number of
  * positions. All give identical bits. */
 enum { NDF_REPAIR_AUTO = 0, NDF_REPAIR_SPARSE = 1, NDF_REPAIR_DENSE = 2 };
 
 /* Device-resident form of the repair for any plan (fixpix uses a median
  * plan with omit_nan = 1): d_output = d_input with the plan's statistic over
  * the snapshot at the `npos` flat indices in d_positions (int64). d_snapshot
- * is caller scratch of `len` elements. Positions may come in any order;
+ * is caller scratch of `len` elements that must not alias d_input or
+ * d_output (INVALID otherwise); d_output may equal d_input (in place). Positions may come in any order;
  * one outside [0, len) gives LENGTH_MISMATCH. Blocking on `stream`. No
  * reference counterpart (the reference has no device). */
 ndf_status ndf_plan_repair_device(ndf_plan *plan, const void *d_input, void *d_output, void *d_snapshot,

@@ -106,13 +106,14 @@ __global__ void __launch_bounds__(256) k_generic(GenericArgs a)
 }
 
 // Sparse repair, one wave per listed position (windows of at most
-// REPAIR_WAVE_MAX_TAPS taps). The wave gathers the window once, 64 taps per
-// step, and compacts the retained keys into its own LDS region (ballot +
-// prefix count). Selection is the same bitwise search as k_generic, but
-// each pass counts the keys below the candidate across the wave: every lane
-// scans n/64 keys and the counts are summed across lanes. An even median
-// takes the (n/2 - 1)-th key and then the smallest key above it, unless that
-// key repeats.
+// REPAIR_WAVE_MAX_TAPS taps). The wave gathers the window once: with KPL > 0
+// tap lane + 64 q goes straight into register q of the lane (omitted taps
+// keep the sentinel key); with KPL = 0 the retained keys are compacted into
+// the wave's LDS region (ballot + prefix count). Selection is the same
+// bitwise search as k_generic, but each pass counts the keys below the
+// candidate across the wave: every lane scans its keys and the counts are
+// summed across lanes. An even median takes the (n/2 - 1)-th key and then
+// the smallest key above it, unless that key repeats.
 constexpr int REPAIR_WAVES = 4;
 constexpr int REPAIR_GATHER = 8;
 
@@ -157,19 +158,8 @@ template <typename Key, int KPL>
 struct WaveKeys {
     static constexpr Key SENTINEL = static_cast<Key>(~Key(0));
     Key r[KPL > 0 ? KPL : 1];
-    const Key *lds;
-    int n;
-
-    __device__ WaveKeys(const Key *keys, int count, int lane) : lds(keys), n(count)
-    {
-        if constexpr (KPL > 0) {
-#pragma unroll
-            for (int q = 0; q < KPL; ++q) {
-                const int j = lane + 64 * q;
-                r[q] = j < n ? keys[j] : SENTINEL;
-            }
-        }
-    }
+    const Key *lds = nullptr;
+    int n = 0;
 
     template <typename F>
     __device__ void each(int lane, F &&f) const
@@ -215,28 +205,44 @@ __global__ void __launch_bounds__(64 * REPAIR_WAVES) k_repair_wave(GenericArgs a
             continue;
         }
         Gather<T, R> g(a, index);
+        WaveKeys<Key, KPL> held;
         int n = 0;
-        // REPAIR_GATHER taps per lane in flight (each a tap -> map -> input
-        // chain of dependent loads) before they are compacted.
-        for (int t0 = 0; t0 < a.count; t0 += 64 * REPAIR_GATHER) {
-            Key gathered[REPAIR_GATHER];
+        if constexpr (KPL > 0) {
+            // Tap t = lane + 64 q lands in register q: every gather of the
+            // window is in flight at once, and omitted taps keep the
+            // sentinel key, which no select pass counts.
+            int kept = 0;
 #pragma unroll
-            for (int u = 0; u < REPAIR_GATHER; ++u) {
-                const int t = t0 + u * 64 + lane;
-                gathered[u] = t < a.count ? g.key(t) : KT::SENTINEL;
+            for (int q = 0; q < KPL; ++q) {
+                const int t = lane + 64 * q;
+                held.r[q] = t < a.count ? g.key(t) : KT::SENTINEL;
+                kept += held.r[q] != KT::SENTINEL;
             }
+            n = wave_sum(kept);
+        } else {
+            // REPAIR_GATHER taps per lane in flight (each a tap -> map ->
+            // input chain of dependent loads), compacted into LDS.
+            for (int t0 = 0; t0 < a.count; t0 += 64 * REPAIR_GATHER) {
+                Key gathered[REPAIR_GATHER];
 #pragma unroll
-            for (int u = 0; u < REPAIR_GATHER; ++u) {
-                const bool keep = gathered[u] != KT::SENTINEL;
-                const uint64_t ballot = __ballot(keep);
-                if (keep) keys[n + __popcll(ballot & ((1ull << lane) - 1))] = gathered[u];
-                n += __popcll(ballot);
+                for (int u = 0; u < REPAIR_GATHER; ++u) {
+                    const int t = t0 + u * 64 + lane;
+                    gathered[u] = t < a.count ? g.key(t) : KT::SENTINEL;
+                }
+#pragma unroll
+                for (int u = 0; u < REPAIR_GATHER; ++u) {
+                    const bool keep = gathered[u] != KT::SENTINEL;
+                    const uint64_t ballot = __ballot(keep);
+                    if (keep) keys[n + __popcll(ballot & ((1ull << lane) - 1))] = gathered[u];
+                    n += __popcll(ballot);
+                }
             }
+            wave_lds_sync();
+            held.lds = keys;
+            held.n = n;
         }
-        wave_lds_sync();
         Key out = KT::CANONICAL_NAN;
         if (n >= a.min_count && n > 0) {
-            const WaveKeys<Key, KPL> held(keys, n, lane);
             if (a.stat == STAT_RANK) {
                 if (a.rank_param < n) out = KT::unkey(wave_select(held, a.rank_param, lane));
             } else if (n & 1) {
@@ -255,7 +261,7 @@ __global__ void __launch_bounds__(64 * REPAIR_WAVES) k_repair_wave(GenericArgs a
             }
         }
         if (lane == 0) static_cast<Key *>(a.output)[index] = out;
-        wave_lds_sync();
+        if constexpr (KPL == 0) wave_lds_sync();
     }
 }
 
@@ -326,7 +332,7 @@ template <typename T, int R>
 void (*repair_kernel_r(int kpl))(GenericArgs)
 {
     return kpl == 1 ? k_repair_wave<T, R, 1> : kpl == 4 ? k_repair_wave<T, R, 4>
-           : kpl == 16 ? k_repair_wave<T, R, 16> : k_repair_wave<T, R, 0>;
+           : kpl == 16 ? k_repair_wave<T, R, 16> : kpl == 64 ? k_repair_wave<T, R, 64> : k_repair_wave<T, R, 0>;
 }
 
 template <typename T>
@@ -343,10 +349,12 @@ hipError_t launch_repair_wave(int dtype_bytes, const GenericArgs &args, hipStrea
     int64_t blocks = (args.len + REPAIR_WAVES - 1) / REPAIR_WAVES;
     if (blocks > 8192) blocks = 8192;
     if (blocks < 1) blocks = 1;
-    const size_t lds = repair_wave_lds_bytes(dtype_bytes, args.count);
     // Keys per lane in registers: 1 (<= 64 taps), 4 (<= 256), 16 (<= 1024),
-    // else 0 (keys read from LDS on every pass).
-    const int kpl = args.count <= 64 ? 1 : args.count <= 256 ? 4 : args.count <= 1024 ? 16 : 0;
+    // 64 (<= 4096 = REPAIR_WAVE_MAX_TAPS), else 0 (keys compacted into LDS
+    // and read there on every pass; only that form uses LDS).
+    const int kpl = args.count <= 64 ? 1 : args.count <= 256 ? 4 : args.count <= 1024 ? 16
+                    : args.count <= 4096 ? 64 : 0;
+    const size_t lds = kpl == 0 ? repair_wave_lds_bytes(dtype_bytes, args.count) : 0;
     void (*fn)(GenericArgs) = dtype_bytes == 4 ? repair_kernel<float>(args.rank, kpl)
                                                : repair_kernel<double>(args.rank, kpl);
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(fn), hipFuncAttributeMaxDynamicSharedMemorySize,

@@ -1780,18 +1780,33 @@ namespace {
 //           (cost ~ positions);
 //   dense   the plan's engine over the whole snapshot, then the estimates
 //           are scattered back (cost ~ array).
-// Auto picks sparse when the plan has no tiled engine, or when the wave
-// kernel applies and at most 1/64 (network-engine windows) or 1/24 (slide-
-// engine windows) of the elements are listed. Measured on 8192^2 float32
-// with 1 % listed: 5x5 sparse 0.87 ms vs dense 1.50 ms; 31x31 sparse
-// 2.99 ms vs dense 13.7 ms.
+// Auto compares estimated costs (measured on MI355X, 8192^2 float32, 1 %
+// of the pixels listed; the paths give identical bits, so the choice only
+// affects time). Sparse, per position, by the keys each lane holds: 1.25 ns
+// (<= 64 taps), 2 ns (<= 256), 4.6 ns (<= 1024), 31 ns (<= 4096), and a
+// thread per position beyond. Dense, per element: the slide engine about
+// 0.2 ns at any window size (31x31 and 63x63 alike), the network engine
+// growing with the taps W (0.023 ns at 25 taps, 0.11 ns at 68, 0.37 ns at
+// 147). Plans without a tiled engine always run sparse.
+double repair_sparse_ns(size_t count)
+{
+    return count <= 64 ? 1.25 : count <= 256 ? 2.0 : count <= 1024 ? 4.6 : count <= 4096 ? 31.0 : 1000.0;
+}
+
+double repair_dense_ns(const ndf_plan *p)
+{
+    const double w = (double)p->count;
+    const double scale = p->dbytes == 8 ? 1.5 : 1.0;
+    if (p->engine == NDF_ENGINE_NETWORK) return scale * 0.0009 * w * (1.0 + w / 60.0);
+    return scale * 0.2;
+}
+
 bool repair_sparse(const ndf_plan *p, size_t npos, int32_t path)
 {
     if (path == NDF_REPAIR_SPARSE) return true;
     if (path == NDF_REPAIR_DENSE) return false;
-    if (!p->slide_ok && !p->network_ok) return true;
-    const double ratio = p->engine == NDF_ENGINE_NETWORK ? 64.0 : 24.0;
-    return p->count <= (size_t)REPAIR_WAVE_MAX_TAPS && (double)npos * ratio <= (double)p->len;
+    if (p->engine == NDF_ENGINE_GENERIC) return true;
+    return (double)npos * repair_sparse_ns(p->count) <= (double)p->len * repair_dense_ns(p);
 }
 
 // Enqueues the repair on `stream`; positions outside the array set
@@ -1810,7 +1825,7 @@ ndf_status repair_launch(ndf_plan *p, const void *d_in, void *d_out, void *d_sna
     if (repair_sparse(p, npos, path)) {
         ndf_status st = prepare_generic_tables(p, err, err_len);
         if (st != NDF_OK) return st;
-        e = hipMemcpyAsync(d_out, d_in, bytes, hipMemcpyDeviceToDevice, stream);
+        if (d_out != d_in) e = hipMemcpyAsync(d_out, d_in, bytes, hipMemcpyDeviceToDevice, stream);
         if (e == hipSuccess && p->all_nan) {
             e = launch_scatter_fill(p->dbytes, d_out, d_pos, (int64_t)npos, extent, nan_bits, flag, stream);
         } else if (e == hipSuccess) {
@@ -1874,8 +1889,15 @@ ndf_status ndf_plan_repair_device(ndf_plan *plan, const void *d_input, void *d_o
     hipStream_t s = static_cast<hipStream_t>(stream);
     if (len == 0) return NDF_OK;
     std::lock_guard<std::mutex> guard(plan->mutex);
+    if (d_snapshot == d_input || d_snapshot == d_output) {
+        return fail(NDF_ERR_INVALID, "the repair snapshot must not alias the input or the output", err, err_len);
+    }
     if (npos == 0) {
-        const hipError_t e = hipMemcpyAsync(d_output, d_input, len * (size_t)plan->dbytes, hipMemcpyDeviceToDevice, s);
+        hipError_t e = hipSuccess;
+        if (d_output != d_input) {
+            e = hipMemcpyAsync(d_output, d_input, len * (size_t)plan->dbytes, hipMemcpyDeviceToDevice, s);
+        }
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
         if (e == hipSuccess) return NDF_OK;
         return fail_hip(e, err, err_len);
     }
@@ -1891,6 +1913,9 @@ ndf_status ndf_plan_time_repair_device(ndf_plan *plan, const void *d_input, void
                                        size_t err_len)
 {
     if (len != plan->len || npos == 0) return fail_code(NDF_ERR_LENGTH_MISMATCH, err, err_len);
+    if (d_snapshot == d_input || d_snapshot == d_output) {
+        return fail(NDF_ERR_INVALID, "the repair snapshot must not alias the input or the output", err, err_len);
+    }
     if (path < NDF_REPAIR_AUTO || path > NDF_REPAIR_DENSE) return fail(NDF_ERR_INVALID, "unknown repair path", err, err_len);
     std::lock_guard<std::mutex> guard(plan->mutex);
     ndf_status st = repair_buffers(plan, err, err_len);

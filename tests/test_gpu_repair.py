@@ -87,7 +87,7 @@ def test_input_snapshot_and_ownership() -> None:
 
 
 @pytest.mark.parametrize("dtype", (np.float32, np.float64))
-@pytest.mark.parametrize("size", (5, 31))
+@pytest.mark.parametrize("size", (5, 31, 41))
 def test_sparse_and_dense_paths_agree(dtype, size: int) -> None:
     """Both device paths through the C ABI, on unsorted positions with duplicates."""
     rng = np.random.default_rng(size)
@@ -193,3 +193,23 @@ def test_rank_four_and_five_filters_and_repair(dtype) -> None:
         got = nf.fixpix(values, mask, method="median", size=size, origin=origin, mode=mode, cval=1.5)
         want = oracle.fixpix_median(values, mask, size, origin=origin, mode=mode, cval=1.5)
         assert np.array_equal(bits(got), bits(want))
+
+
+def test_in_place_repair_and_snapshot_aliasing() -> None:
+    rng = np.random.default_rng(9)
+    values = _cases.special_sample((80, 96), np.float32, seed=9)
+    mask = rng.random(values.shape) < 0.05
+    positions = np.flatnonzero(mask).astype(np.int64)
+    want = oracle.fixpix_median(values, mask, (5, 5), mode="reflect")
+    plan = _core.Plan(values.dtype, values.shape, "median", [5, 5], None, [0, 0], ["reflect"] * 2, np.nan, True,
+                      False, 1, 0)
+    d_buf, d_snap = _core.DeviceArray(values.nbytes), _core.DeviceArray(values.nbytes)
+    d_pos = _core.DeviceArray(positions.nbytes)
+    d_pos.upload(positions)
+    for path in ("sparse", "dense"):
+        print(f"in-place repair path={path}", flush=True)
+        d_buf.upload(values)
+        plan.repair_device(d_buf.address, d_buf.address, d_snap.address, d_pos.address, positions.size, path)
+        assert np.array_equal(bits(d_buf.download(np.empty_like(values))), bits(want)), path
+    with pytest.raises(ValueError, match="alias"):
+        plan.repair_device(d_buf.address, d_snap.address, d_buf.address, d_pos.address, positions.size)
