@@ -107,9 +107,10 @@ __global__ void __launch_bounds__(256) k_generic(GenericArgs a)
 
 // Sparse repair, one wave per listed position (windows of at most
 // REPAIR_WAVE_MAX_TAPS taps). The wave gathers the window once: with KPL > 0
-// tap lane + 64 q goes straight into register q of the lane (omitted taps
-// keep the sentinel key); with KPL = 0 the retained keys are compacted into
-// the wave's LDS region (ballot + prefix count). Selection is the same
+// tap lane + 64 q goes straight into register q of the lane (wide 2-D
+// rectangles: window row q, column lane), and omitted taps keep the
+// sentinel key; with KPL = 0 the retained keys are compacted into the
+// wave's LDS region (ballot + prefix count). Selection is the same
 // bitwise search as k_generic, but each pass counts the keys below the
 // candidate across the wave: every lane scans its keys and the counts are
 // summed across lanes. An even median takes the (n/2 - 1)-th key and then
@@ -173,13 +174,42 @@ struct WaveKeys {
     }
 };
 
-// k-th smallest (0-based) of the wave's keys; requires k < n.
+template <typename Key>
+__device__ inline Key wave_and(Key v)
+{
+    for (int d = 32; d >= 1; d >>= 1) v &= __shfl_xor(v, d);
+    return v;
+}
+
+template <typename Key>
+__device__ inline Key wave_or(Key v)
+{
+    for (int d = 32; d >= 1; d >>= 1) v |= __shfl_xor(v, d);
+    return v;
+}
+
+// k-th smallest (0-based) of the wave's keys; requires k < n. The bits that
+// every retained key shares (AND == OR across the wave) are the answer's
+// prefix, so the search starts at the highest bit where the keys differ.
 template <typename Key, int KPL>
 __device__ Key wave_select(const WaveKeys<Key, KPL> &keys, int64_t k, int lane)
 {
     constexpr int BITS = sizeof(Key) * 8;
-    Key answer = 0;
-    for (int b = BITS - 1; b >= 0; --b) {
+    constexpr Key SENTINEL = WaveKeys<Key, KPL>::SENTINEL;
+    Key all = SENTINEL, any = 0;
+    keys.each(lane, [&](Key v) {
+        if (v != SENTINEL) {
+            all &= v;
+            any |= v;
+        }
+    });
+    all = wave_and(all);
+    any = wave_or(any);
+    const Key differ = all ^ any;
+    if (differ == 0) return all;
+    const int top = BITS - 1 - (sizeof(Key) == 8 ? __clzll((unsigned long long)differ) : __clz((unsigned)differ));
+    Key answer = all & ~((Key(2) << top) - 1);
+    for (int b = top; b >= 0; --b) {
         const Key candidate = answer | (Key(1) << b);
         int below = 0;
         keys.each(lane, [&](Key v) { below += v < candidate; });
@@ -208,15 +238,33 @@ __global__ void __launch_bounds__(64 * REPAIR_WAVES) k_repair_wave(GenericArgs a
         WaveKeys<Key, KPL> held;
         int n = 0;
         if constexpr (KPL > 0) {
-            // Tap t = lane + 64 q lands in register q: every gather of the
-            // window is in flight at once, and omitted taps keep the
-            // sentinel key, which no select pass counts.
             int kept = 0;
+            if (R == 2 && KPL == 64 && a.rect_kx > 32 && a.rect_kx <= 64 && a.rect_ky <= 64) {
+                // Wide 2-D rectangles: register q holds window row q and lane
+                // l column l, so the column map entry is read once per lane,
+                // the row entry is uniform and each key is one coalesced load.
+                const int64_t mx = lane < a.rect_kx ? a.maps[a.map_base[1] + g.coord[1] + lane] : MAP_CVAL;
 #pragma unroll
-            for (int q = 0; q < KPL; ++q) {
-                const int t = lane + 64 * q;
-                held.r[q] = t < a.count ? g.key(t) : KT::SENTINEL;
-                kept += held.r[q] != KT::SENTINEL;
+                for (int q = 0; q < KPL; ++q) {
+                    Key key = KT::SENTINEL;
+                    if (q < a.rect_ky && lane < a.rect_kx) {
+                        const int64_t my = a.maps[a.map_base[0] + g.coord[0] + q];
+                        key = (my < 0 || mx < 0) ? classify<T>((Key)a.cval_bits, a.ignore_inf)
+                                                 : classify<T>(static_cast<const Key *>(a.input)[my + mx], a.ignore_inf);
+                    }
+                    held.r[q] = key;
+                    kept += key != KT::SENTINEL;
+                }
+            } else {
+                // Tap t = lane + 64 q lands in register q: every gather of the
+                // window is in flight at once, and omitted taps keep the
+                // sentinel key, which no select pass counts.
+#pragma unroll
+                for (int q = 0; q < KPL; ++q) {
+                    const int t = lane + 64 * q;
+                    held.r[q] = t < a.count ? g.key(t) : KT::SENTINEL;
+                    kept += held.r[q] != KT::SENTINEL;
+                }
             }
             n = wave_sum(kept);
         } else {

@@ -564,6 +564,10 @@ ndf_status prepare_generic_tables(ndf_plan *p, char *err, size_t err_len)
     g.stat = p->stat;
     g.min_count = p->eff_min_count;
     g.rank_param = (int64_t)std::min(p->parameter, p->count);
+    if (p->rank == 2 && p->footprint.empty()) {
+        g.rect_ky = (int32_t)p->size[0];
+        g.rect_kx = (int32_t)p->size[1];
+    }
     p->generic_ready = true;
     return NDF_OK;
 }
@@ -1782,15 +1786,15 @@ namespace {
 //           are scattered back (cost ~ array).
 // Auto compares estimated costs (measured on MI355X, 8192^2 float32, 1 %
 // of the pixels listed; the paths give identical bits, so the choice only
-// affects time). Sparse, per position, by the keys each lane holds: 1.25 ns
-// (<= 64 taps), 2 ns (<= 256), 4.6 ns (<= 1024), 31 ns (<= 4096), and a
+// affects time). Sparse, per position, by the keys each lane holds: 1.2 ns
+// (<= 64 taps), 2 ns (<= 256), 4.3 ns (<= 1024), 24 ns (<= 4096), and a
 // thread per position beyond. Dense, per element: the slide engine about
 // 0.2 ns at any window size (31x31 and 63x63 alike), the network engine
 // growing with the taps W (0.023 ns at 25 taps, 0.11 ns at 68, 0.37 ns at
 // 147). Plans without a tiled engine always run sparse.
 double repair_sparse_ns(size_t count)
 {
-    return count <= 64 ? 1.25 : count <= 256 ? 2.0 : count <= 1024 ? 4.6 : count <= 4096 ? 31.0 : 1000.0;
+    return count <= 64 ? 1.2 : count <= 256 ? 2.0 : count <= 1024 ? 4.3 : count <= 4096 ? 24.0 : 1000.0;
 }
 
 double repair_dense_ns(const ndf_plan *p)

@@ -45,6 +45,9 @@ struct GenericArgs {
     const int64_t *positions;
     int64_t extent;
     int32_t *error_flag;
+    // Rank-2 rectangular windows: rows and columns of the box (0 otherwise);
+    // tap t is then (t / rect_kx, t % rect_kx) relative to the box corner.
+    int32_t rect_ky, rect_kx;
 };
 
 // Engine 2: LDS-staged tile, block sort of the support, per-lane column slide

@@ -40,7 +40,7 @@ def test_modes_and_fractions_match_oracle(dtype, mode: str) -> None:
     rng = np.random.default_rng(101)
     values = _cases.special_sample((97, 131), dtype, seed=17)
     cval = -2.5 if mode == "constant" else None
-    for fraction, size in ((0.004, 5), (0.05, 3), (0.3, 9), (0.01, 31)):
+    for fraction, size in ((0.004, 5), (0.05, 3), (0.3, 9), (0.01, 31), (0.02, 45)):
         mask = rng.random(values.shape) < fraction
         print(f"fixpix {np.dtype(dtype).name} {mode} fraction={fraction} size={size}", flush=True)
         got = nf.fixpix(values, mask, method="median", size=size, mode=mode, cval=cval)
