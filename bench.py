@@ -338,9 +338,15 @@ def repair_bench(image: np.ndarray, steps: int, warmup: int, check: bool) -> dic
         if check:
             from oracle import oracle
 
-            want = oracle.fixpix_median(image, mask, (size, size), mode="reflect")
+            threads = oracle.default_threads()
+            start = time.perf_counter()
+            want = oracle.fixpix_median(image, mask, (size, size), mode="reflect", threads=threads)
+            cpu_ms = (time.perf_counter() - start) * 1e3
             entry["bit_exact_vs_oracle"] = bool(np.array_equal(outputs["sparse"].view(np.uint32),
                                                                want.view(np.uint32)))
+            entry["cpu"] = {"ms": cpu_ms, "threads": threads, "kind": "port",
+                            "what": "the oracle's restatement of repair_median (direct gather + select per masked "
+                                    "pixel), one run, host threads"}
         result[f"{size}x{size}"] = entry
     for buffer in (d_in, d_out, d_snap, d_pos):
         buffer.free()
