@@ -1786,7 +1786,7 @@ namespace {
 //           are scattered back (cost ~ array).
 // Auto compares estimated costs (measured on MI355X, 8192^2 float32, 1 %
 // of the pixels listed; the paths give identical bits, so the choice only
-// affects time). Sparse, per position, by the keys each lane holds: 1.2 ns
+// affects time). Sparse, per position, by the keys each lane holds: 1.0 ns
 // (<= 64 taps), 2 ns (<= 256), 4.3 ns (<= 1024), 24 ns (<= 4096), and a
 // thread per position beyond. Dense, per element: the slide engine about
 // 0.2 ns at any window size (31x31 and 63x63 alike), the network engine
@@ -1794,7 +1794,7 @@ namespace {
 // 147). Plans without a tiled engine always run sparse.
 double repair_sparse_ns(size_t count)
 {
-    return count <= 64 ? 1.2 : count <= 256 ? 2.0 : count <= 1024 ? 4.3 : count <= 4096 ? 24.0 : 1000.0;
+    return count <= 64 ? 1.0 : count <= 256 ? 2.0 : count <= 1024 ? 4.3 : count <= 4096 ? 24.0 : 1000.0;
 }
 
 double repair_dense_ns(const ndf_plan *p)
