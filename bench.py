@@ -373,7 +373,7 @@ def main() -> int:
                         help="only copy MIB MiB device-to-device 3 times (HBM counter calibration) and exit")
     args = parser.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (a filter shards by row strips with a "
+        print("bench.py measures one GPU: ndfiltrs_amd has no multi-GPU path (a filter shards by row strips with a "
               "halo exchange, which is designed but not built); run with --gpus 1", file=sys.stderr)
         return 2
     from ndfiltrs_amd import _core, bracket, synthetic

@@ -5,8 +5,8 @@ Signature, defaults, option checks, messages and ownership rules follow
 masked windows, over a snapshot of the data in which every masked pixel is
 NaN (``src/repair.rs:107-126``, ``src/filter_nd.rs:174-215``), so repaired
 values never feed another repair. The ``linear`` and ``mean`` methods are
-validated like the reference but are not part of this GPU build: they raise
-``NotImplementedError``.
+validated like the reference but are not implemented in ndfiltrs_amd: they
+raise ``NotImplementedError``.
 """
 
 from __future__ import annotations
@@ -51,7 +51,7 @@ def fixpix(
         means no bad pixels. NaN and infinities are not masked automatically.
     method : {'linear', 'median', 'mean'} or None
         ``None`` selects ``'mean'`` when ``kernel`` is given, else
-        ``'linear'``. Only ``'median'`` runs in this build.
+        ``'linear'``. Only ``'median'`` is implemented (on the GPU).
     size, footprint, origin, mode : optional
         Median window as in ``nanmedian_filter``; defaults 3 per axis, origin
         0 and ``'reflect'``.
@@ -98,7 +98,7 @@ def fixpix(
         raise TypeError("ignore_inf must be a boolean")
     if method != "median":
         raise NotImplementedError(
-            f"fixpix(method={method!r}) is not part of this GPU build; method='median' is"
+            f"fixpix(method={method!r}) is not implemented in ndfiltrs_amd; method='median' is"
         )
     arguments = filter_arguments(
         value,
