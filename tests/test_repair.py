@@ -187,3 +187,86 @@ def test_linear_and_mean_are_not_in_this_build() -> None:
         nf.fixpix(values, mask)
     with pytest.raises(NotImplementedError):
         nf.fixpix(values, mask, kernel=np.ones(3))
+
+
+def brute_fixpix(values, mask, size, *, mode="reflect", cval=np.nan, ignore_inf=False, min_count=1, origin=None,
+                 mutation=None):
+    """Loops over every masked pixel (src/filter_nd.rs:174-215 read literally):
+    samples come from the data with every masked pixel set to NaN; NaN is
+    dropped, infinities too under ignore_inf; fewer than min_count retained
+    samples give NaN; the median is the middle sample or the contract
+    midpoint, in float64, cast once. ``mutation`` plants one mistake."""
+    shape = values.shape
+    size = (size,) * values.ndim if isinstance(size, int) else tuple(size)
+    modes = (mode,) * values.ndim if isinstance(mode, str) else tuple(mode)
+    origin = (0,) * values.ndim if origin is None else tuple(origin)
+    anchors = [n // 2 + o for n, o in zip(size, origin)]
+    taps = [tuple(int(i) - a for i, a in zip(index, anchors)) for index in np.ndindex(size)]
+    live = values.astype(np.float64)
+    if mutation != "no_snapshot":
+        live[mask] = np.nan
+    out = values.copy()
+    for center in np.ndindex(shape):
+        if not mask[center]:
+            continue
+        samples = []
+        for tap in taps:
+            mapped = [_cases._map(c + t, n, m) for c, t, n, m in zip(center, tap, shape, modes)]
+            if None in mapped:
+                if mutation == "cval_dropped":
+                    continue
+                value = float(values.dtype.type(cval))
+            else:
+                value = float(live[tuple(mapped)])
+            if np.isnan(value) or (ignore_inf and np.isinf(value) and mutation != "inf_kept"):
+                continue
+            samples.append(value)
+        samples.sort(key=_cases._total_order)
+        enough = len(samples) >= min_count or mutation == "min_count_ignored"
+        if not samples or not enough:
+            result = np.nan
+        elif len(samples) % 2:
+            result = samples[len(samples) // 2]
+        else:
+            result = _cases._mid(samples[len(samples) // 2 - 1], samples[len(samples) // 2])
+        out[center] = result
+        if mutation == "sequential":
+            live[center] = result
+    return out
+
+
+def test_brute_force_passes_every_fixpix_golden_case() -> None:
+    for entry in KAT["fixpix_median"]:
+        values, mask, expected = _kat_arrays(entry)
+        out = brute_fixpix(values, mask, **entry["kwargs"])
+        assert np.array_equal(bits(out), expected), entry["name"]
+
+
+@pytest.mark.parametrize("mutation", ["no_snapshot", "sequential", "cval_dropped", "inf_kept",
+                                      "min_count_ignored"])
+def test_planted_repair_mistakes_fail_a_golden_case(mutation: str) -> None:
+    """Each plausible mistake in a repair restatement fails at least one of
+    the reference's literal answers, so those answers pin the oracle."""
+    failed = []
+    for entry in KAT["fixpix_median"]:
+        values, mask, expected = _kat_arrays(entry)
+        out = brute_fixpix(values, mask, mutation=mutation, **entry["kwargs"])
+        if not np.array_equal(bits(out), expected):
+            failed.append(entry["name"])
+    assert failed, f"no golden case catches {mutation}"
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_oracle_matches_brute_force_on_small_arrays(mode: str) -> None:
+    rng = np.random.default_rng(MODES.index(mode))
+    for dtype in (np.float32, np.float64):
+        values = _cases.special_sample((9, 11), dtype, seed=MODES.index(mode) + 7)
+        mask = rng.random(values.shape) < 0.3
+        for size, origin, policy in (((3, 3), (0, 0), (False, 1)), ((4, 5), (1, -1), (True, 3)),
+                                     ((1, 6), (0, 2), (True, 1))):
+            ignore_inf, min_count = policy
+            got = oracle.fixpix_median(values, mask, size, mode=mode, cval=2.0, origin=origin,
+                                       ignore_inf=ignore_inf, min_count=min_count)
+            want = brute_fixpix(values, mask, size, mode=mode, cval=2.0, origin=origin, ignore_inf=ignore_inf,
+                                min_count=min_count)
+            assert np.array_equal(bits(got), bits(want)), (dtype, size, origin, policy)
