@@ -335,6 +335,18 @@ def repair_bench(image: np.ndarray, steps: int, warmup: int, check: bool) -> dic
                            "image_mpix_s": image.size / (ms * 1e-3) / 1e6}
         entry["paths_bit_equal"] = bool(np.array_equal(outputs["sparse"].view(np.uint32),
                                                        outputs["dense"].view(np.uint32)))
+        import ndfiltrs_amd as nf
+
+        calls = []
+        for _ in range(3):
+            start = time.perf_counter()
+            public = nf.fixpix(image, mask, method="median", size=size)
+            calls.append(time.perf_counter() - start)
+        entry["end_to_end"] = {"ms": float(np.median(calls)) * 1e3, "repeats": 3,
+                               "bits_equal_device_path": bool(np.array_equal(public.view(np.uint32),
+                                                                             outputs["sparse"].view(np.uint32))),
+                               "what": "nf.fixpix on host arrays (validation, snapshot copies, H2D of data and "
+                                       "positions, automatic path, D2H), median of 3"}
         if check:
             from oracle import oracle
 
