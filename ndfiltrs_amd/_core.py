@@ -365,7 +365,8 @@ def fixpix_median(
     # Snapshot (GIL held): private C-ordered copies of the data and the mask.
     data = np.array(typed, dtype=typed.dtype, order="C", copy=True)
     flags = np.array(bad, dtype=np.uint8, order="C", copy=True)
-    output = np.zeros(data.shape, dtype=data.dtype)
+    # Every element is written by the native call (copied or repaired).
+    output = np.empty(data.shape, dtype=data.dtype)
     err = ctypes.create_string_buffer(256)
     status = _lib.lib.ndf_fixpix_median(
         data.dtype.itemsize,
