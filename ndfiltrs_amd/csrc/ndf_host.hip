@@ -1966,15 +1966,31 @@ ndf_status ndf_fixpix_median(int32_t dtype_bytes, const void *input, void *outpu
     // positions() in src/repair.rs:14-29: the mask must cover the array.
     if (mask_len != plan->len) return fail_code(NDF_ERR_LENGTH_MISMATCH, err, err_len);
     const size_t bytes = plan->len * (size_t)plan->dbytes;
+    // Masks are mostly zero: both scans skip 8-byte words with no set byte.
     std::vector<int64_t> positions;
     size_t npos = 0;
-    for (size_t i = 0; i < plan->len; ++i) npos += mask[i] != 0;
+    const size_t words = plan->len / 8;
+    for (size_t w = 0; w < words; ++w) {
+        uint64_t word;
+        memcpy(&word, mask + 8 * w, 8);
+        if (!word) continue;
+        for (int b = 0; b < 8; ++b) npos += mask[8 * w + b] != 0;
+    }
+    for (size_t i = 8 * words; i < plan->len; ++i) npos += mask[i] != 0;
     try {
         positions.reserve(npos);
     } catch (const std::bad_alloc &) {
         return fail_code(NDF_ERR_MEMORY, err, err_len);
     }
-    for (size_t i = 0; i < plan->len; ++i) {
+    for (size_t w = 0; w < words; ++w) {
+        uint64_t word;
+        memcpy(&word, mask + 8 * w, 8);
+        if (!word) continue;
+        for (int b = 0; b < 8; ++b) {
+            if (mask[8 * w + b]) positions.push_back((int64_t)(8 * w + b));
+        }
+    }
+    for (size_t i = 8 * words; i < plan->len; ++i) {
         if (mask[i]) positions.push_back((int64_t)i);
     }
     if (npos == 0) {
