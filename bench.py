@@ -14,8 +14,9 @@ the fallback tiles. Beside it, with the same K and W:
     16%/84% ranks,
   * the plain kernel on the other workloads (5x5 on 2048^2, annulus
     footprint on 8192^2, 3x7x7 on a 128x1024x1024 cube),
-every output bit-compared (bracket == plain over the whole image; plain ==
-CPU oracle on a strip). Prints one JSON line.
+every result verified by one extra launch into an output buffer filled with
+a poison pattern, then compared bit for bit over the whole image with the
+CPU oracle's result (and the bracket with plain). Prints one JSON line.
 """
 
 from __future__ import annotations
@@ -39,37 +40,38 @@ def log(message: str) -> None:
     print(message, file=sys.stderr, flush=True)
 
 
-def strip_matches_oracle(out: np.ndarray, image: np.ndarray, size, mode: str, rows: int, kind: str, rank: int,
-                         footprint=None, origin=None, ignore_inf=False) -> bool:
-    """Bit-compare the first `rows` output rows (first plane for 3-D) with the
-    oracle run on the input rows those outputs read."""
+# Poison written into every device output before a verification launch: a
+# non-canonical NaN payload that no filter produces, so an output the launch
+# fails to write cannot pass for a correct one.
+POISON = {4: 0x7FA5A5A5, 8: 0x7FF4A5A5A5A5A5A5}
+
+
+def oracle_full(image: np.ndarray, size, mode: str, kind: str = "median", rank: int = 0, footprint=None, origin=None,
+                ignore_inf: bool = False, method: str = "direct") -> tuple[np.ndarray, float]:
+    """The oracle over the whole image with every host thread; returns the
+    result and its wall time in seconds."""
     from oracle import oracle
 
-    if image.ndim == 2:
-        extent = (size[0] if footprint is None else footprint.shape[0]) + 2
-        sub = np.ascontiguousarray(image[: rows + extent])
-        got = out[:rows]
-    else:
-        sub = np.ascontiguousarray(image[:, : rows + size[1] + 2])
-        got = out[:, :rows]
-    kwargs = {"mode": mode, "omit_nan": True, "ignore_inf": ignore_inf, "method": "direct"}
+    kwargs = {"mode": mode, "omit_nan": True, "ignore_inf": ignore_inf, "method": method,
+              "threads": oracle.default_threads()}
     if kind == "rank":
         kwargs.update(statistic="rank", rank=rank)
+    start = time.perf_counter()
     if footprint is not None:
-        expected = oracle.window_filter(sub, footprint.shape, footprint=footprint, origin=origin, **kwargs)
+        out = oracle.window_filter(image, footprint.shape, footprint=footprint, origin=origin, **kwargs)
     else:
-        expected = oracle.window_filter(sub, size, origin=origin, **kwargs)
-    expected = expected[:rows] if image.ndim == 2 else expected[:, :rows]
-    # Rows near the strip's lower edge see the strip's boundary instead of the
-    # image's: compare the rows whose windows lie inside the strip.
-    if mode != "wrap":
-        view = f"u{out.itemsize}"
-        return bool(np.array_equal(got.view(view), expected.view(view)))
-    half = (footprint.shape[0] if footprint is not None else size[-2]) // 2 + 2
-    view = f"u{out.itemsize}"
-    if image.ndim == 2:
-        return bool(np.array_equal(got[half:].view(view), expected[half:].view(view)))
-    return bool(np.array_equal(got[:, half:].view(view), expected[:, half:].view(view)))
+        out = oracle.window_filter(image, size, origin=origin, **kwargs)
+    return out, time.perf_counter() - start
+
+
+def same_bits(a: np.ndarray, b: np.ndarray) -> bool:
+    view = f"u{a.itemsize}"
+    return bool(np.array_equal(a.view(view), b.view(view)))
+
+
+def unwritten(out: np.ndarray) -> int:
+    """Outputs still holding the poison pattern."""
+    return int(np.count_nonzero(out.view(f"u{out.itemsize}") == POISON[out.itemsize]))
 
 
 class Workload:
@@ -118,7 +120,11 @@ class Workload:
             "event_main_kernel_ms": main_ms,
         }
 
-    def output(self) -> np.ndarray:
+    def verify(self, plan) -> np.ndarray:
+        """One launch into a poisoned output buffer; returns its output."""
+        self.d_out.fill(POISON[self.image.itemsize], self.image.itemsize)
+        plan.apply_device(self.d_in.address, self.d_out.address)
+        plan.last_error()
         return self.d_out.download(np.empty_like(self.image))
 
     def free(self) -> None:
@@ -138,20 +144,23 @@ def ideal_seconds(pixels: int, taps: int, itemsize: int) -> float:
 
 
 def bracket_pair(work: Workload, kind: str, size: int, mode: str, parameter: int, weights, steps: int, warmup: int,
-                 check_rows: int) -> dict:
-    """Plain and learned-bracket runs of one statistic; bracket bits == plain bits."""
+                 expected: np.ndarray | None) -> dict:
+    """Plain and learned-bracket runs of one statistic. Each is verified by
+    one more launch into a poisoned output buffer, compared bit for bit with
+    the whole-image oracle result `expected` (and with each other)."""
     plan = work.plan(kind, (size, size), mode, parameter)
     plain = work.run(plan, steps, warmup)
-    plain_out = work.output()
-    plain["bit_exact_vs_oracle_rows"] = check_rows
-    plain["bit_exact_vs_oracle"] = strip_matches_oracle(plain_out, work.image, (size, size), mode, check_rows, kind,
-                                                        parameter)
+    plain_out = work.verify(plan)
+    plain["unwritten_outputs"] = unwritten(plain_out)
     plan.set_bracket(weights, 4096)
     bracketed = work.run(plan, steps, warmup)
+    out = work.verify(plan)
     tiles, fallback = plan.bracket_report()
-    out = work.output()
-    view = f"u{out.itemsize}"
-    bracketed["bits_equal_plain"] = bool(np.array_equal(out.view(view), plain_out.view(view)))
+    bracketed["unwritten_outputs"] = unwritten(out)
+    bracketed["bits_equal_plain"] = same_bits(out, plain_out)
+    if expected is not None:
+        plain["bit_exact_vs_oracle_full"] = same_bits(plain_out, expected)
+        bracketed["bit_exact_vs_oracle_full"] = same_bits(out, expected)
     bracketed["tiles"] = tiles
     bracketed["fallback_tiles"] = fallback
     bracketed["fallback_rate"] = fallback / tiles if tiles else 0.0
@@ -165,20 +174,26 @@ def bracket_pair(work: Workload, kind: str, size: int, mode: str, parameter: int
         "bracket": bracketed,
         "speedup": plain["ms_per_step"] / bracketed["ms_per_step"],
         "engine": json.loads(plan.describe()),
+        "verified": "one launch into a poisoned output per kernel, whole image vs the oracle"
+                    if expected is not None else "one launch into a poisoned output per kernel, bracket vs plain",
     }
 
 
-def plain_config(name: str, work: Workload, steps: int, warmup: int, check_rows: int, size=None, mode="reflect",
-                 footprint=None, origin=None, ignore_inf=False) -> dict:
+def plain_config(name: str, work: Workload, steps: int, warmup: int, size=None, mode="reflect", footprint=None,
+                 origin=None, ignore_inf=False, check: bool = True) -> dict:
     plan = work.plan("median", size, mode, 0, footprint, origin, ignore_inf)
     result = work.run(plan, steps, warmup)
-    out = work.output()
-    result["bit_exact_vs_oracle"] = strip_matches_oracle(out, work.image, size, mode, check_rows, "median", 0,
-                                                         footprint, origin, ignore_inf)
-    result["bit_exact_vs_oracle_rows"] = check_rows
+    out = work.verify(plan)
+    result["unwritten_outputs"] = unwritten(out)
+    if check:
+        expected, seconds = oracle_full(work.image, size, mode, footprint=footprint, origin=origin,
+                                        ignore_inf=ignore_inf)
+        result["bit_exact_vs_oracle_full"] = same_bits(out, expected)
+        result["oracle_seconds"] = seconds
+        del expected
     taps = int(footprint.sum()) if footprint is not None else int(np.prod(size))
     pixels = work.image.size
-    result["roofline_frac"] = ideal_seconds(pixels, taps, out.itemsize) / (result["event_launch_ms"] * 1e-3)
+    result["roofline_frac"] = ideal_seconds(pixels, taps, out.itemsize) / (result["event_main_kernel_ms"] * 1e-3)
     result["engine"] = json.loads(plan.describe())
     result["workload"] = name
     return result
@@ -224,14 +239,21 @@ def cutout_batch(steps: int, warmup: int) -> dict:
     _core.synchronize()
     batch_ms = (time.perf_counter() - start) * 1e3 / steps
     event_ms = b.time_device(d_in.address, d_out.address, 1, steps)
+    # Verification: one more launch of each form into poisoned outputs.
+    d_out.fill(POISON[4], 4)
+    d_ref.fill(POISON[4], 4)
+    b.apply_device(d_in.address, d_out.address)
+    separate()
+    _core.synchronize()
     got = d_out.download(np.empty_like(flat))
     ref = d_ref.download(np.empty_like(flat))
-    equal = bool(np.array_equal(got.view(np.uint32), ref.view(np.uint32)))
-    first = b.unpack(got)[0]
+    equal = same_bits(got, ref)
     from oracle import oracle
 
-    oracle_ok = bool(np.array_equal(first.view(np.uint32), oracle.window_filter(
-        images[0], (sizes[0], sizes[0]), mode=modes[0], omit_nan=True).view(np.uint32)))
+    threads = oracle.default_threads()
+    oracle_ok = all(
+        same_bits(out, oracle.window_filter(image, (size, size), mode=mode, omit_nan=True, threads=threads))
+        for out, image, size, mode in zip(b.unpack(got), images, sizes, modes))
     pixels = flat.size
     ideal = sum(ideal_seconds(img.size, s * s, 4) for img, s in zip(images, sizes))
     for buffer in (d_in, d_out, d_ref):
@@ -249,17 +271,19 @@ def cutout_batch(steps: int, warmup: int) -> dict:
         "autotune": tuned,
         "autotune_seconds": tune_s,
         "bits_equal_separate_calls": equal,
-        "first_cutout_bit_exact_vs_oracle": oracle_ok,
+        "unwritten_outputs": unwritten(got),
+        "bit_exact_vs_oracle_all_cutouts": oracle_ok,
     }
 
 
-def cpu_baseline(image: np.ndarray, size: int, mode: str) -> dict:
+def cpu_baseline(image: np.ndarray, size: int, mode: str) -> tuple[dict, np.ndarray]:
+    """The oracle's block rank histogram over the whole image (all host
+    threads): the CPU baseline, whose result is also the full-size parity
+    reference of the GPU outputs."""
     from oracle import oracle
 
     threads = oracle.default_threads()
-    start = time.perf_counter()
-    oracle.window_filter(image, (size, size), mode=mode, omit_nan=True, method="histogram", threads=threads)
-    seconds = time.perf_counter() - start
+    out, seconds = oracle_full(image, (size, size), mode, method="histogram")
     return {
         "value": image.size / seconds / 1e6,
         "unit": "Mpix/s",
@@ -268,7 +292,7 @@ def cpu_baseline(image: np.ndarray, size: int, mode: str) -> dict:
         "sample": f"the whole {image.shape[0]}x{image.shape[1]} image, nanmedian {size}x{size} {mode}, the "
                   f"oracle's restatement of the reference's block rank histogram (method='histogram'), "
                   f"{threads} threads, {seconds:.2f} s",
-    }
+    }, out
 
 
 def cpu_sample(image: np.ndarray, rows: int, method: str, **kwargs) -> dict:
@@ -329,10 +353,12 @@ def repair_bench(image: np.ndarray, steps: int, warmup: int, check: bool) -> dic
         for path in ("sparse", "dense"):
             ms = plan.time_repair_device(d_in.address, d_out.address, d_snap.address, d_pos.address,
                                          positions.size, path, warmup, steps)
+            d_out.fill(POISON[image.itemsize], image.itemsize)
             plan.repair_device(d_in.address, d_out.address, d_snap.address, d_pos.address, positions.size, path)
             outputs[path] = d_out.download(np.empty_like(image))
             entry[path] = {"ms": ms, "masked_mpix_s": positions.size / (ms * 1e-3) / 1e6,
-                           "image_mpix_s": image.size / (ms * 1e-3) / 1e6}
+                           "image_mpix_s": image.size / (ms * 1e-3) / 1e6,
+                           "unwritten_outputs": unwritten(outputs[path])}
         entry["paths_bit_equal"] = bool(np.array_equal(outputs["sparse"].view(np.uint32),
                                                        outputs["dense"].view(np.uint32)))
         import ndfiltrs_amd as nf
@@ -354,7 +380,7 @@ def repair_bench(image: np.ndarray, steps: int, warmup: int, check: bool) -> dic
             start = time.perf_counter()
             want = oracle.fixpix_median(image, mask, (size, size), mode="reflect", threads=threads)
             cpu_ms = (time.perf_counter() - start) * 1e3
-            entry["bit_exact_vs_oracle"] = bool(np.array_equal(outputs["sparse"].view(np.uint32),
+            entry["bit_exact_vs_oracle_full"] = bool(np.array_equal(outputs["sparse"].view(np.uint32),
                                                                want.view(np.uint32)))
             entry["cpu"] = {"ms": cpu_ms, "threads": threads, "kind": "port",
                             "what": "the oracle's restatement of repair_median (direct gather + select per masked "
@@ -413,12 +439,19 @@ def main() -> int:
     log("generating the 8192x8192 float32 sky (63x63 workload)")
     c5 = Workload(synthetic.sky_quadratic())
     w63, meta63 = bracket.load(bracket.model_path("sky63"))
+    baseline = None
+    expected_median = expected_rank = None
+    if not args.no_cpu:
+        log("CPU baseline and whole-image oracle results (63x63 median and rank 397)")
+        baseline, expected_median = cpu_baseline(c5.image, 63, "reflect")
+        expected_rank, _ = oracle_full(c5.image, (63, 63), "reflect", "rank", 397, method="histogram")
     log("63x63 nanmedian: plain and learned bracket")
-    median63 = bracket_pair(c5, "median", 63, "reflect", 0, w63, args.steps, args.warmup, check_rows=96)
+    median63 = bracket_pair(c5, "median", 63, "reflect", 0, w63, args.steps, args.warmup, expected_median)
     log(json.dumps(median63))
     log("63x63 rank 397 (10%): plain and learned bracket")
-    rank63 = bracket_pair(c5, "rank", 63, "reflect", 397, w63, args.steps, args.warmup, check_rows=32)
+    rank63 = bracket_pair(c5, "rank", 63, "reflect", 397, w63, args.steps, args.warmup, expected_rank)
     log(json.dumps(rank63))
+    del expected_median, expected_rank
     c5_image = c5.image
     c5.free()
 
@@ -429,16 +462,21 @@ def main() -> int:
         w31, _ = bracket.load(bracket.model_path("sky31_f64"))
         for kind, parameter in (("median", 0), ("rank", 154), ("rank", 806)):
             log(f"31x31 float64 {kind} {parameter}: plain and learned bracket")
+            expected = None
+            if not args.no_cpu:
+                expected, _ = oracle_full(c2.image, (31, 31), "nearest", kind, parameter, method="histogram")
             extra[f"31x31_f64_{kind}{parameter if kind == 'rank' else ''}"] = bracket_pair(
-                c2, kind, 31, "nearest", parameter, w31, args.steps, args.warmup, check_rows=64)
+                c2, kind, 31, "nearest", parameter, w31, args.steps, args.warmup, expected)
+            del expected
         if not args.no_cpu:
             extra["31x31_f64_median"]["cpu"] = cpu_sample(c2.image, 1024, "histogram", size=(31, 31),
                                                           mode="nearest", omit_nan=True)
         c2.free()
+        check = not args.no_cpu
         log("5x5 on 2048^2 float32")
         c1 = Workload(synthetic.sky_small())
         extra["5x5_f32"] = plain_config("nanmedian size=5 reflect 2048x2048 float32", c1, args.steps, args.warmup,
-                                        64, size=(5, 5))
+                                        size=(5, 5), check=check)
         if not args.no_cpu:
             extra["5x5_f32"]["cpu"] = cpu_sample(c1.image, 2048, "direct", size=(5, 5), mode="reflect",
                                                  omit_nan=True)
@@ -448,8 +486,8 @@ def main() -> int:
         ring = ((yy * yy + xx * xx) >= 16) & ((yy * yy + xx * xx) <= 36)
         c3a = Workload(synthetic.sky_annulus())
         extra["annulus_f32"] = plain_config("nanmedian annulus r 4..6 origin (1,0) wrap ignore_inf 8192x8192 float32",
-                                            c3a, args.steps, args.warmup, 64, mode="wrap", footprint=ring,
-                                            origin=(1, 0), ignore_inf=True)
+                                            c3a, args.steps, args.warmup, mode="wrap", footprint=ring,
+                                            origin=(1, 0), ignore_inf=True, check=check)
         if not args.no_cpu:
             extra["annulus_f32"]["cpu"] = cpu_sample(c3a.image, 1024, "direct", size=(13, 13),
                                                      footprint=ring, origin=(1, 0), mode="wrap", omit_nan=True,
@@ -458,8 +496,8 @@ def main() -> int:
         log("3x7x7 on the 128x1024x1024 cube")
         c3b = Workload(synthetic.cube())
         extra["cube_3x7x7_f32"] = plain_config("nanmedian size (3,7,7) wrap ignore_inf 128x1024x1024 float32", c3b,
-                                               args.steps, args.warmup, 16, size=(3, 7, 7), mode="wrap",
-                                               ignore_inf=True)
+                                               args.steps, args.warmup, size=(3, 7, 7), mode="wrap",
+                                               ignore_inf=True, check=check)
         if not args.no_cpu:
             extra["cube_3x7x7_f32"]["cpu"] = cpu_sample(c3b.image, 8, "direct", size=(3, 7, 7), mode="wrap",
                                                         omit_nan=True, ignore_inf=True)
@@ -471,10 +509,6 @@ def main() -> int:
         extra["cutouts_512"] = cutout_batch(args.steps, args.warmup)
         log(json.dumps(extra["cutouts_512"]))
 
-    baseline = None
-    if not args.no_cpu:
-        log("timing the CPU baseline")
-        baseline = cpu_baseline(c5_image, 63, "reflect")
     if not args.quick:
         log("end-to-end public call on the 8192x8192 image")
         median63["end_to_end"] = end_to_end(c5_image, size=63, mode="reflect")
@@ -520,7 +554,9 @@ def main() -> int:
             "workload": "learned-bracket nanmedian_filter size=63 mode=reflect on 8192x8192 float32, device-resident",
             "model": os.path.relpath(bracket.model_path("sky63"), ROOT),
             "bits_equal_plain": main_bracket["bits_equal_plain"],
-            "plain_bit_exact_vs_oracle": median63["plain"]["bit_exact_vs_oracle"],
+            "bit_exact_vs_oracle_full": main_bracket.get("bit_exact_vs_oracle_full"),
+            "plain_bit_exact_vs_oracle_full": median63["plain"].get("bit_exact_vs_oracle_full"),
+            "unwritten_outputs": main_bracket["unwritten_outputs"],
             "fallback_rate": main_bracket["fallback_rate"],
             "plain_mpix_s": median63["plain"]["mpix_s"],
             "speedup_vs_plain": median63["speedup"],

@@ -106,6 +106,23 @@ ndf_status ndf_plan_compute_download(ndf_plan *plan, void *output, size_t len, c
 ndf_status ndf_plan_apply_device(ndf_plan *plan, const void *d_input, void *d_output, size_t len,
                                  void *stream, char *err, size_t err_len);
 
+/* Launches of one plan are ordered: each waits for the plan's previous
+ * launch (on whichever stream it ran) before it starts, because launches
+ * share the plan's device scratch (error flag, bracket buffers). */
+
+/* Waits for the plan's last launch and returns DEVICE if one of its kernels
+ * reported an internal inconsistency (a rank walk that left its bounds),
+ * NDF_OK otherwise. Device-resident callers use it after
+ * ndf_plan_apply_device; the host-buffer calls check it themselves. No
+ * reference counterpart. */
+ndf_status ndf_plan_last_error(ndf_plan *plan, char *err, size_t err_len);
+
+/* Test hook (no reference counterpart): when enabled, host-buffer launches
+ * (ndf_plan_apply, ndf_plan_compute_download) first fill the plan's device
+ * output with `bits`, so an output element that no kernel writes keeps a
+ * pattern no filter produces (for example a non-canonical NaN payload). */
+ndf_status ndf_plan_set_output_fill(ndf_plan *plan, int32_t enabled, uint64_t bits);
+
 /* Force an engine (NDF_ENGINE_*); returns INVALID when the engine cannot run
  * this geometry. */
 ndf_status ndf_plan_set_engine(ndf_plan *plan, int32_t engine, char *err, size_t err_len);
@@ -228,6 +245,9 @@ ndf_status ndf_memcpy_to_device(void *d_dst, const void *h_src, size_t bytes);
 ndf_status ndf_memcpy_to_host(void *h_dst, const void *d_src, size_t bytes);
 /* Device-to-device copy (a plain copy kernel; used to calibrate HBM counters). */
 ndf_status ndf_memcpy_device(void *d_dst, const void *d_src, size_t bytes);
+/* Fill `count` elements of `element_bytes` (4 or 8) at d_ptr with `bits`
+ * (a fill kernel; blocking). */
+ndf_status ndf_device_fill(void *d_ptr, size_t count, int32_t element_bytes, uint64_t bits);
 ndf_status ndf_device_synchronize(void);
 /* Times `iterations` launches of ndf_plan_apply_device with HIP events on
  * one stream; writes the mean milliseconds per launch (every kernel of the

@@ -24,6 +24,20 @@ _MSG_LENGTH = "input length does not match the configured shape"
 _MSG_OVERFLOW = "shape or index arithmetic overflowed"
 
 
+# Test hook: when set (a pair of uint32 / uint64 bit patterns), every plan
+# created here fills its device output with that pattern before each
+# host-buffer launch, so an output element no kernel writes cannot inherit a
+# correct value from an earlier launch.
+_OUTPUT_POISON: tuple[int, int] | None = None
+
+
+def set_output_poison(bits: tuple[int, int] | None) -> None:
+    """Enable (``(float32_bits, float64_bits)``) or disable (``None``) the
+    output poison of plans created from now on."""
+    global _OUTPUT_POISON
+    _OUTPUT_POISON = None if bits is None else (int(bits[0]), int(bits[1]))
+
+
 def _raise(status: int, message: str) -> None:
     """Map a status to the exception class the reference raises
     (``src/pyapi/filter.rs:37-45``)."""
@@ -144,6 +158,20 @@ class Plan:
         if status != _lib.OK:
             _raise(status, err.value.decode())
         self._handle = handle
+        if _OUTPUT_POISON is not None:
+            self.set_output_fill(_OUTPUT_POISON[0 if dtype.itemsize == 4 else 1])
+
+    def set_output_fill(self, bits: int | None) -> None:
+        """Fill the device output with ``bits`` before every host-buffer
+        launch (``None`` turns it off)."""
+        _lib.lib.ndf_plan_set_output_fill(self._handle, int(bits is not None), int(bits or 0))
+
+    def last_error(self) -> None:
+        """Wait for the last launch; raise if a kernel reported an internal error."""
+        err = ctypes.create_string_buffer(256)
+        status = _lib.lib.ndf_plan_last_error(self._handle, err, len(err))
+        if status != _lib.OK:
+            _raise(status, err.value.decode())
 
     def __del__(self) -> None:
         handle = getattr(self, "_handle", None)
@@ -422,6 +450,14 @@ class DeviceArray:
         if _lib.lib.ndf_memcpy_to_host(array.ctypes.data, self.ptr, array.nbytes) != _lib.OK:
             raise RuntimeError("device-to-host copy failed")
         return array
+
+    def fill(self, bits: int, element_bytes: int, count: int | None = None) -> None:
+        """Fill ``count`` elements (default: the whole buffer) with ``bits``."""
+        count = self.nbytes // element_bytes if count is None else int(count)
+        if count * element_bytes > self.nbytes:
+            raise ValueError("fill larger than the device buffer")
+        if _lib.lib.ndf_device_fill(self.ptr, count, element_bytes, int(bits)) != _lib.OK:
+            raise RuntimeError("device fill failed")
 
     def free(self) -> None:
         if self.ptr.value:

@@ -104,6 +104,9 @@ def _declare(lib: ctypes.CDLL) -> None:
     lib.ndf_memcpy_to_host.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
     lib.ndf_memcpy_device.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
     lib.ndf_device_synchronize.argtypes = []
+    lib.ndf_device_fill.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int32, ctypes.c_uint64]
+    lib.ndf_plan_last_error.argtypes = [plan_p, *_err]
+    lib.ndf_plan_set_output_fill.argtypes = [plan_p, ctypes.c_int32, ctypes.c_uint64]
     lib.ndf_plan_time_device.argtypes = [
         plan_p,
         ctypes.c_void_p,
@@ -222,6 +225,9 @@ def _declare(lib: ctypes.CDLL) -> None:
         "ndf_memcpy_to_host",
         "ndf_memcpy_device",
         "ndf_device_synchronize",
+        "ndf_device_fill",
+        "ndf_plan_last_error",
+        "ndf_plan_set_output_fill",
         "ndf_plan_time_device",
     ):
         getattr(lib, name).restype = ctypes.c_int32

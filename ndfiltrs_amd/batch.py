@@ -129,6 +129,9 @@ class Batch:
         d_in = _core.DeviceArray(flat.nbytes)
         d_out = _core.DeviceArray(flat.nbytes)
         d_in.upload(flat)
+        poison = _core._OUTPUT_POISON
+        if poison is not None and flat.size:
+            d_out.fill(poison[0 if flat.itemsize == 4 else 1], flat.itemsize)
         self.apply_device(d_in.address, d_out.address)
         out = d_out.download(np.empty_like(flat))
         d_in.free()

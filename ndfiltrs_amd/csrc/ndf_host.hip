@@ -219,6 +219,18 @@ struct ndf_plan {
     // slide, bracket or generic kernel) for ndf_plan_time_device.
     hipEvent_t main_begin = nullptr, main_end = nullptr;
 
+    // Launch order: every launch waits for the plan's previous launch (on
+    // whatever stream that ran) and records `done` at its end, so launches
+    // of one plan never overlap on the plan's scratch (error flag, bracket
+    // buffers, fallback list, repair flag).
+    hipEvent_t done = nullptr;
+    bool done_recorded = false;
+
+    // Test hook: host-buffer launches first fill the device output with
+    // these bits, so an output a kernel fails to write keeps them.
+    bool output_fill = false;
+    uint64_t output_fill_bits = 0;
+
     // Device state.
     std::mutex mutex;
     bool tables_ready = false;
@@ -231,6 +243,7 @@ struct ndf_plan {
 
     ~ndf_plan()
     {
+        if (done) (void)hipEventDestroy(done);
         for (DeviceBuffer *b : {&n_mapz, &n_mapy, &n_mapx, &n_taps}) {
             b->release();
         }
@@ -575,7 +588,9 @@ ndf_status prepare_generic_tables(ndf_plan *p, char *err, size_t err_len)
 ndf_status prepare_engine_tables(ndf_plan *p, char *err, size_t err_len)
 {
     if (p->tables_ready) return NDF_OK;
+    const bool fresh = p->d_error.ptr == nullptr;
     hipError_t e = p->d_error.ensure(sizeof(int32_t));
+    if (e == hipSuccess && fresh) e = hipMemset(p->d_error.ptr, 0, sizeof(int32_t));
     if (e != hipSuccess) return fail_hip(e, err, err_len);
     if (p->engine == NDF_ENGINE_GENERIC) {
         const ndf_status st = prepare_generic_tables(p, err, err_len);
@@ -690,11 +705,36 @@ hipError_t bracket_propose(ndf_plan *p, const void *d_in, hipStream_t stream)
     return e;
 }
 
+// Orders a launch of plan p on `stream` after the plan's previous launch.
+hipError_t plan_begin(ndf_plan *p, hipStream_t stream)
+{
+    if (!p->done) {
+        hipError_t e = hipEventCreateWithFlags(&p->done, hipEventDisableTiming);
+        if (e != hipSuccess) {
+            p->done = nullptr;
+            return e;
+        }
+    }
+    return p->done_recorded ? hipStreamWaitEvent(stream, p->done, 0) : hipSuccess;
+}
+
+hipError_t plan_end(ndf_plan *p, hipStream_t stream)
+{
+    hipError_t e = hipEventRecord(p->done, stream);
+    if (e == hipSuccess) p->done_recorded = true;
+    return e;
+}
+
 ndf_status launch(ndf_plan *p, const void *d_in, void *d_out, hipStream_t stream, char *err, size_t err_len)
 {
-    hipError_t e = hipSuccess;
+    hipError_t e = plan_begin(p, stream);
     const uint64_t nan_bits = p->dbytes == 4 ? 0x7FC00000ull : 0x7FF8000000000000ull;
     p->bracket_launched = false;
+    // The slide kernels report a rank walk that left its bounds through
+    // d_error; it is cleared on the launch stream, ahead of them.
+    if (e == hipSuccess && p->engine == NDF_ENGINE_SLIDE && !p->all_nan) {
+        e = launch_fill(4, p->d_error.ptr, 1, 0, stream);
+    }
     auto mark = [&](hipEvent_t ev) {
         if (ev && e == hipSuccess) e = hipEventRecord(ev, stream);
     };
@@ -749,6 +789,7 @@ ndf_status launch(ndf_plan *p, const void *d_in, void *d_out, hipStream_t stream
         if (e == hipSuccess) e = launch_generic(p->dbytes, args, stream);
         mark(p->main_end);
     }
+    if (e == hipSuccess) e = plan_end(p, stream);
     return e == hipSuccess ? NDF_OK : fail_hip(e, err, err_len);
 }
 
@@ -990,6 +1031,41 @@ ndf_status ndf_plan_apply_device(ndf_plan *plan, const void *d_input, void *d_ou
 
 namespace {
 
+ndf_status read_error_flag(ndf_plan *plan, char *err, size_t err_len);
+
+}  // namespace
+
+ndf_status ndf_plan_last_error(ndf_plan *plan, char *err, size_t err_len)
+{
+    std::lock_guard<std::mutex> guard(plan->mutex);
+    return read_error_flag(plan, err, err_len);
+}
+
+ndf_status ndf_plan_set_output_fill(ndf_plan *plan, int32_t enabled, uint64_t bits)
+{
+    std::lock_guard<std::mutex> guard(plan->mutex);
+    plan->output_fill = enabled != 0;
+    plan->output_fill_bits = bits;
+    return NDF_OK;
+}
+
+namespace {
+
+// Waits for the plan's last launch and reports a rank walk that left its
+// bounds (the slide kernels' internal consistency check).
+ndf_status read_error_flag(ndf_plan *plan, char *err, size_t err_len)
+{
+    hipError_t e = hipSuccess;
+    if (plan->done_recorded) e = hipEventSynchronize(plan->done);
+    int32_t flag = 0;
+    if (e == hipSuccess && plan->d_error.ptr) {
+        e = hipMemcpy(&flag, plan->d_error.ptr, sizeof(flag), hipMemcpyDeviceToHost);
+    }
+    if (e != hipSuccess) return fail_hip(e, err, err_len);
+    if (flag) return fail(NDF_ERR_DEVICE, "internal error: rank walk left its bounds", err, err_len);
+    return NDF_OK;
+}
+
 ndf_status upload_locked(ndf_plan *plan, const void *input, char *err, size_t err_len)
 {
     ndf_status st = prepare_tables(plan, err, err_len);
@@ -1012,18 +1088,17 @@ ndf_status compute_download_locked(ndf_plan *plan, void *output, char *err, size
     plan->uploaded = false;
     const size_t bytes = plan->len * (size_t)plan->dbytes;
     hipError_t e;
-    if ((e = launch_fill(4, plan->d_error.ptr, 1, 0, nullptr)) != hipSuccess) return fail_hip(e, err, err_len);
+    if (plan->output_fill &&
+        (e = launch_fill(plan->dbytes, plan->d_out.ptr, (int64_t)plan->len, plan->output_fill_bits, nullptr)) !=
+            hipSuccess) {
+        return fail_hip(e, err, err_len);
+    }
     ndf_status st = launch(plan, plan->d_in.ptr, plan->d_out.ptr, nullptr, err, err_len);
     if (st != NDF_OK) return st;
     if ((e = hipMemcpy(output, plan->d_out.ptr, bytes, hipMemcpyDeviceToHost)) != hipSuccess) {
         return fail_hip(e, err, err_len);
     }
-    int32_t flag = 0;
-    if ((e = hipMemcpy(&flag, plan->d_error.ptr, sizeof(flag), hipMemcpyDeviceToHost)) != hipSuccess) {
-        return fail_hip(e, err, err_len);
-    }
-    if (flag) return fail(NDF_ERR_DEVICE, "internal error: rank walk left its bounds", err, err_len);
-    return NDF_OK;
+    return read_error_flag(plan, err, err_len);
 }
 
 }  // namespace
@@ -1702,6 +1777,14 @@ ndf_status ndf_memcpy_device(void *d_dst, const void *d_src, size_t bytes)
     return hipMemcpy(d_dst, d_src, bytes, hipMemcpyDeviceToDevice) == hipSuccess ? NDF_OK : NDF_ERR_DEVICE;
 }
 
+ndf_status ndf_device_fill(void *d_ptr, size_t count, int32_t element_bytes, uint64_t bits)
+{
+    if ((element_bytes != 4 && element_bytes != 8) || (count > 0 && !d_ptr)) return NDF_ERR_INVALID;
+    if (count == 0) return NDF_OK;
+    if (launch_fill(element_bytes, d_ptr, (int64_t)count, bits, nullptr) != hipSuccess) return NDF_ERR_DEVICE;
+    return hipDeviceSynchronize() == hipSuccess ? NDF_OK : NDF_ERR_DEVICE;
+}
+
 ndf_status ndf_device_synchronize(void) { return hipDeviceSynchronize() == hipSuccess ? NDF_OK : NDF_ERR_DEVICE; }
 
 ndf_status ndf_plan_time_device(ndf_plan *plan, const void *d_input, void *d_output, size_t len, int32_t warmup,
@@ -1723,9 +1806,6 @@ ndf_status ndf_plan_time_device(ndf_plan *plan, const void *d_input, void *d_out
     }
     for (size_t i = 0; st == NDF_OK && i < marks.size(); ++i) {
         if ((e = hipEventCreate(&marks[i])) != hipSuccess) st = fail_hip(e, err, err_len);
-    }
-    if (st == NDF_OK && (e = launch_fill(4, plan->d_error.ptr, 1, 0, stream)) != hipSuccess) {
-        st = fail_hip(e, err, err_len);
     }
     for (int i = 0; st == NDF_OK && i < warmup; ++i) st = launch(plan, d_input, d_output, stream, err, err_len);
     if (st == NDF_OK && (e = hipEventRecord(start, stream)) != hipSuccess) st = fail_hip(e, err, err_len);
@@ -1750,12 +1830,7 @@ ndf_status ndf_plan_time_device(ndf_plan *plan, const void *d_input, void *d_out
     if (st == NDF_OK) {
         *ms_per_launch = iterations > 0 ? (double)ms / iterations : 0.0;
         if (ms_main_kernel) *ms_main_kernel = iterations > 0 ? main_ms / iterations : 0.0;
-        int32_t flag = 0;
-        if ((e = hipMemcpy(&flag, plan->d_error.ptr, sizeof(flag), hipMemcpyDeviceToHost)) != hipSuccess) {
-            st = fail_hip(e, err, err_len);
-        } else if (flag) {
-            st = fail(NDF_ERR_DEVICE, "internal error: rank walk left its bounds", err, err_len);
-        }
+        st = read_error_flag(plan, err, err_len);
     }
     for (hipEvent_t ev : marks) {
         if (ev) (void)hipEventDestroy(ev);
@@ -1822,13 +1897,18 @@ ndf_status repair_launch(ndf_plan *p, const void *d_in, void *d_out, void *d_sna
     const uint64_t nan_bits = p->dbytes == 4 ? 0x7FC00000ull : 0x7FF8000000000000ull;
     int32_t *flag = static_cast<int32_t *>(p->r_flag.ptr);
     const int64_t extent = (int64_t)p->len;
-    hipError_t e = launch_fill(4, flag, 1, 0, stream);
+    const bool sparse = repair_sparse(p, npos, path);
+    ndf_status st = sparse ? prepare_generic_tables(p, err, err_len) : prepare_tables(p, err, err_len);
+    if (st != NDF_OK) return st;
+    // Both flags are cleared on the repair stream, after the plan's previous
+    // launch, so no earlier or concurrent call can clear a flag this one set.
+    hipError_t e = plan_begin(p, stream);
+    if (e == hipSuccess) e = launch_fill(4, flag, 1, 0, stream);
+    if (e == hipSuccess) e = launch_fill(4, p->d_error.ptr, 1, 0, stream);
     if (e == hipSuccess) e = hipMemcpyAsync(d_snap, d_in, bytes, hipMemcpyDeviceToDevice, stream);
     if (e == hipSuccess) e = launch_scatter_fill(p->dbytes, d_snap, d_pos, (int64_t)npos, extent, nan_bits, flag, stream);
     if (e != hipSuccess) return fail_hip(e, err, err_len);
-    if (repair_sparse(p, npos, path)) {
-        ndf_status st = prepare_generic_tables(p, err, err_len);
-        if (st != NDF_OK) return st;
+    if (sparse) {
         if (d_out != d_in) e = hipMemcpyAsync(d_out, d_in, bytes, hipMemcpyDeviceToDevice, stream);
         if (e == hipSuccess && p->all_nan) {
             e = launch_scatter_fill(p->dbytes, d_out, d_pos, (int64_t)npos, extent, nan_bits, flag, stream);
@@ -1843,16 +1923,16 @@ ndf_status repair_launch(ndf_plan *p, const void *d_in, void *d_out, void *d_sna
             e = p->count <= (size_t)REPAIR_WAVE_MAX_TAPS ? launch_repair_wave(p->dbytes, args, stream)
                                                         : launch_generic(p->dbytes, args, stream);
         }
+        if (e == hipSuccess) e = plan_end(p, stream);
         return e == hipSuccess ? NDF_OK : fail_hip(e, err, err_len);
     }
-    ndf_status st = prepare_tables(p, err, err_len);
-    if (st != NDF_OK) return st;
     // Whole-image statistic of the snapshot into d_out, estimates back into
     // the snapshot (whose unlisted entries are the input), then the result.
     st = launch(p, d_snap, d_out, stream, err, err_len);
     if (st != NDF_OK) return st;
     e = launch_scatter_copy(p->dbytes, d_snap, d_out, d_pos, (int64_t)npos, extent, flag, stream);
     if (e == hipSuccess) e = hipMemcpyAsync(d_out, d_snap, bytes, hipMemcpyDeviceToDevice, stream);
+    if (e == hipSuccess) e = plan_end(p, stream);
     return e == hipSuccess ? NDF_OK : fail_hip(e, err, err_len);
 }
 
@@ -1876,7 +1956,6 @@ ndf_status repair_buffers(ndf_plan *p, char *err, size_t err_len)
     if ((e = p->r_flag.ensure(sizeof(int32_t))) != hipSuccess || (e = p->d_error.ensure(sizeof(int32_t))) != hipSuccess) {
         return fail_hip(e, err, err_len);
     }
-    if ((e = launch_fill(4, p->d_error.ptr, 1, 0, nullptr)) != hipSuccess) return fail_hip(e, err, err_len);
     return NDF_OK;
 }
 

@@ -21,8 +21,24 @@ POLICIES = {
 }
 
 
+# Output poison: NaN payloads that no filter writes (outputs are input
+# values, f64 midpoints cast once, or the canonical NaN).
+POISON32 = 0x7FA5A5A5
+POISON64 = 0x7FF4A5A5A5A5A5A5
+
+
 def bits_of(array: np.ndarray) -> np.ndarray:
     return np.ascontiguousarray(array).view(np.uint32 if array.dtype == np.float32 else np.uint64)
+
+
+def poison_bits(dtype) -> int:
+    return POISON32 if np.dtype(dtype) == np.float32 else POISON64
+
+
+def assert_no_poison(array: np.ndarray, context: str = "") -> None:
+    """Fail when any output still holds the poison pattern (a skipped write)."""
+    left = int(np.count_nonzero(bits_of(array) == poison_bits(array.dtype)))
+    assert left == 0, f"{context}: {left} outputs were never written"
 
 
 def special_sample(shape: Sequence[int], dtype, seed: int, fraction: float = 0.15) -> np.ndarray:

@@ -12,6 +12,7 @@ import pytest
 
 from ndfiltrs_amd import _core, batch, synthetic
 from oracle import oracle
+from tests import _cases
 
 pytestmark = pytest.mark.gpu
 
@@ -103,8 +104,11 @@ def test_graph_replays_follow_input_changes() -> None:
             flat[rng.integers(0, flat.size, size=flat.size // 50)] = rng.normal(100.0, 50.0, size=flat.size // 50)
             d_in.upload(flat)
             single(images[0], sizes[0], modes[0])
+            d_out.fill(_cases.POISON32, 4)
             b.apply_device(d_in.address, d_out.address)
-            got = b.unpack(d_out.download(np.empty_like(flat)))
+            raw = d_out.download(np.empty_like(flat))
+            _cases.assert_no_poison(raw, f"replay {replay}")
+            got = b.unpack(raw)
             for i in (0, 7, 15):
                 expected = single(b.unpack(flat)[i], sizes[i], modes[i])
                 np.testing.assert_array_equal(bits(got[i]), bits(expected), err_msg=f"replay {replay} image {i}")
@@ -121,12 +125,16 @@ def test_autotune_keeps_results() -> None:
     d_out = _core.DeviceArray(flat.nbytes)
     try:
         d_in.upload(flat)
+        d_out.fill(_cases.POISON32, 4)
         b.apply_device(d_in.address, d_out.address)
         before = d_out.download(np.empty_like(flat))
+        _cases.assert_no_poison(before, "before autotune")
         info = b.autotune(d_in.address, d_out.address, repeats=1)
         assert info["tuned"] and info["ok"]
+        d_out.fill(_cases.POISON32, 4)
         b.apply_device(d_in.address, d_out.address)
         after = d_out.download(np.empty_like(flat))
+        _cases.assert_no_poison(after, "after autotune")
         np.testing.assert_array_equal(bits(after), bits(before))
         for i in (0, 13, 39):
             expected = single(images[i], [sizes[i]] * 2, [modes[i]] * 2)

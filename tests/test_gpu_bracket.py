@@ -18,6 +18,7 @@ import pytest
 
 from ndfiltrs_amd import _core, bracket, synthetic
 from oracle import oracle
+from tests import _cases
 
 pytestmark = pytest.mark.gpu
 
@@ -46,15 +47,20 @@ def make_plan(image, kind, size, parameter=0, mode="reflect", footprint=None, om
 
 
 def run(plan, image, weights=None, sort_len=4096):
+    """One launch into a poisoned output buffer: an output the launch does
+    not write keeps the poison (checked here), never an earlier result."""
     d_in = _core.DeviceArray(image.nbytes)
     d_out = _core.DeviceArray(image.nbytes)
     d_in.upload(image)
+    d_out.fill(_cases.poison_bits(image.dtype), image.itemsize)
     plan.set_bracket(weights, sort_len) if weights is not None else plan.set_bracket(None)
     plan.apply_device(d_in.address, d_out.address)
+    plan.last_error()
     out = d_out.download(np.empty_like(image))
     report = plan.bracket_report()
     d_in.free()
     d_out.free()
+    _cases.assert_no_poison(out, "bracket" if weights is not None else "plain")
     return out, report
 
 

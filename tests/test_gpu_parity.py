@@ -76,6 +76,7 @@ ENGINES = ("slide", "generic", "network")
 def assert_bits(actual: np.ndarray | None, expected: np.ndarray, context: str = "") -> None:
     if actual is None:  # engine not applicable to this geometry
         return
+    _cases.assert_no_poison(actual, context)
     a, e = _cases.bits_of(actual), _cases.bits_of(expected)
     if not np.array_equal(a, e):
         bad = np.argwhere(a != e)

@@ -109,8 +109,10 @@ def test_sparse_and_dense_paths_agree(dtype, size: int) -> None:
     d_pos.upload(positions)
     for path in ("sparse", "dense", "auto"):
         print(f"repair path={path} {np.dtype(dtype).name} size={size}", flush=True)
+        d_out.fill(_cases.poison_bits(values.dtype), values.itemsize)
         plan.repair_device(d_in.address, d_out.address, d_snap.address, d_pos.address, positions.size, path)
         got = d_out.download(np.empty_like(values))
+        _cases.assert_no_poison(got, path)
         assert np.array_equal(bits(got), bits(want)), path
 
 
@@ -133,8 +135,11 @@ def _device_repair(plan, values: np.ndarray, positions: np.ndarray, path: str) -
     d_pos = _core.DeviceArray(positions.nbytes)
     d_in.upload(values)
     d_pos.upload(positions)
+    d_out.fill(_cases.poison_bits(values.dtype), values.itemsize)
     plan.repair_device(d_in.address, d_out.address, d_snap.address, d_pos.address, positions.size, path)
-    return d_out.download(np.empty_like(values))
+    out = d_out.download(np.empty_like(values))
+    _cases.assert_no_poison(out, path)
+    return out
 
 
 def test_sparse_thread_fallback_above_the_wave_tap_limit() -> None:
