@@ -106,6 +106,23 @@ ndf_status ndf_plan_compute_download(ndf_plan *plan, void *output, size_t len, c
 ndf_status ndf_plan_apply_device(ndf_plan *plan, const void *d_input, void *d_output, size_t len,
                                  void *stream, char *err, size_t err_len);
 
+/* No-allocation form of ndf_plan_apply_device. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * It calls no hipMalloc, hipFree or other allocating routine: the plan's
+ * device tables and every scratch buffer of the launch (error flag, learned-
+ * bracket block moments, tile features, key brackets and fallback list) live
+ * in one static device array of 32 MiB per GPU, held until the process ends.
+ * A plan whose tables and scratch need more returns INVALID with the sizes
+ * (the learned bracket needs 0.375 bytes per pixel, so 8192 x 8192 fits).
+ * The tables are copied in on every call. Every call on one GPU shares that
+ * GPU's array and it keeps the last call's contents: calls of this function
+ * that overlap on one GPU (on other streams or threads) race, and the caller
+ * orders them. Results are bit-identical to ndf_plan_apply_device.
+ * Asynchronous on `stream`. */
+ndf_status ndf_plan_apply_device_static(ndf_plan *plan, const void *d_input, void *d_output, size_t len,
+                                        void *stream, char *err, size_t err_len);
+
 /* Launches of one plan are ordered: each waits for the plan's previous
  * launch (on whichever stream it ran) before it starts, because launches
  * share the plan's device scratch (error flag, bracket buffers). */

@@ -91,6 +91,14 @@ def _declare(lib: ctypes.CDLL) -> None:
         ctypes.c_void_p,
         *_err,
     ]
+    lib.ndf_plan_apply_device_static.argtypes = [
+        plan_p,
+        ctypes.c_void_p,
+        ctypes.c_void_p,
+        ctypes.c_size_t,
+        ctypes.c_void_p,
+        *_err,
+    ]
     lib.ndf_plan_set_engine.argtypes = [plan_p, ctypes.c_int32, *_err]
     lib.ndf_plan_describe.argtypes = [plan_p, ctypes.c_char_p, ctypes.c_size_t]
     lib.ndf_plan_destroy.argtypes = [plan_p]
@@ -216,6 +224,7 @@ def _declare(lib: ctypes.CDLL) -> None:
         "ndf_plan_upload",
         "ndf_plan_compute_download",
         "ndf_plan_apply_device",
+        "ndf_plan_apply_device_static",
         "ndf_plan_set_engine",
         "ndf_plan_describe",
         "ndf_device_count",

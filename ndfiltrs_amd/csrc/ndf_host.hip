@@ -240,6 +240,12 @@ struct ndf_plan {
     DeviceBuffer r_flag, r_snap, r_pos;  // mask-directed repair: position flag, snapshot, positions
     DeviceBuffer n_mapz, n_mapy, n_mapx, n_taps;
     GenericArgs generic{};
+    // Host copies of the generic engine's tables (maps, tap offsets).
+    bool generic_host_ready = false;
+    std::vector<int64_t> gen_maps;
+    std::vector<int32_t> gen_taps;
+    // Fallback counter of the last bracket launch (plan or static scratch).
+    const int32_t *report_fcount = nullptr;
 
     ~ndf_plan()
     {
@@ -535,13 +541,13 @@ ndf_status prepare_tables(ndf_plan *p, char *err, size_t err_len)
     return st;
 }
 
-// Generic-engine tables (per-axis maps and tap offsets). Also used by the
-// sparse repair path whatever engine the plan runs for whole images.
-ndf_status prepare_generic_tables(ndf_plan *p, char *err, size_t err_len)
+// Generic-engine tables (per-axis maps and tap offsets) on the host, and
+// the generic launch arguments without device pointers.
+void generic_host_tables(ndf_plan *p)
 {
-    if (p->generic_ready) return NDF_OK;
-    hipError_t e;
-    std::vector<int64_t> maps;
+    if (p->generic_host_ready) return;
+    std::vector<int64_t> &maps = p->gen_maps;
+    maps.clear();
     GenericArgs &g = p->generic;
     g = GenericArgs{};
     g.rank = (int32_t)p->rank;
@@ -561,17 +567,12 @@ ndf_status prepare_generic_tables(ndf_plan *p, char *err, size_t err_len)
             maps.push_back(m < 0 ? MAP_CVAL : m * g.stride[a]);
         }
     }
-    std::vector<int32_t> taps(p->count * p->rank);
+    p->gen_taps.assign(p->count * p->rank, 0);
     for (size_t t = 0; t < p->count; ++t) {
         for (size_t a = 0; a < p->rank; ++a) {
-            taps[t * p->rank + a] = (int32_t)(p->taps[t * p->rank + a] - p->low[a]);
+            p->gen_taps[t * p->rank + a] = (int32_t)(p->taps[t * p->rank + a] - p->low[a]);
         }
     }
-    if ((e = upload(p->g_maps, maps)) != hipSuccess || (e = upload(p->g_taps, taps)) != hipSuccess) {
-        return fail_hip(e, err, err_len);
-    }
-    g.maps = static_cast<const int64_t *>(p->g_maps.ptr);
-    g.taps = static_cast<const int32_t *>(p->g_taps.ptr);
     g.cval_bits = p->cval_bits;
     g.ignore_inf = p->eff_ignore_inf;
     g.stat = p->stat;
@@ -581,6 +582,21 @@ ndf_status prepare_generic_tables(ndf_plan *p, char *err, size_t err_len)
         g.rect_ky = (int32_t)p->size[0];
         g.rect_kx = (int32_t)p->size[1];
     }
+    p->generic_host_ready = true;
+}
+
+// Generic-engine tables on the device. Also used by the sparse repair path
+// whatever engine the plan runs for whole images.
+ndf_status prepare_generic_tables(ndf_plan *p, char *err, size_t err_len)
+{
+    if (p->generic_ready) return NDF_OK;
+    generic_host_tables(p);
+    hipError_t e;
+    if ((e = upload(p->g_maps, p->gen_maps)) != hipSuccess || (e = upload(p->g_taps, p->gen_taps)) != hipSuccess) {
+        return fail_hip(e, err, err_len);
+    }
+    p->generic.maps = static_cast<const int64_t *>(p->g_maps.ptr);
+    p->generic.taps = static_cast<const int32_t *>(p->g_taps.ptr);
     p->generic_ready = true;
     return NDF_OK;
 }
@@ -636,38 +652,22 @@ int64_t bracket_tiles(const ndf_plan *p)
     return (int64_t)p->slide.tiles_x * ((p->slide.ny + p->bracket_ty - 1) / p->bracket_ty);
 }
 
-// Allocates the bracket buffers and fills the bracket launch arguments.
-ndf_status prepare_bracket(ndf_plan *p, char *err, size_t err_len)
+// Bracket launch geometry (host only): the plain fallback kernel on the
+// bracket tiles, the bracket kernel and the proposal arguments, without
+// device pointers.
+void bracket_geometry_into(const ndf_plan *p, const SlideArgs &slide, SlideArgs &fb, size_t &fb_lds, SlideArgs &sb,
+                           size_t &sb_lds, BracketArgs &b)
 {
-    if (p->bracket_ready) return NDF_OK;
-    const int64_t tiles = bracket_tiles(p);
-    const int64_t by = (p->slide.ny + 7) / 8, bx = (p->slide.nx + 7) / 8;
-    hipError_t e;
-    if ((e = p->b_blocks.ensure((size_t)(by * bx * 3) * sizeof(double))) != hipSuccess ||
-        (e = p->b_features.ensure((size_t)tiles * FEATURES * sizeof(float))) != hipSuccess ||
-        (e = p->b_moments.ensure((size_t)tiles * 3 * sizeof(double))) != hipSuccess ||
-        (e = p->b_lo.ensure((size_t)tiles * (size_t)p->dbytes)) != hipSuccess ||
-        (e = p->b_hi.ensure((size_t)tiles * (size_t)p->dbytes)) != hipSuccess ||
-        (e = p->b_pred.ensure((size_t)tiles * 2 * sizeof(float))) != hipSuccess ||
-        (e = p->b_fallback.ensure((size_t)tiles * sizeof(int32_t))) != hipSuccess ||
-        (e = p->b_fcount.ensure(sizeof(int32_t))) != hipSuccess) {
-        return fail_hip(e, err, err_len);
-    }
-    if (!p->bracket_weights.empty() && (e = upload(p->b_weights, p->bracket_weights)) != hipSuccess) {
-        return fail_hip(e, err, err_len);
-    }
-    // Plain kernel on the bracket tiles (recomputes fallback tiles); it
-    // shares the plan's device tables.
-    p->slide_fb = with_tile_rows(p->slide, p->bracket_ty);
-    p->slide_fb_lds = slide_lds_bytes(p->dbytes, p->slide_fb.s, p->slide_fb.sort_e, p->slide_fb.hist_stride, 0);
-    BracketArgs &b = p->bargs;
+    const int64_t bx = (slide.nx + 7) / 8;
+    fb = with_tile_rows(slide, p->bracket_ty);
+    fb_lds = slide_lds_bytes(p->dbytes, fb.s, fb.sort_e, fb.hist_stride, 0);
     b = BracketArgs{};
-    b.ny = p->slide.ny;
-    b.nx = p->slide.nx;
+    b.ny = slide.ny;
+    b.nx = slide.nx;
     b.bx = bx;
-    b.ty = p->slide_fb.ty;
-    b.tiles_y = p->slide_fb.tiles_y;
-    b.tiles_x = p->slide_fb.tiles_x;
+    b.ty = fb.ty;
+    b.tiles_y = fb.tiles_y;
+    b.tiles_x = fb.tiles_x;
     const size_t ay = p->rank - 2, ax = p->rank - 1;
     b.low_y = (int32_t)p->low[ay];
     b.high_y = (int32_t)p->high[ay];
@@ -676,31 +676,173 @@ ndf_status prepare_bracket(ndf_plan *p, char *err, size_t err_len)
     b.quantile = p->stat == STAT_RANK
                      ? (float)((double)std::min(p->parameter, p->count) / (double)(p->count > 1 ? p->count - 1 : 1))
                      : 0.5f;
-    b.features = static_cast<float *>(p->b_features.ptr);
-    b.moments = static_cast<double *>(p->b_moments.ptr);
-    b.lo_keys = p->b_lo.ptr;
-    b.hi_keys = p->b_hi.ptr;
-    b.predicted = static_cast<float *>(p->b_pred.ptr);
-    p->slide_b = p->slide_fb;
-    p->slide_b.sort_e = p->bracket_e;
-    p->slide_b.bracket_lo = p->b_lo.ptr;
-    p->slide_b.bracket_hi = p->b_hi.ptr;
-    p->slide_b.fallback_tiles = static_cast<int32_t *>(p->b_fallback.ptr);
-    p->slide_b.fallback_count = static_cast<int32_t *>(p->b_fcount.ptr);
-    p->slide_b_lds = bracket_lds_bytes(p->dbytes, p->slide_fb.s, p->slide_fb.rows, p->bracket_e);
+    sb = fb;
+    sb.sort_e = p->bracket_e;
+    sb_lds = bracket_lds_bytes(p->dbytes, fb.s, fb.rows, p->bracket_e);
+}
+
+void bracket_geometry(ndf_plan *p)
+{
+    bracket_geometry_into(p, p->slide, p->slide_fb, p->slide_fb_lds, p->slide_b, p->slide_b_lds, p->bargs);
+}
+
+// Bytes of the bracket's per-launch buffers, in carve order: block moments
+// (double[3] per 8x8 block), features, tile moments, lo / hi keys,
+// predictions, fallback list, fallback count.
+void bracket_buffer_sizes(const ndf_plan *p, size_t sizes[8])
+{
+    const int64_t tiles = bracket_tiles(p);
+    const int64_t by = (p->slide.ny + 7) / 8, bx = (p->slide.nx + 7) / 8;
+    sizes[0] = (size_t)(by * bx * 3) * sizeof(double);
+    sizes[1] = (size_t)tiles * FEATURES * sizeof(float);
+    sizes[2] = (size_t)tiles * 3 * sizeof(double);
+    sizes[3] = (size_t)tiles * (size_t)p->dbytes;
+    sizes[4] = (size_t)tiles * (size_t)p->dbytes;
+    sizes[5] = (size_t)tiles * 2 * sizeof(float);
+    sizes[6] = (size_t)tiles * sizeof(int32_t);
+    sizes[7] = sizeof(int32_t);
+}
+
+// Points the bracket arguments at its buffers (in bracket_buffer_sizes order).
+void bracket_bind(SlideArgs &slide_b, BracketArgs &b, void *const buffers[8])
+{
+    b.features = static_cast<float *>(buffers[1]);
+    b.moments = static_cast<double *>(buffers[2]);
+    b.lo_keys = buffers[3];
+    b.hi_keys = buffers[4];
+    b.predicted = static_cast<float *>(buffers[5]);
+    slide_b.bracket_lo = buffers[3];
+    slide_b.bracket_hi = buffers[4];
+    slide_b.fallback_tiles = static_cast<int32_t *>(buffers[6]);
+    slide_b.fallback_count = static_cast<int32_t *>(buffers[7]);
+}
+
+// Allocates the bracket buffers and fills the bracket launch arguments.
+ndf_status prepare_bracket(ndf_plan *p, char *err, size_t err_len)
+{
+    if (p->bracket_ready) return NDF_OK;
+    size_t sizes[8];
+    bracket_buffer_sizes(p, sizes);
+    DeviceBuffer *buffers[8] = {&p->b_blocks, &p->b_features, &p->b_moments, &p->b_lo,
+                                &p->b_hi,     &p->b_pred,     &p->b_fallback, &p->b_fcount};
+    hipError_t e = hipSuccess;
+    for (int i = 0; i < 8 && e == hipSuccess; ++i) e = buffers[i]->ensure(sizes[i]);
+    if (e != hipSuccess) return fail_hip(e, err, err_len);
+    if (!p->bracket_weights.empty() && (e = upload(p->b_weights, p->bracket_weights)) != hipSuccess) {
+        return fail_hip(e, err, err_len);
+    }
+    // The plain fallback kernel shares the plan's device tables (p->slide).
+    bracket_geometry(p);
+    void *ptrs[8];
+    for (int i = 0; i < 8; ++i) ptrs[i] = buffers[i]->ptr;
+    bracket_bind(p->slide_b, p->bargs, ptrs);
     p->bracket_ready = true;
     return NDF_OK;
 }
 
-// Features and bracket keys of every tile for the input in d_in.
-hipError_t bracket_propose(ndf_plan *p, const void *d_in, hipStream_t stream)
+// Device pointers of one launch: the launch arguments of every engine with
+// their tables bound, and the bracket and error buffers. A plan launch takes
+// them from the plan's own device buffers; the no-allocation launch from the
+// static scratch array.
+struct LaunchSet {
+    SlideArgs slide{}, slide_b{}, slide_fb{};
+    NetworkArgs network{};
+    GenericArgs generic{};
+    BracketArgs bargs{};
+    size_t slide_b_lds = 0, slide_fb_lds = 0;
+    double *blocks = nullptr;
+    const float *weights = nullptr;
+    int32_t *fallback = nullptr;
+    int32_t *fcount = nullptr;
+    int32_t *error = nullptr;
+};
+
+LaunchSet plan_set(const ndf_plan *p)
 {
-    const int64_t tiles = bracket_tiles(p);
-    hipError_t e = launch_block_stats(p->dbytes, d_in, p->slide.ny, p->slide.nx, p->eff_ignore_inf,
-                                      static_cast<double *>(p->b_blocks.ptr), stream);
-    if (e == hipSuccess) e = launch_tile_features(p->bargs, static_cast<const double *>(p->b_blocks.ptr), tiles, stream);
-    if (e == hipSuccess && p->bracket_on) {
-        e = launch_bracket_predict(p->dbytes, p->bargs, static_cast<const float *>(p->b_weights.ptr), tiles, stream);
+    LaunchSet t;
+    t.slide = p->slide;
+    t.slide_b = p->slide_b;
+    t.slide_fb = p->slide_fb;
+    t.network = p->network;
+    t.generic = p->generic;
+    t.bargs = p->bargs;
+    t.slide_b_lds = p->slide_b_lds;
+    t.slide_fb_lds = p->slide_fb_lds;
+    t.blocks = static_cast<double *>(p->b_blocks.ptr);
+    t.weights = static_cast<const float *>(p->b_weights.ptr);
+    t.fallback = static_cast<int32_t *>(p->b_fallback.ptr);
+    t.fcount = static_cast<int32_t *>(p->b_fcount.ptr);
+    t.error = static_cast<int32_t *>(p->d_error.ptr);
+    return t;
+}
+
+// Enqueues the plan's kernels with the pointers of `t` on `stream`.
+hipError_t enqueue(ndf_plan *p, const LaunchSet &t, const void *d_in, void *d_out, hipStream_t stream)
+{
+    hipError_t e = hipSuccess;
+    const uint64_t nan_bits = p->dbytes == 4 ? 0x7FC00000ull : 0x7FF8000000000000ull;
+    p->bracket_launched = false;
+    auto mark = [&](hipEvent_t ev) {
+        if (ev && e == hipSuccess) e = hipEventRecord(ev, stream);
+    };
+    // The slide kernels report a rank walk that left its bounds through the
+    // error flag; it is cleared on the launch stream, ahead of them.
+    if (p->engine == NDF_ENGINE_SLIDE && !p->all_nan) e = launch_fill(4, t.error, 1, 0, stream);
+    if (e != hipSuccess) return e;
+    if (p->all_nan) {
+        mark(p->main_begin);
+        e = launch_fill(p->dbytes, d_out, (int64_t)p->len, nan_bits, stream);
+        mark(p->main_end);
+    } else if (p->engine == NDF_ENGINE_SLIDE && p->bracket_on) {
+        // Learned bracket: propose (block moments, tile features, MLP),
+        // filter in-bracket samples only, then recompute every fallback tile
+        // with the plain kernel.
+        const int tiles = (int)bracket_tiles(p);
+        p->bracket_launched = true;
+        p->report_fcount = t.fcount;
+        e = launch_block_stats(p->dbytes, d_in, p->slide.ny, p->slide.nx, p->eff_ignore_inf, t.blocks, stream);
+        if (e == hipSuccess) e = launch_tile_features(t.bargs, t.blocks, tiles, stream);
+        if (e == hipSuccess) e = launch_bracket_predict(p->dbytes, t.bargs, t.weights, tiles, stream);
+        if (e == hipSuccess) e = launch_fill(4, t.fcount, 1, 0, stream);
+        if (e == hipSuccess) {
+            SlideArgs args = t.slide_b;
+            args.input = d_in;
+            args.output = d_out;
+            mark(p->main_begin);
+            if (e == hipSuccess) e = launch_bracket_slide(p->dbytes, args, t.slide_b_lds, tiles, stream);
+            mark(p->main_end);
+        }
+        if (e == hipSuccess) {
+            SlideArgs args = t.slide_fb;
+            args.input = d_in;
+            args.output = d_out;
+            args.tile_list = t.fallback;
+            args.tile_count = t.fcount;
+            e = launch_slide(p->dbytes, args, t.slide_fb_lds, tiles, stream);
+        }
+    } else if (p->engine == NDF_ENGINE_NETWORK) {
+        NetworkArgs args = t.network;
+        args.input = d_in;
+        args.output = d_out;
+        mark(p->main_begin);
+        if (e == hipSuccess) e = launch_network(p->dbytes, args, p->network_lds, stream);
+        mark(p->main_end);
+    } else if (p->engine == NDF_ENGINE_SLIDE) {
+        SlideArgs args = t.slide;
+        args.input = d_in;
+        args.output = d_out;
+        mark(p->main_begin);
+        if (e == hipSuccess) {
+            e = launch_slide(p->dbytes, args, p->slide_lds, args.tiles_x * args.tiles_y * (int)args.nz, stream);
+        }
+        mark(p->main_end);
+    } else {
+        GenericArgs args = t.generic;
+        args.input = d_in;
+        args.output = d_out;
+        mark(p->main_begin);
+        if (e == hipSuccess) e = launch_generic(p->dbytes, args, stream);
+        mark(p->main_end);
     }
     return e;
 }
@@ -728,67 +870,7 @@ hipError_t plan_end(ndf_plan *p, hipStream_t stream)
 ndf_status launch(ndf_plan *p, const void *d_in, void *d_out, hipStream_t stream, char *err, size_t err_len)
 {
     hipError_t e = plan_begin(p, stream);
-    const uint64_t nan_bits = p->dbytes == 4 ? 0x7FC00000ull : 0x7FF8000000000000ull;
-    p->bracket_launched = false;
-    // The slide kernels report a rank walk that left its bounds through
-    // d_error; it is cleared on the launch stream, ahead of them.
-    if (e == hipSuccess && p->engine == NDF_ENGINE_SLIDE && !p->all_nan) {
-        e = launch_fill(4, p->d_error.ptr, 1, 0, stream);
-    }
-    auto mark = [&](hipEvent_t ev) {
-        if (ev && e == hipSuccess) e = hipEventRecord(ev, stream);
-    };
-    if (p->all_nan) {
-        mark(p->main_begin);
-        e = launch_fill(p->dbytes, d_out, (int64_t)p->len, nan_bits, stream);
-        mark(p->main_end);
-    } else if (p->engine == NDF_ENGINE_SLIDE && p->bracket_on) {
-        // Learned bracket: propose, filter in-bracket samples only, then
-        // recompute every fallback tile with the plain kernel.
-        const int tiles = (int)bracket_tiles(p);
-        p->bracket_launched = true;
-        e = bracket_propose(p, d_in, stream);
-        if (e == hipSuccess) e = launch_fill(4, p->b_fcount.ptr, 1, 0, stream);
-        if (e == hipSuccess) {
-            SlideArgs args = p->slide_b;
-            args.input = d_in;
-            args.output = d_out;
-            mark(p->main_begin);
-            if (e == hipSuccess) e = launch_bracket_slide(p->dbytes, args, p->slide_b_lds, tiles, stream);
-            mark(p->main_end);
-        }
-        if (e == hipSuccess) {
-            SlideArgs args = p->slide_fb;
-            args.input = d_in;
-            args.output = d_out;
-            args.tile_list = static_cast<const int32_t *>(p->b_fallback.ptr);
-            args.tile_count = static_cast<const int32_t *>(p->b_fcount.ptr);
-            e = launch_slide(p->dbytes, args, p->slide_fb_lds, tiles, stream);
-        }
-    } else if (p->engine == NDF_ENGINE_NETWORK) {
-        NetworkArgs args = p->network;
-        args.input = d_in;
-        args.output = d_out;
-        mark(p->main_begin);
-        if (e == hipSuccess) e = launch_network(p->dbytes, args, p->network_lds, stream);
-        mark(p->main_end);
-    } else if (p->engine == NDF_ENGINE_SLIDE) {
-        SlideArgs args = p->slide;
-        args.input = d_in;
-        args.output = d_out;
-        mark(p->main_begin);
-        if (e == hipSuccess) {
-            e = launch_slide(p->dbytes, args, p->slide_lds, args.tiles_x * args.tiles_y * (int)args.nz, stream);
-        }
-        mark(p->main_end);
-    } else {
-        GenericArgs args = p->generic;
-        args.input = d_in;
-        args.output = d_out;
-        mark(p->main_begin);
-        if (e == hipSuccess) e = launch_generic(p->dbytes, args, stream);
-        mark(p->main_end);
-    }
+    if (e == hipSuccess) e = enqueue(p, plan_set(p), d_in, d_out, stream);
     if (e == hipSuccess) e = plan_end(p, stream);
     return e == hipSuccess ? NDF_OK : fail_hip(e, err, err_len);
 }
@@ -1185,10 +1267,10 @@ ndf_status ndf_plan_bracket_report(ndf_plan *plan, int64_t *tiles, int64_t *fall
     std::lock_guard<std::mutex> guard(plan->mutex);
     *tiles = bracket_supported(plan) ? bracket_tiles(plan) : 0;
     *fallback_tiles = 0;
-    if (!plan->bracket_launched) return NDF_OK;
+    if (!plan->bracket_launched || !plan->report_fcount) return NDF_OK;
     int32_t count = 0;
     if (hipDeviceSynchronize() != hipSuccess ||
-        hipMemcpy(&count, plan->b_fcount.ptr, sizeof(count), hipMemcpyDeviceToHost) != hipSuccess) {
+        hipMemcpy(&count, plan->report_fcount, sizeof(count), hipMemcpyDeviceToHost) != hipSuccess) {
         return NDF_ERR_DEVICE;
     }
     *fallback_tiles = count;
@@ -1223,10 +1305,12 @@ ndf_status ndf_plan_bracket_training_data(ndf_plan *plan, const void *d_input, c
         (e = d_weight.ensure((size_t)tiles * sizeof(float))) != hipSuccess) {
         return fail_hip(e, err, err_len);
     }
-    const bool on = plan->bracket_on;
-    plan->bracket_on = false;  // features only
-    e = bracket_propose(plan, d_input, nullptr);
-    plan->bracket_on = on;
+    // Features only: block moments, then per-tile features.
+    e = launch_block_stats(plan->dbytes, d_input, plan->slide.ny, plan->slide.nx, plan->eff_ignore_inf,
+                           static_cast<double *>(plan->b_blocks.ptr), nullptr);
+    if (e == hipSuccess) {
+        e = launch_tile_features(plan->bargs, static_cast<const double *>(plan->b_blocks.ptr), tiles, nullptr);
+    }
     if (e == hipSuccess) {
         e = launch_tile_labels(plan->dbytes, plan->bargs, d_output, static_cast<float *>(d_targets.ptr),
                                static_cast<float *>(d_weight.ptr), tiles, nullptr);
@@ -2098,6 +2182,155 @@ ndf_status ndf_fixpix_median(int32_t dtype_bytes, const void *input, void *outpu
         return fail_hip(e, err, err_len);
     }
     return NDF_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// No-allocation launches.
+//
+// ndf_plan_apply_device_static runs the plan's kernels with every device
+// table (boundary maps, tap offsets, edge lists, footprint mask, bracket
+// weights) and every scratch buffer (error flag, block moments, tile
+// features and key brackets, fallback list and count) carved out of one
+// file-scope device array, so it never calls hipMalloc or hipFree. Each GPU
+// holds its own copy of the array for the life of the process; its address
+// on a device comes from hipGetSymbolAddress, taken once per device with the
+// device current. All calls on one GPU share that copy and it keeps the
+// last call's contents: the tables are copied in on every call, and the
+// counters that must start at zero are cleared by fill kernels on the call's
+// stream. Calls that overlap on one GPU (on other streams or threads) race
+// on the array; the caller orders them.
+// ---------------------------------------------------------------------------
+
+namespace {
+
+constexpr size_t STATIC_SCRATCH_BYTES = (size_t)32 << 20;
+constexpr int STATIC_MAX_DEVICES = 64;
+
+static __device__ __align__(256) unsigned char scratch[STATIC_SCRATCH_BYTES];
+
+std::mutex g_scratch_mutex;
+void *g_scratch_address[STATIC_MAX_DEVICES] = {};
+
+// Device address of `scratch` on the current device (looked up once per device).
+hipError_t scratch_address(void **out)
+{
+    int device = 0;
+    hipError_t e = hipGetDevice(&device);
+    if (e != hipSuccess) return e;
+    if (device < 0 || device >= STATIC_MAX_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> guard(g_scratch_mutex);
+    if (!g_scratch_address[device]) {
+        void *ptr = nullptr;
+        e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_address[device] = ptr;
+    }
+    *out = g_scratch_address[device];
+    return hipSuccess;
+}
+
+// Sequential 256-byte aligned regions of the scratch array. With a null base
+// it only counts bytes.
+struct Carve {
+    char *base;
+    size_t used = 0;
+    void *take(size_t bytes)
+    {
+        const size_t at = (used + 255) & ~(size_t)255;
+        used = at + bytes;
+        return base ? base + at : nullptr;
+    }
+};
+
+// Lays out one launch of plan p in the scratch array at `base` (null: size
+// only) and copies the plan's host tables there on `stream`. *needed
+// receives the bytes the layout uses. Leaves the plan's own launch
+// arguments untouched.
+hipError_t static_layout(ndf_plan *p, char *base, hipStream_t stream, LaunchSet *t, size_t *needed)
+{
+    Carve c{base};
+    hipError_t e = hipSuccess;
+    auto put = [&](const auto &values) -> void * {
+        const size_t bytes = values.size() * sizeof(values[0]);
+        void *dst = c.take(bytes);
+        if (base && bytes && e == hipSuccess) {
+            e = hipMemcpyAsync(dst, values.data(), bytes, hipMemcpyHostToDevice, stream);
+        }
+        return dst;
+    };
+    *t = LaunchSet{};
+    t->error = static_cast<int32_t *>(c.take(sizeof(int32_t)));
+    if (p->all_nan) {
+        // A fill kernel only.
+    } else if (p->engine == NDF_ENGINE_GENERIC) {
+        generic_host_tables(p);
+        t->generic = p->generic;
+        t->generic.maps = static_cast<const int64_t *>(put(p->gen_maps));
+        t->generic.taps = static_cast<const int32_t *>(put(p->gen_taps));
+    } else if (p->engine == NDF_ENGINE_NETWORK) {
+        NetworkArgs &n = t->network;
+        n = p->network;
+        n.mapz = static_cast<const int64_t *>(put(p->net_mapz));
+        n.mapy = static_cast<const int64_t *>(put(p->net_mapy));
+        n.mapx = static_cast<const int64_t *>(put(p->net_mapx));
+        n.taps = static_cast<const uint32_t *>(put(p->net_taps));
+    } else {
+        SlideArgs &sl = t->slide;
+        sl = p->slide;
+        sl.mapz = static_cast<const int64_t *>(put(p->slide_mapz));
+        sl.mapy = static_cast<const int64_t *>(put(p->slide_mapy));
+        sl.mapx = static_cast<const int64_t *>(put(p->slide_mapx));
+        sl.taps = static_cast<const uint32_t *>(put(p->slide_taps));
+        sl.edge_remove = static_cast<const uint32_t *>(put(p->slide_rm));
+        sl.edge_add = static_cast<const uint32_t *>(put(p->slide_add));
+        const void *mask = put(p->slide_mask);
+        sl.mask = p->slide_mask.empty() ? nullptr : static_cast<const uint8_t *>(mask);
+        sl.error_flag = t->error;
+        if (p->bracket_on) {
+            bracket_geometry_into(p, sl, t->slide_fb, t->slide_fb_lds, t->slide_b, t->slide_b_lds, t->bargs);
+            size_t sizes[8];
+            bracket_buffer_sizes(p, sizes);
+            void *ptrs[8];
+            for (int i = 0; i < 8; ++i) ptrs[i] = c.take(sizes[i]);
+            bracket_bind(t->slide_b, t->bargs, ptrs);
+            t->blocks = static_cast<double *>(ptrs[0]);
+            t->fallback = static_cast<int32_t *>(ptrs[6]);
+            t->fcount = static_cast<int32_t *>(ptrs[7]);
+            t->weights = static_cast<const float *>(put(p->bracket_weights));
+        }
+    }
+    *needed = c.used;
+    return e;
+}
+
+}  // namespace
+
+extern "C" {
+
+ndf_status ndf_plan_apply_device_static(ndf_plan *plan, const void *d_input, void *d_output, size_t len,
+                                        void *stream, char *err, size_t err_len)
+{
+    if (len != plan->len) return fail_code(NDF_ERR_LENGTH_MISMATCH, err, err_len);
+    if (len == 0) return NDF_OK;
+    std::lock_guard<std::mutex> guard(plan->mutex);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    LaunchSet t;
+    size_t needed = 0;
+    (void)static_layout(plan, nullptr, nullptr, &t, &needed);
+    if (needed > STATIC_SCRATCH_BYTES) {
+        char message[192];
+        snprintf(message, sizeof message,
+                 "the no-allocation launch needs %zu bytes of device scratch for this plan; the limit is %zu",
+                 needed, STATIC_SCRATCH_BYTES);
+        return fail(NDF_ERR_INVALID, message, err, err_len);
+    }
+    void *base = nullptr;
+    hipError_t e = scratch_address(&base);
+    if (e == hipSuccess) e = static_layout(plan, static_cast<char *>(base), s, &t, &needed);
+    if (e == hipSuccess) e = enqueue(plan, t, d_input, d_output, s);
+    return e == hipSuccess ? NDF_OK : fail_hip(e, err, err_len);
 }
 
 }  // extern "C"
