@@ -47,7 +47,12 @@ enum {
     NDF_KIND_MEDIAN_NETWORK = 1,
     NDF_KIND_MEDIAN_HISTOGRAM = 2,
     NDF_KIND_RANK = 3,
-    NDF_KIND_RANK_HISTOGRAM = 4
+    NDF_KIND_RANK_HISTOGRAM = 4,
+    /* Minimum / Maximum in IEEE total order (src/reduction.rs:43-46,
+     * 110-119); a rectangle runs the separable extrema engine, as the
+     * reference does (src/plan.rs:270-281). */
+    NDF_KIND_MIN = 5,
+    NDF_KIND_MAX = 6
 };
 
 /* Device engines. AUTO picks by geometry (NETWORK for windows of at most 160
@@ -72,6 +77,25 @@ ndf_status ndf_window_filter(int32_t dtype_bytes, const void *input, void *outpu
                              int32_t omit_nan, int32_t ignore_inf, size_t min_count, size_t parameter,
                              char *err, size_t err_len);
 
+/* Robust-filter kinds (src/pyapi/robust.rs:110-124); only the percentile is
+ * on this path. */
+enum { NDF_ROBUST_PERCENTILE = 0 };
+
+/* One-shot robust filter over host buffers: replaces
+ * _core.robust_filter(array, size, footprint, origin, mode, cval, kind,
+ * parameter, omit_nan, ignore_inf, min_count, ...) (src/pyapi/robust.rs:
+ * 86-158) for kind "percentile": RobustReduction::Percentile{q, policy}
+ * (src/robust.rs:145-151), the linear-interpolated percentile of the
+ * retained samples (src/robust.rs:320-367), q = `parameter` in [0, 100].
+ * Errors in the reference's order: "ignore_inf and min_count require NaN
+ * omission", "unsupported robust filter kind", the window's, then
+ * "percentile must be finite and between zero and 100" and "min_count must
+ * be between one and the number of window elements". */
+ndf_status ndf_robust_filter(int32_t dtype_bytes, const void *input, void *output, size_t rank, const size_t *shape,
+                             const size_t *size, const uint8_t *footprint, const ptrdiff_t *origin,
+                             const int32_t *modes, double cval, int32_t kind, double parameter, int32_t omit_nan,
+                             int32_t ignore_inf, size_t min_count, char *err, size_t err_len);
+
 /* Reusable plan for one shape: replaces WindowFilter::<T>::new
  * (src/plan.rs:260-324). Validation order and messages follow
  * Window::new (src/window.rs:44-137), FilterConfig::from_window
@@ -84,6 +108,14 @@ ndf_status ndf_plan_create(ndf_plan **plan, int32_t dtype_bytes, size_t rank, co
                            const int32_t *modes, double cval, int32_t kind, int32_t omit_nan,
                            int32_t ignore_inf, size_t min_count, size_t parameter, char *err,
                            size_t err_len);
+
+/* Plan form of ndf_robust_filter: replaces RobustConfig::from_window
+ * (src/robust.rs:698-722) for the percentile; apply it with the ndf_plan_*
+ * calls. */
+ndf_status ndf_robust_plan_create(ndf_plan **plan, int32_t dtype_bytes, size_t rank, const size_t *shape,
+                                  const size_t *size, const uint8_t *footprint, const ptrdiff_t *origin,
+                                  const int32_t *modes, double cval, int32_t kind, double parameter, int32_t omit_nan,
+                                  int32_t ignore_inf, size_t min_count, char *err, size_t err_len);
 
 /* Filter host buffers of `len` elements each: replaces WindowFilter::apply_into
  * (src/plan.rs:367-427); LENGTH_MISMATCH when len differs from prod(shape).

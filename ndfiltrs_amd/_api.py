@@ -1,9 +1,11 @@
-"""Public median and rank filters (drop-in signatures of ndfiltrs).
+"""Public median, rank and extrema filters (drop-in signatures of ndfiltrs).
 
 Signatures, defaults and error behaviour follow ``python/ndfiltrs/_api.py``
-(``median_filter`` :86-152, ``nanmedian_filter`` :155-225, ``rank_filter``
-:945-1013, ``get_num_threads`` :1016-1028). Every method string is served by
-the same bit-exact GPU engines; the strings keep their validation.
+(``median_filter`` :86-152, ``nanmedian_filter`` :155-225, ``min_filter``
+:388-435, ``max_filter`` :438-485, ``nanmin_filter`` :488-551,
+``nanmax_filter`` :554-617, ``rank_filter`` :945-1013, ``get_num_threads``
+:1016-1028). Every method string is served by the same bit-exact GPU engines;
+the strings keep their validation.
 """
 
 from __future__ import annotations
@@ -130,6 +132,89 @@ def nanmedian_filter(
         array, size, mode, cval, footprint=footprint, origin=origin, min_count=min_count
     )
     return _run(_median_kind(method, arguments), arguments, ignore_inf=bool(ignore_inf))
+
+
+def min_filter(
+    array: ArrayLike,
+    size: int | tuple[int, ...] | None = None,
+    *,
+    footprint: ArrayLike | None = None,
+    origin: int | tuple[int, ...] = 0,
+    mode: BoundaryMode | tuple[BoundaryMode, ...] = "reflect",
+    cval: float = 0.0,
+) -> NDArray[np.float32] | NDArray[np.float64]:
+    """Smallest sample of each window, in IEEE total order, on the GPU.
+
+    Total order puts -0.0 below +0.0, so a window holding both gives -0.0.
+    A window containing NaN gives canonical NaN; infinities take part. A
+    rectangular window runs the separable extrema engine. Parameters are
+    those of :func:`median_filter` without ``method``.
+    """
+    arguments = filter_arguments(array, size, mode, cval, footprint=footprint, origin=origin, min_count=1)
+    return _run("min", arguments)
+
+
+def max_filter(
+    array: ArrayLike,
+    size: int | tuple[int, ...] | None = None,
+    *,
+    footprint: ArrayLike | None = None,
+    origin: int | tuple[int, ...] = 0,
+    mode: BoundaryMode | tuple[BoundaryMode, ...] = "reflect",
+    cval: float = 0.0,
+) -> NDArray[np.float32] | NDArray[np.float64]:
+    """Largest sample of each window, in IEEE total order, on the GPU.
+
+    A window holding both signed zeros gives +0.0. NaN propagates as
+    canonical NaN; infinities take part. Parameters are those of
+    :func:`min_filter`.
+    """
+    arguments = filter_arguments(array, size, mode, cval, footprint=footprint, origin=origin, min_count=1)
+    return _run("max", arguments)
+
+
+def nanmin_filter(
+    array: ArrayLike,
+    size: int | tuple[int, ...] | None = None,
+    *,
+    footprint: ArrayLike | None = None,
+    origin: int | tuple[int, ...] = 0,
+    mode: BoundaryMode | tuple[BoundaryMode, ...] = "reflect",
+    cval: float = 0.0,
+    ignore_inf: bool = False,
+    min_count: int = 1,
+) -> NDArray[np.float32] | NDArray[np.float64]:
+    """:func:`min_filter` that omits NaN samples (and infinities when
+    ``ignore_inf``); fewer than ``min_count`` retained samples give canonical
+    NaN. ``cval`` samples follow the same rules, and boundary repeats count
+    once per tap."""
+    if not isinstance(ignore_inf, (bool, np.bool_)):
+        raise TypeError("ignore_inf must be a boolean")
+    arguments = filter_arguments(
+        array, size, mode, cval, footprint=footprint, origin=origin, min_count=min_count
+    )
+    return _run("min", arguments, ignore_inf=bool(ignore_inf))
+
+
+def nanmax_filter(
+    array: ArrayLike,
+    size: int | tuple[int, ...] | None = None,
+    *,
+    footprint: ArrayLike | None = None,
+    origin: int | tuple[int, ...] = 0,
+    mode: BoundaryMode | tuple[BoundaryMode, ...] = "reflect",
+    cval: float = 0.0,
+    ignore_inf: bool = False,
+    min_count: int = 1,
+) -> NDArray[np.float32] | NDArray[np.float64]:
+    """:func:`max_filter` that omits NaN samples; other rules as in
+    :func:`nanmin_filter`."""
+    if not isinstance(ignore_inf, (bool, np.bool_)):
+        raise TypeError("ignore_inf must be a boolean")
+    arguments = filter_arguments(
+        array, size, mode, cval, footprint=footprint, origin=origin, min_count=min_count
+    )
+    return _run("max", arguments, ignore_inf=bool(ignore_inf))
 
 
 def rank_filter(

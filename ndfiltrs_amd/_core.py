@@ -90,18 +90,23 @@ class Plan:
         omit_nan: bool,
         ignore_inf: bool,
         min_count: int,
-        parameter: int,
+        parameter: int | float,
     ) -> None:
+        """``kind`` is a window-filter kind (``parameter`` the nonnegative
+        rank of the rank kinds) or ``"percentile"``, the robust percentile
+        (``RobustConfig``, ``src/robust.rs:698-722``) with ``parameter`` the
+        percentile in [0, 100]."""
         self._handle = None
         dtype = np.dtype(dtype)
-        if kind not in _lib.KINDS:
+        robust = kind == "percentile"
+        if not robust and kind not in _lib.KINDS:
             raise ValueError("unsupported window filter kind")
         histogram = kind in ("median_histogram", "rank_histogram")
         ndim = len(shape)
         size = _as_sizes(size, "size")
         origin = [int(value) for value in origin]
         min_count = _as_sizes([min_count], "min_count")[0]
-        parameter = _as_sizes([parameter], "parameter")[0]
+        parameter = float(parameter) if robust else _as_sizes([parameter], "parameter")[0]
         modes = []
         for value in mode:
             if value not in _lib.MODES:
@@ -137,24 +142,31 @@ class Plan:
         c_modes = (ctypes.c_int32 * ndim)(*modes)
         err = ctypes.create_string_buffer(256)
         handle = ctypes.c_void_p()
-        status = _lib.lib.ndf_plan_create(
-            ctypes.byref(handle),
-            dtype.itemsize,
-            ndim,
-            c_shape,
-            c_size,
-            None if fp is None else fp.ctypes.data,
-            c_origin,
-            c_modes,
-            float(cval),
-            _lib.KINDS[kind],
-            int(bool(omit_nan)),
-            int(bool(ignore_inf)),
-            min_count,
-            parameter,
-            err,
-            len(err),
-        )
+        if robust:
+            status = _lib.lib.ndf_robust_plan_create(
+                ctypes.byref(handle), dtype.itemsize, ndim, c_shape, c_size,
+                None if fp is None else fp.ctypes.data, c_origin, c_modes, float(cval), _lib.ROBUST_KINDS[kind],
+                parameter, int(bool(omit_nan)), int(bool(ignore_inf)), min_count, err, len(err),
+            )
+        else:
+            status = _lib.lib.ndf_plan_create(
+                ctypes.byref(handle),
+                dtype.itemsize,
+                ndim,
+                c_shape,
+                c_size,
+                None if fp is None else fp.ctypes.data,
+                c_origin,
+                c_modes,
+                float(cval),
+                _lib.KINDS[kind],
+                int(bool(omit_nan)),
+                int(bool(ignore_inf)),
+                min_count,
+                parameter,
+                err,
+                len(err),
+            )
         if status != _lib.OK:
             _raise(status, err.value.decode())
         self._handle = handle
@@ -317,11 +329,13 @@ def window_filter(
     min_count: int,
     parameter: int,
 ) -> np.ndarray:
-    """Apply one window reduction (median or rank kinds) on the GPU.
+    """Apply one window reduction on the GPU.
 
     ``kind`` is one of ``median``, ``median_network``, ``median_histogram``,
-    ``rank`` or ``rank_histogram``; ``parameter`` is the nonnegative rank for
-    the rank kinds. The input must be C-contiguous.
+    ``rank``, ``rank_histogram``, ``min`` or ``max``; ``parameter`` is the
+    nonnegative rank for the rank kinds. The input must be C-contiguous. The
+    reference's other kinds (means, variances) are not on this path and give
+    ``ValueError("unsupported window filter kind")``.
     """
     if kind not in _lib.KINDS:
         raise ValueError("unsupported window filter kind")
@@ -342,6 +356,51 @@ def window_filter(
         min_count,
         parameter,
     )
+    return plan.apply(typed)
+
+
+def robust_filter(
+    array: np.ndarray,
+    size: Sequence[int],
+    footprint: Sequence[bool] | None,
+    origin: Sequence[int],
+    mode: Sequence[str],
+    cval: float,
+    kind: str,
+    parameter: float,
+    omit_nan: bool,
+    ignore_inf: bool,
+    min_count: int,
+    sigma_lower: float,
+    sigma_upper: float,
+    maxiters: int | None,
+    center: str,
+    spread: str,
+    estimator: str = "sex",
+    modify_sample_size: bool = False,
+    pearson: float = 2.5,
+) -> np.ndarray:
+    """Apply one robust window reduction on the GPU (``_core.robust_filter``,
+    ``src/pyapi/robust.rs:86-158``).
+
+    ``kind="percentile"`` (``parameter`` = q in [0, 100]) is implemented: the
+    linear-interpolated percentile of the retained samples
+    (``src/robust.rs:320-367``). The reference's other robust kinds (MAD,
+    mode, biweight, sigma clipping) are not on this path and raise
+    ``NotImplementedError``; the clipping arguments are accepted for
+    signature compatibility.
+    """
+    if kind in _lib.ROBUST_OTHER_KINDS:
+        raise NotImplementedError(f"robust filter kind {kind!r} is not implemented in ndfiltrs_amd")
+    if kind not in _lib.ROBUST_KINDS:
+        raise ValueError("unsupported robust filter kind")
+    if not omit_nan and (ignore_inf or int(min_count) != 1):
+        raise ValueError("ignore_inf and min_count require NaN omission")
+    typed = _typed_array(array)
+    if not typed.flags.c_contiguous or not typed.flags.aligned:
+        raise ValueError("native input must be aligned and C-contiguous")
+    plan = Plan(typed.dtype, typed.shape, kind, size, footprint, origin, mode, cval, omit_nan, ignore_inf, min_count,
+                parameter)
     return plan.apply(typed)
 
 

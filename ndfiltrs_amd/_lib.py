@@ -29,7 +29,14 @@ KINDS = {
     "median_histogram": 2,
     "rank": 3,
     "rank_histogram": 4,
+    "min": 5,
+    "max": 6,
 }
+# Robust-filter kinds of the reference (src/pyapi/robust.rs:110-140); only
+# the percentile is implemented here.
+ROBUST_KINDS = {"percentile": 0}
+ROBUST_OTHER_KINDS = ("mad", "mode", "biweight_location", "biweight_scale", "sigma_mean", "sigma_median",
+                      "sigma_mode")
 ENGINES = {"auto": 0, "generic": 1, "slide": 2, "network": 3}
 REPAIR_PATHS = {"auto": 0, "sparse": 1, "dense": 2}
 
@@ -77,6 +84,41 @@ def _declare(lib: ctypes.CDLL) -> None:
         ctypes.c_int32,
         ctypes.c_int32,
         ctypes.c_size_t,
+        ctypes.c_size_t,
+        *_err,
+    ]
+    lib.ndf_robust_plan_create.argtypes = [
+        ctypes.POINTER(ctypes.c_void_p),
+        ctypes.c_int32,
+        ctypes.c_size_t,
+        _size_p,
+        _size_p,
+        ctypes.c_void_p,
+        _ssize_p,
+        _i32_p,
+        ctypes.c_double,
+        ctypes.c_int32,
+        ctypes.c_double,
+        ctypes.c_int32,
+        ctypes.c_int32,
+        ctypes.c_size_t,
+        *_err,
+    ]
+    lib.ndf_robust_filter.argtypes = [
+        ctypes.c_int32,
+        ctypes.c_void_p,
+        ctypes.c_void_p,
+        ctypes.c_size_t,
+        _size_p,
+        _size_p,
+        ctypes.c_void_p,
+        _ssize_p,
+        _i32_p,
+        ctypes.c_double,
+        ctypes.c_int32,
+        ctypes.c_double,
+        ctypes.c_int32,
+        ctypes.c_int32,
         ctypes.c_size_t,
         *_err,
     ]
@@ -220,6 +262,8 @@ def _declare(lib: ctypes.CDLL) -> None:
         "ndf_mlp_train",
         "ndf_window_filter",
         "ndf_plan_create",
+        "ndf_robust_filter",
+        "ndf_robust_plan_create",
         "ndf_plan_apply",
         "ndf_plan_upload",
         "ndf_plan_compute_download",

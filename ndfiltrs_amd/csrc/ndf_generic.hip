@@ -88,17 +88,14 @@ __global__ void __launch_bounds__(256) k_generic(GenericArgs a)
             n += g.key(t) != KT::SENTINEL;
         }
         Key out = KT::CANONICAL_NAN;
-        if (n >= a.min_count && n > 0) {
-            if (a.stat == STAT_RANK) {
-                if (a.rank_param < n) {
-                    out = KT::unkey(select_kth(g, a.count, a.rank_param));
-                }
-            } else if (n & 1) {
-                out = KT::unkey(select_kth(g, a.count, n / 2));
+        Pick pick;
+        if (pick_ranks(a.stat, n, a.min_count, a.rank_param, a.q100, pick)) {
+            if (!pick.pair) {
+                out = KT::unkey(select_kth(g, a.count, pick.hi));
             } else {
-                const Key lower = select_kth(g, a.count, n / 2 - 1);
-                const Key upper = select_kth(g, a.count, n / 2);
-                out = median_pair_bits<T>(lower, upper);
+                const Key lower = select_kth(g, a.count, pick.hi - 1);
+                const Key upper = select_kth(g, a.count, pick.hi);
+                out = pair_bits<T>(a.stat, lower, upper, pick.frac);
             }
         }
         static_cast<Key *>(a.output)[index] = out;
@@ -311,22 +308,22 @@ __global__ void __launch_bounds__(64 * REPAIR_WAVES) k_repair_wave(GenericArgs a
             held.n = n;
         }
         Key out = KT::CANONICAL_NAN;
-        if (n >= a.min_count && n > 0) {
-            if (a.stat == STAT_RANK) {
-                if (a.rank_param < n) out = KT::unkey(wave_select(held, a.rank_param, lane));
-            } else if (n & 1) {
-                out = KT::unkey(wave_select(held, n / 2, lane));
+        Pick pick;
+        if (pick_ranks(a.stat, n, a.min_count, a.rank_param, a.q100, pick)) {
+            if (!pick.pair) {
+                out = KT::unkey(wave_select(held, (int)pick.hi, lane));
             } else {
-                const Key lower = wave_select(held, n / 2 - 1, lane);
+                const Key lower = wave_select(held, (int)pick.hi - 1, lane);
                 // Keys <= lower are those below lower + 1 (lower is a
-                // retained key, so lower + 1 does not wrap).
+                // retained key, so lower + 1 does not wrap); when more than
+                // hi of them exist, the key at rank hi equals lower.
                 const int at_most = held.count_below(lower + 1, lane);
                 Key above = KT::SENTINEL;
                 held.each(lane, [&](Key v) {
                     if (v > lower && v < above) above = v;
                 });
                 above = wave_min(above);
-                out = median_pair_bits<T>(lower, at_most > n / 2 ? lower : above);
+                out = pair_bits<T>(a.stat, lower, at_most > pick.hi ? lower : above, pick.frac);
             }
         }
         if (lane == 0) static_cast<Key *>(a.output)[index] = out;

@@ -164,6 +164,10 @@ int lds_limit()
 
 }  // namespace
 
+// Plan kind of the robust percentile (ndf_robust_plan_create); not a
+// window-filter kind of the public ABI.
+constexpr int32_t KIND_PERCENTILE = 64;
+
 struct ndf_plan {
     int32_t dbytes = 4;
     size_t rank = 0;
@@ -177,6 +181,7 @@ struct ndf_plan {
     bool omit = false, ignore_inf = false;
     size_t min_count = 1, parameter = 0;
     int32_t stat = STAT_MEDIAN;
+    double q100 = 0.0;  // STAT_PERCENTILE: percentile / 100 (0 min, 1 max)
     size_t len = 0, count = 0;
     std::vector<int64_t> taps;  // count * rank offsets
     std::vector<int64_t> low, high;
@@ -346,6 +351,7 @@ bool plan_network(ndf_plan *p)
     n.stat = p->stat;
     n.min_count = (int32_t)p->eff_min_count;
     n.rank_param = (int32_t)std::min(p->parameter, p->count);
+    n.q100 = p->q100;
     p->network_lds = lds;
     return true;
 }
@@ -498,6 +504,7 @@ bool plan_slide(ndf_plan *p, int force_e = 0, int max_ty = 256)
     s_args.stat = p->stat;
     s_args.min_count = (int32_t)p->eff_min_count;
     s_args.rank_param = (int32_t)std::min(p->parameter, p->count);
+    s_args.q100 = p->q100;
     p->slide_lds = slide_lds_bytes(p->dbytes, s, sort_e, (int)hist_stride, mask_bytes);
 
     // Bracket tiles (2-D rectangles with support rows of at most 128
@@ -578,6 +585,7 @@ void generic_host_tables(ndf_plan *p)
     g.stat = p->stat;
     g.min_count = p->eff_min_count;
     g.rank_param = (int64_t)std::min(p->parameter, p->count);
+    g.q100 = p->q100;
     if (p->rank == 2 && p->footprint.empty()) {
         g.rect_ky = (int32_t)p->size[0];
         g.rect_kx = (int32_t)p->size[1];
@@ -675,7 +683,8 @@ void bracket_geometry_into(const ndf_plan *p, const SlideArgs &slide, SlideArgs 
     b.high_x = (int32_t)p->high[ax];
     b.quantile = p->stat == STAT_RANK
                      ? (float)((double)std::min(p->parameter, p->count) / (double)(p->count > 1 ? p->count - 1 : 1))
-                     : 0.5f;
+                 : p->stat == STAT_PERCENTILE ? (float)p->q100
+                                              : 0.5f;
     sb = fb;
     sb.sort_e = p->bracket_e;
     sb_lds = bracket_lds_bytes(p->dbytes, fb.s, fb.rows, p->bracket_e);
@@ -879,17 +888,22 @@ ndf_status launch(ndf_plan *p, const void *d_in, void *d_out, hipStream_t stream
 ndf_status build_plan(ndf_plan *p, int32_t dtype_bytes, size_t rank, const size_t *shape, const size_t *size,
                       const uint8_t *footprint, const ptrdiff_t *origin, const int32_t *modes, double cval,
                       int32_t kind, int32_t omit_nan, int32_t ignore_inf, size_t min_count, size_t parameter,
-                      char *err, size_t err_len)
+                      double q, char *err, size_t err_len)
 {
     if (dtype_bytes != 4 && dtype_bytes != 8) {
         return fail(NDF_ERR_INVALID, "array dtype must be float32 or float64", err, err_len);
     }
-    if (kind < NDF_KIND_MEDIAN || kind > NDF_KIND_RANK_HISTOGRAM) return fail_code(NDF_ERR_KIND, err, err_len);
+    const bool robust = kind == KIND_PERCENTILE;
+    if (!robust && (kind < NDF_KIND_MEDIAN || kind > NDF_KIND_MAX)) return fail_code(NDF_ERR_KIND, err, err_len);
     for (size_t a = 0; a < rank; ++a) {
         if (modes[a] < NDF_MODE_REFLECT || modes[a] > NDF_MODE_WRAP) return fail_code(NDF_ERR_MODE, err, err_len);
     }
     if (rank > (size_t)MAX_RANK) return fail(NDF_ERR_INVALID, "array rank exceeds 32", err, err_len);
     const bool histogram = kind == NDF_KIND_MEDIAN_HISTOGRAM || kind == NDF_KIND_RANK_HISTOGRAM;
+    const bool extrema = kind == NDF_KIND_MIN || kind == NDF_KIND_MAX;
+    // Rectangular histogram kinds and rectangular extrema are validated by
+    // validate_rectangular first (src/plan.rs:154-197, 270-312).
+    const bool rectangular_plan = histogram || (extrema && !footprint);
     p->dbytes = dtype_bytes;
     p->rank = rank;
     p->shape.assign(shape, shape + rank);
@@ -901,7 +915,16 @@ ndf_status build_plan(ndf_plan *p, int32_t dtype_bytes, size_t rank, const size_
     p->ignore_inf = ignore_inf != 0;
     p->min_count = min_count;
     p->parameter = parameter;
-    p->stat = (kind == NDF_KIND_RANK || kind == NDF_KIND_RANK_HISTOGRAM) ? STAT_RANK : STAT_MEDIAN;
+    if (kind == NDF_KIND_RANK || kind == NDF_KIND_RANK_HISTOGRAM) {
+        p->stat = STAT_RANK;
+    } else if (extrema || robust) {
+        // The minimum and maximum are the 0th and 100th percentiles: rank 0
+        // and n - 1 of the retained samples, with a zero fraction.
+        p->stat = STAT_PERCENTILE;
+        p->q100 = robust ? q / 100.0 : (kind == NDF_KIND_MAX ? 1.0 : 0.0);
+    } else {
+        p->stat = STAT_MEDIAN;
+    }
     if (dtype_bytes == 4) {
         const float f = (float)cval;
         uint32_t b;
@@ -912,7 +935,7 @@ ndf_status build_plan(ndf_plan *p, int32_t dtype_bytes, size_t rank, const size_
     }
     p->cval = cval;
 
-    if (histogram) {
+    if (rectangular_plan) {
         // src/plan.rs:294-312, validate_rectangular, validate_support.
         if (footprint) return fail(NDF_ERR_INVALID, MSG_HIST_FOOTPRINT, err, err_len);
         if (rank == 0) return fail(NDF_ERR_INVALID, MSG_RANK0, err, err_len);
@@ -939,7 +962,7 @@ ndf_status build_plan(ndf_plan *p, int32_t dtype_bytes, size_t rank, const size_
         }
         // validate_support: block lengths of max(size, 32) per axis.
         size_t support = 1;
-        for (size_t a = 0; a < rank; ++a) {
+        for (size_t a = 0; a < rank && histogram; ++a) {
             const size_t window = std::max<size_t>(size[a], 32);
             size_t blocks = shape[a] / window;
             if (blocks < 1) blocks = 1;
@@ -983,6 +1006,16 @@ ndf_status build_plan(ndf_plan *p, int32_t dtype_bytes, size_t rank, const size_
         count = volume;
     }
     if (count == 0) return fail(NDF_ERR_INVALID, MSG_EMPTY_FOOTPRINT, err, err_len);
+    if (robust) {
+        // RobustReduction::validate (src/robust.rs:189-241).
+        if (!(q >= 0.0 && q <= 100.0)) {
+            return fail(NDF_ERR_INVALID, "percentile must be finite and between zero and 100", err, err_len);
+        }
+        if (p->omit && (min_count == 0 || min_count > count)) {
+            return fail(NDF_ERR_INVALID, "min_count must be between one and the number of window elements", err,
+                        err_len);
+        }
+    }
     if (p->omit && (min_count == 0 || min_count > count)) {
         return fail(NDF_ERR_INVALID, MSG_MIN_COUNT, err, err_len);
     }
@@ -1037,7 +1070,33 @@ ndf_status ndf_plan_create(ndf_plan **plan, int32_t dtype_bytes, size_t rank, co
     ndf_plan *p = new (std::nothrow) ndf_plan();
     if (!p) return fail_code(NDF_ERR_MEMORY, err, err_len);
     const ndf_status st = build_plan(p, dtype_bytes, rank, shape, size, footprint, origin, modes, cval, kind,
-                                     omit_nan, ignore_inf, min_count, parameter, err, err_len);
+                                     omit_nan, ignore_inf, min_count, parameter, 0.0, err, err_len);
+    if (st != NDF_OK) {
+        delete p;
+        return st;
+    }
+    *plan = p;
+    return NDF_OK;
+}
+
+ndf_status ndf_robust_plan_create(ndf_plan **plan, int32_t dtype_bytes, size_t rank, const size_t *shape,
+                                  const size_t *size, const uint8_t *footprint, const ptrdiff_t *origin,
+                                  const int32_t *modes, double cval, int32_t kind, double parameter, int32_t omit_nan,
+                                  int32_t ignore_inf, size_t min_count, char *err, size_t err_len)
+{
+    if (!plan) return fail(NDF_ERR_INVALID, "plan pointer is null", err, err_len);
+    *plan = nullptr;
+    // robust_filter (src/pyapi/robust.rs:98-116): the omission options need
+    // NaN omission, then the kind.
+    if (!omit_nan && (ignore_inf || min_count != 1)) {
+        return fail(NDF_ERR_INVALID, "ignore_inf and min_count require NaN omission", err, err_len);
+    }
+    if (kind != NDF_ROBUST_PERCENTILE) return fail(NDF_ERR_KIND, "unsupported robust filter kind", err, err_len);
+    ndf_plan *p = new (std::nothrow) ndf_plan();
+    if (!p) return fail_code(NDF_ERR_MEMORY, err, err_len);
+    const ndf_status st = build_plan(p, dtype_bytes, rank, shape, size, footprint, origin, modes, cval,
+                                     KIND_PERCENTILE, omit_nan, ignore_inf, omit_nan ? min_count : 1, 0, parameter,
+                                     err, err_len);
     if (st != NDF_OK) {
         delete p;
         return st;
@@ -1220,6 +1279,20 @@ ndf_status ndf_window_filter(int32_t dtype_bytes, const void *input, void *outpu
     ndf_plan *plan = nullptr;
     ndf_status st = ndf_plan_create(&plan, dtype_bytes, rank, shape, size, footprint, origin, modes, cval, kind,
                                     omit_nan, ignore_inf, min_count, parameter, err, err_len);
+    if (st != NDF_OK) return st;
+    st = ndf_plan_apply(plan, input, output, plan->len, err, err_len);
+    ndf_plan_destroy(plan);
+    return st;
+}
+
+ndf_status ndf_robust_filter(int32_t dtype_bytes, const void *input, void *output, size_t rank, const size_t *shape,
+                             const size_t *size, const uint8_t *footprint, const ptrdiff_t *origin,
+                             const int32_t *modes, double cval, int32_t kind, double parameter, int32_t omit_nan,
+                             int32_t ignore_inf, size_t min_count, char *err, size_t err_len)
+{
+    ndf_plan *plan = nullptr;
+    ndf_status st = ndf_robust_plan_create(&plan, dtype_bytes, rank, shape, size, footprint, origin, modes, cval,
+                                           kind, parameter, omit_nan, ignore_inf, min_count, err, err_len);
     if (st != NDF_OK) return st;
     st = ndf_plan_apply(plan, input, output, plan->len, err, err_len);
     ndf_plan_destroy(plan);
@@ -1701,7 +1774,7 @@ ndf_status ndf_batch_create(ndf_batch **batch, int32_t dtype_bytes, size_t count
         char inner[200] = {0};
         const ndf_status st = build_plan(p, dtype_bytes, 2, shapes + 2 * i, sizes + 2 * i, nullptr, origin,
                                          modes + 2 * i, cval, kind, omit_nan, ignore_inf, min_count, parameter,
-                                         inner, sizeof inner);
+                                         0.0, inner, sizeof inner);
         if (st != NDF_OK) {
             char msg[260];
             snprintf(msg, sizeof msg, "batch image %zu: %s", i, inner);

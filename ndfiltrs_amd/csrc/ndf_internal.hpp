@@ -38,6 +38,7 @@ struct GenericArgs {
     int32_t stat;
     int64_t min_count;
     int64_t rank_param;
+    double q100;  // percentile / 100 (STAT_PERCENTILE)
     // Sparse mode (mask-directed repair): when `positions` is set, `len`
     // counts positions and thread i computes the output at flat index
     // positions[i] of an array of `extent` elements. A position outside
@@ -81,6 +82,7 @@ struct SlideArgs {
     int32_t stat;
     int32_t min_count;
     int32_t rank_param;
+    double q100;          // percentile / 100 (STAT_PERCENTILE)
     int32_t *error_flag;  // set when a rank walk leaves its bounds (plain mode)
     // Tile selection: tiles 0 .. tiles_x*tiles_y*nz - 1, or tile_list[0 .. *tile_count).
     const int32_t *tile_list;
@@ -112,6 +114,7 @@ struct NetworkArgs {
     int32_t stat;
     int32_t min_count;
     int32_t rank_param;
+    double q100;  // percentile / 100 (STAT_PERCENTILE)
 };
 
 // Network size (live wires) for a window of `count` taps, 0 if too large.

@@ -168,23 +168,17 @@ __device__ __forceinline__ void network_tile(const NetworkArgs &a, int tile,
             n += k[i] != KT::SENTINEL;
         }
         Key out = KT::CANONICAL_NAN;
-        if (n >= a.min_count && n > 0) {
-            int lo_index, hi_index;
-            if (a.stat == STAT_RANK) {
-                lo_index = hi_index = a.rank_param;
-            } else {
-                hi_index = n >> 1;
-                lo_index = (n & 1) ? hi_index : hi_index - 1;
-            }
-            if (hi_index < n) {
-                Key lo = 0, hi = 0;
+        Pick pick;
+        if (pick_ranks(a.stat, n, a.min_count, a.rank_param, a.q100, pick)) {
+            const int hi_index = (int)pick.hi;
+            const int lo_index = pick.pair ? hi_index - 1 : hi_index;
+            Key lo = 0, hi = 0;
 #pragma unroll
-                for (int i = 0; i < LIVE; ++i) {
-                    lo = i == lo_index ? k[i] : lo;
-                    hi = i == hi_index ? k[i] : hi;
-                }
-                out = lo_index == hi_index ? KT::unkey(hi) : median_pair_bits<T>(lo, hi);
+            for (int i = 0; i < LIVE; ++i) {
+                lo = i == lo_index ? k[i] : lo;
+                hi = i == hi_index ? k[i] : hi;
             }
+            out = pick.pair ? pair_bits<T>(a.stat, lo, hi, pick.frac) : KT::unkey(hi);
         }
         output[(z0 * a.ny + y) * a.nx + x] = out;
     }

@@ -965,15 +965,7 @@ __device__ __forceinline__ bool slide_tile(const SlideArgs &a, int tile, unsigne
         const int y_first = wave * a.tw;
         if (y0 + y_first < a.ny) {
             const bool col_valid = x0 + lane < a.nx;
-            auto target_of = [&](int n) {
-                if (n < a.min_count || n <= 0) {
-                    return -1;
-                }
-                if (a.stat == STAT_RANK) {
-                    return a.rank_param < n ? a.rank_param : -1;
-                }
-                return n >> 1;
-            };
+            auto target_of = [&](int n) { return (int)target_rank(a.stat, n, a.min_count, a.rank_param, a.q100); };
 
             // Initial cursor: two levels of 16-bin histograms over the ranks.
             int m = 1, c = 0, n = 0;
@@ -1094,20 +1086,21 @@ __device__ __forceinline__ bool slide_tile(const SlideArgs &a, int tile, unsigne
                     c += dc;
                 }
                 Key out = KT::CANONICAL_NAN;
-                const int k = target_of(n);
-                if (k >= 0) {
+                Pick pick;
+                if (pick_ranks(a.stat, n, a.min_count, a.rank_param, a.q100, pick)) {
+                    const int k = (int)pick.hi;
                     const uint32_t base = ((uint32_t)y << 8) | (uint32_t)lane;
                     if (!walk_to(m, c, k, pos1, retained, g, base)) {
                         failed = true;
                         break;
                     }
-                    if (a.stat == STAT_MEDIAN && (n & 1) == 0) {
+                    if (pick.pair) {
                         const int m2 = previous_member(m, pos1, g, base);
                         if (m2 < 1) {
                             failed = true;
                             break;
                         }
-                        out = median_pair_bits<T>(keys[m2 - 1], keys[m - 1]);
+                        out = pair_bits<T>(a.stat, keys[m2 - 1], keys[m - 1], pick.frac);
                     } else {
                         out = KT::unkey(keys[m - 1]);
                     }
@@ -1446,8 +1439,8 @@ __global__ void __launch_bounds__(NT, E >= 32 ? 1 : (sizeof(T) == 8 && E >= 16 ?
                 int est = 0;
                 bool have = false;
                 if (n >= a.min_count && n > 0 && inside > 0) {
-                    const int k0 = a.stat == STAT_RANK ? a.rank_param : (n >> 1);
-                    if (k0 < n && k0 >= below && k0 < below + inside) {
+                    const int k0 = (int)target_rank(a.stat, n, a.min_count, a.rank_param, a.q100);
+                    if (k0 >= 0 && k0 < n && k0 >= below && k0 < below + inside) {
                         const float rel = ((float)(k0 - below) + 0.5f) / (float)inside;
                         est = 1 + (int)(rel * (float)retained);
                         est = est < 1 ? 1 : (est > retained ? retained : est);
@@ -1539,7 +1532,7 @@ __global__ void __launch_bounds__(NT, E >= 32 ? 1 : (sizeof(T) == 8 && E >= 16 ?
                     }
                     // Nearest candidate by window count: the last one at or
                     // below the target, or the next one if that is closer.
-                    const int k0 = have ? (a.stat == STAT_RANK ? a.rank_param : (n >> 1)) : 0;
+                    const int k0 = have ? (int)target_rank(a.stat, n, a.min_count, a.rank_param, a.q100) : 0;
                     int pick = 0;
 #pragma unroll
                     for (int j = 1; j < START; ++j) {
@@ -1591,20 +1584,18 @@ __global__ void __launch_bounds__(NT, E >= 32 ? 1 : (sizeof(T) == 8 && E >= 16 ?
                     }
                 }
                 Key out = KT::CANONICAL_NAN;
-                int k = -1;
-                if (n >= a.min_count && n > 0) {
-                    k = a.stat == STAT_RANK ? (a.rank_param < n ? a.rank_param : -1) : (n >> 1);
-                }
-                if (k >= 0) {
+                Pick pick;
+                if (pick_ranks(a.stat, n, a.min_count, a.rank_param, a.q100, pick)) {
+                    const int k = (int)pick.hi;
                     const uint32_t base = ((uint32_t)y << 8) | (uint32_t)lane;
                     if (k < below || k >= below + inside || !walk_to(m, c, k, pos1, retained, g, base)) {
                         failed = true;
-                    } else if (a.stat == STAT_MEDIAN && (n & 1) == 0) {
+                    } else if (pick.pair) {
                         const int m2 = previous_member(m, pos1, g, base);
                         if (m2 < 1) {
                             failed = true;
                         } else {
-                            out = median_pair_bits<T>(keys[m2 - 1], keys[m - 1]);
+                            out = pair_bits<T>(a.stat, keys[m2 - 1], keys[m - 1], pick.frac);
                         }
                     } else {
                         out = KT::unkey(keys[m - 1]);

@@ -232,6 +232,137 @@ static uint64_t midpoint_bits(uint64_t lo_bits, uint64_t hi_bits, int dbytes)
 }
 
 /* ------------------------------------------------------------------ */
+/* Percentile interpolation (src/robust.rs:320-356), with the two-term     */
+/* finite weighted sum it calls for opposite signs (src/convolution.rs:    */
+/* 203-384, normalize = false) and Neumaier summation (src/mean.rs:6-26).  */
+/* Every step is one IEEE f64 operation; the Makefile forbids contraction.  */
+/* ------------------------------------------------------------------ */
+
+static void split_f64(double value, double *mantissa, int *exponent)
+{
+    uint64_t bits;
+    memcpy(&bits, &value, 8);
+    const int field = (int)((bits >> 52) & 0x7FF);
+    const uint64_t fraction = bits & ((1ull << 52) - 1);
+    if (field == 0) {
+        const int leading = 63 - __builtin_clzll(fraction);
+        *mantissa = copysign((double)fraction / (double)(1ull << leading), value);
+        *exponent = leading - 1074;
+    } else {
+        const uint64_t m = (bits & ((1ull << 63) | ((1ull << 52) - 1))) | (1023ull << 52);
+        memcpy(mantissa, &m, 8);
+        *exponent = field - 1023;
+    }
+}
+
+static double pow2_bits(int biased)
+{
+    const uint64_t b = (uint64_t)biased << 52;
+    double d;
+    memcpy(&d, &b, 8);
+    return d;
+}
+
+static double scale_f64(double value, int exponent)
+{
+    if (value == 0.0) return value;
+    double mantissa;
+    int own;
+    split_f64(value, &mantissa, &own);
+    const int target = own + exponent;
+    if (target > 1023) return copysign(INFINITY, value);
+    if (target < -1075) return copysign(0.0, value);
+    if (target >= -1022) return mantissa * pow2_bits(target + 1023);
+    return (mantissa * 2.2250738585072014e-308) * pow2_bits(target + 1022 + 1023);
+}
+
+typedef struct {
+    double value, correction;
+} neumaier_t;
+
+static void neumaier_add(neumaier_t *s, double x)
+{
+    const double next = s->value + x;
+    if (fabs(s->value) >= fabs(x)) {
+        s->correction += (s->value - next) + x;
+    } else {
+        s->correction += (x - next) + s->value;
+    }
+    s->value = next;
+}
+
+static double finite_weighted_pair(double v0, double w0, double v1, double w1)
+{
+    const double values[2] = {v0, v1}, weights[2] = {w0, w1};
+    neumaier_t sum = {0.0, 0.0};
+    int fallback = 0;
+    for (int i = 0; i < 2; ++i) {
+        const double product = values[i] * weights[i];
+        if (!isfinite(product) || (values[i] != 0.0 && fabs(product) < 2.2250738585072014e-308)) {
+            fallback = 1;
+            break;
+        }
+        neumaier_add(&sum, product);
+    }
+    const double total = sum.value + sum.correction;
+    if (!fallback && isfinite(total)) return total;
+    int have = 0, maximum = 0;
+    for (int i = 0; i < 2; ++i) {
+        if (values[i] == 0.0) continue;
+        double m;
+        int ev, ew;
+        split_f64(values[i], &m, &ev);
+        split_f64(weights[i], &m, &ew);
+        if (!have || ev + ew > maximum) maximum = ev + ew;
+        have = 1;
+    }
+    if (!have) return 0.0;
+    const int exponent = maximum + 1 + 2; /* bit length of (pairs - 1) = 1, plus 2 */
+    neumaier_t scaled = {0.0, 0.0};
+    for (int i = 0; i < 2; ++i) {
+        if (values[i] == 0.0) continue;
+        double mv, mw;
+        int ev, ew;
+        split_f64(values[i], &mv, &ev);
+        split_f64(weights[i], &mw, &ew);
+        neumaier_add(&scaled, scale_f64(mv * mw, ev + ew - exponent));
+    }
+    return scale_f64(scaled.value + scaled.correction, exponent);
+}
+
+static double clamp_f64(double value, double lower, double upper)
+{
+    double x = value;
+    if (x < lower) x = lower;
+    if (x > upper) x = upper;
+    return x;
+}
+
+double ndfo_interpolate(double lower, double upper, double fraction)
+{
+    uint64_t lb, ub;
+    memcpy(&lb, &lower, 8);
+    memcpy(&ub, &upper, 8);
+    if (fraction == 0.0 || lb == ub) return lower;
+    if (fraction == 1.0) return upper;
+    if (isinf(lower) || isinf(upper)) {
+        if (lower == -INFINITY && upper == INFINITY) return NAN;
+        return isinf(lower) ? lower : upper;
+    }
+    if ((signbit(lower) != 0) != (signbit(upper) != 0)) {
+        if (!isfinite(upper - lower)) {
+            const double half_lower = lower / 2.0;
+            const double half_upper = upper / 2.0;
+            return clamp_f64((half_lower + (half_upper - half_lower) * fraction) * 2.0, lower, upper);
+        }
+        return finite_weighted_pair(lower, 1.0 - fraction, upper, fraction);
+    }
+    const double difference = upper - lower;
+    const double result = fraction <= 0.5 ? lower + difference * fraction : upper - difference * (1.0 - fraction);
+    return clamp_f64(result, lower, upper);
+}
+
+/* ------------------------------------------------------------------ */
 /* Geometry (src/window.rs:44-137)                                      */
 /* ------------------------------------------------------------------ */
 
@@ -410,6 +541,7 @@ typedef struct {
     int ignore_inf;
     size_t min_count;
     const size_t *positions; /* NULL: every output; else outputs positions[begin..end) only */
+    double q;                /* percentile in [0, 100] (NDFO_STAT_PERCENTILE) */
 } direct_job_t;
 
 typedef struct {
@@ -418,6 +550,41 @@ typedef struct {
     int status;
 } direct_part_t;
 
+static int compare_keys(const void *a, const void *b)
+{
+    const uint64_t x = *(const uint64_t *)a, y = *(const uint64_t *)b;
+    return x < y ? -1 : x > y;
+}
+
+static uint64_t double_to_bits(double value, int dbytes)
+{
+    if (isnan(value)) return canonical_nan(dbytes);
+    if (dbytes == 4) {
+        float f = (float)value;
+        uint32_t b;
+        memcpy(&b, &f, 4);
+        return b;
+    }
+    uint64_t b;
+    memcpy(&b, &value, 8);
+    return b;
+}
+
+/* percentile() of src/robust.rs:358-367 over the retained keys: sort, rank
+ * (q / 100) * (n - 1), interpolate the floor and the next sample (clamped to
+ * the last), cast the f64 result once. */
+static uint64_t percentile_keys(uint64_t *keys, size_t n, double q, int dbytes)
+{
+    qsort(keys, n, sizeof(uint64_t), compare_keys);
+    if (q == 100.0) return unkey(keys[n - 1], dbytes);
+    const double rank = (q / 100.0) * (double)(n - 1);
+    const size_t lower = (size_t)floor(rank);
+    const size_t upper = lower + 1 < n - 1 ? lower + 1 : n - 1;
+    const double value = ndfo_interpolate(bits_to_double(unkey(keys[lower], dbytes), dbytes),
+                                          bits_to_double(unkey(keys[upper], dbytes), dbytes), rank - (double)lower);
+    return double_to_bits(value, dbytes);
+}
+
 static uint64_t finish_keys(uint64_t *keys, size_t n, int nan_seen, const direct_job_t *job)
 {
     const int db = job->dbytes;
@@ -425,6 +592,20 @@ static uint64_t finish_keys(uint64_t *keys, size_t n, int nan_seen, const direct
         if (nan_seen) return canonical_nan(db);
     } else if (n < job->min_count) {
         return canonical_nan(db);
+    }
+    if (job->statistic == NDFO_STAT_PERCENTILE) {
+        if (n == 0) return canonical_nan(db);
+        return percentile_keys(keys, n, job->q, db);
+    }
+    if (job->statistic == NDFO_STAT_MIN || job->statistic == NDFO_STAT_MAX) {
+        /* Minimum / Maximum: total-order extreme of the retained samples
+         * (src/reduction.rs:110-119); none retained gives NaN. */
+        if (n == 0) return canonical_nan(db);
+        uint64_t best = keys[0];
+        for (size_t i = 1; i < n; ++i) {
+            if (job->statistic == NDFO_STAT_MIN ? keys[i] < best : keys[i] > best) best = keys[i];
+        }
+        return unkey(best, db);
     }
     if (job->statistic == NDFO_STAT_RANK) {
         if (job->parameter >= n) return canonical_nan(db);
@@ -985,6 +1166,15 @@ int ndfo_filter(int dtype_bytes, const void *input, void *output, size_t rank, c
                 double cval, int32_t statistic, size_t parameter, int32_t omit_nan, int32_t ignore_inf,
                 size_t min_count, int32_t method, int32_t nthreads, char *err, size_t err_len)
 {
+    return ndfo_filter_q(dtype_bytes, input, output, rank, shape, size, footprint, origin, modes, cval, statistic,
+                         parameter, 0.0, omit_nan, ignore_inf, min_count, method, nthreads, err, err_len);
+}
+
+int ndfo_filter_q(int dtype_bytes, const void *input, void *output, size_t rank, const size_t *shape,
+                  const size_t *size, const uint8_t *footprint, const ptrdiff_t *origin, const int32_t *modes,
+                  double cval, int32_t statistic, size_t parameter, double q, int32_t omit_nan, int32_t ignore_inf,
+                  size_t min_count, int32_t method, int32_t nthreads, char *err, size_t err_len)
+{
     if (dtype_bytes != 4 && dtype_bytes != 8) return fail(NDFO_INVALID, "dtype must be float32 or float64", err, err_len);
     for (size_t a = 0; a < rank; ++a) {
         if (modes[a] < 0 || modes[a] > 4) return fail(NDFO_INVALID, "unsupported boundary mode", err, err_len);
@@ -997,6 +1187,18 @@ int ndfo_filter(int dtype_bytes, const void *input, void *output, size_t rank, c
     window_t w;
     int st = window_build(&w, rank, shape, size, footprint, origin, modes, err, err_len);
     if (st != NDFO_OK) return st;
+    if (statistic == NDFO_STAT_PERCENTILE) {
+        /* RobustReduction::validate (src/robust.rs:189-241). */
+        if (!isfinite(q) || q < 0.0 || q > 100.0) {
+            window_free(&w);
+            return fail(NDFO_INVALID, "percentile must be finite and between zero and 100", err, err_len);
+        }
+        if (omit_nan && (min_count == 0 || min_count > w.count)) {
+            window_free(&w);
+            return fail(NDFO_INVALID, "min_count must be between one and the number of window elements", err,
+                        err_len);
+        }
+    }
     if (omit_nan && (min_count == 0 || min_count > w.count)) {
         window_free(&w);
         return fail(NDFO_INVALID, MSG_MIN_COUNT, err, err_len);
@@ -1006,7 +1208,7 @@ int ndfo_filter(int dtype_bytes, const void *input, void *output, size_t rank, c
         return NDFO_OK;
     }
     direct_job_t job = {dtype_bytes, input, output, &w, cval_to_bits(cval, dtype_bytes), statistic, parameter,
-                        omit_nan != 0, ignore_inf != 0, min_count, NULL};
+                        omit_nan != 0, ignore_inf != 0, min_count, NULL, q};
     if (nthreads < 1) nthreads = 1;
     if (nthreads > 256) nthreads = 256;
     if ((size_t)nthreads > w.len) nthreads = (int)w.len;
@@ -1025,6 +1227,169 @@ int ndfo_filter(int dtype_bytes, const void *input, void *output, size_t rank, c
     }
     window_free(&w);
     return st == NDFO_OK ? NDFO_OK : fail_code(st, err, err_len);
+}
+
+/* ------------------------------------------------------------------ */
+/* Separable extrema (src/extrema.rs:125-436)                             */
+/* ------------------------------------------------------------------ */
+
+/* A window summary: the extreme key so far, a propagated NaN, or nothing
+ * retained yet (src/extrema.rs:11-62). Equal keys are equal bits, so which
+ * of two equal extremes survives does not matter. */
+typedef struct {
+    uint64_t key;
+    unsigned char nan;
+    unsigned char empty;
+} summary_t;
+
+static summary_t summary_of(uint64_t bits, int dbytes, int omit, int ignore_inf)
+{
+    summary_t out = {0, 0, 0};
+    if (!omit) {
+        if (bits_is_nan(bits, dbytes)) {
+            out.nan = 1;
+        } else {
+            out.key = key_of(bits, dbytes);
+        }
+        return out;
+    }
+    if (bits_is_nan(bits, dbytes) || (ignore_inf && bits_is_inf(bits, dbytes))) {
+        out.empty = 1;
+    } else {
+        out.key = key_of(bits, dbytes);
+    }
+    return out;
+}
+
+static summary_t summary_combine(summary_t a, summary_t b, int maximum)
+{
+    if (a.nan || b.nan) {
+        summary_t n = {0, 1, 0};
+        return n;
+    }
+    if (a.empty) return b;
+    if (b.empty) return a;
+    if (maximum ? a.key < b.key : a.key > b.key) return b;
+    return a;
+}
+
+static int retained_bits(uint64_t bits, int dbytes, int omit, int ignore_inf)
+{
+    if (!omit) return 1;
+    return !bits_is_nan(bits, dbytes) && !(ignore_inf && bits_is_inf(bits, dbytes));
+}
+
+int ndfo_extrema(int dtype_bytes, const void *input, void *output, size_t rank, const size_t *shape,
+                 const size_t *size, const ptrdiff_t *origin, const int32_t *modes, double cval, int32_t maximum,
+                 int32_t omit_nan, int32_t ignore_inf, size_t min_count, char *err, size_t err_len)
+{
+    const int db = dtype_bytes;
+    if (db != 4 && db != 8) return fail(NDFO_INVALID, "dtype must be float32 or float64", err, err_len);
+    if (rank == 0) return fail(NDFO_INVALID, MSG_RANK0, err, err_len);
+    if (rank > 64) return fail(NDFO_INVALID, "array rank exceeds 64", err, err_len);
+    for (size_t a = 0; a < rank; ++a) {
+        if (size[a] == 0) return fail(NDFO_INVALID, MSG_SIZE0, err, err_len);
+        if (modes[a] < 0 || modes[a] > 4) return fail(NDFO_INVALID, "unsupported boundary mode", err, err_len);
+    }
+    size_t volume = 1, len = 1;
+    for (size_t a = 0; a < rank; ++a) {
+        if (!checked_mul(volume, size[a], &volume)) return fail_code(NDFO_OVERFLOW, err, err_len);
+    }
+    int64_t anchors[64];
+    for (size_t a = 0; a < rank; ++a) {
+        const int64_t length = (int64_t)size[a], shift = (int64_t)origin[a];
+        if (shift < -(length / 2) || shift > (length - 1) / 2) return fail(NDFO_INVALID, MSG_ORIGIN, err, err_len);
+        anchors[a] = length / 2 + shift;
+    }
+    if (omit_nan && (min_count == 0 || min_count > volume)) return fail(NDFO_INVALID, MSG_MIN_COUNT, err, err_len);
+    for (size_t a = 0; a < rank; ++a) {
+        if (!checked_mul(len, shape[a], &len)) return fail_code(NDFO_OVERFLOW, err, err_len);
+    }
+    if (len == 0) return NDFO_OK;
+    size_t strides[64];
+    strides[rank - 1] = 1;
+    for (size_t a = rank - 1; a-- > 0;) strides[a] = strides[a + 1] * shape[a + 1];
+    const uint64_t cbits = cval_to_bits(cval, db);
+    const summary_t csum = summary_of(cbits, db, omit_nan, ignore_inf);
+    const int cret = retained_bits(cbits, db, omit_nan, ignore_inf);
+    size_t longest = 0;
+    for (size_t a = 0; a < rank; ++a) {
+        if (shape[a] + size[a] - 1 > longest) longest = shape[a] + size[a] - 1;
+    }
+    summary_t *prev = malloc(sizeof(summary_t) * len), *next = malloc(sizeof(summary_t) * len);
+    size_t *pc = malloc(sizeof(size_t) * len), *nc = malloc(sizeof(size_t) * len);
+    summary_t *pad = malloc(sizeof(summary_t) * longest), *pre = malloc(sizeof(summary_t) * longest),
+              *suf = malloc(sizeof(summary_t) * longest);
+    size_t *padc = malloc(sizeof(size_t) * longest), *prec = malloc(sizeof(size_t) * longest),
+           *sufc = malloc(sizeof(size_t) * longest);
+    int status = NDFO_OK;
+    if (!prev || !next || !pc || !nc || !pad || !pre || !suf || !padc || !prec || !sufc) {
+        status = NDFO_MEMORY;
+        goto done;
+    }
+    for (size_t i = 0; i < len; ++i) {
+        const uint64_t bits = load_bits(input, db, i);
+        prev[i] = summary_of(bits, db, omit_nan, ignore_inf);
+        pc[i] = (size_t)retained_bits(bits, db, omit_nan, ignore_inf);
+    }
+    size_t prior = 1;
+    for (size_t axis = 0; axis < rank; ++axis) {
+        const size_t n = shape[axis], k = size[axis], stride = strides[axis];
+        const size_t padded = n + k - 1, lines = len / n, block = n * stride;
+        for (size_t line = 0; line < lines; ++line) {
+            const size_t base = (line / stride) * block + line % stride;
+            for (size_t p = 0; p < padded; ++p) {
+                int64_t mapped;
+                status = ndfo_map_index((int64_t)p - anchors[axis], n, modes[axis], &mapped);
+                if (status != NDFO_OK) goto done;
+                if (mapped >= 0) {
+                    pad[p] = prev[base + (size_t)mapped * stride];
+                    padc[p] = pc[base + (size_t)mapped * stride];
+                } else {
+                    pad[p] = csum;
+                    padc[p] = prior * (size_t)cret;
+                }
+            }
+            for (size_t p = 0; p < padded; ++p) {
+                const int start = p % k == 0;
+                pre[p] = start ? pad[p] : summary_combine(pre[p - 1], pad[p], maximum);
+                prec[p] = start ? padc[p] : prec[p - 1] + padc[p];
+            }
+            for (size_t p = padded; p-- > 0;) {
+                const int start = p == padded - 1 || p % k == k - 1;
+                suf[p] = start ? pad[p] : summary_combine(pad[p], suf[p + 1], maximum);
+                sufc[p] = start ? padc[p] : sufc[p + 1] + padc[p];
+            }
+            for (size_t x = 0; x < n; ++x) {
+                const size_t end = x + k - 1;
+                next[base + x * stride] = summary_combine(suf[x], pre[end], maximum);
+                nc[base + x * stride] = x / k == end / k ? prec[end] - prec[x] + padc[x] : sufc[x] + prec[end];
+            }
+        }
+        summary_t *ts = prev;
+        prev = next;
+        next = ts;
+        size_t *tc = pc;
+        pc = nc;
+        nc = tc;
+        prior *= k;
+    }
+    for (size_t i = 0; i < len; ++i) {
+        const int none = prev[i].nan || prev[i].empty || (omit_nan && pc[i] < min_count);
+        store_bits(output, db, i, none ? canonical_nan(db) : unkey(prev[i].key, db));
+    }
+done:
+    free(prev);
+    free(next);
+    free(pc);
+    free(nc);
+    free(pad);
+    free(pre);
+    free(suf);
+    free(padc);
+    free(prec);
+    free(sufc);
+    return status == NDFO_OK ? NDFO_OK : fail_code(status, err, err_len);
 }
 
 /* fixpix(method="median"): src/repair.rs:107-126 (fixpix_median) with
@@ -1079,7 +1444,7 @@ int ndfo_fixpix_median(int dtype_bytes, const void *input, void *output, const u
         }
     }
     direct_job_t job = {dtype_bytes, snapshot, output, &w, cval_to_bits(cval, dtype_bytes), NDFO_STAT_MEDIAN, 0,
-                        1, ignore_inf != 0, min_count, positions};
+                        1, ignore_inf != 0, min_count, positions, 0.0};
     if (nthreads < 1) nthreads = 1;
     if (nthreads > 256) nthreads = 256;
     if ((size_t)nthreads > count) nthreads = (int)count;

@@ -8,6 +8,8 @@
  *   - boundary mapping            src/boundary.rs:29-71
  *   - window taps / gather        src/window.rs:44-137, 211-294
  *   - NaN/inf policy + selection  src/reduction.rs:78-130
+ *   - percentile interpolation    src/robust.rs:320-367, src/convolution.rs:203-384
+ *   - separable extrema           src/extrema.rs:125-436
  *   - total-order keys            src/median_network.rs:9-37
  *   - even-count midpoint         docs/semantics.md:74-85, tests/_reference.py:36-47
  *   - mask-directed median repair src/repair.rs:14-29, 107-126,
@@ -39,7 +41,10 @@ enum {
 /* Boundary modes in the order of src/boundary.rs:16-27. */
 enum { NDFO_REFLECT = 0, NDFO_CONSTANT = 1, NDFO_NEAREST = 2, NDFO_MIRROR = 3, NDFO_WRAP = 4 };
 
-enum { NDFO_STAT_MEDIAN = 0, NDFO_STAT_RANK = 1 };
+/* Statistics: median, fixed rank, the linear-interpolated percentile of
+ * src/robust.rs:320-367, and the total-order minimum / maximum of
+ * src/reduction.rs:110-119. */
+enum { NDFO_STAT_MEDIAN = 0, NDFO_STAT_RANK = 1, NDFO_STAT_PERCENTILE = 2, NDFO_STAT_MIN = 3, NDFO_STAT_MAX = 4 };
 enum { NDFO_METHOD_DIRECT = 0, NDFO_METHOD_HISTOGRAM = 1 };
 
 /* Maps `index` on an axis of `length`; *mapped = -1 marks a constant
@@ -66,6 +71,26 @@ int ndfo_filter(int dtype_bytes, const void *input, void *output, size_t rank,
                 int32_t statistic, size_t parameter, int32_t omit_nan, int32_t ignore_inf,
                 size_t min_count, int32_t method, int32_t nthreads, char *err,
                 size_t err_len);
+
+/* ndfo_filter with the percentile q (0..100) of NDFO_STAT_PERCENTILE, whose
+ * validation follows RobustReduction::validate (src/robust.rs:189-241). The
+ * histogram method serves the median and rank statistics only. */
+int ndfo_filter_q(int dtype_bytes, const void *input, void *output, size_t rank,
+                  const size_t *shape, const size_t *size, const uint8_t *footprint,
+                  const ptrdiff_t *origin, const int32_t *modes, double cval,
+                  int32_t statistic, size_t parameter, double q, int32_t omit_nan, int32_t ignore_inf,
+                  size_t min_count, int32_t method, int32_t nthreads, char *err,
+                  size_t err_len);
+
+/* interpolate() of src/robust.rs:320-356. */
+double ndfo_interpolate(double lower, double upper, double fraction);
+
+/* Rectangular minimum (maximum != 0: maximum) by the reference's exact
+ * separable van Herk / Gil-Werman passes (src/extrema.rs:125-436), a second
+ * algorithm beside the direct NDFO_STAT_MIN / NDFO_STAT_MAX. */
+int ndfo_extrema(int dtype_bytes, const void *input, void *output, size_t rank, const size_t *shape,
+                 const size_t *size, const ptrdiff_t *origin, const int32_t *modes, double cval, int32_t maximum,
+                 int32_t omit_nan, int32_t ignore_inf, size_t min_count, char *err, size_t err_len);
 
 /* Mask-directed median repair (fixpix(method="median")): outputs whose
  * mask byte is nonzero become the Omit median of their window over the input

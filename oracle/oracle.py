@@ -20,6 +20,11 @@ LIBRARY = os.path.join(HERE, "_build", "libndf_oracle.so")
 MODES = ("reflect", "constant", "nearest", "mirror", "wrap")
 STAT_MEDIAN = 0
 STAT_RANK = 1
+STAT_PERCENTILE = 2
+STAT_MIN = 3
+STAT_MAX = 4
+STATISTICS = {"median": STAT_MEDIAN, "rank": STAT_RANK, "percentile": STAT_PERCENTILE, "min": STAT_MIN,
+              "max": STAT_MAX}
 METHOD_DIRECT = 0
 METHOD_HISTOGRAM = 1
 
@@ -77,6 +82,17 @@ def _load() -> ctypes.CDLL:
             ctypes.c_size_t,
         ]
         lib.ndfo_filter.restype = ctypes.c_int
+        lib.ndfo_filter_q.argtypes = lib.ndfo_filter.argtypes[:12] + [ctypes.c_double] + lib.ndfo_filter.argtypes[12:]
+        lib.ndfo_filter_q.restype = ctypes.c_int
+        lib.ndfo_interpolate.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double]
+        lib.ndfo_interpolate.restype = ctypes.c_double
+        lib.ndfo_extrema.argtypes = [
+            ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t),
+            ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_ssize_t), ctypes.POINTER(ctypes.c_int32),
+            ctypes.c_double, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_size_t, ctypes.c_char_p,
+            ctypes.c_size_t,
+        ]
+        lib.ndfo_extrema.restype = ctypes.c_int
         _lib = lib
     return _lib
 
@@ -99,6 +115,10 @@ def map_index(index: int, length: int, mode: str) -> int | None:
 
 def midpoint(lower: float, upper: float) -> float:
     return _load().ndfo_midpoint(lower, upper)
+
+
+def interpolate(lower: float, upper: float, fraction: float) -> float:
+    return _load().ndfo_interpolate(lower, upper, fraction)
 
 
 def key32(bits: int) -> int:
@@ -134,13 +154,18 @@ def window_filter(
     cval: float = 0.0,
     statistic: str = "median",
     rank: int = 0,
+    q: float = 0.0,
     omit_nan: bool = False,
     ignore_inf: bool = False,
     min_count: int = 1,
     method: str = "direct",
     threads: int | None = None,
 ) -> np.ndarray:
-    """Run the oracle on a C-contiguous float32/float64 array."""
+    """Run the oracle on a C-contiguous float32/float64 array.
+
+    ``statistic`` is median, rank (``rank``), percentile (``q`` in 0..100),
+    min or max. ``method`` is direct, histogram (median and rank) or
+    separable (min and max over a rectangle)."""
     array = np.ascontiguousarray(array)
     if array.dtype not in (np.float32, np.float64):
         raise TypeError("oracle supports float32 and float64")
@@ -159,7 +184,18 @@ def window_filter(
         fp_arr = np.ascontiguousarray(np.asarray(footprint).ravel() != 0, dtype=np.uint8)
         fp = fp_arr.ctypes.data
     err = ctypes.create_string_buffer(256)
-    status = _load().ndfo_filter(
+    if method == "separable":
+        if statistic not in ("min", "max") or footprint is not None:
+            raise ValueError("the separable method serves rectangular min and max only")
+        status = _load().ndfo_extrema(
+            array.dtype.itemsize, array.ctypes.data, out.ctypes.data, ndim, shape, sizes, origins, modes,
+            float(cval), int(statistic == "max"), int(bool(omit_nan)), int(bool(ignore_inf)), int(min_count), err,
+            len(err),
+        )
+        if status != 0:
+            raise OracleError(status, err.value.decode())
+        return out
+    status = _load().ndfo_filter_q(
         array.dtype.itemsize,
         array.ctypes.data,
         out.ctypes.data,
@@ -170,8 +206,9 @@ def window_filter(
         origins,
         modes,
         float(cval),
-        STAT_RANK if statistic == "rank" else STAT_MEDIAN,
+        STATISTICS[statistic],
         int(rank),
+        float(q),
         int(bool(omit_nan)),
         int(bool(ignore_inf)),
         int(min_count),
