@@ -46,16 +46,21 @@ def log(message: str) -> None:
 POISON = {4: 0x7FA5A5A5, 8: 0x7FF4A5A5A5A5A5A5}
 
 
-def oracle_full(image: np.ndarray, size, mode: str, kind: str = "median", rank: int = 0, footprint=None, origin=None,
-                ignore_inf: bool = False, method: str = "direct") -> tuple[np.ndarray, float]:
+def oracle_full(image: np.ndarray, size, mode: str, kind: str = "median", rank: float = 0, footprint=None,
+                origin=None, ignore_inf: bool = False, method: str = "direct") -> tuple[np.ndarray, float]:
     """The oracle over the whole image with every host thread; returns the
-    result and its wall time in seconds."""
+    result and its wall time in seconds. `rank` is the rank of kind "rank"
+    and the percentile of kind "percentile"."""
     from oracle import oracle
 
     kwargs = {"mode": mode, "omit_nan": True, "ignore_inf": ignore_inf, "method": method,
               "threads": oracle.default_threads()}
     if kind == "rank":
-        kwargs.update(statistic="rank", rank=rank)
+        kwargs.update(statistic="rank", rank=int(rank))
+    elif kind == "percentile":
+        kwargs.update(statistic="percentile", q=float(rank))
+    elif kind in ("min", "max"):
+        kwargs.update(statistic=kind)
     start = time.perf_counter()
     if footprint is not None:
         out = oracle.window_filter(image, footprint.shape, footprint=footprint, origin=origin, **kwargs)
@@ -180,20 +185,27 @@ def bracket_pair(work: Workload, kind: str, size: int, mode: str, parameter: int
 
 
 def plain_config(name: str, work: Workload, steps: int, warmup: int, size=None, mode="reflect", footprint=None,
-                 origin=None, ignore_inf=False, check: bool = True) -> dict:
-    plan = work.plan("median", size, mode, 0, footprint, origin, ignore_inf)
+                 origin=None, ignore_inf=False, check: bool = True, kind: str = "median",
+                 oracle_method: str = "direct") -> dict:
+    plan = work.plan(kind, size, mode, 0, footprint, origin, ignore_inf)
     result = work.run(plan, steps, warmup)
     out = work.verify(plan)
     result["unwritten_outputs"] = unwritten(out)
     if check:
-        expected, seconds = oracle_full(work.image, size, mode, footprint=footprint, origin=origin,
-                                        ignore_inf=ignore_inf)
+        expected, seconds = oracle_full(work.image, size, mode, kind, footprint=footprint, origin=origin,
+                                        ignore_inf=ignore_inf, method=oracle_method)
         result["bit_exact_vs_oracle_full"] = same_bits(out, expected)
         result["oracle_seconds"] = seconds
         del expected
     taps = int(footprint.sum()) if footprint is not None else int(np.prod(size))
     pixels = work.image.size
-    result["roofline_frac"] = ideal_seconds(pixels, taps, out.itemsize) / (result["event_main_kernel_ms"] * 1e-3)
+    if kind in ("min", "max"):
+        # The separable engine's bound is HBM: 2 * sizeof(T) algorithmic
+        # bytes per output (one read, one write).
+        result["hbm_gb_s"] = pixels * 2 * out.itemsize / (result["event_main_kernel_ms"] * 1e-3) / 1e9
+        result["roofline_frac"] = result["hbm_gb_s"] * 1e9 / HBM_PEAK
+    else:
+        result["roofline_frac"] = ideal_seconds(pixels, taps, out.itemsize) / (result["event_main_kernel_ms"] * 1e-3)
     result["engine"] = json.loads(plan.describe())
     result["workload"] = name
     return result
@@ -452,10 +464,25 @@ def main() -> int:
     rank63 = bracket_pair(c5, "rank", 63, "reflect", 397, w63, args.steps, args.warmup, expected_rank)
     log(json.dumps(rank63))
     del expected_median, expected_rank
+    extra = {}
+    if not args.quick:
+        expected = None
+        if not args.no_cpu:
+            log("whole-image oracle result: nanpercentile 10 63x63")
+            expected, _ = oracle_full(c5.image, (63, 63), "reflect", "percentile", 10.0, method="histogram")
+        log("63x63 nanpercentile 10: plain and learned bracket")
+        extra["percentile10_63x63"] = bracket_pair(c5, "percentile", 63, "reflect", 10.0, w63, args.steps,
+                                                   args.warmup, expected)
+        del expected
+        log(json.dumps(extra["percentile10_63x63"]))
+        for kind in ("min", "max"):
+            log(f"63x63 nan{kind} on the 8192x8192 sky (extrema engine)")
+            extra[f"{kind}_63x63"] = plain_config(f"nan{kind}_filter size=63 reflect 8192x8192 float32", c5, args.steps,
+                                                  args.warmup, size=(63, 63), kind=kind, check=not args.no_cpu,
+                                                  oracle_method="separable")
     c5_image = c5.image
     c5.free()
 
-    extra = {}
     if not args.quick:
         log("generating the 4096x4096 float64 sky (31x31 workload)")
         c2 = Workload(synthetic.sky_gradient_f64())

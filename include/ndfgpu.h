@@ -55,11 +55,12 @@ enum {
     NDF_KIND_MAX = 6
 };
 
-/* Device engines. AUTO picks by geometry (NETWORK for windows of at most 160
- * taps in float32 or 64 in float64, else SLIDE where the tile can be staged,
- * else GENERIC); the others force one engine and exist so tests can check
- * engines against each other. */
-enum { NDF_ENGINE_AUTO = 0, NDF_ENGINE_GENERIC = 1, NDF_ENGINE_SLIDE = 2, NDF_ENGINE_NETWORK = 3 };
+/* Device engines. AUTO picks by geometry (EXTREMA for rectangular minimum /
+ * maximum plans that need no retained count, NETWORK for windows of at most
+ * 160 taps in float32 or 64 in float64, else SLIDE where the tile can be
+ * staged, else GENERIC); the others force one engine and exist so tests can
+ * check engines against each other. */
+enum { NDF_ENGINE_AUTO = 0, NDF_ENGINE_GENERIC = 1, NDF_ENGINE_SLIDE = 2, NDF_ENGINE_NETWORK = 3, NDF_ENGINE_EXTREMA = 4 };
 
 /* One-shot filter over host buffers.
  * Replaces _core.window_filter(array, kind, size, footprint, origin, mode,

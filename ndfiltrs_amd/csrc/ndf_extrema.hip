@@ -1,0 +1,232 @@
+// Extrema engine: rectangular minimum / maximum filters (rank <= 3) by
+// separable passes over an LDS-staged tile.
+//
+// The reference computes rectangular extrema with exact separable van Herk /
+// Gil-Werman passes over whole lines (src/extrema.rs:125-436), one axis at a
+// time through image-sized buffers. Here one workgroup owns an output tile
+// of tx columns x ty rows of one plane and stages its support (kz planes of
+// ty + ky - 1 rows of tx + kx - 1 samples) in LDS once, so each input sample
+// crosses HBM about once and the intermediate results never leave the CU:
+//   * every sample becomes an unsigned total-order key; a propagated NaN
+//     becomes the key that dominates the reduction (0 for a minimum, all
+//     ones for a maximum) and an omitted sample the one that loses, so the
+//     policy rides inside the reduction;
+//   * the x pass and the y pass each take the extreme over k samples by
+//     doubling: log2(k) steps of m_2s[i] = op(m_s[i], m_s[i + s]), then
+//     op(m_P[i], m_P[i + k - P]) for the largest power of two P <= k (the
+//     two windows overlap, which an idempotent op allows); the steps ping-
+//     pong between two LDS buffers, one element per thread, with a barrier
+//     per step;
+//   * the z pass folds the kz planes per output.
+// The extreme key is a retained sample's key (its bits unchanged), or one of
+// the two marks (canonical NaN). Equal keys are equal bits, so which of two
+// equal extremes wins cannot change a result: the output is bit-identical to
+// the direct total-order extreme of the retained samples. Windows that need
+// a retained count (min_count > 1) run on the selection engines instead.
+#include "ndf_device.hpp"
+#include "ndf_internal.hpp"
+
+namespace ndf {
+namespace {
+
+constexpr int EXTREMA_THREADS = 256;
+
+template <typename Key, bool MAXIMUM>
+__device__ __forceinline__ Key pick_extreme(Key a, Key b)
+{
+    if constexpr (MAXIMUM) {
+        return a > b ? a : b;
+    } else {
+        return a < b ? a : b;
+    }
+}
+
+__device__ inline void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Support columns one lane stages per row (sx <= 64 * STAGE_COLUMNS).
+constexpr int STAGE_COLUMNS = 8;
+// Rows a wave stages per batch (all loads of a batch are issued first).
+constexpr int STAGE_ROWS = 4;
+
+template <typename T, bool MAXIMUM>
+__global__ void __launch_bounds__(EXTREMA_THREADS) k_extrema(ExtremaArgs a)
+{
+    using KT = KeyTraits<T>;
+    using Key = typename KT::Key;
+    extern __shared__ __align__(16) unsigned char lds[];
+    const int plane = a.sy * a.sx;   // staging buffer: kz planes of sy x sx
+    const int yplane = a.sy * a.tx;  // y-pass partner: kz planes of sy x tx
+    Key *stage = reinterpret_cast<Key *>(lds);
+    Key *partner = stage + a.kz * plane;
+    const Key *input = static_cast<const Key *>(a.input);
+    Key *output = static_cast<Key *>(a.output);
+    // Marks: an omitted (or padding) sample never wins; a propagated NaN always does.
+    const Key lose = MAXIMUM ? (Key)0 : KT::SENTINEL;
+    const Key win = MAXIMUM ? KT::SENTINEL : (Key)0;
+    auto classify_key = [&](Key bits) -> Key {
+        if (KT::is_nan(bits)) return a.omit ? lose : win;
+        if (a.omit && a.ignore_inf && KT::is_inf(bits)) return lose;
+        return KT::key(bits);
+    };
+    const Key cval_key = classify_key((Key)a.cval_bits);
+    int px = 1;
+    while (2 * px <= a.kx) px <<= 1;
+    int py = 1;
+    while (2 * py <= a.ky) py <<= 1;
+    const int tiles = a.tiles_x * a.tiles_y * (int)a.nz;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int rows = a.kz * a.sy;
+    for (int tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+        int rest = tile;
+        const int txi = rest % a.tiles_x;
+        rest /= a.tiles_x;
+        const int tyi = rest % a.tiles_y;
+        const int64_t z0 = rest / a.tiles_y;
+        const int64_t x0 = (int64_t)txi * a.tx, y0 = (int64_t)tyi * a.ty;
+        // Column maps of this lane, shared by every row of the tile.
+        int64_t mx[STAGE_COLUMNS];
+#pragma unroll
+        for (int j = 0; j < STAGE_COLUMNS; ++j) {
+            const int x = lane + 64 * j;
+            mx[j] = x < a.sx ? a.mapx[x0 + x] : MAP_PAD;
+        }
+        __syncthreads();  // the previous tile is done with both buffers
+        // ---- stage the support as keys: each wave owns rows wave + 4 i ----
+        for (int row0 = wave; row0 < rows; row0 += STAGE_ROWS * (EXTREMA_THREADS / 64)) {
+            Key raw[STAGE_ROWS][STAGE_COLUMNS];
+            int64_t my[STAGE_ROWS];
+#pragma unroll
+            for (int u = 0; u < STAGE_ROWS; ++u) {
+                const int row = row0 + u * (EXTREMA_THREADS / 64);
+                const int z = row / a.sy;
+                const int y = row - z * a.sy;
+                const int64_t mz = row < rows ? a.mapz[z0 + z] : MAP_PAD;
+                const int64_t myy = row < rows ? a.mapy[y0 + y] : MAP_PAD;
+                my[u] = (mz == MAP_PAD || myy == MAP_PAD) ? MAP_PAD : ((mz < 0 || myy < 0) ? MAP_CVAL : mz + myy);
+#pragma unroll
+                for (int j = 0; j < STAGE_COLUMNS; ++j) {
+                    const bool real = my[u] >= 0 && mx[j] >= 0;
+                    raw[u][j] = input[real ? my[u] + mx[j] : 0];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < STAGE_ROWS; ++u) {
+                const int row = row0 + u * (EXTREMA_THREADS / 64);
+                if (row >= rows) continue;
+#pragma unroll
+                for (int j = 0; j < STAGE_COLUMNS; ++j) {
+                    const int x = lane + 64 * j;
+                    if (x >= a.sx) continue;
+                    const bool pad = my[u] == MAP_PAD || mx[j] == MAP_PAD;
+                    const bool cval = !pad && (my[u] < 0 || mx[j] < 0);
+                    stage[row * a.sx + x] = pad ? lose : (cval ? cval_key : classify_key(raw[u][j]));
+                }
+            }
+        }
+        // ---- x pass, in place: each wave doubles the rows it staged ----
+        // Chunks of 64 consecutive columns go in increasing order and every
+        // lane of a chunk reads before any writes, so a chunk never reads a
+        // value a lower chunk already overwrote; only wave-level ordering is
+        // needed between steps.
+        wave_lds_sync();
+        for (int s = 1; s <= px; s <<= 1) {
+            const int shift = s < px ? s : a.kx - px;
+            const int columns = s < px ? a.sx : a.tx;
+            for (int r = wave; r < rows; r += EXTREMA_THREADS / 64) {
+                Key *row = stage + r * a.sx;
+                for (int i = lane; i < columns; i += 64) {
+                    const Key v = row[i];
+                    const Key w = i + shift < a.sx ? row[i + shift] : v;
+                    row[i] = pick_extreme<Key, MAXIMUM>(v, w);
+                }
+            }
+            wave_lds_sync();
+        }
+        __syncthreads();
+        // ---- y pass: extreme over ky along columns, ping-pong ----
+        const Key *src = stage;
+        int src_stride = a.sx, src_plane = plane;
+        Key *dst = partner;
+        for (int s = 1; s < py; s <<= 1) {
+            for (int zr = wave; zr < rows; zr += EXTREMA_THREADS / 64) {
+                const int z = zr / a.sy;
+                const int r = zr - z * a.sy;
+                const Key *in = src + z * src_plane + r * src_stride;
+                Key *out = dst + z * yplane + r * a.tx;
+                for (int i = lane; i < a.tx; i += 64) {
+                    const Key v = in[i];
+                    out[i] = r + s < a.sy ? pick_extreme<Key, MAXIMUM>(v, in[i + s * src_stride]) : v;
+                }
+            }
+            __syncthreads();
+            // Later steps ping-pong between the partner and the staging
+            // buffer, both laid out sy x tx from here on.
+            Key *next = dst == partner ? stage : partner;
+            src = dst;
+            src_stride = a.tx;
+            src_plane = yplane;
+            dst = next;
+        }
+        // ---- final y step, z fold and output ----
+        const int dy = a.ky - py;
+        for (int e = threadIdx.x; e < a.ty * a.tx; e += EXTREMA_THREADS) {
+            const int r = e / a.tx;
+            const int i = e - r * a.tx;
+            const int64_t y = y0 + r, x = x0 + i;
+            Key best = lose;
+            for (int z = 0; z < a.kz; ++z) {
+                const Key *col = src + z * src_plane + r * src_stride + i;
+                best = pick_extreme<Key, MAXIMUM>(best, pick_extreme<Key, MAXIMUM>(col[0], col[dy * src_stride]));
+            }
+            if (y < a.ny && x < a.nx) {
+                output[(z0 * a.ny + y) * a.nx + x] = (best == win || best == lose) ? KT::CANONICAL_NAN : KT::unkey(best);
+            }
+        }
+    }
+}
+
+template <typename T, bool MAXIMUM>
+hipError_t launch_typed(const ExtremaArgs &args, size_t lds_bytes, hipStream_t stream)
+{
+    const void *fn = reinterpret_cast<const void *>(&k_extrema<T, MAXIMUM>);
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+    if (e != hipSuccess) return e;
+    int per_cu = 0, device = 0, cus = 0;
+    if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, EXTREMA_THREADS, lds_bytes)) != hipSuccess ||
+        (e = hipGetDevice(&device)) != hipSuccess ||
+        (e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device)) != hipSuccess) {
+        return e;
+    }
+    const int64_t tiles = (int64_t)args.tiles_x * args.tiles_y * args.nz;
+    int64_t blocks = (int64_t)(per_cu < 1 ? 1 : per_cu) * (cus < 1 ? 1 : cus) * 2;
+    blocks = blocks < tiles ? blocks : tiles;
+    hipLaunchKernelGGL((k_extrema<T, MAXIMUM>), dim3((unsigned)(blocks < 1 ? 1 : blocks)), dim3(EXTREMA_THREADS),
+                       lds_bytes, stream, args);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+size_t extrema_lds_bytes(int dtype_bytes, int kz, int sy, int sx, int tx)
+{
+    return (size_t)kz * (size_t)sy * ((size_t)sx + (size_t)tx) * (size_t)dtype_bytes;
+}
+
+int extrema_max_support_columns() { return 64 * STAGE_COLUMNS; }
+
+hipError_t launch_extrema(int dtype_bytes, const ExtremaArgs &args, size_t lds_bytes, hipStream_t stream)
+{
+    if (dtype_bytes == 4) {
+        return args.maximum ? launch_typed<float, true>(args, lds_bytes, stream)
+                            : launch_typed<float, false>(args, lds_bytes, stream);
+    }
+    return args.maximum ? launch_typed<double, true>(args, lds_bytes, stream)
+                        : launch_typed<double, false>(args, lds_bytes, stream);
+}
+
+}  // namespace ndf

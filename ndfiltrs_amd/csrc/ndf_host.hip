@@ -199,6 +199,13 @@ struct ndf_plan {
     std::vector<int64_t> net_mapz, net_mapy, net_mapx;
     std::vector<uint32_t> net_taps;
 
+    // Extrema geometry (rectangular minimum / maximum).
+    bool extrema_ok = false;
+    ExtremaArgs extrema{};
+    size_t extrema_lds = 0;
+    std::vector<int64_t> ext_mapz, ext_mapy, ext_mapx;
+    DeviceBuffer e_mapz, e_mapy, e_mapx;
+
     // Slide geometry (host side).
     SlideArgs slide{};
     size_t slide_lds = 0;
@@ -255,7 +262,7 @@ struct ndf_plan {
     ~ndf_plan()
     {
         if (done) (void)hipEventDestroy(done);
-        for (DeviceBuffer *b : {&n_mapz, &n_mapy, &n_mapx, &n_taps}) {
+        for (DeviceBuffer *b : {&n_mapz, &n_mapy, &n_mapx, &n_taps, &e_mapz, &e_mapy, &e_mapx}) {
             b->release();
         }
         for (DeviceBuffer *b : {&g_maps, &g_taps, &s_mapz, &s_mapy, &s_mapx, &s_taps, &s_rm, &s_add, &s_mask, &d_in,
@@ -353,6 +360,91 @@ bool plan_network(ndf_plan *p)
     n.rank_param = (int32_t)std::min(p->parameter, p->count);
     n.q100 = p->q100;
     p->network_lds = lds;
+    return true;
+}
+
+// Maps for the tiled engines: entry i is the input offset of coordinate
+// i + low along an axis of n (MAP_CVAL outside under constant mode); entries
+// past the last window coordinate (n + k - 1) only feed padded outputs.
+void build_map(std::vector<int64_t> &map, int64_t length, int64_t n, int64_t low, int64_t k, int32_t mode,
+               int64_t stride)
+{
+    map.assign((size_t)length, MAP_PAD);
+    for (int64_t i = 0; i < length && i < n + k - 1; ++i) {
+        const int64_t m = map_coordinate(i + low, n, mode);
+        map[(size_t)i] = m < 0 ? MAP_CVAL : m * stride;
+    }
+}
+
+// Plan the extrema engine (rectangular minimum / maximum of rank <= 3 that
+// needs no retained count); false when it does not apply or the tile does
+// not fit LDS.
+bool plan_extrema(ndf_plan *p)
+{
+    if (p->kind != NDF_KIND_MIN && p->kind != NDF_KIND_MAX) return false;
+    if (!p->footprint.empty() || p->rank > 3 || p->len == 0) return false;
+    if (p->omit && p->min_count > 1) return false;
+    int64_t dims[3] = {1, 1, 1}, lo[3] = {0, 0, 0}, box[3] = {1, 1, 1};
+    int32_t modes[3] = {NDF_MODE_NEAREST, NDF_MODE_NEAREST, NDF_MODE_NEAREST};
+    const size_t pad = 3 - p->rank;
+    for (size_t a = 0; a < p->rank; ++a) {
+        dims[pad + a] = (int64_t)p->shape[a];
+        lo[pad + a] = p->low[a];
+        box[pad + a] = p->high[a] - p->low[a] + 1;
+        modes[pad + a] = p->modes[a];
+    }
+    const int64_t kz = box[0], ky = box[1], kx = box[2];
+    const int limit = lds_limit();
+    // The tile with the least support per output that fits LDS; ties keep
+    // the larger tile. A tile above half of LDS allows one workgroup per CU
+    // (no second workgroup to hide the staging latency): it must save a
+    // fifth of the support to be chosen.
+    int best_tx = 0, best_ty = 0;
+    double best_ratio = 0.0;
+    const int64_t rows_cap = dims[1];
+    for (int tx : {256, 128, 64}) {
+        if (tx > 64 && (int64_t)(tx / 2) >= dims[2]) continue;
+        for (int ty : {64, 48, 32, 24, 16, 12, 8, 6, 4, 2, 1}) {
+            if (ty > rows_cap && ty > 1) continue;
+            const int64_t sx = tx + kx - 1, sy = ty + ky - 1;
+            if (sx > extrema_max_support_columns() || sy > (1 << 20)) continue;
+            const size_t lds = extrema_lds_bytes(p->dbytes, (int)kz, (int)sy, (int)sx, tx);
+            if (lds > (size_t)limit) continue;
+            const double ratio = (double)(kz * sy * sx) / (double)(ty * tx) * (2 * lds > (size_t)limit ? 1.2 : 1.0);
+            if (best_tx == 0 || ratio < best_ratio * 0.999) {
+                best_tx = tx;
+                best_ty = ty;
+                best_ratio = ratio;
+            }
+        }
+    }
+    if (best_tx == 0) return false;
+    const int64_t tiles_x = (dims[2] + best_tx - 1) / best_tx;
+    const int64_t tiles_y = (dims[1] + best_ty - 1) / best_ty;
+    if (tiles_x * tiles_y * dims[0] > 0x7FFFFFFF) return false;
+    const int64_t in_sy = dims[2], in_sz = dims[1] * dims[2];
+    build_map(p->ext_mapz, dims[0] + kz - 1, dims[0], lo[0], kz, modes[0], in_sz);
+    build_map(p->ext_mapy, tiles_y * best_ty + ky - 1, dims[1], lo[1], ky, modes[1], in_sy);
+    build_map(p->ext_mapx, tiles_x * best_tx + kx - 1, dims[2], lo[2], kx, modes[2], 1);
+    ExtremaArgs &e = p->extrema;
+    e = ExtremaArgs{};
+    e.nz = dims[0];
+    e.ny = dims[1];
+    e.nx = dims[2];
+    e.kz = (int32_t)kz;
+    e.ky = (int32_t)ky;
+    e.kx = (int32_t)kx;
+    e.tx = best_tx;
+    e.ty = best_ty;
+    e.sx = (int32_t)(best_tx + kx - 1);
+    e.sy = (int32_t)(best_ty + ky - 1);
+    e.tiles_x = (int32_t)tiles_x;
+    e.tiles_y = (int32_t)tiles_y;
+    e.cval_bits = p->cval_bits;
+    e.ignore_inf = p->eff_ignore_inf;
+    e.omit = p->omit;
+    e.maximum = p->kind == NDF_KIND_MAX;
+    p->extrema_lds = extrema_lds_bytes(p->dbytes, e.kz, e.sy, e.sx, e.tx);
     return true;
 }
 
@@ -619,6 +711,15 @@ ndf_status prepare_engine_tables(ndf_plan *p, char *err, size_t err_len)
     if (p->engine == NDF_ENGINE_GENERIC) {
         const ndf_status st = prepare_generic_tables(p, err, err_len);
         if (st != NDF_OK) return st;
+    } else if (p->engine == NDF_ENGINE_EXTREMA) {
+        if ((e = upload(p->e_mapz, p->ext_mapz)) != hipSuccess || (e = upload(p->e_mapy, p->ext_mapy)) != hipSuccess ||
+            (e = upload(p->e_mapx, p->ext_mapx)) != hipSuccess) {
+            return fail_hip(e, err, err_len);
+        }
+        ExtremaArgs &x = p->extrema;
+        x.mapz = static_cast<const int64_t *>(p->e_mapz.ptr);
+        x.mapy = static_cast<const int64_t *>(p->e_mapy.ptr);
+        x.mapx = static_cast<const int64_t *>(p->e_mapx.ptr);
     } else if (p->engine == NDF_ENGINE_NETWORK) {
         if ((e = upload(p->n_mapz, p->net_mapz)) != hipSuccess || (e = upload(p->n_mapy, p->net_mapy)) != hipSuccess ||
             (e = upload(p->n_mapx, p->net_mapx)) != hipSuccess || (e = upload(p->n_taps, p->net_taps)) != hipSuccess) {
@@ -755,6 +856,7 @@ ndf_status prepare_bracket(ndf_plan *p, char *err, size_t err_len)
 // static scratch array.
 struct LaunchSet {
     SlideArgs slide{}, slide_b{}, slide_fb{};
+    ExtremaArgs extrema{};
     NetworkArgs network{};
     GenericArgs generic{};
     BracketArgs bargs{};
@@ -773,6 +875,7 @@ LaunchSet plan_set(const ndf_plan *p)
     t.slide_b = p->slide_b;
     t.slide_fb = p->slide_fb;
     t.network = p->network;
+    t.extrema = p->extrema;
     t.generic = p->generic;
     t.bargs = p->bargs;
     t.slide_b_lds = p->slide_b_lds;
@@ -829,6 +932,13 @@ hipError_t enqueue(ndf_plan *p, const LaunchSet &t, const void *d_in, void *d_ou
             args.tile_count = t.fcount;
             e = launch_slide(p->dbytes, args, t.slide_fb_lds, tiles, stream);
         }
+    } else if (p->engine == NDF_ENGINE_EXTREMA) {
+        ExtremaArgs args = t.extrema;
+        args.input = d_in;
+        args.output = d_out;
+        mark(p->main_begin);
+        if (e == hipSuccess) e = launch_extrema(p->dbytes, args, p->extrema_lds, stream);
+        mark(p->main_end);
     } else if (p->engine == NDF_ENGINE_NETWORK) {
         NetworkArgs args = t.network;
         args.input = d_in;
@@ -1052,7 +1162,10 @@ ndf_status build_plan(ndf_plan *p, int32_t dtype_bytes, size_t rank, const size_
     p->all_nan = p->stat == STAT_RANK && parameter >= count;
     p->slide_ok = plan_slide(p);
     p->network_ok = plan_network(p);
-    p->engine = p->network_ok ? NDF_ENGINE_NETWORK : (p->slide_ok ? NDF_ENGINE_SLIDE : NDF_ENGINE_GENERIC);
+    p->extrema_ok = plan_extrema(p);
+    p->engine = p->extrema_ok   ? NDF_ENGINE_EXTREMA
+                : p->network_ok ? NDF_ENGINE_NETWORK
+                                : (p->slide_ok ? NDF_ENGINE_SLIDE : NDF_ENGINE_GENERIC);
     return NDF_OK;
 }
 
@@ -1112,7 +1225,17 @@ ndf_status ndf_plan_set_engine(ndf_plan *plan, int32_t engine, char *err, size_t
     std::lock_guard<std::mutex> guard(plan->mutex);
     int32_t chosen;
     if (engine == NDF_ENGINE_AUTO) {
-        chosen = plan->network_ok ? NDF_ENGINE_NETWORK : (plan->slide_ok ? NDF_ENGINE_SLIDE : NDF_ENGINE_GENERIC);
+        chosen = plan->extrema_ok   ? NDF_ENGINE_EXTREMA
+                 : plan->network_ok ? NDF_ENGINE_NETWORK
+                                    : (plan->slide_ok ? NDF_ENGINE_SLIDE : NDF_ENGINE_GENERIC);
+    } else if (engine == NDF_ENGINE_EXTREMA) {
+        if (!plan->extrema_ok) {
+            return fail(NDF_ERR_INVALID,
+                        "the extrema engine needs a rectangular minimum or maximum of rank 3 or less without a "
+                        "min_count above 1",
+                        err, err_len);
+        }
+        chosen = NDF_ENGINE_EXTREMA;
     } else if (engine == NDF_ENGINE_GENERIC) {
         chosen = NDF_ENGINE_GENERIC;
     } else if (engine == NDF_ENGINE_SLIDE) {
@@ -1136,10 +1259,20 @@ ndf_status ndf_plan_set_engine(ndf_plan *plan, int32_t engine, char *err, size_t
 
 ndf_status ndf_plan_describe(const ndf_plan *plan, char *buffer, size_t buffer_len)
 {
-    const char *engine = plan->all_nan ? "fill"
-                         : plan->engine == NDF_ENGINE_SLIDE ? "slide"
+    const char *engine = plan->all_nan                       ? "fill"
+                         : plan->engine == NDF_ENGINE_SLIDE   ? "slide"
                          : plan->engine == NDF_ENGINE_NETWORK ? "network"
+                         : plan->engine == NDF_ENGINE_EXTREMA ? "extrema"
                                                               : "generic";
+    if (plan->engine == NDF_ENGINE_EXTREMA) {
+        snprintf(buffer, buffer_len,
+                 "{\"engine\": \"%s\", \"count\": %zu, \"tile\": [%d, %d], \"support\": [%d, %d, %d], "
+                 "\"lds_bytes\": %zu, \"tiles\": %lld}",
+                 engine, plan->count, plan->extrema.ty, plan->extrema.tx, plan->extrema.kz, plan->extrema.sy,
+                 plan->extrema.sx, plan->extrema_lds,
+                 (long long)plan->extrema.tiles_x * plan->extrema.tiles_y * plan->extrema.nz);
+        return NDF_OK;
+    }
     if (plan->engine == NDF_ENGINE_NETWORK) {
         snprintf(buffer, buffer_len,
                  "{\"engine\": \"%s\", \"count\": %zu, \"tile\": [%d, 64], \"network_wires\": %d, "
@@ -2342,6 +2475,12 @@ hipError_t static_layout(ndf_plan *p, char *base, hipStream_t stream, LaunchSet 
         t->generic = p->generic;
         t->generic.maps = static_cast<const int64_t *>(put(p->gen_maps));
         t->generic.taps = static_cast<const int32_t *>(put(p->gen_taps));
+    } else if (p->engine == NDF_ENGINE_EXTREMA) {
+        ExtremaArgs &x = t->extrema;
+        x = p->extrema;
+        x.mapz = static_cast<const int64_t *>(put(p->ext_mapz));
+        x.mapy = static_cast<const int64_t *>(put(p->ext_mapy));
+        x.mapx = static_cast<const int64_t *>(put(p->ext_mapx));
     } else if (p->engine == NDF_ENGINE_NETWORK) {
         NetworkArgs &n = t->network;
         n = p->network;

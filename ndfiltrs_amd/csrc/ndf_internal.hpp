@@ -117,6 +117,28 @@ struct NetworkArgs {
     double q100;  // percentile / 100 (STAT_PERCENTILE)
 };
 
+// Engine 4: rectangular minimum / maximum (rank <= 3) by separable passes
+// over an LDS tile of tx columns x ty rows of one plane; its support is kz
+// planes of sy = ty + ky - 1 rows of sx = tx + kx - 1 samples. Maps as for
+// the slide engine (index i <-> coordinate i + low of the window).
+struct ExtremaArgs {
+    const void *input;
+    void *output;
+    int64_t nz, ny, nx;
+    const int64_t *mapz, *mapy, *mapx;
+    int32_t kz, ky, kx;
+    int32_t tx, ty, sx, sy;
+    int32_t tiles_x, tiles_y;
+    uint64_t cval_bits;
+    int32_t ignore_inf;
+    int32_t omit;     // 0: a NaN in the window gives NaN (propagate)
+    int32_t maximum;  // 0: minimum
+};
+
+size_t extrema_lds_bytes(int dtype_bytes, int kz, int sy, int sx, int tx);
+int extrema_max_support_columns();
+hipError_t launch_extrema(int dtype_bytes, const ExtremaArgs &args, size_t lds_bytes, hipStream_t stream);
+
 // Network size (live wires) for a window of `count` taps, 0 if too large.
 int network_live(int dtype_bytes, int count);
 size_t network_lds_bytes(int dtype_bytes, int s, int count);

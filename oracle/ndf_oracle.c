@@ -833,6 +833,7 @@ typedef struct {
     size_t min_count;
     int statistic;
     size_t parameter;
+    double q; /* NDFO_STAT_PERCENTILE */
 } hjob_t;
 
 typedef struct {
@@ -878,6 +879,20 @@ static uint64_t histo_finish(const histo_t *h, const uint64_t *value_bits, const
         return value_bits[histo_select(h, job->parameter)];
     }
     if (h->count == 0) return canonical_nan(db);
+    if (job->statistic == NDFO_STAT_MIN) return value_bits[histo_select(h, 0)];
+    if (job->statistic == NDFO_STAT_MAX) return value_bits[histo_select(h, h->count - 1)];
+    if (job->statistic == NDFO_STAT_PERCENTILE) {
+        /* percentile(), src/robust.rs:358-367, on the ranked window. */
+        const size_t n = h->count;
+        if (job->q == 100.0) return value_bits[histo_select(h, n - 1)];
+        const double rank = (job->q / 100.0) * (double)(n - 1);
+        const size_t lower = (size_t)floor(rank);
+        const size_t upper = lower + 1 < n - 1 ? lower + 1 : n - 1;
+        const double value = ndfo_interpolate(bits_to_double(value_bits[histo_select(h, lower)], db),
+                                              bits_to_double(value_bits[histo_select(h, upper)], db),
+                                              rank - (double)lower);
+        return double_to_bits(value, db);
+    }
     if (h->count % 2 == 1) {
         return value_bits[histo_select(h, h->count / 2)];
     }
@@ -1079,8 +1094,8 @@ static uint64_t cval_to_bits(double cval, int dbytes)
 
 static int filter_histogram(int dbytes, const void *input, void *output, size_t rank, const size_t *shape,
                             const size_t *size, const uint8_t *footprint, const ptrdiff_t *origin,
-                            const int32_t *modes, double cval, int32_t statistic, size_t parameter, int omit,
-                            int ignore_inf, size_t min_count, int nthreads, char *err, size_t err_len)
+                            const int32_t *modes, double cval, int32_t statistic, size_t parameter, double q,
+                            int omit, int ignore_inf, size_t min_count, int nthreads, char *err, size_t err_len)
 {
     /* src/plan.rs:294-312 then validate_rectangular (:154-197). */
     if (footprint) return fail(NDFO_INVALID, MSG_HIST_FOOTPRINT, err, err_len);
@@ -1124,6 +1139,7 @@ static int filter_histogram(int dbytes, const void *input, void *output, size_t 
     job.min_count = min_count;
     job.statistic = statistic;
     job.parameter = parameter;
+    job.q = q;
     job.strides[rank - 1] = 1;
     for (size_t a = rank - 1; a-- > 0;) job.strides[a] = job.strides[a + 1] * shape[a + 1];
     block_lengths(rank, shape, size, job.block);
@@ -1180,8 +1196,11 @@ int ndfo_filter_q(int dtype_bytes, const void *input, void *output, size_t rank,
         if (modes[a] < 0 || modes[a] > 4) return fail(NDFO_INVALID, "unsupported boundary mode", err, err_len);
     }
     if (method == NDFO_METHOD_HISTOGRAM) {
+        if (statistic == NDFO_STAT_PERCENTILE && (!isfinite(q) || q < 0.0 || q > 100.0)) {
+            return fail(NDFO_INVALID, "percentile must be finite and between zero and 100", err, err_len);
+        }
         return filter_histogram(dtype_bytes, input, output, rank, shape, size, footprint, origin, modes, cval,
-                                statistic, parameter, omit_nan != 0, ignore_inf != 0, min_count, nthreads, err,
+                                statistic, parameter, q, omit_nan != 0, ignore_inf != 0, min_count, nthreads, err,
                                 err_len);
     }
     window_t w;

@@ -73,8 +73,8 @@ int ndfo_filter(int dtype_bytes, const void *input, void *output, size_t rank,
                 size_t err_len);
 
 /* ndfo_filter with the percentile q (0..100) of NDFO_STAT_PERCENTILE, whose
- * validation follows RobustReduction::validate (src/robust.rs:189-241). The
- * histogram method serves the median and rank statistics only. */
+ * validation follows RobustReduction::validate (src/robust.rs:189-241). Both
+ * methods serve every statistic; the histogram method needs a rectangle. */
 int ndfo_filter_q(int dtype_bytes, const void *input, void *output, size_t rank,
                   const size_t *shape, const size_t *size, const uint8_t *footprint,
                   const ptrdiff_t *origin, const int32_t *modes, double cval,

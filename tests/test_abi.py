@@ -56,6 +56,17 @@ def _plan(dtype, shape, size, **kw) -> dict:
     return json.loads(plan.describe())
 
 
+def test_extrema_plans_take_the_extrema_engine() -> None:
+    for kind in ("min", "max"):
+        info = _plan(np.float32, (8192, 8192), (63, 63), kind=kind)
+        assert info["engine"] == "extrema" and info["lds_bytes"] <= 163840, info
+        info = _plan(np.float64, (128, 1024, 1024), (3, 7, 7), kind=kind)
+        assert info["engine"] == "extrema", info
+    # A retained count above one needs a selection engine.
+    plan = _core.Plan(np.float32, (512, 512), "min", (5, 5), None, (0, 0), ("reflect",) * 2, 0.0, True, False, 3, 0)
+    assert json.loads(plan.describe())["engine"] != "extrema"
+
+
 def test_engine_choice_for_the_benchmark_shapes() -> None:
     c5 = _plan(np.float32, (8192, 8192), (63, 63))
     assert c5["engine"] == "slide" and c5["tile"][1] == 64 and c5["lds_bytes"] <= 163840

@@ -1,8 +1,8 @@
 """GPU parity of the percentile, minimum and maximum statistics and of the
 grey-scale morphology built on them.
 
-Every engine that can run a geometry (network, slide, generic; forced in
-turn) gives the oracle's bits on special-value inputs for percentiles 0 to
+Every engine that can run a geometry (extrema, network, slide, generic;
+forced in turn) gives the oracle's bits on special-value inputs for percentiles 0 to
 100 and for the extrema; the public calls give the reference's literal
 answers; morphology matches SciPy on finite data, as the reference's tests
 check (tests/test_morphology.py:74-147).
@@ -25,7 +25,7 @@ from tests.test_robust_oracle import check_literal, literal_cases
 
 pytestmark = pytest.mark.gpu
 
-ENGINES = ("network", "slide", "generic")
+ENGINES = ("extrema", "network", "slide", "generic")
 
 
 @pytest.fixture(scope="module", autouse=True)
@@ -239,3 +239,29 @@ def test_bracket_serves_every_statistic(dtype, kind: str) -> None:
     expected = oracle.window_filter(image, (31, 31), statistic=statistic, q=30.0, omit_nan=True, mode="reflect")
     assert_bits(outputs[0], expected, f"{kind} plain")
     assert_bits(outputs[1], expected, f"{kind} bracket")
+
+
+EXTREMA_CASES = [
+    ((1000,), (7,), (0,), np.float32),
+    ((257, 300), (63, 63), (0, 0), np.float32),
+    ((300, 257), (64, 2), (-32, 0), np.float64),
+    ((130, 140), (1, 129), (0, 64), np.float32),
+    ((6, 50, 70), (3, 7, 7), (1, 0, -3), np.float32),
+    ((5, 40, 33), (5, 4, 1), (-2, 1, 0), np.float64),
+    ((2, 3, 700), (2, 3, 5), (0, 1, 2), np.float32),
+]
+
+
+@pytest.mark.parametrize(("shape", "size", "origin", "dtype"), EXTREMA_CASES)
+def test_extrema_engine_matches_the_separable_oracle(shape, size, origin, dtype) -> None:
+    values = _cases.special_sample(shape, dtype, seed=int(np.prod(size)) % 1000, fraction=0.02)
+    for kind, mode, omit, ignore_inf, cval in itertools.product(("min", "max"), _cases.MODES, (False, True),
+                                                               (False, True), (np.nan, 1.5)):
+        if ignore_inf and not omit:
+            continue
+        expected = oracle.window_filter(values, size, origin=origin, statistic=kind, omit_nan=omit,
+                                        ignore_inf=ignore_inf, mode=mode, cval=cval, method="separable")
+        plan = _core.Plan(values.dtype, values.shape, kind, size, None, origin, (mode,) * values.ndim, cval, omit,
+                          ignore_inf, 1, 0)
+        assert '"engine": "extrema"' in plan.describe(), plan.describe()
+        assert_bits(plan.apply(values), expected, f"{kind} {mode} omit={omit} ignore_inf={ignore_inf} cval={cval}")

@@ -377,3 +377,21 @@ def test_extrema_literals(method: str, dtype) -> None:
             check_literal(out, want)
         check_literal(run([7.0], statistic, (3,), "reflect", omit_nan=True, min_count=3), 7.0)
         check_literal(run([1.0, np.nan, 3.0], statistic, (3,), "reflect"), math.nan)
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("statistic,q", [("percentile", 0.0), ("percentile", 10.0), ("percentile", 50.0),
+                                         ("percentile", 100.0 / 7), ("percentile", 100.0), ("min", 0.0),
+                                         ("max", 0.0)])
+def test_histogram_method_equals_direct(dtype, statistic: str, q: float) -> None:
+    """The block rank histogram serves percentiles and extrema too (the
+    whole-image parity reference of the benchmark); it equals the direct
+    method bit for bit."""
+    values = _cases.special_sample((47, 61), dtype, seed=9, fraction=0.1)
+    for mode, omit, ignore_inf, min_count in (("reflect", True, False, 1), ("constant", False, False, 1),
+                                              ("wrap", True, True, 9), ("mirror", True, False, 40)):
+        kwargs = dict(statistic=statistic, q=q, mode=mode, omit_nan=omit, ignore_inf=ignore_inf,
+                      min_count=min_count, cval=0.5)
+        direct = oracle.window_filter(values, (7, 6), **kwargs)
+        histogram = oracle.window_filter(values, (7, 6), method="histogram", **kwargs)
+        assert np.array_equal(_cases.bits_of(direct), _cases.bits_of(histogram)), (statistic, q, mode)
