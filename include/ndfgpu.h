@@ -283,6 +283,25 @@ ndf_status ndf_batch_time_device(ndf_batch *batch, const void *d_input, void *d_
 ndf_status ndf_batch_describe(ndf_batch *batch, char *buffer, size_t buffer_len);
 void ndf_batch_destroy(ndf_batch *batch);
 
+/* Median filter of mesh-background grids: replaces the grid stage of the
+ * reference's mesh background, MeshBackground::filter
+ * (src/background.rs:797-855) with MeshFilterEdge (:102-111) and
+ * MeshThreshold (:113-123). background / rms are float64 grids of
+ * prod(mesh_shape) values (C order); every mesh takes the median of the
+ * meshes in its filter_size window (NaN if one is NaN; an even count
+ * averages the middle two, src/background.rs:940-963), the window shrunk to
+ * stay centred (SHRINK) or truncated at the grid (TRUNCATE). ALL replaces
+ * every mesh, DEVIATION meshes with |median - background| >= threshold,
+ * ABOVE meshes with background > threshold; a replaced mesh takes the RMS
+ * median too. A filter of 1 on every axis copies the grids. Host buffers;
+ * blocking. "filter threshold must be finite" for a nonfinite threshold
+ * with DEVIATION or ABOVE. */
+enum { NDF_MESH_SHRINK = 0, NDF_MESH_TRUNCATE = 1 };
+enum { NDF_MESH_ALL = 0, NDF_MESH_DEVIATION = 1, NDF_MESH_ABOVE = 2 };
+ndf_status ndf_mesh_median_filter(const double *background, const double *rms, double *out_background,
+                                  double *out_rms, size_t ndim, const size_t *mesh_shape, const size_t *filter_size,
+                                  int32_t edge, int32_t threshold_kind, double threshold, char *err, size_t err_len);
+
 /* Replaces get_num_threads (src/pyapi/filter.rs:377-380): the host thread
  * count a CPU caller would use (RAYON_NUM_THREADS, else the CPU count). */
 size_t ndf_get_num_threads(void);

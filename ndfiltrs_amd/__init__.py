@@ -3,13 +3,14 @@
 Drop-in for ndfiltrs' ``median_filter``, ``nanmedian_filter``,
 ``rank_filter``, ``percentile_filter``, ``nanpercentile_filter``,
 ``min_filter``, ``max_filter``, ``nanmin_filter``, ``nanmax_filter``, the
-grey-scale morphology built on them, ``get_num_threads`` and
-``fixpix(method="median")``; results are bit-exact with the reference CPU
+grey-scale morphology built on them, the mesh-background grid median
+filter, ``get_num_threads`` and ``fixpix(method="median")``; results are bit-exact with the reference CPU
 semantics. Computation runs in hand-written HIP kernels in ``libndfgpu.so``;
 there is no CPU fallback.
 """
 
 from . import _core
+from ._background import mesh_median_filter
 from ._api import (
     get_num_threads,
     max_filter,
@@ -42,6 +43,7 @@ __all__ = [
     "grey_opening",
     "max_filter",
     "median_filter",
+    "mesh_median_filter",
     "min_filter",
     "morphological_gradient",
     "morphological_laplace",

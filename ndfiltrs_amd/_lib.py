@@ -122,6 +122,10 @@ def _declare(lib: ctypes.CDLL) -> None:
         ctypes.c_size_t,
         *_err,
     ]
+    lib.ndf_mesh_median_filter.argtypes = [
+        ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, _size_p, _size_p,
+        ctypes.c_int32, ctypes.c_int32, ctypes.c_double, *_err,
+    ]
     lib.ndf_plan_apply.argtypes = [plan_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, *_err]
     lib.ndf_plan_upload.argtypes = [plan_p, ctypes.c_void_p, ctypes.c_size_t, *_err]
     lib.ndf_plan_compute_download.argtypes = [plan_p, ctypes.c_void_p, ctypes.c_size_t, *_err]
@@ -264,6 +268,7 @@ def _declare(lib: ctypes.CDLL) -> None:
         "ndf_plan_create",
         "ndf_robust_filter",
         "ndf_robust_plan_create",
+        "ndf_mesh_median_filter",
         "ndf_plan_apply",
         "ndf_plan_upload",
         "ndf_plan_compute_download",

@@ -1411,6 +1411,108 @@ done:
     return status == NDFO_OK ? NDFO_OK : fail_code(status, err, err_len);
 }
 
+/* ------------------------------------------------------------------ */
+/* Mesh-grid median filter (MeshBackground::filter, src/background.rs:    */
+/* 797-855, with median_of / median_sorted, :940-963)                     */
+/* ------------------------------------------------------------------ */
+
+static int compare_f64_total(const void *a, const void *b)
+{
+    uint64_t x, y;
+    memcpy(&x, a, 8);
+    memcpy(&y, b, 8);
+    x = ndfo_key64(x);
+    y = ndfo_key64(y);
+    return x < y ? -1 : x > y;
+}
+
+/* Median of f64 values, NaN when empty or when any is NaN; an even count
+ * takes (a + b) / 2, or a / 2 + b / 2 when that overflows. Reorders. */
+static double mesh_median(double *values, size_t n)
+{
+    if (n == 0) return NAN;
+    for (size_t i = 0; i < n; ++i) {
+        if (isnan(values[i])) return NAN;
+    }
+    qsort(values, n, sizeof(double), compare_f64_total);
+    if (n % 2 == 1) return values[n / 2];
+    const double a = values[n / 2 - 1], b = values[n / 2];
+    const double mid = (a + b) / 2.0;
+    return isfinite(mid) ? mid : a / 2.0 + b / 2.0;
+}
+
+int ndfo_mesh_filter(const double *background, const double *rms, double *out_background, double *out_rms,
+                     size_t ndim, const size_t *mesh_shape, const size_t *filter_size, int32_t edge,
+                     int32_t threshold_kind, double threshold, char *err, size_t err_len)
+{
+    if (ndim == 0 || ndim > 64) return fail(NDFO_INVALID, "mesh grids need 1 to 64 dimensions", err, err_len);
+    if ((threshold_kind == 1 || threshold_kind == 2) && !isfinite(threshold)) {
+        return fail(NDFO_INVALID, "filter threshold must be finite", err, err_len);
+    }
+    size_t len = 1, volume = 1, all_one = 1;
+    for (size_t a = 0; a < ndim; ++a) {
+        if (filter_size[a] == 0) return fail(NDFO_INVALID, "filter sizes must be positive", err, err_len);
+        if (!checked_mul(len, mesh_shape[a], &len) || !checked_mul(volume, filter_size[a], &volume)) {
+            return fail_code(NDFO_OVERFLOW, err, err_len);
+        }
+        all_one &= filter_size[a] == 1;
+    }
+    memcpy(out_background, background, len * sizeof(double));
+    memcpy(out_rms, rms, len * sizeof(double));
+    if (all_one || len == 0) return NDFO_OK;
+    double *wb = malloc(sizeof(double) * volume), *wr = malloc(sizeof(double) * volume);
+    if (!wb || !wr) {
+        free(wb);
+        free(wr);
+        return fail_code(NDFO_MEMORY, err, err_len);
+    }
+    size_t index[64], low[64], high[64];
+    for (size_t flat = 0; flat < len; ++flat) {
+        size_t rest = flat;
+        for (size_t a = ndim; a-- > 0;) {
+            index[a] = rest % mesh_shape[a];
+            rest /= mesh_shape[a];
+        }
+        for (size_t a = 0; a < ndim; ++a) {
+            const size_t k = index[a], n = mesh_shape[a];
+            size_t half = filter_size[a] / 2;
+            if (edge == 0) { /* shrink: the window stays centred */
+                if (half > k) half = k;
+                if (half > n - 1 - k) half = n - 1 - k;
+                low[a] = k - half;
+                high[a] = k + half;
+            } else { /* truncate: drop positions outside the grid */
+                low[a] = k > half ? k - half : 0;
+                high[a] = k + half < n - 1 ? k + half : n - 1;
+            }
+        }
+        size_t count = 1;
+        for (size_t a = 0; a < ndim; ++a) count *= high[a] - low[a] + 1;
+        for (size_t position = 0; position < count; ++position) {
+            size_t r = position, offset = 0, stride = 1;
+            for (size_t a = ndim; a-- > 0;) {
+                const size_t width = high[a] - low[a] + 1;
+                offset += (low[a] + r % width) * stride;
+                r /= width;
+                stride *= mesh_shape[a];
+            }
+            wb[position] = background[offset];
+            wr[position] = rms[offset];
+        }
+        const double median = mesh_median(wb, count);
+        int replace = 1;
+        if (threshold_kind == 1) replace = fabs(median - background[flat]) >= threshold;
+        if (threshold_kind == 2) replace = background[flat] > threshold;
+        if (replace) {
+            out_background[flat] = median;
+            out_rms[flat] = mesh_median(wr, count);
+        }
+    }
+    free(wb);
+    free(wr);
+    return NDFO_OK;
+}
+
 /* fixpix(method="median"): src/repair.rs:107-126 (fixpix_median) with
  * positions() at src/repair.rs:14-29 and repair_median at
  * src/filter_nd.rs:174-215. The output starts as a copy of the input; every

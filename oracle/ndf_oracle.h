@@ -10,6 +10,7 @@
  *   - NaN/inf policy + selection  src/reduction.rs:78-130
  *   - percentile interpolation    src/robust.rs:320-367, src/convolution.rs:203-384
  *   - separable extrema           src/extrema.rs:125-436
+ *   - mesh-grid median filter     src/background.rs:797-855, 940-963
  *   - total-order keys            src/median_network.rs:9-37
  *   - even-count midpoint         docs/semantics.md:74-85, tests/_reference.py:36-47
  *   - mask-directed median repair src/repair.rs:14-29, 107-126,
@@ -91,6 +92,16 @@ double ndfo_interpolate(double lower, double upper, double fraction);
 int ndfo_extrema(int dtype_bytes, const void *input, void *output, size_t rank, const size_t *shape,
                  const size_t *size, const ptrdiff_t *origin, const int32_t *modes, double cval, int32_t maximum,
                  int32_t omit_nan, int32_t ignore_inf, size_t min_count, char *err, size_t err_len);
+
+/* Median filter of a mesh background grid and its RMS grid
+ * (MeshBackground::filter, src/background.rs:797-855): each mesh takes the
+ * median (NaN if any sample is NaN; an even count averages the middle two)
+ * of its window, shrunk to stay centred (edge 0) or truncated at the grid
+ * (edge 1); threshold_kind 0 replaces every mesh, 1 meshes whose median
+ * deviates by at least `threshold`, 2 meshes whose background is above it. */
+int ndfo_mesh_filter(const double *background, const double *rms, double *out_background, double *out_rms,
+                     size_t ndim, const size_t *mesh_shape, const size_t *filter_size, int32_t edge,
+                     int32_t threshold_kind, double threshold, char *err, size_t err_len);
 
 /* Mask-directed median repair (fixpix(method="median")): outputs whose
  * mask byte is nonzero become the Omit median of their window over the input

@@ -277,3 +277,36 @@ def fixpix_median(
     if status != 0:
         raise OracleError(status, err.value.decode())
     return out
+
+
+MESH_EDGES = ("shrink", "truncate")
+MESH_THRESHOLDS = ("all", "deviation", "above")
+
+
+def mesh_filter(background: np.ndarray, rms: np.ndarray, filter_size: Sequence[int], *, edge: str = "shrink",
+                threshold_kind: str = "all", threshold: float = 0.0) -> tuple[np.ndarray, np.ndarray]:
+    """Median filter of float64 mesh grids (``ndfo_mesh_filter``)."""
+    background = np.ascontiguousarray(background, dtype=np.float64)
+    rms = np.ascontiguousarray(rms, dtype=np.float64)
+    lib = _load()
+    if not hasattr(lib, "_mesh_declared"):
+        lib.ndfo_mesh_filter.argtypes = [
+            ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
+            ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t), ctypes.c_int32, ctypes.c_int32,
+            ctypes.c_double, ctypes.c_char_p, ctypes.c_size_t,
+        ]
+        lib.ndfo_mesh_filter.restype = ctypes.c_int
+        lib._mesh_declared = True
+    ndim = background.ndim
+    out_b = np.empty_like(background)
+    out_r = np.empty_like(rms)
+    err = ctypes.create_string_buffer(256)
+    status = lib.ndfo_mesh_filter(
+        background.ctypes.data, rms.ctypes.data, out_b.ctypes.data, out_r.ctypes.data, ndim,
+        (ctypes.c_size_t * ndim)(*background.shape), (ctypes.c_size_t * ndim)(*filter_size),
+        MESH_EDGES.index(edge), MESH_THRESHOLDS.index(threshold_kind), float(threshold), err, len(err),
+    )
+    if status != 0:
+        raise OracleError(status, err.value.decode())
+    return out_b, out_r
+
