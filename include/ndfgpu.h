@@ -197,6 +197,18 @@ ndf_status ndf_fixpix_median(int32_t dtype_bytes, const void *input, void *outpu
                              const uint8_t *footprint, const ptrdiff_t *origin, const int32_t *modes, double cval,
                              int32_t ignore_inf, size_t min_count, char *err, size_t err_len);
 
+/* ndf_fixpix_median in two halves on a median plan (omit_nan = 1), so a
+ * caller can snapshot under its own lock and release it for the compute, as
+ * the reference does with the GIL (src/pyapi/repair.rs:23-83).
+ * ndf_fixpix_upload copies `len` elements of input and `len` mask bytes to
+ * the device (the mask's nonzero positions are compacted there); the
+ * caller's buffers may change once it returns. ndf_fixpix_compute_download
+ * repairs the uploaded data and writes all `len` elements of the result to
+ * `output`. Blocking. */
+ndf_status ndf_fixpix_upload(ndf_plan *plan, const void *input, const uint8_t *mask, size_t len, char *err,
+                             size_t err_len);
+ndf_status ndf_fixpix_compute_download(ndf_plan *plan, void *output, size_t len, char *err, size_t err_len);
+
 /* Repair paths: SPARSE evaluates only the listed positions (one wave each
  * for windows of up to 4096 taps, else one thread each), DENSE runs the
  * plan's engine over the whole snapshot and keeps the listed outputs, AUTO

@@ -9,7 +9,12 @@ released while the device computes.
 
 from __future__ import annotations
 
+import atexit
 import ctypes
+import json
+import os
+import threading
+from collections import OrderedDict
 from collections.abc import Sequence
 
 import numpy as np
@@ -187,8 +192,9 @@ class Plan:
 
     def __del__(self) -> None:
         handle = getattr(self, "_handle", None)
-        if handle is not None and handle.value:
-            _lib.lib.ndf_plan_destroy(handle)
+        library = getattr(_lib, "lib", None) if _lib is not None else None
+        if handle is not None and handle.value and library is not None:
+            library.ndf_plan_destroy(handle)
             self._handle = None
 
     def set_engine(self, engine: str) -> None:
@@ -316,6 +322,112 @@ class Plan:
         return (ms.value, main.value) if main_kernel else ms.value
 
 
+# ---------------------------------------------------------------------------
+# Plan reuse. A plan owns its device buffers and pinned staging, and planning
+# a large window takes milliseconds, so public calls reuse plans keyed by
+# everything that defines one (the reference plans per call but documents
+# WindowFilter reuse, src/plan.rs:199-204). A plan is taken out of the cache
+# for the duration of a call, so concurrent calls never share one.
+# ---------------------------------------------------------------------------
+
+_CACHE_PLANS = 4
+_CACHE_BYTES = 4 << 30
+
+
+class _PlanCache:
+    def __init__(self) -> None:
+        self._lock = threading.Lock()
+        self._idle: OrderedDict[tuple, list[tuple[Plan, int]]] = OrderedDict()
+
+    def take(self, key: tuple) -> Plan | None:
+        with self._lock:
+            plans = self._idle.get(key)
+            if not plans:
+                return None
+            plan, _ = plans.pop()
+            if not plans:
+                del self._idle[key]
+            return plan
+
+    def give(self, key: tuple, plan: Plan, nbytes: int) -> None:
+        with self._lock:
+            self._idle.setdefault(key, []).append((plan, nbytes))
+            self._idle.move_to_end(key)
+            while self._idle:
+                count = sum(len(v) for v in self._idle.values())
+                total = sum(b for v in self._idle.values() for _, b in v)
+                if count <= _CACHE_PLANS and total <= _CACHE_BYTES:
+                    break
+                oldest = next(iter(self._idle))
+                victims = self._idle[oldest]
+                victims.pop(0)
+                if not victims:
+                    del self._idle[oldest]
+
+    def clear(self) -> None:
+        with self._lock:
+            self._idle.clear()
+
+
+_PLANS = _PlanCache()
+# Plans release their device memory before the interpreter tears modules down.
+atexit.register(lambda: _PLANS.clear())
+
+
+def clear_plan_cache() -> None:
+    """Release the plans (and their device memory) kept for reuse."""
+    _PLANS.clear()
+
+
+# Learned-bracket models shipped with the package, by (dtype, window): a
+# public call whose plan runs the slide engine on such a window uses the
+# model (results are identical with or without it).
+_MODEL_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "models")
+_SHIPPED_MODELS = {
+    ("float32", (63, 63)): "bracket_sky_63x63.json",
+    ("float64", (31, 31)): "bracket_sky_31x31_f64.json",
+}
+_MODEL_WEIGHTS: dict[str, np.ndarray] = {}
+_BRACKET_MIN_PIXELS = 1 << 20
+
+
+def _shipped_model(dtype: np.dtype, size: Sequence[int]) -> np.ndarray | None:
+    name = _SHIPPED_MODELS.get((np.dtype(dtype).name, tuple(int(v) for v in size)))
+    if name is None:
+        return None
+    if name not in _MODEL_WEIGHTS:
+        path = os.path.join(_MODEL_DIR, name)
+        if not os.path.exists(path):
+            return None
+        with open(path) as handle:
+            _MODEL_WEIGHTS[name] = np.asarray(json.load(handle)["weights"], dtype=np.float32)
+    return _MODEL_WEIGHTS[name]
+
+
+def _cached_apply(typed: np.ndarray, kind: str, size, footprint, origin, mode, cval: float, omit_nan: bool,
+                  ignore_inf: bool, min_count: int, parameter) -> np.ndarray:
+    """Run one public call on a cached (or new) plan."""
+    key = (kind, typed.dtype.str, typed.shape, tuple(int(v) for v in size),
+           None if footprint is None else tuple(bool(v) for v in footprint), tuple(int(v) for v in origin),
+           tuple(mode), np.float64(cval).tobytes(), bool(omit_nan), bool(ignore_inf), int(min_count),
+           float(parameter) if kind == "percentile" else int(parameter))
+    plan = _PLANS.take(key)
+    if plan is None:
+        plan = Plan(typed.dtype, typed.shape, kind, size, footprint, origin, mode, cval, omit_nan, ignore_inf,
+                    min_count, parameter)
+        if (footprint is None and typed.ndim == 2 and omit_nan and typed.size >= _BRACKET_MIN_PIXELS
+                and '"engine": "slide"' in plan.describe()):
+            weights = _shipped_model(typed.dtype, size)
+            if weights is not None:
+                try:
+                    plan.set_bracket(weights)
+                except ValueError:
+                    pass
+    output = plan.apply(typed)
+    _PLANS.give(key, plan, 2 * typed.nbytes)
+    return output
+
+
 def window_filter(
     array: np.ndarray,
     kind: str,
@@ -342,21 +454,8 @@ def window_filter(
     typed = _typed_array(array)
     if not typed.flags.c_contiguous:
         raise ValueError("native input must be C-contiguous")
-    plan = Plan(
-        typed.dtype,
-        typed.shape,
-        kind,
-        size,
-        footprint,
-        origin,
-        mode,
-        cval,
-        omit_nan,
-        ignore_inf,
-        min_count,
-        parameter,
-    )
-    return plan.apply(typed)
+    return _cached_apply(typed, kind, size, footprint, origin, mode, cval, omit_nan, ignore_inf, min_count,
+                         parameter)
 
 
 def robust_filter(
@@ -399,9 +498,8 @@ def robust_filter(
     typed = _typed_array(array)
     if not typed.flags.c_contiguous or not typed.flags.aligned:
         raise ValueError("native input must be aligned and C-contiguous")
-    plan = Plan(typed.dtype, typed.shape, kind, size, footprint, origin, mode, cval, omit_nan, ignore_inf, min_count,
-                parameter)
-    return plan.apply(typed)
+    return _cached_apply(typed, kind, size, footprint, origin, mode, cval, omit_nan, ignore_inf, min_count,
+                         parameter)
 
 
 def fixpix_median(
@@ -449,32 +547,30 @@ def fixpix_median(
             raise ValueError(_MSG_SIZE0)
         if len(fp) != volume:
             raise ValueError(_MSG_LENGTH)
-    # Snapshot (GIL held): private C-ordered copies of the data and the mask.
-    data = np.array(typed, dtype=typed.dtype, order="C", copy=True)
-    flags = np.array(bad, dtype=np.uint8, order="C", copy=True)
-    # Every element is written by the native call (copied or repaired).
-    output = np.empty(data.shape, dtype=data.dtype)
+    if fp is not None:
+        fp = fp.astype(bool).tolist()
+    data = typed if typed.flags.c_contiguous else np.ascontiguousarray(typed)
+    flags = bad if bad.flags.c_contiguous else np.ascontiguousarray(bad)
+    if flags.size != data.size:
+        raise ValueError(_MSG_LENGTH)
+    key = ("fixpix", data.dtype.str, data.shape, tuple(size), None if fp is None else tuple(fp), tuple(origin),
+           tuple(mode), np.float64(cval).tobytes(), bool(ignore_inf), int(min_count))
+    plan = _PLANS.take(key)
+    if plan is None:
+        plan = Plan(data.dtype, data.shape, "median", size, fp, origin, mode, cval, True, ignore_inf, min_count, 0)
     err = ctypes.create_string_buffer(256)
-    status = _lib.lib.ndf_fixpix_median(
-        data.dtype.itemsize,
-        data.ctypes.data,
-        output.ctypes.data,
-        flags.ctypes.data,
-        flags.size,
-        ndim,
-        (ctypes.c_size_t * ndim)(*data.shape),
-        (ctypes.c_size_t * ndim)(*size),
-        None if fp is None else fp.ctypes.data,
-        (ctypes.c_ssize_t * ndim)(*origin),
-        (ctypes.c_int32 * ndim)(*modes),
-        float(cval),
-        int(bool(ignore_inf)),
-        min_count,
-        err,
-        len(err),
-    )
+    # Snapshot (GIL held): the data and the mask go to the device; the mask's
+    # positions are compacted there.
+    status = _lib.lib_gil.ndf_fixpix_upload(plan._handle, data.ctypes.data, flags.ctypes.data, plan.len, err,
+                                            len(err))
     if status != _lib.OK:
         _raise(status, err.value.decode())
+    # Every element is written by the native call (copied or repaired).
+    output = np.empty(data.shape, dtype=data.dtype)
+    status = _lib.lib.ndf_fixpix_compute_download(plan._handle, output.ctypes.data, plan.len, err, len(err))
+    if status != _lib.OK:
+        _raise(status, err.value.decode())
+    _PLANS.give(key, plan, 2 * data.nbytes)
     return output
 
 

@@ -242,6 +242,8 @@ def _declare(lib: ctypes.CDLL) -> None:
         ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_size_t,
         _size_p, _size_p, ctypes.c_void_p, _ssize_p, _i32_p, ctypes.c_double, ctypes.c_int32, ctypes.c_size_t, *_err,
     ]
+    lib.ndf_fixpix_upload.argtypes = [plan_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, *_err]
+    lib.ndf_fixpix_compute_download.argtypes = [plan_p, ctypes.c_void_p, ctypes.c_size_t, *_err]
     lib.ndf_plan_repair_device.argtypes = [
         plan_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
         ctypes.c_size_t, ctypes.c_int32, ctypes.c_void_p, *_err,
@@ -252,6 +254,8 @@ def _declare(lib: ctypes.CDLL) -> None:
     ]
     for name in (
         "ndf_fixpix_median",
+        "ndf_fixpix_upload",
+        "ndf_fixpix_compute_download",
         "ndf_plan_repair_device",
         "ndf_plan_time_repair_device",
         "ndf_batch_create",

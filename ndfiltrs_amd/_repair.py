@@ -120,7 +120,10 @@ def fixpix(
             raise TypeError("mask must have a boolean or numeric dtype")
     if bad is None or not np.any(bad):
         return value.copy(order="C") if copy else value
-    bad = np.ascontiguousarray(bad != 0, dtype=np.bool_).view(np.uint8)
+    if bad.dtype == np.bool_ and bad.flags.c_contiguous:
+        bad = bad.view(np.uint8)  # one byte per element already, 0 or 1
+    else:
+        bad = np.ascontiguousarray(bad != 0, dtype=np.bool_).view(np.uint8)
     return _core.fixpix_median(
         arguments.array,
         bad,

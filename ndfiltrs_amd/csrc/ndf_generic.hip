@@ -331,6 +331,76 @@ __global__ void __launch_bounds__(64 * REPAIR_WAVES) k_repair_wave(GenericArgs a
     }
 }
 
+// Mask bytes a thread reads per step (one 16-byte load for full chunks).
+constexpr int MASK_CHUNK = 16;
+
+__device__ inline int chunk_flags(const uint8_t *mask, int64_t len, int64_t chunk, uint32_t &bits)
+{
+    const int64_t begin = chunk * MASK_CHUNK;
+    bits = 0;
+    if (begin + MASK_CHUNK <= len) {
+        const uint4 v = *reinterpret_cast<const uint4 *>(mask + begin);
+        const uint32_t words[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int w = 0; w < 4; ++w) {
+#pragma unroll
+            for (int b = 0; b < 4; ++b) bits |= (uint32_t)(((words[w] >> (8 * b)) & 0xFFu) != 0) << (4 * w + b);
+        }
+    } else {
+        for (int b = 0; begin + b < len; ++b) bits |= (uint32_t)(mask[begin + b] != 0) << b;
+    }
+    return __popc(bits);
+}
+
+// Number of nonzero mask bytes (one atomic per block).
+__global__ void __launch_bounds__(256) k_mask_count(const uint8_t *mask, int64_t len, unsigned long long *count)
+{
+    __shared__ unsigned long long partial[4];
+    const int64_t chunks = (len + MASK_CHUNK - 1) / MASK_CHUNK;
+    const int64_t step = (int64_t)gridDim.x * blockDim.x;
+    unsigned long long n = 0;
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < chunks; c += step) {
+        uint32_t bits;
+        n += (unsigned long long)chunk_flags(mask, len, c, bits);
+    }
+    for (int off = 32; off > 0; off >>= 1) n += __shfl_down(n, off);
+    if ((threadIdx.x & 63) == 0) partial[threadIdx.x >> 6] = n;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(count, partial[0] + partial[1] + partial[2] + partial[3]);
+}
+
+// Flat indices of the nonzero mask bytes (positions() of src/repair.rs:14-
+// 29, in an order the repair does not depend on): each wave reserves room
+// for its chunks with one atomic, each thread writes its chunk's indices.
+__global__ void __launch_bounds__(256) k_mask_positions(const uint8_t *mask, int64_t len, int64_t *positions,
+                                                        unsigned long long *cursor)
+{
+    const int64_t chunks = (len + MASK_CHUNK - 1) / MASK_CHUNK;
+    const int lane = threadIdx.x & 63;
+    const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
+    const int64_t wave = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    for (int64_t c0 = wave * 64; c0 < chunks; c0 += waves * 64) {
+        const int64_t c = c0 + lane;
+        uint32_t bits = 0;
+        const int n = c < chunks ? chunk_flags(mask, len, c, bits) : 0;
+        int prefix = n;
+        for (int off = 1; off < 64; off <<= 1) {
+            const int v = __shfl_up(prefix, off);
+            if (lane >= off) prefix += v;
+        }
+        const int total = __shfl(prefix, 63);
+        unsigned long long base = 0;
+        if (lane == 63 && total > 0) base = atomicAdd(cursor, (unsigned long long)total);
+        base = __shfl(base, 63);
+        int64_t at = (int64_t)base + prefix - n;
+        while (bits) {
+            const int b = __builtin_ctz(bits);
+            bits &= bits - 1;
+            positions[at++] = c * MASK_CHUNK + b;
+        }
+    }
+}
+
 template <typename Key>
 __global__ void __launch_bounds__(256) k_fill(Key *out, int64_t len, Key bits)
 {
@@ -427,6 +497,22 @@ hipError_t launch_repair_wave(int dtype_bytes, const GenericArgs &args, hipStrea
                                        (int)lds);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(fn, dim3((unsigned)blocks), dim3(64 * REPAIR_WAVES), lds, stream, args);
+    return hipGetLastError();
+}
+
+hipError_t launch_mask_count(const uint8_t *mask, int64_t len, unsigned long long *count, hipStream_t stream)
+{
+    const int64_t chunks = (len + MASK_CHUNK - 1) / MASK_CHUNK;
+    hipLaunchKernelGGL(k_mask_count, dim3((unsigned)grid_for(chunks)), dim3(256), 0, stream, mask, len, count);
+    return hipGetLastError();
+}
+
+hipError_t launch_mask_positions(const uint8_t *mask, int64_t len, int64_t *positions, unsigned long long *cursor,
+                                 hipStream_t stream)
+{
+    const int64_t chunks = (len + MASK_CHUNK - 1) / MASK_CHUNK;
+    hipLaunchKernelGGL(k_mask_positions, dim3((unsigned)grid_for(chunks)), dim3(256), 0, stream, mask, len,
+                       positions, cursor);
     return hipGetLastError();
 }
 

@@ -250,6 +250,10 @@ struct ndf_plan {
     bool uploaded = false;
     DeviceBuffer g_maps, g_taps, s_mapz, s_mapy, s_mapx, s_taps, s_rm, s_add, s_mask, d_in, d_out, d_error;
     DeviceBuffer r_flag, r_snap, r_pos;  // mask-directed repair: position flag, snapshot, positions
+    DeviceBuffer r_mask, r_count;        // host-buffer repair: uploaded mask, position counter
+    size_t fix_npos = 0;                 // positions of the uploaded mask
+    bool fix_uploaded = false;
+    Staging staging;                     // pinned buffers of the host-buffer calls
     DeviceBuffer n_mapz, n_mapy, n_mapx, n_taps;
     GenericArgs generic{};
     // Host copies of the generic engine's tables (maps, tap offsets).
@@ -266,7 +270,7 @@ struct ndf_plan {
             b->release();
         }
         for (DeviceBuffer *b : {&g_maps, &g_taps, &s_mapz, &s_mapy, &s_mapx, &s_taps, &s_rm, &s_add, &s_mask, &d_in,
-                                &d_out, &d_error, &r_flag, &r_snap, &r_pos, &b_weights, &b_blocks, &b_features, &b_moments, &b_lo, &b_hi,
+                                &d_out, &d_error, &r_flag, &r_snap, &r_pos, &r_mask, &r_count, &b_weights, &b_blocks, &b_features, &b_moments, &b_lo, &b_hi,
                                 &b_pred, &b_fallback, &b_fcount}) {
             b->release();
         }
@@ -1349,7 +1353,7 @@ ndf_status upload_locked(ndf_plan *plan, const void *input, char *err, size_t er
     if ((e = plan->d_in.ensure(bytes)) != hipSuccess || (e = plan->d_out.ensure(bytes)) != hipSuccess) {
         return fail_hip(e, err, err_len);
     }
-    if ((e = hipMemcpy(plan->d_in.ptr, input, bytes, hipMemcpyHostToDevice)) != hipSuccess) {
+    if ((e = staged_upload(plan->staging, plan->d_in.ptr, input, bytes, nullptr)) != hipSuccess) {
         return fail_hip(e, err, err_len);
     }
     plan->uploaded = true;
@@ -1369,7 +1373,7 @@ ndf_status compute_download_locked(ndf_plan *plan, void *output, char *err, size
     }
     ndf_status st = launch(plan, plan->d_in.ptr, plan->d_out.ptr, nullptr, err, err_len);
     if (st != NDF_OK) return st;
-    if ((e = hipMemcpy(output, plan->d_out.ptr, bytes, hipMemcpyDeviceToHost)) != hipSuccess) {
+    if ((e = staged_download(plan->staging, output, plan->d_out.ptr, bytes, nullptr)) != hipSuccess) {
         return fail_hip(e, err, err_len);
     }
     return read_error_flag(plan, err, err_len);
@@ -2322,6 +2326,95 @@ ndf_status ndf_plan_time_repair_device(ndf_plan *plan, const void *d_input, void
     return st;
 }
 
+namespace {
+
+// Snapshot half of the host-buffer repair: the data and the mask go to the
+// device, and the mask's flat indices are compacted there. The caller's
+// buffers are free to change once it returns.
+ndf_status fixpix_upload_locked(ndf_plan *p, const void *input, const uint8_t *mask, char *err, size_t err_len)
+{
+    const size_t bytes = p->len * (size_t)p->dbytes;
+    hipError_t e;
+    p->fix_uploaded = false;
+    if ((e = p->d_in.ensure(bytes)) != hipSuccess || (e = p->d_out.ensure(bytes)) != hipSuccess ||
+        (e = p->r_mask.ensure(p->len)) != hipSuccess ||
+        (e = p->r_count.ensure(2 * sizeof(unsigned long long))) != hipSuccess) {
+        return fail_hip(e, err, err_len);
+    }
+    if ((e = staged_upload(p->staging, p->d_in.ptr, input, bytes, nullptr)) != hipSuccess ||
+        (e = staged_upload(p->staging, p->r_mask.ptr, mask, p->len, nullptr)) != hipSuccess) {
+        return fail_hip(e, err, err_len);
+    }
+    unsigned long long *counter = static_cast<unsigned long long *>(p->r_count.ptr);
+    unsigned long long count = 0;
+    if ((e = launch_fill(8, counter, 2, 0, nullptr)) != hipSuccess ||
+        (e = launch_mask_count(static_cast<const uint8_t *>(p->r_mask.ptr), (int64_t)p->len, counter, nullptr)) !=
+            hipSuccess ||
+        (e = hipMemcpy(&count, counter, sizeof(count), hipMemcpyDeviceToHost)) != hipSuccess) {
+        return fail_hip(e, err, err_len);
+    }
+    p->fix_npos = (size_t)count;
+    if (count > 0) {
+        if ((e = p->r_pos.ensure((size_t)count * sizeof(int64_t))) != hipSuccess ||
+            (e = launch_mask_positions(static_cast<const uint8_t *>(p->r_mask.ptr), (int64_t)p->len,
+                                       static_cast<int64_t *>(p->r_pos.ptr), counter + 1, nullptr)) != hipSuccess) {
+            return fail_hip(e, err, err_len);
+        }
+    }
+    p->fix_uploaded = true;
+    return NDF_OK;
+}
+
+// Compute half: repair the uploaded data at the mask's positions and copy
+// the result into `output`.
+ndf_status fixpix_compute_locked(ndf_plan *p, void *output, char *err, size_t err_len)
+{
+    if (!p->fix_uploaded) return fail(NDF_ERR_INVALID, "no input has been uploaded to this plan", err, err_len);
+    p->fix_uploaded = false;
+    const size_t bytes = p->len * (size_t)p->dbytes;
+    const void *result = p->d_in.ptr;
+    hipError_t e;
+    if (p->fix_npos > 0) {
+        if ((e = p->r_snap.ensure(bytes)) != hipSuccess) return fail_hip(e, err, err_len);
+        ndf_status st = repair_buffers(p, err, err_len);
+        if (st == NDF_OK) {
+            st = repair_launch(p, p->d_in.ptr, p->d_out.ptr, p->r_snap.ptr, static_cast<const int64_t *>(p->r_pos.ptr),
+                               p->fix_npos, NDF_REPAIR_AUTO, nullptr, err, err_len);
+        }
+        if (st == NDF_OK) st = repair_finish(p, nullptr, err, err_len);
+        if (st != NDF_OK) return st;
+        result = p->d_out.ptr;
+    }
+    if ((e = staged_download(p->staging, output, result, bytes, nullptr)) != hipSuccess) return fail_hip(e, err, err_len);
+    return NDF_OK;
+}
+
+}  // namespace
+
+ndf_status ndf_fixpix_upload(ndf_plan *plan, const void *input, const uint8_t *mask, size_t len, char *err,
+                             size_t err_len)
+{
+    if (len != plan->len) return fail_code(NDF_ERR_LENGTH_MISMATCH, err, err_len);
+    std::lock_guard<std::mutex> guard(plan->mutex);
+    if (len == 0) {
+        plan->fix_uploaded = true;
+        plan->fix_npos = 0;
+        return NDF_OK;
+    }
+    return fixpix_upload_locked(plan, input, mask, err, err_len);
+}
+
+ndf_status ndf_fixpix_compute_download(ndf_plan *plan, void *output, size_t len, char *err, size_t err_len)
+{
+    if (len != plan->len) return fail_code(NDF_ERR_LENGTH_MISMATCH, err, err_len);
+    std::lock_guard<std::mutex> guard(plan->mutex);
+    if (len == 0) {
+        plan->fix_uploaded = false;
+        return NDF_OK;
+    }
+    return fixpix_compute_locked(plan, output, err, err_len);
+}
+
 ndf_status ndf_fixpix_median(int32_t dtype_bytes, const void *input, void *output, const uint8_t *mask,
                              size_t mask_len, size_t rank, const size_t *shape, const size_t *size,
                              const uint8_t *footprint, const ptrdiff_t *origin, const int32_t *modes, double cval,
@@ -2334,60 +2427,9 @@ ndf_status ndf_fixpix_median(int32_t dtype_bytes, const void *input, void *outpu
     std::unique_ptr<ndf_plan> owner(plan);
     // positions() in src/repair.rs:14-29: the mask must cover the array.
     if (mask_len != plan->len) return fail_code(NDF_ERR_LENGTH_MISMATCH, err, err_len);
-    const size_t bytes = plan->len * (size_t)plan->dbytes;
-    // Masks are mostly zero: both scans skip 8-byte words with no set byte.
-    std::vector<int64_t> positions;
-    size_t npos = 0;
-    const size_t words = plan->len / 8;
-    for (size_t w = 0; w < words; ++w) {
-        uint64_t word;
-        memcpy(&word, mask + 8 * w, 8);
-        if (!word) continue;
-        for (int b = 0; b < 8; ++b) npos += mask[8 * w + b] != 0;
-    }
-    for (size_t i = 8 * words; i < plan->len; ++i) npos += mask[i] != 0;
-    try {
-        positions.reserve(npos);
-    } catch (const std::bad_alloc &) {
-        return fail_code(NDF_ERR_MEMORY, err, err_len);
-    }
-    for (size_t w = 0; w < words; ++w) {
-        uint64_t word;
-        memcpy(&word, mask + 8 * w, 8);
-        if (!word) continue;
-        for (int b = 0; b < 8; ++b) {
-            if (mask[8 * w + b]) positions.push_back((int64_t)(8 * w + b));
-        }
-    }
-    for (size_t i = 8 * words; i < plan->len; ++i) {
-        if (mask[i]) positions.push_back((int64_t)i);
-    }
-    if (npos == 0) {
-        if (bytes) memcpy(output, input, bytes);
-        return NDF_OK;
-    }
-    std::lock_guard<std::mutex> guard(plan->mutex);
-    hipError_t e;
-    if ((e = plan->d_in.ensure(bytes)) != hipSuccess || (e = plan->d_out.ensure(bytes)) != hipSuccess ||
-        (e = plan->r_snap.ensure(bytes)) != hipSuccess) {
-        return fail_hip(e, err, err_len);
-    }
-    if ((e = upload(plan->r_pos, positions)) != hipSuccess ||
-        (e = hipMemcpy(plan->d_in.ptr, input, bytes, hipMemcpyHostToDevice)) != hipSuccess) {
-        return fail_hip(e, err, err_len);
-    }
-    st = repair_buffers(plan, err, err_len);
-    if (st == NDF_OK) {
-        st = repair_launch(plan, plan->d_in.ptr, plan->d_out.ptr, plan->r_snap.ptr,
-                           static_cast<const int64_t *>(plan->r_pos.ptr), npos, NDF_REPAIR_AUTO, nullptr, err,
-                           err_len);
-    }
-    if (st == NDF_OK) st = repair_finish(plan, nullptr, err, err_len);
-    if (st != NDF_OK) return st;
-    if ((e = hipMemcpy(output, plan->d_out.ptr, bytes, hipMemcpyDeviceToHost)) != hipSuccess) {
-        return fail_hip(e, err, err_len);
-    }
-    return NDF_OK;
+    st = ndf_fixpix_upload(plan, input, mask, plan->len, err, err_len);
+    if (st == NDF_OK) st = ndf_fixpix_compute_download(plan, output, plan->len, err, err_len);
+    return st;
 }
 
 }  // extern "C"

@@ -163,6 +163,12 @@ size_t repair_wave_lds_bytes(int dtype_bytes, int count);
 hipError_t launch_repair_wave(int dtype_bytes, const GenericArgs &args, hipStream_t stream);
 hipError_t launch_slide(int dtype_bytes, const SlideArgs &args, size_t lds_bytes, int max_tiles, hipStream_t stream);
 hipError_t launch_fill(int dtype_bytes, void *output, int64_t len, uint64_t bits, hipStream_t stream);
+// Nonzero bytes of a mask (added to *count), and their flat indices
+// (written from *cursor on, in no particular order). The mask is 16-byte
+// aligned.
+hipError_t launch_mask_count(const uint8_t *mask, int64_t len, unsigned long long *count, hipStream_t stream);
+hipError_t launch_mask_positions(const uint8_t *mask, int64_t len, int64_t *positions, unsigned long long *cursor,
+                                 hipStream_t stream);
 // out[positions[i]] = bits for i < n; positions outside [0, extent) are
 // skipped and set *error_flag.
 hipError_t launch_scatter_fill(int dtype_bytes, void *output, const int64_t *positions, int64_t n, int64_t extent,
@@ -185,5 +191,22 @@ hipError_t slide_batch_grid(int dtype_bytes, int sort_e, size_t lds_bytes, int *
 hipError_t launch_slide_batch(int dtype_bytes, int sort_e, const SlideArgs *images, const int64_t *offsets,
                               const int32_t *work, int nwork, int32_t *counter, const void *input, void *output,
                               int32_t *error_flag, size_t lds_bytes, int grid, hipStream_t stream);
+
+// Pinned double buffers for large host <-> device copies (ndf_transfer.hip).
+constexpr size_t STAGING_CHUNK = (size_t)16 << 20;
+constexpr size_t STAGING_MIN_BYTES = (size_t)8 << 20;
+struct Staging {
+    void *pinned[2] = {nullptr, nullptr};
+    hipEvent_t done[2] = {nullptr, nullptr};
+    Staging() = default;
+    Staging(const Staging &) = delete;
+    Staging &operator=(const Staging &) = delete;
+    ~Staging();
+    hipError_t ensure();
+};
+// Copies host -> device; returns once the host buffer may change.
+hipError_t staged_upload(Staging &s, void *d_dst, const void *h_src, size_t bytes, hipStream_t stream);
+// Copies device -> host after the work already enqueued on `stream`; blocking.
+hipError_t staged_download(Staging &s, void *h_dst, const void *d_src, size_t bytes, hipStream_t stream);
 
 }  // namespace ndf
