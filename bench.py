@@ -411,6 +411,45 @@ def load_traffic() -> dict:
     return {}
 
 
+def ring_footprint() -> np.ndarray:
+    yy, xx = np.mgrid[-6:7, -6:7]
+    return ((yy * yy + xx * xx) >= 16) & ((yy * yy + xx * xx) <= 36)
+
+
+# Single workloads for profiler runs: (image factory, plan arguments, model).
+PROFILE_WORKLOADS = {
+    "c1_5x5": ("sky_small", dict(kind="median", size=(5, 5), mode="reflect"), None),
+    "c3a_annulus": ("sky_annulus", dict(kind="median", size=None, mode="wrap", footprint="ring", origin=(1, 0),
+                                        ignore_inf=True), None),
+    "c3b_cube": ("cube", dict(kind="median", size=(3, 7, 7), mode="wrap", ignore_inf=True), None),
+    "c5_median_plain": ("sky_quadratic", dict(kind="median", size=(63, 63), mode="reflect"), None),
+    "c5_median_bracket": ("sky_quadratic", dict(kind="median", size=(63, 63), mode="reflect"), "sky63"),
+    "c2_median_plain": ("sky_gradient_f64", dict(kind="median", size=(31, 31), mode="nearest"), None),
+    "c2_median_bracket": ("sky_gradient_f64", dict(kind="median", size=(31, 31), mode="nearest"), "sky31_f64"),
+    "min63": ("sky_quadratic", dict(kind="min", size=(63, 63), mode="reflect"), None),
+    "min5": ("sky_quadratic", dict(kind="min", size=(5, 5), mode="reflect"), None),
+}
+
+
+def profile_workload(name: str, steps: int, warmup: int) -> dict:
+    """Run one workload's launch `warmup + steps` times (no checks): the
+    command a profiler wraps."""
+    from ndfiltrs_amd import bracket, synthetic
+
+    factory, kwargs, model = PROFILE_WORKLOADS[name]
+    kwargs = dict(kwargs)
+    if kwargs.get("footprint") == "ring":
+        kwargs["footprint"] = ring_footprint()
+    work = Workload(getattr(synthetic, factory)())
+    plan = work.plan(kwargs.pop("kind"), kwargs.pop("size"), kwargs.pop("mode"), **kwargs)
+    if model:
+        plan.set_bracket(bracket.load(bracket.model_path(model))[0], 4096)
+    result = work.run(plan, steps, warmup)
+    result.update(workload=name, engine=json.loads(plan.describe()))
+    work.free()
+    return result
+
+
 def main() -> int:
     parser = argparse.ArgumentParser()
     parser.add_argument("--gpus", type=int, default=1)
@@ -419,6 +458,8 @@ def main() -> int:
     parser.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     parser.add_argument("--quick", action="store_true", help="only the 63x63 lines")
     parser.add_argument("--fixpix", action="store_true", help="only the fixpix median repair lines")
+    parser.add_argument("--workload", choices=sorted(PROFILE_WORKLOADS),
+                        help="only time this workload's launch (for profiler runs) and exit")
     parser.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                         help="only copy MIB MiB device-to-device 3 times (HBM counter calibration) and exit")
     args = parser.parse_args()
@@ -443,6 +484,9 @@ def main() -> int:
         print(json.dumps({"copy_calibration_bytes": nbytes, "copies": 3}), flush=True)
         return 0
 
+    if args.workload:
+        print(json.dumps(profile_workload(args.workload, args.steps, args.warmup)), flush=True)
+        return 0
     if args.fixpix:
         log("fixpix median on the 8192x8192 sky: sparse and dense repair paths")
         print(json.dumps(repair_bench(synthetic.sky_quadratic(), args.steps, args.warmup, check=not args.no_cpu)),

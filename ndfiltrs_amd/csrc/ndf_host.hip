@@ -1286,10 +1286,12 @@ ndf_status ndf_plan_describe(const ndf_plan *plan, char *buffer, size_t buffer_l
     } else if (plan->engine == NDF_ENGINE_SLIDE) {
         snprintf(buffer, buffer_len,
                  "{\"engine\": \"%s\", \"count\": %zu, \"tile\": [%d, %d], \"support\": %d, \"sort_len\": %d, "
-                 "\"lds_bytes\": %zu, \"edge_taps\": %d, \"tiles\": %lld, \"bracket_tile_rows\": %d}",
+                 "\"lds_bytes\": %zu, \"edge_taps\": %d, \"tiles\": %lld, \"bracket_tile_rows\": %d, "
+                 "\"bracket\": %s}",
                  engine, plan->count, plan->slide.ty, SLIDE_TILE_X, plan->slide.s, 256 * plan->slide.sort_e, plan->slide_lds,
                  plan->slide.nedge,
-                 (long long)plan->slide.tiles_x * plan->slide.tiles_y * plan->slide.nz, plan->bracket_ty);
+                 (long long)plan->slide.tiles_x * plan->slide.tiles_y * plan->slide.nz, plan->bracket_ty,
+                 plan->bracket_on ? "true" : "false");
     } else {
         snprintf(buffer, buffer_len, "{\"engine\": \"%s\", \"count\": %zu}", engine, plan->count);
     }

@@ -198,3 +198,37 @@ def test_gold_literal_windows() -> None:
                                       omit_nan=omit, ignore_inf=False, min_count=1)
             got = nf.nanmedian_filter(array) if omit else nf.median_filter(array)
             assert_bits(got, want, f"gold window {index}")
+
+
+@pytest.mark.parametrize("function,kind,extra", [("nanmedian_filter", "median", {}),
+                                                 ("nanpercentile_filter", "percentile", {"percentile": 10.0})])
+def test_public_call_with_the_shipped_bracket_matches_oracle(function: str, kind: str, extra: dict) -> None:
+    """The drop-in call on a 63x63 float32 window of at least 1024^2 pixels
+    runs the shipped learned bracket (cached plan) and still gives the
+    oracle's bits; a second call with other data reuses the plan."""
+    _core.clear_plan_cache()
+    for seed in (109, 110):
+        image = synthetic.sky_quadratic_crop((1024, 1088), seed=seed)
+        if kind == "median":
+            got = nf.nanmedian_filter(image, 63)
+        else:
+            got = nf.nanpercentile_filter(image, extra["percentile"], 63)
+        expected = oracle.window_filter(image, (63, 63), mode="reflect", statistic=kind, q=extra.get("percentile", 0),
+                                        omit_nan=True, method="histogram")
+        assert_bits(got, expected, f"public {function} seed {seed}")
+    plans = [plan for entries in _core._PLANS._idle.values() for plan, _ in entries]
+    assert len(plans) == 1 and '"bracket": true' in plans[0].describe(), [p.describe() for p in plans]
+
+
+def test_public_fixpix_reuses_plans_and_matches_oracle() -> None:
+    _core.clear_plan_cache()
+    rng = np.random.default_rng(111)
+    for seed in (112, 113):
+        image = synthetic.sky_small((300, 420), seed=seed)
+        mask = rng.random(image.shape) < 0.02
+        got = nf.fixpix(image, mask, method="median", size=5)
+        want = oracle.fixpix_median(image, mask, (5, 5), mode="reflect")
+        assert_bits(got, want, f"fixpix seed {seed}")
+    # A mask given as an integer array takes the converting path.
+    got = nf.fixpix(image, mask.astype(np.int32) * 7, method="median", size=5)
+    assert_bits(got, want, "fixpix integer mask")
