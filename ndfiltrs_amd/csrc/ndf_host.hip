@@ -320,7 +320,10 @@ bool plan_network(ndf_plan *p)
     const int64_t sy = ty + ky - 1;
     const int64_t rows = kz * sy;
     const int64_t s = rows * sx;
-    const size_t lds = network_lds_bytes(p->dbytes, (int)s, (int)p->count);
+    // Sentinel run read by the taps past the count: window bases span
+    // (ty - 1) * sx + 64 keys.
+    const int64_t pad_keys = (int64_t)p->count < live ? (ty - 1) * sx + 64 : 0;
+    const size_t lds = network_lds_bytes(p->dbytes, (int)s, (int)pad_keys, live);
     if (lds > (size_t)lds_limit() || s > (1 << 24)) return false;
     const int64_t tiles_x = (dims[2] + 63) / 64;
     const int64_t tiles_y = (dims[1] + ty - 1) / ty;
@@ -343,6 +346,9 @@ bool plan_network(ndf_plan *p)
         for (size_t a = 0; a < p->rank; ++a) d[pad + a] = p->taps[t * p->rank + a] - p->low[a];
         p->net_taps.push_back((uint32_t)((d[0] * sy + d[1]) * sx + d[2]));
     }
+    // The kernels read `live` taps; those past the count read the sentinel
+    // run that starts at offset s.
+    p->net_taps.resize((size_t)live, (uint32_t)s);
     NetworkArgs &n = p->network;
     n = NetworkArgs{};
     n.nz = dims[0];
@@ -357,6 +363,7 @@ bool plan_network(ndf_plan *p)
     n.tiles_y = (int32_t)tiles_y;
     n.count = (int32_t)p->count;
     n.live = live;
+    n.pad = (int32_t)pad_keys;
     n.cval_bits = p->cval_bits;
     n.ignore_inf = p->eff_ignore_inf;
     n.stat = p->stat;

@@ -108,6 +108,7 @@ struct NetworkArgs {
     int32_t sx, sy, rows, s;
     int32_t ty, tiles_x, tiles_y;
     int32_t count, live;
+    int32_t pad;  // sentinel keys after the support (taps past count read there)
     const uint32_t *taps;
     uint64_t cval_bits;
     int32_t ignore_inf;
@@ -141,7 +142,7 @@ hipError_t launch_extrema(int dtype_bytes, const ExtremaArgs &args, size_t lds_b
 
 // Network size (live wires) for a window of `count` taps, 0 if too large.
 int network_live(int dtype_bytes, int count);
-size_t network_lds_bytes(int dtype_bytes, int s, int count);
+size_t network_lds_bytes(int dtype_bytes, int s, int pad, int live);
 hipError_t launch_network(int dtype_bytes, const NetworkArgs &args, size_t lds_bytes, hipStream_t stream);
 // Batched network kernel for images whose windows have at most `live` taps:
 // `images` one NetworkArgs per image (input/output unused; taps in global

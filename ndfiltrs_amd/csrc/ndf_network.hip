@@ -14,7 +14,7 @@
 //     live wires (comparators into sentinel-only wires are no-ops);
 //   * sentinels sort last, so the retained count n is the number of
 //     non-sentinel keys and the statistic is read at a data-dependent index
-//     with a select chain.
+//     with a binary tree of selects.
 // The result is the exact sample (or the f64 midpoint of two), bit-identical
 // to the slide and generic engines.
 #include <array>
@@ -113,6 +113,40 @@ __device__ __forceinline__ void sort_keys(Key (&k)[LIVE])
     }
 }
 
+// a when take_a (selector 0x07060504: every byte from the first operand),
+// else b (0x03020100). A byte permute instead of a conditional keeps the
+// compiler from merging the two selected operands into one indexed access.
+__device__ __forceinline__ uint32_t pick_word(uint32_t a, uint32_t b, uint32_t selector)
+{
+    return __builtin_amdgcn_perm(a, b, selector);
+}
+__device__ __forceinline__ uint64_t pick_word(uint64_t a, uint64_t b, uint32_t selector)
+{
+    const uint32_t lo = __builtin_amdgcn_perm((uint32_t)a, (uint32_t)b, selector);
+    const uint32_t hi = __builtin_amdgcn_perm((uint32_t)(a >> 32), (uint32_t)(b >> 32), selector);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// k[index] for a per-lane index below N: a binary tree of selects on the
+// index bits, lowest first (N - 1 selects instead of the N compares and N
+// selects of a linear scan).
+template <typename Key, int N>
+__device__ __forceinline__ Key select_index(const Key (&k)[N], uint32_t index)
+{
+    if constexpr (N == 1) {
+        return k[0];
+    } else {
+        constexpr int M = (N + 1) / 2;
+        Key t[M];
+        const uint32_t selector = (index & 1u) != 0 ? 0x07060504u : 0x03020100u;
+#pragma unroll
+        for (int j = 0; j < M; ++j) {
+            t[j] = 2 * j + 1 < N ? pick_word(k[2 * j + 1 < N ? 2 * j + 1 : 2 * j], k[2 * j], selector) : k[2 * j];
+        }
+        return select_index<Key, M>(t, index >> 1);
+    }
+}
+
 // One tile: stage the support (classified keys) into LDS, then one output
 // per thread and row step. `taps` holds the window's support offsets.
 template <typename T, int LIVE>
@@ -148,36 +182,40 @@ __device__ __forceinline__ void network_tile(const NetworkArgs &a, int tile,
             support[row * a.sx + x] = pad ? KT::SENTINEL : (cval ? cval_key : classify<T>(raw, a.ignore_inf));
         }
     }
+    // Taps past the window's count read offset a.s: every window base
+    // (r * sx + lane, r < ty, lane < 64) lands in this sentinel run.
+    for (int i = tid; i < a.pad; i += NETWORK_THREADS) {
+        support[a.s + i] = KT::SENTINEL;
+    }
     __syncthreads();
     const int64_t x = x0 + lane;
+    const bool may_pair = a.stat != STAT_RANK;
     for (int r = wave; r < a.ty; r += 4) {
         const int64_t y = y0 + r;
         if (y >= a.ny || x >= a.nx) {
             continue;
         }
+        // Every tap table holds LIVE entries (those past the window's count
+        // point into the sentinel run), so the loads are unconditional.
         Key k[LIVE];
         const Key *window = support + r * a.sx + lane;
-#pragma unroll
-        for (int i = 0; i < LIVE; ++i) {
-            k[i] = i < a.count ? window[taps[i]] : KT::SENTINEL;
-        }
-        sort_keys<Key, LIVE>(k);
         int n = 0;
 #pragma unroll
         for (int i = 0; i < LIVE; ++i) {
+            k[i] = window[taps[i]];
             n += k[i] != KT::SENTINEL;
         }
+        sort_keys<Key, LIVE>(k);
         Key out = KT::CANONICAL_NAN;
         Pick pick;
-        if (pick_ranks(a.stat, n, a.min_count, a.rank_param, a.q100, pick)) {
-            const int hi_index = (int)pick.hi;
-            const int lo_index = pick.pair ? hi_index - 1 : hi_index;
-            Key lo = 0, hi = 0;
-#pragma unroll
-            for (int i = 0; i < LIVE; ++i) {
-                lo = i == lo_index ? k[i] : lo;
-                hi = i == hi_index ? k[i] : hi;
-            }
+        const bool ok = pick_ranks(a.stat, n, a.min_count, a.rank_param, a.q100, pick);
+        const uint32_t hi_index = ok ? (uint32_t)pick.hi : 0u;
+        const Key hi = select_index<Key, LIVE>(k, hi_index);
+        Key lo = hi;
+        if (may_pair) {
+            lo = select_index<Key, LIVE>(k, ok && pick.pair ? hi_index - 1u : hi_index);
+        }
+        if (ok) {
             out = pick.pair ? pair_bits<T>(a.stat, lo, hi, pick.frac) : KT::unkey(hi);
         }
         output[(z0 * a.ny + y) * a.nx + x] = out;
@@ -191,8 +229,8 @@ __global__ void __launch_bounds__(NETWORK_THREADS) k_network(NetworkArgs a)
     using Key = typename KeyTraits<T>::Key;
     extern __shared__ __align__(16) unsigned char lds[];
     Key *support = reinterpret_cast<Key *>(lds);
-    uint32_t *taps = reinterpret_cast<uint32_t *>(support + a.s);
-    for (int t = threadIdx.x; t < a.count; t += NETWORK_THREADS) {
+    uint32_t *taps = reinterpret_cast<uint32_t *>(support + a.s + a.pad);
+    for (int t = threadIdx.x; t < LIVE; t += NETWORK_THREADS) {
         taps[t] = a.taps[t];
     }
     const int tiles = a.tiles_x * a.tiles_y * (int)a.nz;
@@ -267,15 +305,11 @@ hipError_t launch_live(const NetworkArgs &args, size_t lds_bytes, hipStream_t st
     if (e != hipSuccess) {
         return e;
     }
-    int per_cu = 0, device = 0, cus = 0;
-    if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, NETWORK_THREADS, lds_bytes)) != hipSuccess ||
-        (e = hipGetDevice(&device)) != hipSuccess ||
-        (e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device)) != hipSuccess) {
-        return e;
-    }
+    // One workgroup per tile (tiles cost the same, and the dispatcher keeps
+    // every CU at the occupancy registers and LDS allow); the tile loop
+    // strides only past 2^24 tiles.
     const int64_t tiles = (int64_t)args.tiles_x * args.tiles_y * args.nz;
-    int64_t blocks = (int64_t)(per_cu < 1 ? 1 : per_cu) * (cus < 1 ? 1 : cus) * 4;
-    blocks = blocks < tiles ? blocks : tiles;
+    const int64_t blocks = tiles < ((int64_t)1 << 24) ? tiles : ((int64_t)1 << 24);
     hipLaunchKernelGGL((k_network<T, LIVE>), dim3((unsigned)(blocks < 1 ? 1 : blocks)), dim3(NETWORK_THREADS),
                        lds_bytes, stream, args);
     return hipGetLastError();
@@ -289,7 +323,7 @@ int network_live(int dtype_bytes, int count)
     if (count > limit) {
         return 0;
     }
-    for (int live : {9, 16, 25, 32, 49, 64, 96, 128, 160}) {
+    for (int live : {9, 16, 25, 32, 49, 64, 72, 96, 128, 160}) {
         if (count <= live) {
             return live;
         }
@@ -297,9 +331,9 @@ int network_live(int dtype_bytes, int count)
     return 0;
 }
 
-size_t network_lds_bytes(int dtype_bytes, int s, int count)
+size_t network_lds_bytes(int dtype_bytes, int s, int pad, int live)
 {
-    return (size_t)s * (size_t)dtype_bytes + (size_t)count * 4;
+    return ((size_t)s + (size_t)pad) * (size_t)dtype_bytes + (size_t)live * 4;
 }
 
 hipError_t network_batch_grid(int dtype_bytes, int live, size_t lds_bytes, int *grid)
@@ -315,6 +349,8 @@ hipError_t network_batch_grid(int dtype_bytes, int live, size_t lds_bytes, int *
         NDF_NETWORK_GRID(32)
         NDF_NETWORK_GRID(49)
         NDF_NETWORK_GRID(64)
+    case 72:
+        return dtype_bytes == 4 ? batch_grid_live<float, 72>(lds_bytes, grid) : hipErrorInvalidValue;
     case 96:
         return dtype_bytes == 4 ? batch_grid_live<float, 96>(lds_bytes, grid) : hipErrorInvalidValue;
     case 128:
@@ -343,6 +379,10 @@ hipError_t launch_network_batch(int dtype_bytes, int live, const NetworkArgs *im
         NDF_NETWORK_BATCH(32)
         NDF_NETWORK_BATCH(49)
         NDF_NETWORK_BATCH(64)
+    case 72:
+        return dtype_bytes == 4 ? launch_batch_live<float, 72>(images, offsets, work, nwork, counter, input, output,
+                                                               lds_bytes, grid, stream)
+                                : hipErrorInvalidValue;
     case 96:
         return dtype_bytes == 4 ? launch_batch_live<float, 96>(images, offsets, work, nwork, counter, input, output,
                                                                lds_bytes, grid, stream)
@@ -373,6 +413,8 @@ hipError_t launch_network(int dtype_bytes, const NetworkArgs &args, size_t lds_b
         NDF_NETWORK_CASE(32)
         NDF_NETWORK_CASE(49)
         NDF_NETWORK_CASE(64)
+    case 72:
+        return dtype_bytes == 4 ? launch_live<float, 72>(args, lds_bytes, stream) : hipErrorInvalidValue;
     case 96:
         return dtype_bytes == 4 ? launch_live<float, 96>(args, lds_bytes, stream) : hipErrorInvalidValue;
     case 128:
