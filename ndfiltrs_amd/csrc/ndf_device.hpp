@@ -59,6 +59,17 @@ struct KeyTraits<double> {
     __device__ __host__ static inline uint64_t from_double(double d) { return bits(d); }
 };
 
+// Entry of a read-only table (the per-axis maps) at a wave-uniform index,
+// read through the constant address space so the compiler issues scalar
+// loads that batch under one wait (a generic pointer gives a vector load and
+// a wait per entry). Tables are never written while a kernel runs.
+template <typename V>
+__device__ inline V uniform_entry(const V *table, int64_t index)
+{
+    return reinterpret_cast<const __attribute__((address_space(4))) V *>(
+        reinterpret_cast<uintptr_t>(table))[index];
+}
+
 // Retention under InvalidPolicy::Omit (src/reduction.rs:85-101). The
 // propagate policy is expressed as Omit{ignore_inf=false, min_count=count}:
 // a window with a NaN then has fewer retained samples than taps.

@@ -79,7 +79,7 @@ __global__ void __launch_bounds__(EXTREMA_THREADS) k_extrema(ExtremaArgs a)
     int py = 1;
     while (2 * py <= a.ky) py <<= 1;
     const int tiles = a.tiles_x * a.tiles_y * (int)a.nz;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int rows = a.kz * a.sy;
     for (int tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
         int rest = tile;
@@ -105,8 +105,10 @@ __global__ void __launch_bounds__(EXTREMA_THREADS) k_extrema(ExtremaArgs a)
                 const int row = row0 + u * (EXTREMA_THREADS / 64);
                 const int z = row / a.sy;
                 const int y = row - z * a.sy;
-                const int64_t mz = row < rows ? a.mapz[z0 + z] : MAP_PAD;
-                const int64_t myy = row < rows ? a.mapy[y0 + y] : MAP_PAD;
+                const int64_t mz_at = uniform_entry(a.mapz, z0 + (row < rows ? z : 0));
+                const int64_t my_at = uniform_entry(a.mapy, y0 + (row < rows ? y : 0));
+                const int64_t mz = row < rows ? mz_at : MAP_PAD;
+                const int64_t myy = row < rows ? my_at : MAP_PAD;
                 my[u] = (mz == MAP_PAD || myy == MAP_PAD) ? MAP_PAD : ((mz < 0 || myy < 0) ? MAP_CVAL : mz + myy);
 #pragma unroll
                 for (int j = 0; j < STAGE_COLUMNS; ++j) {

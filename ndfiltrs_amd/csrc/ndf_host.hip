@@ -608,6 +608,9 @@ bool plan_slide(ndf_plan *p, int force_e = 0, int max_ty = 256)
     s_args.min_count = (int32_t)p->eff_min_count;
     s_args.rank_param = (int32_t)std::min(p->parameter, p->count);
     s_args.q100 = p->q100;
+    // A min_count above half the window (Propagate: all of it) leaves most
+    // windows of a sky with NaNs without a result; such tiles skip the sort.
+    s_args.count_gate = rect && sx <= 128 && 2 * p->eff_min_count > p->count ? 1 : 0;
     p->slide_lds = slide_lds_bytes(p->dbytes, s, sort_e, (int)hist_stride, mask_bytes);
 
     // Bracket tiles (2-D rectangles with support rows of at most 128

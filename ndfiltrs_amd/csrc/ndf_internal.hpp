@@ -83,6 +83,9 @@ struct SlideArgs {
     int32_t min_count;
     int32_t rank_param;
     double q100;          // percentile / 100 (STAT_PERCENTILE)
+    // Rectangles with a high min_count: count every window's retained
+    // samples before the sort and skip it when no window has a result.
+    int32_t count_gate;
     int32_t *error_flag;  // set when a rank walk leaves its bounds (plain mode)
     // Tile selection: tiles 0 .. tiles_x*tiles_y*nz - 1, or tile_list[0 .. *tile_count).
     const int32_t *tile_list;

@@ -157,7 +157,7 @@ __device__ __forceinline__ void network_tile(const NetworkArgs &a, int tile,
     using Key = typename KT::Key;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
-    const int wave = tid >> 6;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const Key *input = static_cast<const Key *>(a.input);
     Key *output = static_cast<Key *>(a.output);
     const Key cval_key = classify<T>((Key)a.cval_bits, a.ignore_inf);
@@ -172,8 +172,8 @@ __device__ __forceinline__ void network_tile(const NetworkArgs &a, int tile,
     for (int row = wave; row < a.rows; row += 4) {
         const int z = row / a.sy;
         const int y = row - z * a.sy;
-        const int64_t mz = a.mapz[z0 + z];
-        const int64_t my = a.mapy[y0 + y];
+        const int64_t mz = uniform_entry(a.mapz, z0 + z);
+        const int64_t my = uniform_entry(a.mapy, y0 + y);
         for (int x = lane; x < a.sx; x += 64) {
             const int64_t mx = a.mapx[x0 + x];
             const bool pad = mz == MAP_PAD || my == MAP_PAD || mx == MAP_PAD;

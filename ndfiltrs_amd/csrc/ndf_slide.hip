@@ -814,8 +814,12 @@ __device__ __forceinline__ void stage_support(const SlideArgs &a, typename KeyTr
             const int z = row >> a.zshift;
             const int y = row & ((1 << a.zshift) - 1);
             const bool live = row < a.rows && y < a.sy;  // padding rows between z planes are skipped
-            const int64_t mz = live ? a.mapz[z0 + z] : MAP_PAD;
-            const int64_t my = live ? a.mapy[y0 + y] : MAP_PAD;
+            // Unconditional uniform loads (a dead row reads the tile's first
+            // entries, which exist).
+            const int64_t mz_at = uniform_entry(a.mapz, z0 + (live ? z : 0));
+            const int64_t my_at = uniform_entry(a.mapy, y0 + (live ? y : 0));
+            const int64_t mz = live ? mz_at : MAP_PAD;
+            const int64_t my = live ? my_at : MAP_PAD;
             const bool pad = mz == MAP_PAD || my == MAP_PAD;
             rkind[r] = pad ? 2 : (mz < 0 || my < 0 ? 1 : 0);
             roff[r] = pad ? 0 : mz + my;
@@ -929,6 +933,62 @@ __device__ __forceinline__ bool slide_tile(const SlideArgs &a, int tile, unsigne
             // Cannot happen: the planner sizes the sort for the whole support.
             __syncthreads();
             return false;
+        }
+
+        // ---- count gate: no window with a result, no sort ----
+        // Each lane sums the retained samples of its two support columns over
+        // the window rows (every plane), slides the sums down its wave's rows
+        // and adds the kx columns of its window with a wave prefix scan.
+        if (a.count_gate) {
+            const int y_first = wave * a.tw;
+            const bool has1 = lane + 64 < a.sx;
+            auto kept_at = [&](int row, int col) -> int {
+                return (int)(codes[row * a.sx + col] != CODE_OMIT);
+            };
+            int col0 = 0, col1 = 0;
+            for (int z = 0; z < a.kz; ++z) {
+                for (int dy = 0; dy < a.ky; ++dy) {
+                    const int row = z * a.zrows + y_first + dy;
+                    col0 += kept_at(row, lane);
+                    col1 += has1 ? kept_at(row, lane + 64) : 0;
+                }
+            }
+            const int last = lane + a.kx - 1;  // last support column of this lane's window
+            bool any = false;
+            for (int y = y_first; y < y_first + a.tw; ++y) {
+                if (y > y_first) {
+                    for (int z = 0; z < a.kz; ++z) {
+                        const int leave = z * a.zrows + y - 1;
+                        const int enter = leave + a.ky;
+                        col0 += kept_at(enter, lane) - kept_at(leave, lane);
+                        col1 += has1 ? kept_at(enter, lane + 64) - kept_at(leave, lane + 64) : 0;
+                    }
+                }
+                int p0 = col0, p1 = col1;
+                for (int off = 1; off < 64; off <<= 1) {
+                    const int u0 = __shfl_up(p0, off), u1 = __shfl_up(p1, off);
+                    p0 += lane >= off ? u0 : 0;
+                    p1 += lane >= off ? u1 : 0;
+                }
+                const int total0 = __shfl(p0, 63);
+                const int hi0 = __shfl(p0, last & 63);
+                const int hi1 = __shfl(p1, (last - 64) & 63);
+                const int before = __shfl(p0, (lane + 63) & 63);
+                const int n = (last < 64 ? hi0 : total0 + hi1) - (lane ? before : 0);
+                Pick pick;
+                any = any || pick_ranks(a.stat, n, a.min_count, a.rank_param, a.q100, pick);
+            }
+            if (!__syncthreads_or(any ? 1 : 0)) {
+                // Every window of the tile lacks a result: canonical NaN.
+                const bool col_valid = x0 + lane < a.nx;
+                for (int y = y_first; y < y_first + a.tw && y0 + y < a.ny; ++y) {
+                    if (col_valid) {
+                        output[z0 * a.ny * a.nx + (y0 + y) * a.nx + x0 + lane] = KT::CANONICAL_NAN;
+                    }
+                }
+                __syncthreads();
+                return true;
+            }
         }
 
         // ---- sort the retained samples (padded with sentinels to N) ----
@@ -1284,7 +1344,7 @@ __global__ void __launch_bounds__(NT, E >= 32 ? 1 : (sizeof(T) == 8 && E >= 16 ?
     const Key cval_key = classify<T>((Key)a.cval_bits, a.ignore_inf);
     const Geometry g{1, a.ky, a.kx, 8, 0xFFFFu, nullptr, a.ky < 128 && a.kx < 128 && a.ty + a.ky <= 257};
     const uint64_t kxmask = a.kx >= 64 ? ~0ull : ((1ull << a.kx) - 1ull);
-    const int64_t mz = a.mapz[0];
+    const int64_t mz = uniform_entry(a.mapz, 0);
     const uint64_t lanemask_lt = (1ull << lane) - 1ull;
 
     const int chunk = (ntiles + 7) / 8;
@@ -1315,7 +1375,8 @@ __global__ void __launch_bounds__(NT, E >= 32 ? 1 : (sizeof(T) == 8 && E >= 16 ?
             int64_t my[R];
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                my[r] = row0 + r < a.rows ? a.mapy[y0 + row0 + r] : MAP_PAD;
+                const int64_t at = uniform_entry(a.mapy, y0 + (row0 + r < a.rows ? row0 + r : 0));
+                my[r] = row0 + r < a.rows ? at : MAP_PAD;
             }
             Key kv[R][2];
 #pragma unroll

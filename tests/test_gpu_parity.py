@@ -261,6 +261,48 @@ def test_large_windows_match_oracle(shape, size, dtype, mode: str) -> None:
     assert_bits(nf.rank_filter(values, count // 7, **kwargs), expected, "propagate rank")
 
 
+@pytest.mark.parametrize(
+    ("shape", "size", "dtype", "mode"),
+    [
+        ((400, 300), (31, 31), np.float32, "reflect"),
+        ((330, 270), (63, 63), np.float32, "nearest"),
+        ((300, 200), (31, 31), np.float64, "mirror"),
+        ((9, 120, 140), (3, 9, 9), np.float32, "wrap"),
+    ],
+)
+def test_count_gate_keeps_tiles_with_results(shape, size, dtype, mode: str) -> None:
+    """Propagate and high-min_count plans skip the sort of tiles whose
+    windows all lack a result: NaNs cover the left part of the image (every
+    window there has one), the right part is clean, and one band holds
+    single NaNs so windows there sit just below or at min_count."""
+    rng = np.random.default_rng(sum(shape) + size[-1])
+    values = rng.normal(1000.0, 10.0, size=shape)
+    width = shape[-1]
+    left = values[..., : width // 3]
+    left[rng.random(left.shape) < 0.05] = np.nan
+    band = values[..., width // 2: width // 2 + 8]
+    band[rng.random(band.shape) < 0.002] = np.nan
+    values = values.astype(dtype)
+    count = int(np.prod(size))
+    modes = (mode,) * len(shape)
+    kwargs = {"size": size, "mode": mode}
+    expected = _cases.oracle_call("median_filter", values, kwargs, method="histogram")
+    assert_bits(nf.median_filter(values, **kwargs), expected, "propagate median")
+    rank = count // 5
+    expected = _cases.oracle_call("rank_filter", values, kwargs, rank, method="histogram")
+    assert_bits(nf.rank_filter(values, rank, **kwargs), expected, "propagate rank")
+    for min_count in (count - 1, count // 2 + 1):
+        expected = oracle.window_filter(values, size, mode=mode, statistic="median", omit_nan=True,
+                                        min_count=min_count, method="histogram")
+        plan = _core.Plan(values.dtype, values.shape, "median", size, None, (0,) * len(shape), modes, 0.0, True,
+                          False, min_count, 0)
+        assert_bits(plan.apply(values), expected, f"omit min_count {min_count}")
+    # Everything covered: every tile takes the gate.
+    values[..., :] = np.where(rng.random(shape) < 0.3, np.nan, values)
+    expected = _cases.oracle_call("rank_filter", values, kwargs, rank, method="histogram")
+    assert_bits(nf.rank_filter(values, rank, **kwargs), expected, "all NaN windows")
+
+
 def test_window_larger_than_array() -> None:
     values = _cases.special_sample((13, 17), np.float32, seed=99)
     for mode in _cases.MODES:
