@@ -75,7 +75,7 @@ def test_engine_choice_for_the_benchmark_shapes() -> None:
     yy, xx = np.mgrid[-6:7, -6:7]
     ring = (((yy**2 + xx**2) >= 16) & ((yy**2 + xx**2) <= 36)).ravel().tolist()
     c3 = _plan(np.float32, (8192, 8192), (13, 13), footprint=ring, origin=[1, 0], mode=["wrap"] * 2)
-    assert c3["engine"] == "network" and c3["count"] == 68 and c3["network_wires"] == 96
+    assert c3["engine"] == "network" and c3["count"] == 68 and c3["network_wires"] == 72
     cube = _plan(np.float32, (128, 1024, 1024), (3, 7, 7), mode=["wrap"] * 3)
     assert cube["engine"] == "network" and cube["count"] == 147 and cube["network_wires"] == 160
     five = _plan(np.float64, (2, 2, 2, 2, 3), (3, 3, 3, 3, 5))
