@@ -56,11 +56,20 @@ enum {
 };
 
 /* Device engines. AUTO picks by geometry (EXTREMA for rectangular minimum /
- * maximum plans that need no retained count, NETWORK for windows of at most
- * 160 taps in float32 or 64 in float64, else SLIDE where the tile can be
- * staged, else GENERIC); the others force one engine and exist so tests can
- * check engines against each other. */
-enum { NDF_ENGINE_AUTO = 0, NDF_ENGINE_GENERIC = 1, NDF_ENGINE_SLIDE = 2, NDF_ENGINE_NETWORK = 3, NDF_ENGINE_EXTREMA = 4 };
+ * maximum plans that need no retained count; STACK for box windows with
+ * compiled slice networks: 2-D 3x3, 5x5, 7x7, 9x9 and 3-D 3x3x3, 3x5x5,
+ * 3x7x7, float64 up to 49 taps; NETWORK for other windows of at most 160
+ * taps in float32 or 64 in float64; else SLIDE where the tile can be staged,
+ * else GENERIC); the others force one engine and exist so tests can check
+ * engines against each other. */
+enum {
+    NDF_ENGINE_AUTO = 0,
+    NDF_ENGINE_GENERIC = 1,
+    NDF_ENGINE_SLIDE = 2,
+    NDF_ENGINE_NETWORK = 3,
+    NDF_ENGINE_EXTREMA = 4,
+    NDF_ENGINE_STACK = 5
+};
 
 /* One-shot filter over host buffers.
  * Replaces _core.window_filter(array, kind, size, footprint, origin, mode,

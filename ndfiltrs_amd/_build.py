@@ -11,7 +11,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIBRARY = os.path.join(HERE, "libndfgpu.so")
 SOURCES = ("ndf_host.hip", "ndf_generic.hip", "ndf_slide.hip", "ndf_bracket.hip", "ndf_network.hip",
-           "ndf_extrema.hip", "ndf_mesh.hip", "ndf_transfer.hip")
+           "ndf_stack.hip", "ndf_extrema.hip", "ndf_mesh.hip", "ndf_transfer.hip")
 
 # Strict IEEE for the f64 midpoint (no contraction, denormals kept), packed
 # FP32 instructions disabled, gfx950 only.

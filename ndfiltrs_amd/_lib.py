@@ -37,7 +37,7 @@ KINDS = {
 ROBUST_KINDS = {"percentile": 0}
 ROBUST_OTHER_KINDS = ("mad", "mode", "biweight_location", "biweight_scale", "sigma_mean", "sigma_median",
                       "sigma_mode")
-ENGINES = {"auto": 0, "generic": 1, "slide": 2, "network": 3, "extrema": 4}
+ENGINES = {"auto": 0, "generic": 1, "slide": 2, "network": 3, "extrema": 4, "stack": 5}
 REPAIR_PATHS = {"auto": 0, "sparse": 1, "dense": 2}
 
 _size_p = ctypes.POINTER(ctypes.c_size_t)

@@ -1,0 +1,294 @@
+"""Generate the comparator networks of the stack engine (ndf_stack_networks.inc).
+
+The stack engine computes a median / rank over a box window of K0 slices
+along the slowest window axis, each slice an SH x SW box (SH = 1 for a 2-D
+window, whose slices are row segments). Per output it merges K0 sorted
+slices, so the networks here are:
+
+  col    sorts one column of SH keys (wires 0 .. SH-1);
+  slice  merges SW sorted columns (column c on wires c*SH .. c*SH+SH-1) into
+         one sorted slice; `slice_out[i]` is the wire holding rank i;
+  half   merges K0 sorted slices (slice s, in rank order, on wires
+         s*L .. s*L+L-1 with L = SH*SW) and keeps only the comparators that
+         ranks 0 .. W/2 depend on (W = K0*L); `half_out[r]` is rank r's wire.
+         Omitted samples carry the all-ones key and sort last, so a median's
+         two middle ranks never exceed W/2;
+  full   the same merge with every rank (rank and percentile statistics).
+
+Merges are Batcher's odd-even merge (K. E. Batcher, "Sorting networks and
+their applications", 1968): merge the odd- and the even-indexed elements of
+both lists recursively, then compare neighbours once; it works for lists of
+any lengths. A merge tree is searched over its shapes for the fewest
+comparators. `verify()` proves every network by the 0-1 principle: a
+comparator network sorts (or selects a rank of) every input of its class iff
+it does so for every 0-1 input of that class, and the class here is "lists
+already sorted", whose 0-1 inputs are 0^a 1^(n-a) per list. Where the 0-1
+inputs of a whole network are too many, every two-list merge it is built
+from is proven instead (a tree of correct merges is correct, and dropping
+comparators outside the cone of the kept ranks leaves those ranks unchanged)
+and the whole network is checked on random inputs.
+
+Run `python gen_stack_networks.py` in this directory to rewrite the header.
+"""
+
+from __future__ import annotations
+
+import itertools
+import os
+import sys
+
+import numpy as np
+
+# (K0, SH, SW): slices along the slowest window axis, slice rows, slice columns.
+CONFIGS = [
+    (3, 1, 3),
+    (5, 1, 5),
+    (7, 1, 7),
+    (9, 1, 9),
+    (3, 3, 3),
+    (3, 5, 5),
+    (3, 7, 7),
+]
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+HEADER = os.path.join(HERE, "ndf_stack_networks.inc")
+
+
+def merge(xs: list[int], ys: list[int]) -> tuple[list[tuple[int, int]], list[int]]:
+    """Batcher's odd-even merge of the sorted wire lists xs and ys (any
+    lengths). Returns the comparators (min goes to the first wire) and the
+    wires of the merged list in rank order."""
+    m, n = len(xs), len(ys)
+    if m == 0:
+        return [], list(ys)
+    if n == 0:
+        return [], list(xs)
+    if m == 1 and n == 1:
+        return [(xs[0], ys[0])], [xs[0], ys[0]]
+    c_odd, v = merge(xs[0::2], ys[0::2])
+    c_even, w = merge(xs[1::2], ys[1::2])
+    comps = c_odd + c_even
+    out = [v[0]]
+    i = 0
+    while i < len(w) or i + 1 < len(v):
+        if i < len(w) and i + 1 < len(v):
+            comps.append((w[i], v[i + 1]))
+            out += [w[i], v[i + 1]]
+        elif i < len(w):
+            out.append(w[i])
+        else:
+            out.append(v[i + 1])
+        i += 1
+    return comps, out
+
+
+def merge_tree(lists: list[list[int]], shape) -> tuple[list[tuple[int, int]], list[int]]:
+    """Merge sorted wire lists along a tree: `shape` is an index or a pair of
+    shapes."""
+    if isinstance(shape, int):
+        return [], lists[shape]
+    ca, a = merge_tree(lists, shape[0])
+    cb, b = merge_tree(lists, shape[1])
+    cm, out = merge(a, b)
+    return ca + cb + cm, out
+
+
+def tree_shapes(items: tuple[int, ...]):
+    """Every binary tree over the items in this order (Catalan many; the
+    configurations keep this small)."""
+    if len(items) == 1:
+        yield items[0]
+        return
+    for split in range(1, len(items)):
+        for left in tree_shapes(items[:split]):
+            for right in tree_shapes(items[split:]):
+                yield (left, right)
+
+
+def cone(comps: list[tuple[int, int]], outputs: list[int]) -> list[tuple[int, int]]:
+    """The comparators that the given output wires depend on, in order."""
+    need = set(outputs)
+    kept = []
+    for a, b in reversed(comps):
+        if a in need or b in need:
+            kept.append((a, b))
+            need.add(a)
+            need.add(b)
+    kept.reverse()
+    return kept
+
+
+def sort_network(n: int) -> tuple[list[tuple[int, int]], list[int]]:
+    """Sorting network on wires 0..n-1 as a merge tree of single wires (the
+    cheapest tree shape)."""
+    best = None
+    for shape in tree_shapes(tuple(range(n))) if n <= 9 else [balanced(tuple(range(n)))]:
+        comps, out = merge_tree([[i] for i in range(n)], shape)
+        if best is None or len(comps) < len(best[0]):
+            best = (comps, out)
+    return best
+
+
+def balanced(items: tuple[int, ...]):
+    if len(items) == 1:
+        return items[0]
+    h = (len(items) + 1) // 2
+    return (balanced(items[:h]), balanced(items[h:]))
+
+
+def build(k0: int, sh: int, sw: int) -> dict:
+    length = sh * sw
+    width = k0 * length
+    col, col_out = sort_network(sh) if sh > 1 else ([], [0])
+    # Slice: merge SW sorted columns; columns are symmetric, so the balanced
+    # tree and the left-deep tree are the candidates.
+    columns = [list(range(c * sh, (c + 1) * sh)) for c in range(sw)]
+    best = None
+    for shape in tree_shapes(tuple(range(sw))) if sw <= 9 else [balanced(tuple(range(sw)))]:
+        comps, out = merge_tree(columns, shape)
+        if best is None or len(comps) < len(best[0]):
+            best = (comps, out)
+    slice_comps, slice_out = best
+    # Output merges over the K0 slices: the fewest comparators for the lower
+    # half (the median's ranks), and for the full order.
+    slices = [list(range(s * length, (s + 1) * length)) for s in range(k0)]
+    best_half = best_full = None
+    for shape in tree_shapes(tuple(range(k0))):
+        comps, out = merge_tree(slices, shape)
+        half = cone(comps, out[: width // 2 + 1])
+        if best_half is None or len(half) < len(best_half[0]):
+            best_half = (half, out[: width // 2 + 1])
+        if best_full is None or len(comps) < len(best_full[0]):
+            best_full = (comps, out)
+    return dict(k0=k0, sh=sh, sw=sw, length=length, width=width, col=col, col_out=col_out,
+                slice=slice_comps, slice_out=slice_out, half=best_half[0], half_out=best_half[1],
+                full=best_full[0], full_out=best_full[1])
+
+
+def apply(comps: list[tuple[int, int]], values: np.ndarray) -> np.ndarray:
+    """Apply comparators to rows of `values` (one input per row)."""
+    v = values.copy()
+    for a, b in comps:
+        lo = np.minimum(v[:, a], v[:, b])
+        hi = np.maximum(v[:, a], v[:, b])
+        v[:, a] = lo
+        v[:, b] = hi
+    return v
+
+
+def sorted_01(lists: int, length: int) -> np.ndarray:
+    """Every 0-1 input made of `lists` sorted lists of `length` (rows)."""
+    rows = []
+    for zeros in itertools.product(range(length + 1), repeat=lists):
+        row = []
+        for z in zeros:
+            row += [0] * z + [1] * (length - z)
+        rows.append(row)
+    return np.array(rows, dtype=np.uint8)
+
+
+EXHAUSTIVE_LIMIT = 1 << 18
+
+
+def verify_merge(m: int, n: int) -> None:
+    """Batcher's merge of sorted lists of m and n keys, on every 0-1 input."""
+    comps, out = merge(list(range(m)), list(range(m, m + n)))
+    x = np.array([[0] * a + [1] * (m - a) + [0] * b + [1] * (n - b) for a in range(m + 1) for b in range(n + 1)],
+                 dtype=np.uint8)
+    assert np.array_equal(apply(comps, x)[:, out], np.sort(x, axis=1)), f"merge({m}, {n}) is wrong"
+
+
+def merge_sizes(lists: int, length: int) -> set[tuple[int, int]]:
+    """Every (m, n) a merge tree over `lists` lists of `length` may merge."""
+    return {(a * length, b * length) for a in range(1, lists) for b in range(1, lists - a + 1)}
+
+
+def verify(net: dict, random_trials: int = 2000, seed: int = 0) -> None:
+    """Raise AssertionError unless every network of `net` is correct."""
+    sh, sw, k0, length, width = net["sh"], net["sw"], net["k0"], net["length"], net["width"]
+    for m, n in merge_sizes(sw, sh) | merge_sizes(k0, length):
+        verify_merge(m, n)
+    # Column sort: all 2^SH 0-1 inputs.
+    if sh > 1:
+        x = np.array(list(itertools.product((0, 1), repeat=sh)), dtype=np.uint8)
+        y = apply(net["col"], x)[:, net["col_out"]]
+        assert np.array_equal(y, np.sort(x, axis=1)), "column network does not sort"
+    # Slice merge: every 0-1 input of SW sorted columns (when few enough).
+    if (sh + 1) ** sw <= EXHAUSTIVE_LIMIT:
+        x = sorted_01(sw, sh)
+        y = apply(net["slice"], x)[:, net["slice_out"]]
+        assert np.array_equal(y, np.sort(x, axis=1)), "slice network does not merge"
+    # Output merges: every 0-1 input of K0 sorted slices (when few enough).
+    if (length + 1) ** k0 <= EXHAUSTIVE_LIMIT:
+        x = sorted_01(k0, length)
+        expect = np.sort(x, axis=1)
+        y = apply(net["full"], x)[:, net["full_out"]]
+        assert np.array_equal(y, expect), "full network does not merge"
+        y = apply(net["half"], x)[:, net["half_out"]]
+        assert np.array_equal(y, expect[:, : width // 2 + 1]), "half network does not give the lower half"
+    # The whole pipeline on random keys with ties and sentinels.
+    rng = np.random.default_rng(seed)
+    keys = rng.integers(0, 6, size=(random_trials, width)).astype(np.uint32)
+    keys[rng.random(keys.shape) < 0.2] = 0xFFFFFFFF
+    cols = keys.reshape(random_trials, k0, sw, sh)
+    if sh > 1:
+        flat = cols.reshape(-1, sh)
+        cols = apply(net["col"], flat)[:, net["col_out"]].reshape(random_trials, k0, sw, sh)
+    sl = cols.reshape(-1, length)
+    sl = apply(net["slice"], sl)[:, net["slice_out"]].reshape(random_trials, width)
+    expect = np.sort(keys, axis=1)
+    assert np.array_equal(apply(net["full"], sl)[:, net["full_out"]], expect)
+    assert np.array_equal(apply(net["half"], sl)[:, net["half_out"]], expect[:, : width // 2 + 1])
+
+
+def _pairs(name: str, comps: list[tuple[int, int]]) -> str:
+    body = ", ".join(f"{{{a}, {b}}}" for a, b in comps) if comps else "{0, 0}"
+    return f"    static constexpr uint8_t {name}[][2] = {{{body}}};\n    static constexpr int {name}_count = {len(comps)};\n"
+
+
+def _list(name: str, values: list[int]) -> str:
+    return f"    static constexpr uint8_t {name}[] = {{{', '.join(str(v) for v in values)}}};\n"
+
+
+def render() -> str:
+    out = [
+        "// Comparator networks of the stack engine. Generated by gen_stack_networks.py",
+        "// (same directory); edit that script, not this file.",
+        "// Net<K0, SH, SW>: K0 slices of SH x SW keys; W = K0 * SH * SW taps.",
+        "#pragma once",
+        "",
+        "#include <stdint.h>",
+        "",
+        "namespace ndf {",
+        "namespace stack_net {",
+        "",
+        "template <int K0, int SH, int SW>",
+        "struct Net;",
+        "",
+    ]
+    for k0, sh, sw in CONFIGS:
+        net = build(k0, sh, sw)
+        out.append(f"template <>\nstruct Net<{k0}, {sh}, {sw}> {{")
+        out.append(f"    static constexpr int L = {net['length']}, W = {net['width']}, H = {net['width'] // 2 + 1};")
+        text = _pairs("col", net["col"]) + _list("col_out", net["col_out"])
+        text += _pairs("slice", net["slice"]) + _list("slice_out", net["slice_out"])
+        text += _pairs("half", net["half"]) + _list("half_out", net["half_out"])
+        text += _pairs("full", net["full"]) + _list("full_out", net["full_out"])
+        out.append(text.rstrip("\n"))
+        out.append("};\n")
+    out += ["}  // namespace stack_net", "}  // namespace ndf", ""]
+    return "\n".join(out)
+
+
+def main() -> None:
+    for cfg in CONFIGS:
+        net = build(*cfg)
+        verify(net)
+        print(f"Net{cfg}: W={net['width']} col={len(net['col'])} slice={len(net['slice'])} "
+              f"half={len(net['half'])} full={len(net['full'])}", file=sys.stderr)
+    with open(HEADER, "w") as f:
+        f.write(render())
+
+
+if __name__ == "__main__":
+    main()

@@ -199,6 +199,12 @@ struct ndf_plan {
     std::vector<int64_t> net_mapz, net_mapy, net_mapx;
     std::vector<uint32_t> net_taps;
 
+    // Stack geometry (box windows with compiled slice networks).
+    bool stack_ok = false;
+    StackArgs stack{};
+    std::vector<int64_t> stk_maps, stk_mapr, stk_mapx;
+    DeviceBuffer k_maps, k_mapr, k_mapx;
+
     // Extrema geometry (rectangular minimum / maximum).
     bool extrema_ok = false;
     ExtremaArgs extrema{};
@@ -266,7 +272,7 @@ struct ndf_plan {
     ~ndf_plan()
     {
         if (done) (void)hipEventDestroy(done);
-        for (DeviceBuffer *b : {&n_mapz, &n_mapy, &n_mapx, &n_taps, &e_mapz, &e_mapy, &e_mapx}) {
+        for (DeviceBuffer *b : {&n_mapz, &n_mapy, &n_mapx, &n_taps, &e_mapz, &e_mapy, &e_mapx, &k_maps, &k_mapr, &k_mapx}) {
             b->release();
         }
         for (DeviceBuffer *b : {&g_maps, &g_taps, &s_mapz, &s_mapy, &s_mapx, &s_taps, &s_rm, &s_add, &s_mask, &d_in,
@@ -372,6 +378,93 @@ bool plan_network(ndf_plan *p)
     n.q100 = p->q100;
     p->network_lds = lds;
     return true;
+}
+
+void build_map(std::vector<int64_t> &map, int64_t length, int64_t n, int64_t low, int64_t k, int32_t mode,
+               int64_t stride);
+
+// Plan the stack engine (box windows of rank 2 or 3 with compiled networks);
+// false when it does not apply. A window with more than one plane slides
+// along z (slices ky x kx, rows = y); a single-plane window slides along y
+// (slices 1 x kx, rows = z planes).
+bool plan_stack(ndf_plan *p)
+{
+    if (p->rank < 2 || p->rank > 3 || p->len == 0) return false;
+    int64_t dims[3] = {1, 1, 1}, lo[3] = {0, 0, 0}, box[3] = {1, 1, 1};
+    int32_t modes[3] = {NDF_MODE_NEAREST, NDF_MODE_NEAREST, NDF_MODE_NEAREST};
+    const size_t pad = 3 - p->rank;
+    int64_t volume = 1;
+    for (size_t a = 0; a < p->rank; ++a) {
+        dims[pad + a] = (int64_t)p->shape[a];
+        lo[pad + a] = p->low[a];
+        box[pad + a] = p->high[a] - p->low[a] + 1;
+        modes[pad + a] = p->modes[a];
+        volume *= box[pad + a];
+    }
+    // Every position of the taps' bounding box must be a tap.
+    if (volume != (int64_t)p->count) return false;
+    const bool planes = box[0] > 1;
+    const int k0 = (int)(planes ? box[0] : box[1]);
+    const int sh = planes ? (int)box[1] : 1;
+    const int sw = (int)box[2];
+    if (!stack_supported(p->dbytes, k0, sh, sw)) return false;
+    const int64_t ns = planes ? dims[0] : dims[1];
+    const int64_t nr = planes ? dims[1] : dims[0];
+    const int64_t tiles_x = (dims[2] + 63) / 64;
+    const int64_t in_sy = dims[2], in_sz = dims[1] * dims[2];
+    // Runs along the slide axis: whole columns when that gives 8192 waves,
+    // else shorter runs (at least 2 * k0 steps, so the k0 - 1 slices each run
+    // builds ahead of its first output stay a small share).
+    const int64_t columns = tiles_x * nr;
+    int64_t run = ns;
+    if (columns < 8192) {
+        const int64_t wanted = (8192 + columns - 1) / columns;
+        run = std::max<int64_t>((ns + wanted - 1) / wanted, 2 * k0);
+        run = std::min(run, ns);
+    }
+    const int64_t units = columns * ((ns + run - 1) / run);
+    if (units > stack_max_units() || run > INT32_MAX) return false;
+    if (planes) {
+        build_map(p->stk_maps, ns + k0 - 1, dims[0], lo[0], k0, modes[0], in_sz);
+        build_map(p->stk_mapr, nr + sh - 1, dims[1], lo[1], sh, modes[1], in_sy);
+    } else {
+        build_map(p->stk_maps, ns + k0 - 1, dims[1], lo[1], k0, modes[1], in_sy);
+        build_map(p->stk_mapr, nr, dims[0], 0, 1, NDF_MODE_NEAREST, in_sz);
+    }
+    build_map(p->stk_mapx, tiles_x * 64 + sw - 1, dims[2], lo[2], sw, modes[2], 1);
+    StackArgs &s = p->stack;
+    s = StackArgs{};
+    s.ns = ns;
+    s.nr = nr;
+    s.nx = dims[2];
+    s.out_ss = planes ? dims[1] * dims[2] : dims[2];
+    s.out_sr = planes ? dims[2] : dims[1] * dims[2];
+    s.units = units;
+    s.k0 = k0;
+    s.sh = sh;
+    s.sw = sw;
+    s.sx = 64 + sw - 1;
+    s.tiles_x = (int32_t)tiles_x;
+    s.run = (int32_t)run;
+    s.cval_bits = p->cval_bits;
+    s.ignore_inf = p->eff_ignore_inf;
+    s.stat = p->stat;
+    s.min_count = (int32_t)p->eff_min_count;
+    s.rank_param = (int32_t)std::min(p->parameter, p->count);
+    s.q100 = p->q100;
+    return true;
+}
+
+// The engine a plan runs unless one is forced: extrema for rectangular
+// minimum / maximum, then the stack engine for box windows it has networks
+// for, the network engine for other small windows, the slide engine for
+// large ones, the generic engine for the rest.
+int32_t auto_engine(const ndf_plan *p)
+{
+    return p->extrema_ok   ? NDF_ENGINE_EXTREMA
+           : p->stack_ok   ? NDF_ENGINE_STACK
+           : p->network_ok ? NDF_ENGINE_NETWORK
+                           : (p->slide_ok ? NDF_ENGINE_SLIDE : NDF_ENGINE_GENERIC);
 }
 
 // Maps for the tiled engines: entry i is the input offset of coordinate
@@ -744,6 +837,15 @@ ndf_status prepare_engine_tables(ndf_plan *p, char *err, size_t err_len)
         n.mapy = static_cast<const int64_t *>(p->n_mapy.ptr);
         n.mapx = static_cast<const int64_t *>(p->n_mapx.ptr);
         n.taps = static_cast<const uint32_t *>(p->n_taps.ptr);
+    } else if (p->engine == NDF_ENGINE_STACK) {
+        if ((e = upload(p->k_maps, p->stk_maps)) != hipSuccess || (e = upload(p->k_mapr, p->stk_mapr)) != hipSuccess ||
+            (e = upload(p->k_mapx, p->stk_mapx)) != hipSuccess) {
+            return fail_hip(e, err, err_len);
+        }
+        StackArgs &k = p->stack;
+        k.maps = static_cast<const int64_t *>(p->k_maps.ptr);
+        k.mapr = static_cast<const int64_t *>(p->k_mapr.ptr);
+        k.mapx = static_cast<const int64_t *>(p->k_mapx.ptr);
     } else if (p->engine == NDF_ENGINE_SLIDE) {
         if ((e = upload(p->s_mapz, p->slide_mapz)) != hipSuccess ||
             (e = upload(p->s_mapy, p->slide_mapy)) != hipSuccess ||
@@ -872,6 +974,7 @@ struct LaunchSet {
     SlideArgs slide{}, slide_b{}, slide_fb{};
     ExtremaArgs extrema{};
     NetworkArgs network{};
+    StackArgs stack{};
     GenericArgs generic{};
     BracketArgs bargs{};
     size_t slide_b_lds = 0, slide_fb_lds = 0;
@@ -889,6 +992,7 @@ LaunchSet plan_set(const ndf_plan *p)
     t.slide_b = p->slide_b;
     t.slide_fb = p->slide_fb;
     t.network = p->network;
+    t.stack = p->stack;
     t.extrema = p->extrema;
     t.generic = p->generic;
     t.bargs = p->bargs;
@@ -959,6 +1063,13 @@ hipError_t enqueue(ndf_plan *p, const LaunchSet &t, const void *d_in, void *d_ou
         args.output = d_out;
         mark(p->main_begin);
         if (e == hipSuccess) e = launch_network(p->dbytes, args, p->network_lds, stream);
+        mark(p->main_end);
+    } else if (p->engine == NDF_ENGINE_STACK) {
+        StackArgs args = t.stack;
+        args.input = d_in;
+        args.output = d_out;
+        mark(p->main_begin);
+        if (e == hipSuccess) e = launch_stack(p->dbytes, args, stream);
         mark(p->main_end);
     } else if (p->engine == NDF_ENGINE_SLIDE) {
         SlideArgs args = t.slide;
@@ -1176,10 +1287,9 @@ ndf_status build_plan(ndf_plan *p, int32_t dtype_bytes, size_t rank, const size_
     p->all_nan = p->stat == STAT_RANK && parameter >= count;
     p->slide_ok = plan_slide(p);
     p->network_ok = plan_network(p);
+    p->stack_ok = plan_stack(p);
     p->extrema_ok = plan_extrema(p);
-    p->engine = p->extrema_ok   ? NDF_ENGINE_EXTREMA
-                : p->network_ok ? NDF_ENGINE_NETWORK
-                                : (p->slide_ok ? NDF_ENGINE_SLIDE : NDF_ENGINE_GENERIC);
+    p->engine = auto_engine(p);
     return NDF_OK;
 }
 
@@ -1239,9 +1349,15 @@ ndf_status ndf_plan_set_engine(ndf_plan *plan, int32_t engine, char *err, size_t
     std::lock_guard<std::mutex> guard(plan->mutex);
     int32_t chosen;
     if (engine == NDF_ENGINE_AUTO) {
-        chosen = plan->extrema_ok   ? NDF_ENGINE_EXTREMA
-                 : plan->network_ok ? NDF_ENGINE_NETWORK
-                                    : (plan->slide_ok ? NDF_ENGINE_SLIDE : NDF_ENGINE_GENERIC);
+        chosen = auto_engine(plan);
+    } else if (engine == NDF_ENGINE_STACK) {
+        if (!plan->stack_ok) {
+            return fail(NDF_ERR_INVALID,
+                        "the stack engine needs a box window of rank 2 or 3 with compiled slice networks "
+                        "(2-D 3x3, 5x5, 7x7, 9x9; 3-D 3x3x3, 3x5x5, 3x7x7)",
+                        err, err_len);
+        }
+        chosen = NDF_ENGINE_STACK;
     } else if (engine == NDF_ENGINE_EXTREMA) {
         if (!plan->extrema_ok) {
             return fail(NDF_ERR_INVALID,
@@ -1276,6 +1392,7 @@ ndf_status ndf_plan_describe(const ndf_plan *plan, char *buffer, size_t buffer_l
     const char *engine = plan->all_nan                       ? "fill"
                          : plan->engine == NDF_ENGINE_SLIDE   ? "slide"
                          : plan->engine == NDF_ENGINE_NETWORK ? "network"
+                         : plan->engine == NDF_ENGINE_STACK   ? "stack"
                          : plan->engine == NDF_ENGINE_EXTREMA ? "extrema"
                                                               : "generic";
     if (plan->engine == NDF_ENGINE_EXTREMA) {
@@ -1285,6 +1402,15 @@ ndf_status ndf_plan_describe(const ndf_plan *plan, char *buffer, size_t buffer_l
                  engine, plan->count, plan->extrema.ty, plan->extrema.tx, plan->extrema.kz, plan->extrema.sy,
                  plan->extrema.sx, plan->extrema_lds,
                  (long long)plan->extrema.tiles_x * plan->extrema.tiles_y * plan->extrema.nz);
+        return NDF_OK;
+    }
+    if (plan->engine == NDF_ENGINE_STACK) {
+        const StackArgs &k = plan->stack;
+        snprintf(buffer, buffer_len,
+                 "{\"engine\": \"%s\", \"count\": %zu, \"slices\": [%d, %d, %d], \"run\": %d, \"units\": %lld, "
+                 "\"lds_bytes\": %zu, \"bracket_tile_rows\": %d}",
+                 engine, plan->count, k.k0, k.sh, k.sw, k.run, (long long)k.units,
+                 stack_lds_bytes(plan->dbytes, k.sh, k.sx), plan->bracket_ty);
         return NDF_OK;
     }
     if (plan->engine == NDF_ENGINE_NETWORK) {
@@ -2183,6 +2309,7 @@ double repair_dense_ns(const ndf_plan *p)
     const double w = (double)p->count;
     const double scale = p->dbytes == 8 ? 1.5 : 1.0;
     if (p->engine == NDF_ENGINE_NETWORK) return scale * 0.0009 * w * (1.0 + w / 60.0);
+    if (p->engine == NDF_ENGINE_STACK) return scale * 0.0009 * w * (1.0 + w / 150.0);
     return scale * 0.2;
 }
 
@@ -2535,6 +2662,12 @@ hipError_t static_layout(ndf_plan *p, char *base, hipStream_t stream, LaunchSet 
         x.mapz = static_cast<const int64_t *>(put(p->ext_mapz));
         x.mapy = static_cast<const int64_t *>(put(p->ext_mapy));
         x.mapx = static_cast<const int64_t *>(put(p->ext_mapx));
+    } else if (p->engine == NDF_ENGINE_STACK) {
+        StackArgs &k = t->stack;
+        k = p->stack;
+        k.maps = static_cast<const int64_t *>(put(p->stk_maps));
+        k.mapr = static_cast<const int64_t *>(put(p->stk_mapr));
+        k.mapx = static_cast<const int64_t *>(put(p->stk_mapx));
     } else if (p->engine == NDF_ENGINE_NETWORK) {
         NetworkArgs &n = t->network;
         n = p->network;

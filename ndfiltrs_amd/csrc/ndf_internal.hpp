@@ -121,6 +121,40 @@ struct NetworkArgs {
     double q100;  // percentile / 100 (STAT_PERCENTILE)
 };
 
+// Engine 5: box windows of K0 slices along their slowest axis (3-D: kz
+// planes of ky x kx; 2-D: ky rows of 1 x kx), one sorted slice built per
+// output step and K0 sorted slices merged per output (ndf_stack.hip). A wave
+// owns a unit: 64 columns of one row of the row axis and a run of `run`
+// positions along the slide axis. Map entry i of an axis is coordinate
+// i + low of its window (the row axis of a 2-D window has no window: its
+// entry r is row r's offset).
+struct StackArgs {
+    const void *input;
+    void *output;
+    const int64_t *maps;  // slide axis: entries [0, ns + k0 - 1)
+    const int64_t *mapr;  // row axis: entries [0, nr + sh - 1)
+    const int64_t *mapx;  // columns: entries [0, tiles_x * 64 + sw - 1)
+    int64_t ns, nr, nx;
+    int64_t out_ss, out_sr;  // output strides of the slide and row axes
+    int64_t units;           // tiles_x * nr * ceil(ns / run)
+    int32_t k0, sh, sw;
+    int32_t sx;              // 64 + sw - 1 staged columns
+    int32_t tiles_x;
+    int32_t run;
+    uint64_t cval_bits;
+    int32_t ignore_inf;
+    int32_t stat;
+    int32_t min_count;
+    int32_t rank_param;
+    double q100;  // percentile / 100 (STAT_PERCENTILE)
+};
+
+// Whether a (k0, sh, sw) box has compiled networks for this dtype.
+bool stack_supported(int dtype_bytes, int k0, int sh, int sw);
+size_t stack_lds_bytes(int dtype_bytes, int sh, int sx);
+int64_t stack_max_units();
+hipError_t launch_stack(int dtype_bytes, const StackArgs &args, hipStream_t stream);
+
 // Engine 4: rectangular minimum / maximum (rank <= 3) by separable passes
 // over an LDS tile of tx columns x ty rows of one plane; its support is kz
 // planes of sy = ty + ky - 1 rows of sx = tx + kx - 1 samples. Maps as for

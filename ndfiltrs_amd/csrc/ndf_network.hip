@@ -22,6 +22,7 @@
 
 #include "ndf_device.hpp"
 #include "ndf_internal.hpp"
+#include "ndf_select.hpp"
 
 namespace ndf {
 namespace {
@@ -110,40 +111,6 @@ __device__ __forceinline__ void sort_keys(Key (&k)[LIVE])
         constexpr size_t len = total - BASE < NETWORK_CHUNK ? total - BASE : NETWORK_CHUNK;
         sort_chunk<Key, LIVE, BASE>(k, std::make_index_sequence<len>{});
         sort_keys<Key, LIVE, BASE + NETWORK_CHUNK>(k);
-    }
-}
-
-// a when take_a (selector 0x07060504: every byte from the first operand),
-// else b (0x03020100). A byte permute instead of a conditional keeps the
-// compiler from merging the two selected operands into one indexed access.
-__device__ __forceinline__ uint32_t pick_word(uint32_t a, uint32_t b, uint32_t selector)
-{
-    return __builtin_amdgcn_perm(a, b, selector);
-}
-__device__ __forceinline__ uint64_t pick_word(uint64_t a, uint64_t b, uint32_t selector)
-{
-    const uint32_t lo = __builtin_amdgcn_perm((uint32_t)a, (uint32_t)b, selector);
-    const uint32_t hi = __builtin_amdgcn_perm((uint32_t)(a >> 32), (uint32_t)(b >> 32), selector);
-    return ((uint64_t)hi << 32) | lo;
-}
-
-// k[index] for a per-lane index below N: a binary tree of selects on the
-// index bits, lowest first (N - 1 selects instead of the N compares and N
-// selects of a linear scan).
-template <typename Key, int N>
-__device__ __forceinline__ Key select_index(const Key (&k)[N], uint32_t index)
-{
-    if constexpr (N == 1) {
-        return k[0];
-    } else {
-        constexpr int M = (N + 1) / 2;
-        Key t[M];
-        const uint32_t selector = (index & 1u) != 0 ? 0x07060504u : 0x03020100u;
-#pragma unroll
-        for (int j = 0; j < M; ++j) {
-            t[j] = 2 * j + 1 < N ? pick_word(k[2 * j + 1 < N ? 2 * j + 1 : 2 * j], k[2 * j], selector) : k[2 * j];
-        }
-        return select_index<Key, M>(t, index >> 1);
     }
 }
 

@@ -1,0 +1,310 @@
+// Stack engine: box windows, sorted slices merged per output.
+//
+// A box window of K0 x SH x SW taps (3-D: kz planes of ky x kx; 2-D: ky rows
+// of 1 x kx) is K0 slices along its slowest axis. Neighbouring outputs along
+// that axis share K0 - 1 slices, so each wave slides along it:
+//   * per step it builds ONE new sorted slice: the lanes stage the slice's
+//     columns (SH samples at one x, classified through the boundary maps),
+//     sort each column in registers, exchange them through a wave-private LDS
+//     row, and each lane merges the SW sorted columns of its own window;
+//   * the output merges the K0 sorted slices with a merge network. A median
+//     only reads ranks 0 .. W/2 (omitted samples carry the all-ones key and
+//     sort last), so its network keeps only the comparators those ranks
+//     depend on; rank and percentile statistics merge the full order.
+// Per output this is one slice build plus one K0-way merge instead of a
+// W-wire sort: for 3 x 7 x 7 about 880 comparators against 2,257, for 5 x 5
+// about 94 against 140. The networks (and their 0-1 proofs) come from
+// gen_stack_networks.py. The reference computes the same statistic per
+// window with its median network or histogram (src/median_network.rs:65-209,
+// src/filter_nd.rs); results are bit-identical to every other engine.
+#include <utility>
+
+#include "ndf_device.hpp"
+#include "ndf_internal.hpp"
+#include "ndf_select.hpp"
+#include "ndf_stack_networks.inc"
+
+namespace ndf {
+namespace {
+
+constexpr int STACK_THREADS = 256;
+constexpr int STACK_WAVES = STACK_THREADS / 64;
+
+enum NetPart { NET_COL, NET_SLICE, NET_HALF, NET_FULL };
+
+template <class Net, int PART>
+struct Part;
+template <class Net>
+struct Part<Net, NET_COL> {
+    static constexpr int count = Net::col_count;
+    static constexpr int lo(int i) { return Net::col[i][0]; }
+    static constexpr int hi(int i) { return Net::col[i][1]; }
+};
+template <class Net>
+struct Part<Net, NET_SLICE> {
+    static constexpr int count = Net::slice_count;
+    static constexpr int lo(int i) { return Net::slice[i][0]; }
+    static constexpr int hi(int i) { return Net::slice[i][1]; }
+};
+template <class Net>
+struct Part<Net, NET_HALF> {
+    static constexpr int count = Net::half_count;
+    static constexpr int lo(int i) { return Net::half[i][0]; }
+    static constexpr int hi(int i) { return Net::half[i][1]; }
+};
+template <class Net>
+struct Part<Net, NET_FULL> {
+    static constexpr int count = Net::full_count;
+    static constexpr int lo(int i) { return Net::full[i][0]; }
+    static constexpr int hi(int i) { return Net::full[i][1]; }
+};
+
+// Comparators are applied in chunks of 512 so no fold expression exceeds the
+// compiler's nesting limit.
+constexpr int STACK_CHUNK = 512;
+
+template <class P, typename Key, int N, int BASE, size_t... I>
+__device__ __forceinline__ void apply_chunk(Key (&w)[N], std::index_sequence<I...>)
+{
+    (
+        [&] {
+            constexpr int a = P::lo(BASE + (int)I);
+            constexpr int b = P::hi(BASE + (int)I);
+            static_assert(a < N && b < N, "comparator outside the register array");
+            const Key x = w[a], y = w[b];
+            w[a] = x < y ? x : y;
+            w[b] = x < y ? y : x;
+        }(),
+        ...);
+}
+
+template <class P, typename Key, int N, int BASE = 0>
+__device__ __forceinline__ void apply_net(Key (&w)[N])
+{
+    if constexpr (BASE < P::count) {
+        constexpr int len = P::count - BASE < STACK_CHUNK ? P::count - BASE : STACK_CHUNK;
+        apply_chunk<P, Key, N, BASE>(w, std::make_index_sequence<(size_t)len>{});
+        apply_net<P, Key, N, BASE + STACK_CHUNK>(w);
+    }
+}
+
+// Combined offset of a slide-axis and a row-axis map entry: MAP_PAD when
+// either is padding, MAP_CVAL when either is outside under constant mode.
+__device__ __forceinline__ int64_t combine(int64_t a, int64_t b)
+{
+    if (a == MAP_PAD || b == MAP_PAD) return MAP_PAD;
+    if (a < 0 || b < 0) return MAP_CVAL;
+    return a + b;
+}
+
+template <typename T, int K0, int SH, int SW>
+__global__ void __launch_bounds__(STACK_THREADS) k_stack(StackArgs a)
+{
+    using Net = stack_net::Net<K0, SH, SW>;
+    using KT = KeyTraits<T>;
+    using Key = typename KT::Key;
+    constexpr int L = Net::L, W = Net::W, H = Net::H;
+    static_assert(L == SH * SW && W == K0 * L && H == W / 2 + 1, "network shape");
+    // LDS rows per wave: the SH sorted keys of each column, and its retained
+    // count (one row of single keys when SH == 1).
+    constexpr int ROWS = SH > 1 ? SH + 1 : 1;
+    extern __shared__ __align__(16) unsigned char lds[];
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int sx = a.sx;
+    Key *col = reinterpret_cast<Key *>(lds) + (size_t)wave * ROWS * sx;
+
+    // XCD-aware order: the grid is a multiple of 8 blocks, dispatched round
+    // robin over the 8 XCDs; XCD j runs the contiguous block range
+    // [j * nb / 8, (j + 1) * nb / 8), whose units share input rows in its L2.
+    const int nb8 = (int)(gridDim.x >> 3);
+    const int64_t block = (int64_t)(blockIdx.x & 7) * nb8 + (blockIdx.x >> 3);
+    const int64_t unit = block * STACK_WAVES + wave;
+    if (unit >= a.units) return;  // wave-uniform; the kernel has no barrier
+    const int64_t xc = unit % a.tiles_x;
+    const int64_t rest = unit / a.tiles_x;
+    const int64_t r = rest % a.nr;
+    const int64_t p0 = (rest / a.nr) * a.run;
+    const int64_t p1 = p0 + a.run < a.ns ? p0 + a.run : a.ns;
+    const int64_t x0 = xc * 64;
+
+    const Key *input = static_cast<const Key *>(a.input);
+    Key *output = static_cast<Key *>(a.output);
+    const Key cval_key = classify<T>((Key)a.cval_bits, a.ignore_inf);
+    const int64_t mxa = a.mapx[x0 + lane];
+    const bool second = lane + 64 < sx;
+    const int64_t mxb = second ? a.mapx[x0 + lane + 64] : MAP_PAD;
+    int64_t mr[SH];
+#pragma unroll
+    for (int dy = 0; dy < SH; ++dy) mr[dy] = uniform_entry(a.mapr, r + dy);
+
+    auto sample = [&](int64_t base, int64_t mx) -> Key {
+        const bool pad = base == MAP_PAD || mx == MAP_PAD;
+        const bool cv = !pad && (base == MAP_CVAL || mx < 0);
+        const Key raw = input[pad || cv ? 0 : base + mx];
+        return pad ? KT::SENTINEL : (cv ? cval_key : classify<T>(raw, a.ignore_inf));
+    };
+
+    // Stages column c (map entry mx) of slide position q into LDS.
+    auto stage = [&](int64_t ms, int c, int64_t mx) {
+        Key ck[SH];
+#pragma unroll
+        for (int dy = 0; dy < SH; ++dy) ck[dy] = sample(combine(ms, mr[dy]), mx);
+        if constexpr (SH > 1) {
+            int n = 0;
+#pragma unroll
+            for (int dy = 0; dy < SH; ++dy) n += ck[dy] != KT::SENTINEL;
+            apply_net<Part<Net, NET_COL>>(ck);
+#pragma unroll
+            for (int dy = 0; dy < SH; ++dy) col[dy * sx + c] = ck[Net::col_out[dy]];
+            col[SH * sx + c] = (Key)n;
+        } else {
+            col[c] = ck[0];
+        }
+    };
+
+    // Builds the sorted slice of window coordinate index q (along the slide
+    // axis) for this lane's window; `count` receives its retained samples.
+    auto build = [&](Key (&dst)[L], int &count, int64_t q) {
+        const int64_t ms = uniform_entry(a.maps, q);
+        wave_lds_sync();  // the previous slice's reads of `col` are done
+        stage(ms, lane, mxa);
+        if (second) stage(ms, lane + 64, mxb);
+        wave_lds_sync();
+        Key v[L];
+#pragma unroll
+        for (int dx = 0; dx < SW; ++dx) {
+#pragma unroll
+            for (int dy = 0; dy < SH; ++dy) v[dx * SH + dy] = col[dy * sx + lane + dx];
+        }
+        int n = 0;
+        if constexpr (SH > 1) {
+#pragma unroll
+            for (int dx = 0; dx < SW; ++dx) n += (int)col[SH * sx + lane + dx];
+        } else {
+#pragma unroll
+            for (int i = 0; i < L; ++i) n += v[i] != KT::SENTINEL;
+        }
+        apply_net<Part<Net, NET_SLICE>>(v);
+#pragma unroll
+        for (int i = 0; i < L; ++i) dst[i] = v[Net::slice_out[i]];
+        count = n;
+    };
+
+    Key S[K0][L];
+    int cnt[K0];
+#pragma unroll
+    for (int s = 0; s + 1 < K0; ++s) build(S[s], cnt[s], p0 + s);
+
+    const int64_t x = x0 + lane;
+#pragma unroll 1
+    for (int64_t p = p0; p < p1; ++p) {
+        build(S[K0 - 1], cnt[K0 - 1], p + K0 - 1);
+        Key w[W];
+        int n = 0;
+#pragma unroll
+        for (int s = 0; s < K0; ++s) {
+            n += cnt[s];
+#pragma unroll
+            for (int i = 0; i < L; ++i) w[s * L + i] = S[s][i];
+        }
+        Key out = KT::CANONICAL_NAN;
+        if (a.stat == STAT_MEDIAN) {
+            apply_net<Part<Net, NET_HALF>>(w);
+            Key h[H];
+#pragma unroll
+            for (int i = 0; i < H; ++i) h[i] = w[Net::half_out[i]];
+            const bool ok = n >= a.min_count && n > 0;
+            const int hi = n >> 1;
+            const bool pair = (n & 1) == 0;
+            const int lo = pair ? hi - 1 : hi;
+            // Few omitted samples put both ranks in the top F of the half:
+            // a short select tree when the whole wave qualifies.
+            constexpr int F = H < 16 ? H : 16;
+            const bool narrow = !ok || lo >= H - F;
+            Key klo, khi;
+            if (__builtin_amdgcn_ballot_w64(!narrow) == 0) {
+                Key t[F];
+#pragma unroll
+                for (int j = 0; j < F; ++j) t[j] = h[H - F + j];
+                khi = select_index<Key, F>(t, ok ? (uint32_t)(hi - (H - F)) : 0u);
+                klo = select_index<Key, F>(t, ok ? (uint32_t)(lo - (H - F)) : 0u);
+            } else {
+                khi = select_index<Key, H>(h, ok ? (uint32_t)hi : 0u);
+                klo = select_index<Key, H>(h, ok ? (uint32_t)lo : 0u);
+            }
+            if (ok) out = pair ? median_pair_bits<T>(klo, khi) : KT::unkey(khi);
+        } else {
+            apply_net<Part<Net, NET_FULL>>(w);
+            Key f[W];
+#pragma unroll
+            for (int i = 0; i < W; ++i) f[i] = w[Net::full_out[i]];
+            Pick pick;
+            const bool ok = pick_ranks(a.stat, n, a.min_count, a.rank_param, a.q100, pick);
+            const uint32_t hi_index = ok ? (uint32_t)pick.hi : 0u;
+            const Key khi = select_index<Key, W>(f, hi_index);
+            Key klo = khi;
+            if (a.stat != STAT_RANK) klo = select_index<Key, W>(f, ok && pick.pair ? hi_index - 1u : hi_index);
+            if (ok) out = pick.pair ? pair_bits<T>(a.stat, klo, khi, pick.frac) : KT::unkey(khi);
+        }
+        if (x < a.nx) output[p * a.out_ss + r * a.out_sr + x] = out;
+#pragma unroll
+        for (int s = 0; s + 1 < K0; ++s) {
+            cnt[s] = cnt[s + 1];
+#pragma unroll
+            for (int i = 0; i < L; ++i) S[s][i] = S[s + 1][i];
+        }
+    }
+}
+
+template <typename T, int K0, int SH, int SW>
+hipError_t launch_shape(const StackArgs &args, hipStream_t stream)
+{
+    const size_t lds = stack_lds_bytes((int)sizeof(T), SH, args.sx);
+    const int64_t blocks = (args.units + STACK_WAVES - 1) / STACK_WAVES;
+    const int64_t grid = (blocks + 7) / 8 * 8;
+    hipLaunchKernelGGL((k_stack<T, K0, SH, SW>), dim3((unsigned)grid), dim3(STACK_THREADS), lds, stream, args);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+size_t stack_lds_bytes(int dtype_bytes, int sh, int sx)
+{
+    return (size_t)STACK_WAVES * (size_t)(sh > 1 ? sh + 1 : 1) * (size_t)sx * (size_t)dtype_bytes;
+}
+
+bool stack_supported(int dtype_bytes, int k0, int sh, int sw)
+{
+    const int w = k0 * sh * sw;
+    if (dtype_bytes == 8 && w > 49) return false;
+    return (sh == 1 && k0 == sw && (k0 == 3 || k0 == 5 || k0 == 7 || k0 == 9)) ||
+           (k0 == 3 && sh == sw && (sh == 3 || sh == 5 || sh == 7));
+}
+
+int64_t stack_max_units() { return ((int64_t)1 << 31) - 8 * STACK_WAVES; }
+
+hipError_t launch_stack(int dtype_bytes, const StackArgs &args, hipStream_t stream)
+{
+#define NDF_STACK_CASE(K0, SH, SW)                                                              \
+    if (args.k0 == K0 && args.sh == SH && args.sw == SW) {                                      \
+        return dtype_bytes == 4 ? launch_shape<float, K0, SH, SW>(args, stream)                 \
+                                : launch_shape<double, K0, SH, SW>(args, stream);               \
+    }
+#define NDF_STACK_CASE_F32(K0, SH, SW)                                                          \
+    if (args.k0 == K0 && args.sh == SH && args.sw == SW) {                                      \
+        return dtype_bytes == 4 ? launch_shape<float, K0, SH, SW>(args, stream) : hipErrorInvalidValue; \
+    }
+    NDF_STACK_CASE(3, 1, 3)
+    NDF_STACK_CASE(5, 1, 5)
+    NDF_STACK_CASE(7, 1, 7)
+    NDF_STACK_CASE_F32(9, 1, 9)
+    NDF_STACK_CASE(3, 3, 3)
+    NDF_STACK_CASE_F32(3, 5, 5)
+    NDF_STACK_CASE_F32(3, 7, 7)
+#undef NDF_STACK_CASE
+#undef NDF_STACK_CASE_F32
+    return hipErrorInvalidValue;
+}
+
+}  // namespace ndf

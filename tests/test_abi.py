@@ -77,7 +77,10 @@ def test_engine_choice_for_the_benchmark_shapes() -> None:
     c3 = _plan(np.float32, (8192, 8192), (13, 13), footprint=ring, origin=[1, 0], mode=["wrap"] * 2)
     assert c3["engine"] == "network" and c3["count"] == 68 and c3["network_wires"] == 72
     cube = _plan(np.float32, (128, 1024, 1024), (3, 7, 7), mode=["wrap"] * 3)
-    assert cube["engine"] == "network" and cube["count"] == 147 and cube["network_wires"] == 160
+    assert cube["engine"] == "stack" and cube["count"] == 147 and cube["slices"] == [3, 7, 7]
+    assert cube["run"] == 128 and cube["units"] == 1024 * 16
+    c1 = _plan(np.float32, (2048, 2048), (5, 5))
+    assert c1["engine"] == "stack" and c1["slices"] == [5, 1, 5] and c1["run"] == 10 and c1["units"] == 32 * 205
     five = _plan(np.float64, (2, 2, 2, 2, 3), (3, 3, 3, 3, 5))
     assert five["engine"] == "generic"
     fill = _plan(np.float32, (10, 10), (3, 3), kind="rank", parameter=9)
@@ -93,12 +96,12 @@ def test_plan_rejects_length_mismatch_without_a_device() -> None:
     assert err.value.decode() == "input length does not match the configured shape"
 
 
-@pytest.mark.parametrize("engine", ["generic", "slide", "network", "auto"])
+@pytest.mark.parametrize("engine", ["generic", "slide", "network", "stack", "auto"])
 def test_engines_can_be_forced(engine: str) -> None:
     plan = _core.Plan(np.float32, (40, 40), "median", (5, 5), None, (0, 0), ("reflect",) * 2, 0.0, False, False, 1, 0)
     plan.set_engine(engine)
     described = json.loads(plan.describe())["engine"]
-    assert described == ("network" if engine == "auto" else engine)
+    assert described == ("stack" if engine == "auto" else engine)
 
 
 def test_network_engine_limits() -> None:
@@ -110,6 +113,8 @@ def test_network_engine_limits() -> None:
     f64 = _core.Plan(np.float64, (40, 40), "median", (9, 9), None, (0, 0), ("reflect",) * 2, 0.0, True, False, 1, 0)
     assert json.loads(f64.describe())["engine"] == "slide"  # 81 taps > 64 for float64
     f32 = _core.Plan(np.float32, (40, 40), "median", (9, 9), None, (0, 0), ("reflect",) * 2, 0.0, True, False, 1, 0)
+    assert json.loads(f32.describe())["engine"] == "stack"  # a 9x9 box has slice networks
+    f32.set_engine("network")
     assert json.loads(f32.describe())["network_wires"] == 96
     for size, wires in (((11, 11), 128), ((12, 12), 160)):
         plan = _core.Plan(np.float32, (40, 40), "median", size, None, (0, 0), ("reflect",) * 2, 0.0, True, False, 1, 0)

@@ -3,8 +3,8 @@
 Every test calls the C-ABI library (libndfgpu.so) on the device. Inputs carry
 ties, signed zeros, infinities, NaN payloads and the largest finite value
 (tests/test_histogram_median.py:34-46 law); every engine that can run a
-geometry (slide, generic, and network for small windows) is forced in turn
-and must give the oracle's bits.
+geometry (slide, generic, network for small windows, stack for box windows)
+is forced in turn and must give the oracle's bits.
 """
 
 from __future__ import annotations
@@ -63,14 +63,14 @@ def run_plan(values: np.ndarray, function: str, kwargs: dict, rank: int | None =
         except ValueError as error:
             # The network engine takes small windows only; other engines must
             # always be available where the tests force them.
-            if engine == "network" and "network engine needs" in str(error):
+            if engine in ("network", "stack") and f"{engine} engine needs" in str(error):
                 return None
             raise
     return plan.apply(np.ascontiguousarray(values))
 
 
 # Every engine that can run a geometry must give the oracle's bits.
-ENGINES = ("slide", "generic", "network")
+ENGINES = ("slide", "generic", "network", "stack")
 
 
 def assert_bits(actual: np.ndarray | None, expected: np.ndarray, context: str = "") -> None:
@@ -95,7 +95,7 @@ def test_gpu_matches_golden_cases(entry: dict) -> None:
     array = _cases.kat_input(entry)
     kwargs = _cases.kat_kwargs(entry)
     _cases.check_kat(entry, _cases.public_call(nf, entry["function"], array, kwargs, entry.get("rank")))
-    for engine in ("generic", "slide", "network"):
+    for engine in ("generic", "slide", "network", "stack"):
         try:
             out = run_plan(array, entry["function"], kwargs, entry.get("rank"), engine)
         except ValueError as error:
