@@ -50,6 +50,10 @@ CONFIGS = [
     (3, 7, 7),
 ]
 
+# Live wire counts of the network engine (one window of up to LIVE taps,
+# padded with all-ones keys, sorted per output).
+SORT_LIVES = [9, 16, 25, 32, 49, 64, 72, 96, 128, 160]
+
 HERE = os.path.dirname(os.path.abspath(__file__))
 HEADER = os.path.join(HERE, "ndf_stack_networks.inc")
 
@@ -165,6 +169,40 @@ def build(k0: int, sh: int, sw: int) -> dict:
                 full=best_full[0], full_out=best_full[1])
 
 
+def build_sort(live: int) -> dict:
+    """Sorting network of the network engine: a balanced merge tree of single
+    wires; `half` keeps the comparators ranks 0 .. live/2 depend on."""
+    comps, out = merge_tree([[i] for i in range(live)], balanced(tuple(range(live))))
+    half_ranks = out[: live // 2 + 1]
+    return dict(live=live, full=comps, full_out=out, half=cone(comps, half_ranks), half_out=half_ranks)
+
+
+def tree_merge_sizes(items: tuple[int, ...]) -> set[tuple[int, int]]:
+    """The (m, n) list lengths a balanced merge tree of single wires merges."""
+    if len(items) == 1:
+        return set()
+    h = (len(items) + 1) // 2
+    return {(h, len(items) - h)} | tree_merge_sizes(items[:h]) | tree_merge_sizes(items[h:])
+
+
+def verify_sort(net: dict, random_trials: int = 3000, seed: int = 1) -> None:
+    """Every two-list merge of the tree on every 0-1 input, then the whole
+    network (full and half) on random keys with ties and sentinels, and on
+    every 0-1 input when there are few enough."""
+    live = net["live"]
+    for m, n in tree_merge_sizes(tuple(range(live))):
+        verify_merge(m, n)
+    rng = np.random.default_rng(seed)
+    keys = rng.integers(0, 6, size=(random_trials, live)).astype(np.uint32)
+    keys[rng.random(keys.shape) < 0.2] = 0xFFFFFFFF
+    if 2**live <= EXHAUSTIVE_LIMIT:
+        keys = np.concatenate([keys, np.array(list(itertools.product((0, 1), repeat=live)), dtype=np.uint32)])
+    expect = np.sort(keys, axis=1)
+    assert np.array_equal(apply(net["full"], keys)[:, net["full_out"]], expect), "sort network does not sort"
+    assert np.array_equal(apply(net["half"], keys)[:, net["half_out"]], expect[:, : live // 2 + 1]), \
+        "half sort network does not give the lower half"
+
+
 def apply(comps: list[tuple[int, int]], values: np.ndarray) -> np.ndarray:
     """Apply comparators to rows of `values` (one input per row)."""
     v = values.copy()
@@ -265,7 +303,20 @@ def render() -> str:
         "template <int K0, int SH, int SW>",
         "struct Net;",
         "",
+        "// Sort<LIVE>: sorting network of the network engine on LIVE wires (a",
+        "// window's taps padded with all-ones keys).",
+        "template <int LIVE>",
+        "struct Sort;",
+        "",
     ]
+    for live in SORT_LIVES:
+        net = build_sort(live)
+        out.append(f"template <>\nstruct Sort<{live}> {{")
+        out.append(f"    static constexpr int W = {live}, H = {live // 2 + 1};")
+        text = _pairs("half", net["half"]) + _list("half_out", net["half_out"])
+        text += _pairs("full", net["full"]) + _list("full_out", net["full_out"])
+        out.append(text.rstrip("\n"))
+        out.append("};\n")
     for k0, sh, sw in CONFIGS:
         net = build(k0, sh, sw)
         out.append(f"template <>\nstruct Net<{k0}, {sh}, {sw}> {{")
@@ -281,6 +332,10 @@ def render() -> str:
 
 
 def main() -> None:
+    for live in SORT_LIVES:
+        net = build_sort(live)
+        verify_sort(net)
+        print(f"Sort<{live}>: half={len(net['half'])} full={len(net['full'])}", file=sys.stderr)
     for cfg in CONFIGS:
         net = build(*cfg)
         verify(net)

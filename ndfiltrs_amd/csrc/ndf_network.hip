@@ -9,114 +9,90 @@
 //     classifying each sample once (omitted samples become the sentinel key,
 //     constant-mode samples the cval key);
 //   * each thread gathers its window's keys from LDS into registers (taps
-//     beyond the window are sentinels) and sorts them with a Batcher
-//     odd-even merge network generated at compile time and pruned to the
-//     live wires (comparators into sentinel-only wires are no-ops);
+//     beyond the window are sentinels) and sorts them with a network of
+//     Batcher odd-even merges (a balanced merge tree of single wires,
+//     generated with its 0-1 proof by gen_stack_networks.py);
 //   * sentinels sort last, so the retained count n is the number of
 //     non-sentinel keys and the statistic is read at a data-dependent index
-//     with a binary tree of selects.
+//     with a binary tree of selects. A median never reads past rank LIVE/2,
+//     so its kernels keep only the comparators the lower half depends on.
 // The result is the exact sample (or the f64 midpoint of two), bit-identical
 // to the slide and generic engines.
-#include <array>
 #include <utility>
 
 #include "ndf_device.hpp"
 #include "ndf_internal.hpp"
 #include "ndf_select.hpp"
+#include "ndf_stack_networks.inc"
 
 namespace ndf {
 namespace {
 
 constexpr int NETWORK_THREADS = 256;
 
-// Batcher odd-even merge sort on WIRES (a power of two) wires, keeping only
-// the comparators whose upper wire is below LIVE.
-struct Comparator {
-    int lo, hi;
+// The window's sorting network (ndf_stack_networks.inc): a balanced merge
+// tree of single wires; the median applies only the comparators that ranks
+// 0 .. LIVE/2 depend on (omitted samples carry the all-ones key and sort
+// last, so a median's ranks never pass LIVE/2).
+template <int LIVE>
+struct SortFull {
+    static constexpr int count = stack_net::Sort<LIVE>::full_count;
+    static constexpr int lo(int i) { return stack_net::Sort<LIVE>::full[i][0]; }
+    static constexpr int hi(int i) { return stack_net::Sort<LIVE>::full[i][1]; }
+};
+template <int LIVE>
+struct SortHalf {
+    static constexpr int count = stack_net::Sort<LIVE>::half_count;
+    static constexpr int lo(int i) { return stack_net::Sort<LIVE>::half[i][0]; }
+    static constexpr int hi(int i) { return stack_net::Sort<LIVE>::half[i][1]; }
 };
 
-template <int WIRES, int LIVE>
-constexpr int batcher_count()
-{
-    int c = 0;
-    for (int p = 1; p < WIRES; p <<= 1) {
-        for (int k = p; k >= 1; k >>= 1) {
-            for (int j = k % p; j + k < WIRES; j += 2 * k) {
-                for (int i = 0; i < k && i < WIRES - j - k; ++i) {
-                    if ((i + j) / (2 * p) == (i + j + k) / (2 * p) && i + j + k < LIVE) {
-                        ++c;
-                    }
-                }
-            }
-        }
-    }
-    return c;
-}
+// Which statistics a kernel instance computes: the median only, the others
+// only, or either (chosen per image at run time).
+enum StatSet { STATS_MEDIAN, STATS_OTHER, STATS_ANY };
 
-template <int WIRES, int LIVE>
-constexpr std::array<Comparator, batcher_count<WIRES, LIVE>()> batcher_pairs()
+// The output bits of one window (k: its LIVE keys, n of them retained).
+template <typename T, int LIVE, int STATS>
+__device__ __forceinline__ typename KeyTraits<T>::Key window_result(const NetworkArgs &a,
+                                                                     typename KeyTraits<T>::Key (&k)[LIVE], int n)
 {
-    std::array<Comparator, batcher_count<WIRES, LIVE>()> out{};
-    int c = 0;
-    for (int p = 1; p < WIRES; p <<= 1) {
-        for (int k = p; k >= 1; k >>= 1) {
-            for (int j = k % p; j + k < WIRES; j += 2 * k) {
-                for (int i = 0; i < k && i < WIRES - j - k; ++i) {
-                    if ((i + j) / (2 * p) == (i + j + k) / (2 * p) && i + j + k < LIVE) {
-                        out[c] = Comparator{i + j, i + j + k};
-                        ++c;
-                    }
-                }
-            }
+    using KT = KeyTraits<T>;
+    using Key = typename KT::Key;
+    using Sort = stack_net::Sort<LIVE>;
+    Key out = KT::CANONICAL_NAN;
+    if (STATS == STATS_MEDIAN || (STATS == STATS_ANY && a.stat == STAT_MEDIAN)) {
+        if constexpr (STATS != STATS_OTHER) {
+            apply_net<SortHalf<LIVE>>(k);
+            Key h[Sort::H];
+#pragma unroll
+            for (int i = 0; i < Sort::H; ++i) h[i] = k[Sort::half_out[i]];
+            const bool ok = n >= a.min_count && n > 0;
+            const int hi = n >> 1;
+            const bool pair = (n & 1) == 0;
+            const int lo = pair ? hi - 1 : hi;
+            Key klo, khi;
+            median_ranks<Key, Sort::H>(h, ok, lo, hi, klo, khi);
+            if (ok) out = pair ? median_pair_bits<T>(klo, khi) : KT::unkey(khi);
         }
+    } else if constexpr (STATS != STATS_MEDIAN) {
+        apply_net<SortFull<LIVE>>(k);
+        Key f[LIVE];
+#pragma unroll
+        for (int i = 0; i < LIVE; ++i) f[i] = k[Sort::full_out[i]];
+        Pick pick;
+        const bool ok = pick_ranks(a.stat, n, a.min_count, a.rank_param, a.q100, pick);
+        const uint32_t hi_index = ok ? (uint32_t)pick.hi : 0u;
+        const Key hi = select_index<Key, LIVE>(f, hi_index);
+        Key lo = hi;
+        if (a.stat != STAT_RANK) lo = select_index<Key, LIVE>(f, ok && pick.pair ? hi_index - 1u : hi_index);
+        if (ok) out = pick.pair ? pair_bits<T>(a.stat, lo, hi, pick.frac) : KT::unkey(hi);
     }
     return out;
 }
 
-constexpr int wires_for(int live)
-{
-    int w = 1;
-    while (w < live) {
-        w <<= 1;
-    }
-    return w;
-}
-
-template <int LIVE>
-constexpr auto NETWORK = batcher_pairs<wires_for(LIVE), LIVE>();
-
-// The comparators are applied in chunks of at most 1024 so that no single
-// fold expression exceeds the compiler's nesting limit.
-constexpr size_t NETWORK_CHUNK = 1024;
-
-template <typename Key, int LIVE, size_t BASE, size_t... I>
-__device__ __forceinline__ void sort_chunk(Key (&k)[LIVE], std::index_sequence<I...>)
-{
-    (
-        [&] {
-            constexpr int a = NETWORK<LIVE>[BASE + I].lo;
-            constexpr int b = NETWORK<LIVE>[BASE + I].hi;
-            const Key x = k[a], y = k[b];
-            k[a] = x < y ? x : y;
-            k[b] = x < y ? y : x;
-        }(),
-        ...);
-}
-
-template <typename Key, int LIVE, size_t BASE = 0>
-__device__ __forceinline__ void sort_keys(Key (&k)[LIVE])
-{
-    constexpr size_t total = NETWORK<LIVE>.size();
-    if constexpr (BASE < total) {
-        constexpr size_t len = total - BASE < NETWORK_CHUNK ? total - BASE : NETWORK_CHUNK;
-        sort_chunk<Key, LIVE, BASE>(k, std::make_index_sequence<len>{});
-        sort_keys<Key, LIVE, BASE + NETWORK_CHUNK>(k);
-    }
-}
-
 // One tile: stage the support (classified keys) into LDS, then one output
 // per thread and row step. `taps` holds the window's support offsets.
-template <typename T, int LIVE>
+template <typename T, int LIVE, int STATS>
 __device__ __forceinline__ void network_tile(const NetworkArgs &a, int tile,
                                              typename KeyTraits<T>::Key *support, const uint32_t *taps)
 {
@@ -156,7 +132,6 @@ __device__ __forceinline__ void network_tile(const NetworkArgs &a, int tile,
     }
     __syncthreads();
     const int64_t x = x0 + lane;
-    const bool may_pair = a.stat != STAT_RANK;
     for (int r = wave; r < a.ty; r += 4) {
         const int64_t y = y0 + r;
         if (y >= a.ny || x >= a.nx) {
@@ -172,25 +147,13 @@ __device__ __forceinline__ void network_tile(const NetworkArgs &a, int tile,
             k[i] = window[taps[i]];
             n += k[i] != KT::SENTINEL;
         }
-        sort_keys<Key, LIVE>(k);
-        Key out = KT::CANONICAL_NAN;
-        Pick pick;
-        const bool ok = pick_ranks(a.stat, n, a.min_count, a.rank_param, a.q100, pick);
-        const uint32_t hi_index = ok ? (uint32_t)pick.hi : 0u;
-        const Key hi = select_index<Key, LIVE>(k, hi_index);
-        Key lo = hi;
-        if (may_pair) {
-            lo = select_index<Key, LIVE>(k, ok && pick.pair ? hi_index - 1u : hi_index);
-        }
-        if (ok) {
-            out = pick.pair ? pair_bits<T>(a.stat, lo, hi, pick.frac) : KT::unkey(hi);
-        }
+        const Key out = window_result<T, LIVE, STATS>(a, k, n);
         output[(z0 * a.ny + y) * a.nx + x] = out;
     }
 }
 
 // One image: workgroups stride over the tiles.
-template <typename T, int LIVE>
+template <typename T, int LIVE, int STATS>
 __global__ void __launch_bounds__(NETWORK_THREADS) k_network(NetworkArgs a)
 {
     using Key = typename KeyTraits<T>::Key;
@@ -202,7 +165,7 @@ __global__ void __launch_bounds__(NETWORK_THREADS) k_network(NetworkArgs a)
     }
     const int tiles = a.tiles_x * a.tiles_y * (int)a.nz;
     for (int tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
-        network_tile<T, LIVE>(a, tile, support, taps);
+        network_tile<T, LIVE, STATS>(a, tile, support, taps);
     }
 }
 
@@ -232,7 +195,7 @@ __global__ void __launch_bounds__(NETWORK_THREADS) k_network_batch(const Network
         NetworkArgs a = images[image];
         a.input = static_cast<const Key *>(input) + offsets[image];
         a.output = static_cast<Key *>(output) + offsets[image];
-        network_tile<T, LIVE>(a, work[2 * w + 1], support, a.taps);
+        network_tile<T, LIVE, STATS_ANY>(a, work[2 * w + 1], support, a.taps);
     }
 }
 
@@ -264,10 +227,10 @@ hipError_t batch_grid_live(size_t lds_bytes, int *grid)
     return hipSuccess;
 }
 
-template <typename T, int LIVE>
-hipError_t launch_live(const NetworkArgs &args, size_t lds_bytes, hipStream_t stream)
+template <typename T, int LIVE, int STATS>
+hipError_t launch_stats(const NetworkArgs &args, size_t lds_bytes, hipStream_t stream)
 {
-    const void *fn = reinterpret_cast<const void *>(&k_network<T, LIVE>);
+    const void *fn = reinterpret_cast<const void *>(&k_network<T, LIVE, STATS>);
     hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
     if (e != hipSuccess) {
         return e;
@@ -277,9 +240,16 @@ hipError_t launch_live(const NetworkArgs &args, size_t lds_bytes, hipStream_t st
     // strides only past 2^24 tiles.
     const int64_t tiles = (int64_t)args.tiles_x * args.tiles_y * args.nz;
     const int64_t blocks = tiles < ((int64_t)1 << 24) ? tiles : ((int64_t)1 << 24);
-    hipLaunchKernelGGL((k_network<T, LIVE>), dim3((unsigned)(blocks < 1 ? 1 : blocks)), dim3(NETWORK_THREADS),
-                       lds_bytes, stream, args);
+    hipLaunchKernelGGL((k_network<T, LIVE, STATS>), dim3((unsigned)(blocks < 1 ? 1 : blocks)),
+                       dim3(NETWORK_THREADS), lds_bytes, stream, args);
     return hipGetLastError();
+}
+
+template <typename T, int LIVE>
+hipError_t launch_live(const NetworkArgs &args, size_t lds_bytes, hipStream_t stream)
+{
+    return args.stat == STAT_MEDIAN ? launch_stats<T, LIVE, STATS_MEDIAN>(args, lds_bytes, stream)
+                                    : launch_stats<T, LIVE, STATS_OTHER>(args, lds_bytes, stream);
 }
 
 }  // namespace

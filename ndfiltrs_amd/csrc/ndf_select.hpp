@@ -1,10 +1,14 @@
 // Register-array helpers shared by the register engines (network, stack):
-// reading a sorted register array at a per-lane index, and a wave-level
-// LDS ordering point.
+// applying a compile-time comparator network, reading a sorted register
+// array at a per-lane index (and a median's two ranks from a lower half),
+// and a wave-level LDS ordering point.
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <stddef.h>
 #include <stdint.h>
+
+#include <utility>
 
 namespace ndf {
 
@@ -40,6 +44,67 @@ __device__ __forceinline__ Key select_index(const Key (&k)[N], uint32_t index)
         }
         return select_index<Key, M>(t, index >> 1);
     }
+}
+
+// Applies the comparator network P (P::count comparators; comparator i puts
+// the smaller key on wire P::lo(i) and the larger on P::hi(i)) to the
+// register array w. Comparators are applied in chunks of 512 so no fold
+// expression exceeds the compiler's nesting limit.
+constexpr int NET_CHUNK = 512;
+
+template <class P, typename Key, int N, int BASE, size_t... I>
+__device__ __forceinline__ void apply_chunk(Key (&w)[N], std::index_sequence<I...>)
+{
+    (
+        [&] {
+            constexpr int a = P::lo(BASE + (int)I);
+            constexpr int b = P::hi(BASE + (int)I);
+            static_assert(a < N && b < N, "comparator outside the register array");
+            const Key x = w[a], y = w[b];
+            w[a] = x < y ? x : y;
+            w[b] = x < y ? y : x;
+        }(),
+        ...);
+}
+
+template <class P, typename Key, int N, int BASE = 0>
+__device__ __forceinline__ void apply_net(Key (&w)[N])
+{
+    if constexpr (BASE < P::count) {
+        constexpr int len = P::count - BASE < NET_CHUNK ? P::count - BASE : NET_CHUNK;
+        apply_chunk<P, Key, N, BASE>(w, std::make_index_sequence<(size_t)len>{});
+        apply_net<P, Key, N, BASE + NET_CHUNK>(w);
+    }
+}
+
+// Reads ranks lo <= hi of the lower half h (both below H; `need` false:
+// any ranks). Few omitted samples put both in the top ranks of the half, so
+// the wave reads from the top 4, else the top 16, when every lane qualifies,
+// and from all H otherwise (wave-uniform branches).
+template <typename Key, int H, int F>
+__device__ __forceinline__ bool top_ranks(const Key (&h)[H], bool need, int lo, int hi, Key &klo, Key &khi)
+{
+    const bool fits = !need || lo >= H - F;
+    if (__builtin_amdgcn_ballot_w64(!fits) != 0) return false;
+    Key t[F];
+#pragma unroll
+    for (int j = 0; j < F; ++j) t[j] = h[H - F + j];
+    khi = select_index<Key, F>(t, need ? (uint32_t)(hi - (H - F)) : 0u);
+    klo = select_index<Key, F>(t, need ? (uint32_t)(lo - (H - F)) : 0u);
+    return true;
+}
+
+template <typename Key, int H>
+__device__ __forceinline__ void median_ranks(const Key (&h)[H], bool need, int lo, int hi, Key &klo, Key &khi)
+{
+    if constexpr (H > 4) {
+        if (top_ranks<Key, H, 4>(h, need, lo, hi, klo, khi)) return;
+    }
+    if constexpr (H > 16) {
+        if (top_ranks<Key, H, 16>(h, need, lo, hi, klo, khi)) return;
+    }
+    khi = select_index<Key, H>(h, need ? (uint32_t)hi : 0u);
+    klo = select_index<Key, H>(h, need ? (uint32_t)lo : 0u);
 }
 
 // Orders the LDS accesses of one wave around this point: a wave's LDS

@@ -59,35 +59,6 @@ struct Part<Net, NET_FULL> {
     static constexpr int hi(int i) { return Net::full[i][1]; }
 };
 
-// Comparators are applied in chunks of 512 so no fold expression exceeds the
-// compiler's nesting limit.
-constexpr int STACK_CHUNK = 512;
-
-template <class P, typename Key, int N, int BASE, size_t... I>
-__device__ __forceinline__ void apply_chunk(Key (&w)[N], std::index_sequence<I...>)
-{
-    (
-        [&] {
-            constexpr int a = P::lo(BASE + (int)I);
-            constexpr int b = P::hi(BASE + (int)I);
-            static_assert(a < N && b < N, "comparator outside the register array");
-            const Key x = w[a], y = w[b];
-            w[a] = x < y ? x : y;
-            w[b] = x < y ? y : x;
-        }(),
-        ...);
-}
-
-template <class P, typename Key, int N, int BASE = 0>
-__device__ __forceinline__ void apply_net(Key (&w)[N])
-{
-    if constexpr (BASE < P::count) {
-        constexpr int len = P::count - BASE < STACK_CHUNK ? P::count - BASE : STACK_CHUNK;
-        apply_chunk<P, Key, N, BASE>(w, std::make_index_sequence<(size_t)len>{});
-        apply_net<P, Key, N, BASE + STACK_CHUNK>(w);
-    }
-}
-
 // Combined offset of a slide-axis and a row-axis map entry: MAP_PAD when
 // either is padding, MAP_CVAL when either is outside under constant mode.
 __device__ __forceinline__ int64_t combine(int64_t a, int64_t b)
@@ -97,7 +68,10 @@ __device__ __forceinline__ int64_t combine(int64_t a, int64_t b)
     return a + b;
 }
 
-template <typename T, int K0, int SH, int SW>
+// MEDIAN: the statistic is the median (the lower-half network); otherwise
+// rank or percentile (the full network). Separate kernels keep the median's
+// register count to what its network needs.
+template <typename T, int K0, int SH, int SW, bool MEDIAN>
 __global__ void __launch_bounds__(STACK_THREADS) k_stack(StackArgs a)
 {
     using Net = stack_net::Net<K0, SH, SW>;
@@ -138,18 +112,30 @@ __global__ void __launch_bounds__(STACK_THREADS) k_stack(StackArgs a)
 #pragma unroll
     for (int dy = 0; dy < SH; ++dy) mr[dy] = uniform_entry(a.mapr, r + dy);
 
-    auto sample = [&](int64_t base, int64_t mx) -> Key {
+    // Raw bits of one sample (a harmless in-bounds read where the sample is
+    // padding or cval), and its key once the bits have arrived.
+    auto fetch = [&](int64_t base, int64_t mx) -> Key { return input[base < 0 || mx < 0 ? 0 : base + mx]; };
+    auto keyed = [&](Key bits, int64_t base, int64_t mx) -> Key {
         const bool pad = base == MAP_PAD || mx == MAP_PAD;
         const bool cv = !pad && (base == MAP_CVAL || mx < 0);
-        const Key raw = input[pad || cv ? 0 : base + mx];
-        return pad ? KT::SENTINEL : (cv ? cval_key : classify<T>(raw, a.ignore_inf));
+        return pad ? KT::SENTINEL : (cv ? cval_key : classify<T>(bits, a.ignore_inf));
     };
 
-    // Stages column c (map entry mx) of slide position q into LDS.
-    auto stage = [&](int64_t ms, int c, int64_t mx) {
-        Key ck[SH];
+    // The lane's raw samples of one slide position (map entry ms): column
+    // `lane` in raw[0 .. SH), column `lane + 64` (halo lanes only) in
+    // raw[SH .. 2 SH). Loaded one step ahead, so the loads are in flight
+    // during the output merge; keys are formed when the slice is built.
+    auto load = [&](Key (&raw)[2 * SH], int64_t ms) {
 #pragma unroll
-        for (int dy = 0; dy < SH; ++dy) ck[dy] = sample(combine(ms, mr[dy]), mx);
+        for (int dy = 0; dy < SH; ++dy) raw[dy] = fetch(combine(ms, mr[dy]), mxa);
+        if (second) {
+#pragma unroll
+            for (int dy = 0; dy < SH; ++dy) raw[SH + dy] = fetch(combine(ms, mr[dy]), mxb);
+        }
+    };
+
+    // Stages column c (its SH samples in ck) into LDS.
+    auto stage = [&](Key (&ck)[SH], int c) {
         if constexpr (SH > 1) {
             int n = 0;
 #pragma unroll
@@ -163,13 +149,22 @@ __global__ void __launch_bounds__(STACK_THREADS) k_stack(StackArgs a)
         }
     };
 
-    // Builds the sorted slice of window coordinate index q (along the slide
-    // axis) for this lane's window; `count` receives its retained samples.
-    auto build = [&](Key (&dst)[L], int &count, int64_t q) {
-        const int64_t ms = uniform_entry(a.maps, q);
+    // Builds the sorted slice of the loaded samples `raw` for this lane's
+    // window; `count` receives its retained samples.
+    auto build = [&](Key (&dst)[L], int &count, const Key (&raw)[2 * SH], int64_t ms) {
         wave_lds_sync();  // the previous slice's reads of `col` are done
-        stage(ms, lane, mxa);
-        if (second) stage(ms, lane + 64, mxb);
+        {
+            Key ck[SH];
+#pragma unroll
+            for (int dy = 0; dy < SH; ++dy) ck[dy] = keyed(raw[dy], combine(ms, mr[dy]), mxa);
+            stage(ck, lane);
+        }
+        if (second) {
+            Key ck[SH];
+#pragma unroll
+            for (int dy = 0; dy < SH; ++dy) ck[dy] = keyed(raw[SH + dy], combine(ms, mr[dy]), mxb);
+            stage(ck, lane + 64);
+        }
         wave_lds_sync();
         Key v[L];
 #pragma unroll
@@ -193,13 +188,23 @@ __global__ void __launch_bounds__(STACK_THREADS) k_stack(StackArgs a)
 
     Key S[K0][L];
     int cnt[K0];
+    Key raw[2 * SH];
 #pragma unroll
-    for (int s = 0; s + 1 < K0; ++s) build(S[s], cnt[s], p0 + s);
+    for (int s = 0; s + 1 < K0; ++s) {
+        const int64_t ms = uniform_entry(a.maps, p0 + s);
+        load(raw, ms);
+        build(S[s], cnt[s], raw, ms);
+    }
+    int64_t ms = uniform_entry(a.maps, p0 + K0 - 1);
+    load(raw, ms);
 
     const int64_t x = x0 + lane;
 #pragma unroll 1
     for (int64_t p = p0; p < p1; ++p) {
-        build(S[K0 - 1], cnt[K0 - 1], p + K0 - 1);
+        build(S[K0 - 1], cnt[K0 - 1], raw, ms);
+        // The next step's samples (the last step reloads a valid position).
+        ms = uniform_entry(a.maps, p + 1 < p1 ? p + K0 : p + K0 - 1);
+        load(raw, ms);
         Key w[W];
         int n = 0;
 #pragma unroll
@@ -209,7 +214,7 @@ __global__ void __launch_bounds__(STACK_THREADS) k_stack(StackArgs a)
             for (int i = 0; i < L; ++i) w[s * L + i] = S[s][i];
         }
         Key out = KT::CANONICAL_NAN;
-        if (a.stat == STAT_MEDIAN) {
+        if constexpr (MEDIAN) {
             apply_net<Part<Net, NET_HALF>>(w);
             Key h[H];
 #pragma unroll
@@ -218,21 +223,8 @@ __global__ void __launch_bounds__(STACK_THREADS) k_stack(StackArgs a)
             const int hi = n >> 1;
             const bool pair = (n & 1) == 0;
             const int lo = pair ? hi - 1 : hi;
-            // Few omitted samples put both ranks in the top F of the half:
-            // a short select tree when the whole wave qualifies.
-            constexpr int F = H < 16 ? H : 16;
-            const bool narrow = !ok || lo >= H - F;
             Key klo, khi;
-            if (__builtin_amdgcn_ballot_w64(!narrow) == 0) {
-                Key t[F];
-#pragma unroll
-                for (int j = 0; j < F; ++j) t[j] = h[H - F + j];
-                khi = select_index<Key, F>(t, ok ? (uint32_t)(hi - (H - F)) : 0u);
-                klo = select_index<Key, F>(t, ok ? (uint32_t)(lo - (H - F)) : 0u);
-            } else {
-                khi = select_index<Key, H>(h, ok ? (uint32_t)hi : 0u);
-                klo = select_index<Key, H>(h, ok ? (uint32_t)lo : 0u);
-            }
+            median_ranks<Key, H>(h, ok, lo, hi, klo, khi);
             if (ok) out = pair ? median_pair_bits<T>(klo, khi) : KT::unkey(khi);
         } else {
             apply_net<Part<Net, NET_FULL>>(w);
@@ -263,7 +255,13 @@ hipError_t launch_shape(const StackArgs &args, hipStream_t stream)
     const size_t lds = stack_lds_bytes((int)sizeof(T), SH, args.sx);
     const int64_t blocks = (args.units + STACK_WAVES - 1) / STACK_WAVES;
     const int64_t grid = (blocks + 7) / 8 * 8;
-    hipLaunchKernelGGL((k_stack<T, K0, SH, SW>), dim3((unsigned)grid), dim3(STACK_THREADS), lds, stream, args);
+    if (args.stat == STAT_MEDIAN) {
+        hipLaunchKernelGGL((k_stack<T, K0, SH, SW, true>), dim3((unsigned)grid), dim3(STACK_THREADS), lds, stream,
+                           args);
+    } else {
+        hipLaunchKernelGGL((k_stack<T, K0, SH, SW, false>), dim3((unsigned)grid), dim3(STACK_THREADS), lds, stream,
+                           args);
+    }
     return hipGetLastError();
 }
 
