@@ -83,9 +83,15 @@ CASES = [
     ("slide_63x63_rank", (lambda: synthetic.sky_quadratic_crop((512, 512), seed=205),
                           lambda: synthetic.sky_quadratic_crop((200, 130), seed=206)),
      dict(kind="rank", size=(63, 63), parameter=397), None),
-    ("network_5x5", (lambda: synthetic.sky_small((512, 512), seed=207),
-                     lambda: synthetic.sky_small((100, 150), seed=208)),
+    ("stack_5x5", (lambda: synthetic.sky_small((512, 512), seed=207),
+                   lambda: synthetic.sky_small((100, 150), seed=208)),
      dict(kind="median", size=(5, 5)), None),
+    ("network_5x5", (lambda: synthetic.sky_small((512, 512), seed=217),
+                     lambda: synthetic.sky_small((100, 150), seed=218)),
+     dict(kind="median", size=(5, 5), engine="network"), None),
+    ("stack_cube", (lambda: synthetic.cube((6, 48, 80), seed=219),
+                    lambda: synthetic.cube((3, 20, 24), seed=220)),
+     dict(kind="median", size=(3, 7, 7), mode="wrap", ignore_inf=True), None),
     ("network_annulus", (lambda: synthetic.sky_annulus((384, 384), seed=209),
                          lambda: synthetic.sky_annulus((64, 96), seed=210)),
      dict(kind="median", size=None, mode="wrap", footprint=ring(), ignore_inf=True), None),
