@@ -164,9 +164,38 @@ def build(k0: int, sh: int, sw: int) -> dict:
             best_half = (half, out[: width // 2 + 1])
         if best_full is None or len(comps) < len(best_full[0]):
             best_full = (comps, out)
-    return dict(k0=k0, sh=sh, sw=sw, length=length, width=width, col=col, col_out=col_out,
-                slice=slice_comps, slice_out=slice_out, half=best_half[0], half_out=best_half[1],
-                full=best_full[0], full_out=best_full[1])
+    net = dict(k0=k0, sh=sh, sw=sw, length=length, width=width, col=col, col_out=col_out,
+               slice=slice_comps, slice_out=slice_out, half=best_half[0], half_out=best_half[1],
+               full=best_full[0], full_out=best_full[1])
+    if sh == 1:
+        net.update(build_pair(k0, length))
+    return net
+
+
+def build_pair(k0: int, length: int) -> dict:
+    """Networks of a 2-D window's two-output step: outputs p and p + 1 share
+    the k0 - 1 row slices p + 1 .. p + k0 - 1 (the core). `core` merges them
+    (slice s on wires s*L ..), keeping what the final merges read; `fin_half`
+    / `fin_full` merge the sorted core (ranks on wires 0 .. C-1) with one
+    more sorted slice (wires C .. W-1) into the lower half / the full order.
+    `core_out[r]` is the wire of the core's rank r."""
+    width = k0 * length
+    core_width = (k0 - 1) * length
+    fin, fin_out = merge(list(range(core_width)), list(range(core_width, width)))
+    fin_half = cone(fin, fin_out[: width // 2 + 1])
+    used_half = {w for pair in fin_half for w in pair} | set(fin_out[: width // 2 + 1])
+    slices = [list(range(s * length, (s + 1) * length)) for s in range(k0 - 1)]
+    best = None
+    for shape in tree_shapes(tuple(range(k0 - 1))):
+        comps, out = merge_tree(slices, shape)
+        if best is None or len(comps) < len(best[0]):
+            best = (comps, out)
+    core, core_out = best
+    # The median's core keeps only the ranks its final merge reads.
+    need = sorted(r for r in range(core_width) if r in used_half)
+    core_half = cone(core, [core_out[r] for r in need])
+    return dict(core_width=core_width, core=core, core_half=core_half, core_out=core_out,
+                fin_half=fin_half, fin_half_out=fin_out[: width // 2 + 1], fin_full=fin, fin_full_out=fin_out)
 
 
 def build_sort(live: int) -> dict:
@@ -264,6 +293,8 @@ def verify(net: dict, random_trials: int = 2000, seed: int = 0) -> None:
         assert np.array_equal(y, expect), "full network does not merge"
         y = apply(net["half"], x)[:, net["half_out"]]
         assert np.array_equal(y, expect[:, : width // 2 + 1]), "half network does not give the lower half"
+    if sh == 1:
+        verify_pair(net, seed)
     # The whole pipeline on random keys with ties and sentinels.
     rng = np.random.default_rng(seed)
     keys = rng.integers(0, 6, size=(random_trials, width)).astype(np.uint32)
@@ -277,6 +308,35 @@ def verify(net: dict, random_trials: int = 2000, seed: int = 0) -> None:
     expect = np.sort(keys, axis=1)
     assert np.array_equal(apply(net["full"], sl)[:, net["full_out"]], expect)
     assert np.array_equal(apply(net["half"], sl)[:, net["half_out"]], expect[:, : width // 2 + 1])
+
+
+def verify_pair(net: dict, seed: int = 0, random_trials: int = 3000) -> None:
+    """The two-output networks: every merge they are built from on every 0-1
+    input, the core + final pipeline exhaustively over sorted 0-1 slices when
+    few enough, and on random keys."""
+    k0, length, width = net["k0"], net["length"], net["width"]
+    core_width = net["core_width"]
+    verify_merge(core_width, length)
+    for m, n in merge_sizes(k0 - 1, length):
+        verify_merge(m, n)
+
+    def run(slices_keys: np.ndarray, extra: np.ndarray, half: bool) -> np.ndarray:
+        core = apply(net["core_half" if half else "core"], slices_keys)[:, net["core_out"]]
+        both = np.concatenate([core, extra], axis=1)
+        part = "fin_half" if half else "fin_full"
+        return apply(net[part], both)[:, net[part + "_out"]]
+
+    rng = np.random.default_rng(seed + 7)
+    keys = rng.integers(0, 5, size=(random_trials, width)).astype(np.uint32)
+    keys[rng.random(keys.shape) < 0.2] = 0xFFFFFFFF
+    keys = np.sort(keys.reshape(random_trials, k0, length), axis=2).reshape(random_trials, width)
+    if (length + 1) ** k0 <= EXHAUSTIVE_LIMIT:
+        keys = np.concatenate([keys, sorted_01(k0, length).astype(np.uint32)])
+    expect = np.sort(keys, axis=1)
+    got = run(keys[:, :core_width], keys[:, core_width:], False)
+    assert np.array_equal(got, expect), "two-output full networks do not merge"
+    got = run(keys[:, :core_width], keys[:, core_width:], True)
+    assert np.array_equal(got, expect[:, : width // 2 + 1]), "two-output half networks miss the lower half"
 
 
 def _pairs(name: str, comps: list[tuple[int, int]]) -> str:
@@ -325,6 +385,12 @@ def render() -> str:
         text += _pairs("slice", net["slice"]) + _list("slice_out", net["slice_out"])
         text += _pairs("half", net["half"]) + _list("half_out", net["half_out"])
         text += _pairs("full", net["full"]) + _list("full_out", net["full_out"])
+        if sh == 1:
+            text += f"    static constexpr int C = {net['core_width']};\n"
+            text += _pairs("core", net["core"]) + _pairs("core_half", net["core_half"])
+            text += _list("core_out", net["core_out"])
+            text += _pairs("fin_half", net["fin_half"]) + _list("fin_half_out", net["fin_half_out"])
+            text += _pairs("fin_full", net["fin_full"]) + _list("fin_full_out", net["fin_full_out"])
         out.append(text.rstrip("\n"))
         out.append("};\n")
     out += ["}  // namespace stack_net", "}  // namespace ndf", ""]
@@ -339,8 +405,12 @@ def main() -> None:
     for cfg in CONFIGS:
         net = build(*cfg)
         verify(net)
+        extra = ""
+        if cfg[1] == 1:
+            extra = (f" pair: core={len(net['core'])} core_half={len(net['core_half'])} "
+                     f"fin_half={len(net['fin_half'])} fin_full={len(net['fin_full'])}")
         print(f"Net{cfg}: W={net['width']} col={len(net['col'])} slice={len(net['slice'])} "
-              f"half={len(net['half'])} full={len(net['full'])}", file=sys.stderr)
+              f"half={len(net['half'])} full={len(net['full'])}{extra}", file=sys.stderr)
     with open(HEADER, "w") as f:
         f.write(render())
 

@@ -30,7 +30,7 @@ namespace {
 constexpr int STACK_THREADS = 256;
 constexpr int STACK_WAVES = STACK_THREADS / 64;
 
-enum NetPart { NET_COL, NET_SLICE, NET_HALF, NET_FULL };
+enum NetPart { NET_COL, NET_SLICE, NET_HALF, NET_FULL, NET_CORE, NET_CORE_HALF, NET_FIN_HALF, NET_FIN_FULL };
 
 template <class Net, int PART>
 struct Part;
@@ -57,6 +57,31 @@ struct Part<Net, NET_FULL> {
     static constexpr int count = Net::full_count;
     static constexpr int lo(int i) { return Net::full[i][0]; }
     static constexpr int hi(int i) { return Net::full[i][1]; }
+};
+
+template <class Net>
+struct Part<Net, NET_CORE> {
+    static constexpr int count = Net::core_count;
+    static constexpr int lo(int i) { return Net::core[i][0]; }
+    static constexpr int hi(int i) { return Net::core[i][1]; }
+};
+template <class Net>
+struct Part<Net, NET_CORE_HALF> {
+    static constexpr int count = Net::core_half_count;
+    static constexpr int lo(int i) { return Net::core_half[i][0]; }
+    static constexpr int hi(int i) { return Net::core_half[i][1]; }
+};
+template <class Net>
+struct Part<Net, NET_FIN_HALF> {
+    static constexpr int count = Net::fin_half_count;
+    static constexpr int lo(int i) { return Net::fin_half[i][0]; }
+    static constexpr int hi(int i) { return Net::fin_half[i][1]; }
+};
+template <class Net>
+struct Part<Net, NET_FIN_FULL> {
+    static constexpr int count = Net::fin_full_count;
+    static constexpr int lo(int i) { return Net::fin_full[i][0]; }
+    static constexpr int hi(int i) { return Net::fin_full[i][1]; }
 };
 
 // Combined offset of a slide-axis and a row-axis map entry: MAP_PAD when
@@ -186,8 +211,59 @@ __global__ void __launch_bounds__(STACK_THREADS) k_stack(StackArgs a)
         count = n;
     };
 
-    Key S[K0][L];
-    int cnt[K0];
+    // The statistic of one window from its sorted lower half h (median) or
+    // its sorted keys f (other statistics), n retained.
+    auto median_of = [&](const Key (&h)[H], int n) -> Key {
+        const bool ok = n >= a.min_count && n > 0;
+        const int hi = n >> 1;
+        const bool pair = (n & 1) == 0;
+        const int lo = pair ? hi - 1 : hi;
+        Key klo, khi;
+        median_ranks<Key, H>(h, ok, lo, hi, klo, khi);
+        return ok ? (pair ? median_pair_bits<T>(klo, khi) : KT::unkey(khi)) : KT::CANONICAL_NAN;
+    };
+    auto statistic_of = [&](const Key (&f)[W], int n) -> Key {
+        Pick pick;
+        const bool ok = pick_ranks(a.stat, n, a.min_count, a.rank_param, a.q100, pick);
+        const uint32_t hi_index = ok ? (uint32_t)pick.hi : 0u;
+        const Key khi = select_index<Key, W>(f, hi_index);
+        Key klo = khi;
+        if (a.stat != STAT_RANK) klo = select_index<Key, W>(f, ok && pick.pair ? hi_index - 1u : hi_index);
+        return ok ? (pick.pair ? pair_bits<T>(a.stat, klo, khi, pick.frac) : KT::unkey(khi)) : KT::CANONICAL_NAN;
+    };
+
+    // Slots: K0 slices, and one more for the two-output step of 2-D windows.
+    constexpr bool PAIRS = SH == 1;
+    constexpr int SLOTS = PAIRS ? K0 + 1 : K0;
+    Key S[SLOTS][L];
+    int cnt[SLOTS];
+    // One window's result from slots [first, first + K0).
+    auto emit_one = [&](int first) -> Key {
+        Key w[W];
+        int n = 0;
+#pragma unroll
+        for (int s = 0; s < K0; ++s) {
+            n += cnt[first + s];
+#pragma unroll
+            for (int i = 0; i < L; ++i) w[s * L + i] = S[first + s][i];
+        }
+        if constexpr (MEDIAN) {
+            apply_net<Part<Net, NET_HALF>>(w);
+            Key h[H];
+#pragma unroll
+            for (int i = 0; i < H; ++i) h[i] = w[Net::half_out[i]];
+            return median_of(h, n);
+        } else {
+            apply_net<Part<Net, NET_FULL>>(w);
+            Key f[W];
+#pragma unroll
+            for (int i = 0; i < W; ++i) f[i] = w[Net::full_out[i]];
+            return statistic_of(f, n);
+        }
+    };
+
+    const int64_t last_q = a.ns + K0 - 2;  // last map entry of the slide axis
+    const int64_t x = x0 + lane;
     Key raw[2 * SH];
 #pragma unroll
     for (int s = 0; s + 1 < K0; ++s) {
@@ -197,48 +273,77 @@ __global__ void __launch_bounds__(STACK_THREADS) k_stack(StackArgs a)
     }
     int64_t ms = uniform_entry(a.maps, p0 + K0 - 1);
     load(raw, ms);
-
-    const int64_t x = x0 + lane;
+    int64_t p = p0;
+    if constexpr (PAIRS) {
+        // Two outputs per step: rows p and p + 1 share the K0 - 1 slices in
+        // slots 1 .. K0 - 1 (the core), merged once; each output merges the
+        // sorted core with its own last slice (slot 0, slot K0).
+        Key raw1[2 * SH];
+        int64_t ms1 = uniform_entry(a.maps, p0 + K0 <= last_q ? p0 + K0 : last_q);
+        load(raw1, ms1);
 #pragma unroll 1
-    for (int64_t p = p0; p < p1; ++p) {
+        for (; p + 1 < p1; p += 2) {
+            build(S[K0 - 1], cnt[K0 - 1], raw, ms);
+            build(S[K0], cnt[K0], raw1, ms1);
+            // The next step's samples (clamped to the slide axis's map).
+            ms = uniform_entry(a.maps, p + K0 + 1 <= last_q ? p + K0 + 1 : last_q);
+            load(raw, ms);
+            ms1 = uniform_entry(a.maps, p + K0 + 2 <= last_q ? p + K0 + 2 : last_q);
+            load(raw1, ms1);
+            constexpr int C = Net::C;
+            Key core[C];
+            int ncore = 0;
+#pragma unroll
+            for (int s = 1; s < K0; ++s) {
+                ncore += cnt[s];
+#pragma unroll
+                for (int i = 0; i < L; ++i) core[(s - 1) * L + i] = S[s][i];
+            }
+            if constexpr (MEDIAN) {
+                apply_net<Part<Net, NET_CORE_HALF>>(core);
+            } else {
+                apply_net<Part<Net, NET_CORE>>(core);
+            }
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                const int extra = j == 0 ? 0 : K0;
+                Key u[W];
+#pragma unroll
+                for (int i = 0; i < C; ++i) u[i] = core[Net::core_out[i]];
+#pragma unroll
+                for (int i = 0; i < L; ++i) u[C + i] = S[extra][i];
+                const int n = ncore + cnt[extra];
+                Key out;
+                if constexpr (MEDIAN) {
+                    apply_net<Part<Net, NET_FIN_HALF>>(u);
+                    Key h[H];
+#pragma unroll
+                    for (int i = 0; i < H; ++i) h[i] = u[Net::fin_half_out[i]];
+                    out = median_of(h, n);
+                } else {
+                    apply_net<Part<Net, NET_FIN_FULL>>(u);
+                    Key f[W];
+#pragma unroll
+                    for (int i = 0; i < W; ++i) f[i] = u[Net::fin_full_out[i]];
+                    out = statistic_of(f, n);
+                }
+                if (x < a.nx) output[(p + j) * a.out_ss + r * a.out_sr + x] = out;
+            }
+#pragma unroll
+            for (int s = 0; s + 2 < SLOTS; ++s) {
+                cnt[s] = cnt[s + 2];
+#pragma unroll
+                for (int i = 0; i < L; ++i) S[s][i] = S[s + 2][i];
+            }
+        }
+    }
+    // One output per step (3-D windows; the odd last row of a 2-D run).
+#pragma unroll 1
+    for (; p < p1; ++p) {
         build(S[K0 - 1], cnt[K0 - 1], raw, ms);
-        // The next step's samples (the last step reloads a valid position).
-        ms = uniform_entry(a.maps, p + 1 < p1 ? p + K0 : p + K0 - 1);
+        ms = uniform_entry(a.maps, p + K0 <= last_q ? p + K0 : last_q);
         load(raw, ms);
-        Key w[W];
-        int n = 0;
-#pragma unroll
-        for (int s = 0; s < K0; ++s) {
-            n += cnt[s];
-#pragma unroll
-            for (int i = 0; i < L; ++i) w[s * L + i] = S[s][i];
-        }
-        Key out = KT::CANONICAL_NAN;
-        if constexpr (MEDIAN) {
-            apply_net<Part<Net, NET_HALF>>(w);
-            Key h[H];
-#pragma unroll
-            for (int i = 0; i < H; ++i) h[i] = w[Net::half_out[i]];
-            const bool ok = n >= a.min_count && n > 0;
-            const int hi = n >> 1;
-            const bool pair = (n & 1) == 0;
-            const int lo = pair ? hi - 1 : hi;
-            Key klo, khi;
-            median_ranks<Key, H>(h, ok, lo, hi, klo, khi);
-            if (ok) out = pair ? median_pair_bits<T>(klo, khi) : KT::unkey(khi);
-        } else {
-            apply_net<Part<Net, NET_FULL>>(w);
-            Key f[W];
-#pragma unroll
-            for (int i = 0; i < W; ++i) f[i] = w[Net::full_out[i]];
-            Pick pick;
-            const bool ok = pick_ranks(a.stat, n, a.min_count, a.rank_param, a.q100, pick);
-            const uint32_t hi_index = ok ? (uint32_t)pick.hi : 0u;
-            const Key khi = select_index<Key, W>(f, hi_index);
-            Key klo = khi;
-            if (a.stat != STAT_RANK) klo = select_index<Key, W>(f, ok && pick.pair ? hi_index - 1u : hi_index);
-            if (ok) out = pick.pair ? pair_bits<T>(a.stat, klo, khi, pick.frac) : KT::unkey(khi);
-        }
+        const Key out = emit_one(0);
         if (x < a.nx) output[p * a.out_ss + r * a.out_sr + x] = out;
 #pragma unroll
         for (int s = 0; s + 1 < K0; ++s) {
