@@ -54,6 +54,11 @@ CONFIGS = [
 # padded with all-ones keys, sorted per output).
 SORT_LIVES = [9, 16, 25, 32, 49, 64, 72, 96, 128, 160]
 
+# (core, extra) wire classes of the network engine's two-row step: a
+# footprint whose windows at rows y and y + 1 share `core` taps and differ
+# by `extra` taps each.
+PAIR_CLASSES = [(16, 8), (32, 12), (48, 20), (64, 24), (96, 32), (128, 32)]
+
 HERE = os.path.dirname(os.path.abspath(__file__))
 HEADER = os.path.join(HERE, "ndf_stack_networks.inc")
 
@@ -204,6 +209,50 @@ def build_sort(live: int) -> dict:
     comps, out = merge_tree([[i] for i in range(live)], balanced(tuple(range(live))))
     half_ranks = out[: live // 2 + 1]
     return dict(live=live, full=comps, full_out=out, half=cone(comps, half_ranks), half_out=half_ranks)
+
+
+def build_pair_sort(core: int, extra: int) -> dict:
+    """Networks of the network engine's two-row step. The core (the taps
+    both windows share, padded with all-ones keys to `core` wires) and each
+    window's extra taps (`extra` wires) are sorted by merge-tree sorts; `fin`
+    merges the sorted core (wires 0 .. core-1) with one sorted extra (wires
+    core ..). The median's networks keep only what ranks 0 .. W/2 of the
+    final merge depend on (W = core + extra)."""
+    width = core + extra
+    fin, fin_out = merge(list(range(core)), list(range(core, width)))
+    fin_half = cone(fin, fin_out[: width // 2 + 1])
+    used = {w for pair in fin_half for w in pair} | set(fin_out[: width // 2 + 1])
+    core_net = build_sort(core)
+    extra_net = build_sort(extra)
+    core_need = [core_net["full_out"][r] for r in range(core) if r in used]
+    extra_need = [extra_net["full_out"][r] for r in range(extra) if core + r in used]
+    return dict(core=core, extra=extra, width=width,
+                core_full=core_net["full"], core_half=cone(core_net["full"], core_need), core_out=core_net["full_out"],
+                extra_full=extra_net["full"], extra_half=cone(extra_net["full"], extra_need),
+                extra_out=extra_net["full_out"],
+                fin_full=fin, fin_full_out=fin_out, fin_half=fin_half, fin_half_out=fin_out[: width // 2 + 1])
+
+
+def verify_pair_sort(net: dict, random_trials: int = 3000, seed: int = 5) -> None:
+    """The two-row networks: every merge they are built from on every 0-1
+    input, and the pipeline (sort core, sort extra, merge) on random keys
+    with ties and sentinels against np.sort."""
+    core, extra, width = net["core"], net["extra"], net["width"]
+    verify_merge(core, extra)
+    for n in (core, extra):
+        for m, k in tree_merge_sizes(tuple(range(n))):
+            verify_merge(m, k)
+    rng = np.random.default_rng(seed)
+    keys = rng.integers(0, 6, size=(random_trials, width)).astype(np.uint32)
+    keys[rng.random(keys.shape) < 0.2] = 0xFFFFFFFF
+    expect = np.sort(keys, axis=1)
+    for half in (False, True):
+        tag = "half" if half else "full"
+        c = apply(net["core_" + tag], keys[:, :core])[:, net["core_out"]]
+        e = apply(net["extra_" + tag], keys[:, core:])[:, net["extra_out"]]
+        got = apply(net["fin_" + tag], np.concatenate([c, e], axis=1))[:, net["fin_" + tag + "_out"]]
+        want = expect[:, : width // 2 + 1] if half else expect
+        assert np.array_equal(got, want), f"two-row {tag} networks ({core}, {extra}) do not sort"
 
 
 def tree_merge_sizes(items: tuple[int, ...]) -> set[tuple[int, int]]:
@@ -368,6 +417,11 @@ def render() -> str:
         "template <int LIVE>",
         "struct Sort;",
         "",
+        "// PairSort<C, E>: the network engine's two-row step (C shared taps, E",
+        "// taps per window beyond them).",
+        "template <int C, int E>",
+        "struct PairSort;",
+        "",
     ]
     for live in SORT_LIVES:
         net = build_sort(live)
@@ -375,6 +429,17 @@ def render() -> str:
         out.append(f"    static constexpr int W = {live}, H = {live // 2 + 1};")
         text = _pairs("half", net["half"]) + _list("half_out", net["half_out"])
         text += _pairs("full", net["full"]) + _list("full_out", net["full_out"])
+        out.append(text.rstrip("\n"))
+        out.append("};\n")
+    for core, extra in PAIR_CLASSES:
+        net = build_pair_sort(core, extra)
+        out.append(f"template <>\nstruct PairSort<{core}, {extra}> {{")
+        out.append(f"    static constexpr int C = {core}, E = {extra}, W = {core + extra}, H = {(core + extra) // 2 + 1};")
+        text = ""
+        for part in ("core", "extra", "fin"):
+            text += _pairs(part + "_full", net[part + "_full"]) + _pairs(part + "_half", net[part + "_half"])
+        text += _list("core_out", net["core_out"]) + _list("extra_out", net["extra_out"])
+        text += _list("fin_full_out", net["fin_full_out"]) + _list("fin_half_out", net["fin_half_out"])
         out.append(text.rstrip("\n"))
         out.append("};\n")
     for k0, sh, sw in CONFIGS:
@@ -402,6 +467,12 @@ def main() -> None:
         net = build_sort(live)
         verify_sort(net)
         print(f"Sort<{live}>: half={len(net['half'])} full={len(net['full'])}", file=sys.stderr)
+    for core, extra in PAIR_CLASSES:
+        net = build_pair_sort(core, extra)
+        verify_pair_sort(net)
+        print(f"PairSort<{core}, {extra}>: core half={len(net['core_half'])} extra half={len(net['extra_half'])} "
+              f"fin half={len(net['fin_half'])} per output {(len(net['core_half']) + 2 * len(net['extra_half']) + 2 * len(net['fin_half'])) / 2}",
+              file=sys.stderr)
     for cfg in CONFIGS:
         net = build(*cfg)
         verify(net)

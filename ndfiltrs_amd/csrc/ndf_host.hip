@@ -197,7 +197,8 @@ struct ndf_plan {
     NetworkArgs network{};
     size_t network_lds = 0;
     std::vector<int64_t> net_mapz, net_mapy, net_mapx;
-    std::vector<uint32_t> net_taps;
+    std::vector<uint32_t> net_taps;       // one output per step: `live` entries
+    std::vector<uint32_t> net_pair_taps;  // two-row step: core, extra of row r, extra of row r + 1
 
     // Stack geometry (box windows with compiled slice networks).
     bool stack_ok = false;
@@ -328,7 +329,7 @@ bool plan_network(ndf_plan *p)
     const int64_t s = rows * sx;
     // Sentinel run read by the taps past the count: window bases span
     // (ty - 1) * sx + 64 keys.
-    const int64_t pad_keys = (int64_t)p->count < live ? (ty - 1) * sx + 64 : 0;
+    int64_t pad_keys = (int64_t)p->count < live ? (ty - 1) * sx + 64 : 0;
     const size_t lds = network_lds_bytes(p->dbytes, (int)s, (int)pad_keys, live);
     if (lds > (size_t)lds_limit() || s > (1 << 24)) return false;
     const int64_t tiles_x = (dims[2] + 63) / 64;
@@ -355,6 +356,47 @@ bool plan_network(ndf_plan *p)
     // The kernels read `live` taps; those past the count read the sentinel
     // run that starts at offset s.
     p->net_taps.resize((size_t)live, (uint32_t)s);
+    // Two-row step: rows r and r + 1 of a tile share the taps of
+    // T ∩ (T + one row) (the core); row r has T \ (T + one row) of its own and
+    // row r + 1 (T + one row) \ T, all as offsets from row r.
+    p->net_pair_taps.clear();
+    int pair_c = 0, pair_e = 0;
+    {
+        std::vector<std::array<int64_t, 3>> rel;
+        for (size_t t = 0; t < p->count; ++t) {
+            std::array<int64_t, 3> d = {0, 0, 0};
+            for (size_t a = 0; a < p->rank; ++a) d[pad + a] = p->taps[t * p->rank + a] - p->low[a];
+            rel.push_back(d);
+        }
+        std::sort(rel.begin(), rel.end());
+        auto has = [&](const std::array<int64_t, 3> &d) { return std::binary_search(rel.begin(), rel.end(), d); };
+        std::vector<uint32_t> core, extra0, extra1;
+        auto offset = [&](int64_t dz, int64_t dy, int64_t dx) { return (uint32_t)((dz * sy + dy) * sx + dx); };
+        for (const auto &d : rel) {
+            const std::array<int64_t, 3> up = {d[0], d[1] - 1, d[2]};  // the same sample seen from row r + 1
+            if (d[1] >= 1 && has(up)) {
+                core.push_back(offset(d[0], d[1], d[2]));
+            } else {
+                extra0.push_back(offset(d[0], d[1], d[2]));
+            }
+            const std::array<int64_t, 3> down = {d[0], d[1] + 1, d[2]};
+            if (!has(down)) extra1.push_back(offset(d[0], d[1] + 1, d[2]));
+        }
+        if (extra0.size() == extra1.size() &&
+            network_pair_class(p->dbytes, (int)core.size(), (int)extra0.size(), live, p->stat == STAT_MEDIAN,
+                               &pair_c, &pair_e)) {
+            core.resize((size_t)pair_c, (uint32_t)s);
+            extra0.resize((size_t)pair_e, (uint32_t)s);
+            extra1.resize((size_t)pair_e, (uint32_t)s);
+            // Padded entries read the sentinel run after the support.
+            pad_keys = (ty - 1) * sx + 64;
+            p->net_pair_taps = core;
+            p->net_pair_taps.insert(p->net_pair_taps.end(), extra0.begin(), extra0.end());
+            p->net_pair_taps.insert(p->net_pair_taps.end(), extra1.begin(), extra1.end());
+        } else {
+            pair_c = pair_e = 0;
+        }
+    }
     NetworkArgs &n = p->network;
     n = NetworkArgs{};
     n.nz = dims[0];
@@ -369,6 +411,8 @@ bool plan_network(ndf_plan *p)
     n.tiles_y = (int32_t)tiles_y;
     n.count = (int32_t)p->count;
     n.live = live;
+    n.pair_c = pair_c;
+    n.pair_e = pair_e;
     n.pad = (int32_t)pad_keys;
     n.cval_bits = p->cval_bits;
     n.ignore_inf = p->eff_ignore_inf;
@@ -376,7 +420,8 @@ bool plan_network(ndf_plan *p)
     n.min_count = (int32_t)p->eff_min_count;
     n.rank_param = (int32_t)std::min(p->parameter, p->count);
     n.q100 = p->q100;
-    p->network_lds = lds;
+    p->network_lds = pair_c ? network_lds_bytes(p->dbytes, (int)s, (int)pad_keys, pair_c + 2 * pair_e) : lds;
+    if (p->network_lds > (size_t)lds_limit()) return false;
     return true;
 }
 
@@ -829,7 +874,8 @@ ndf_status prepare_engine_tables(ndf_plan *p, char *err, size_t err_len)
         x.mapx = static_cast<const int64_t *>(p->e_mapx.ptr);
     } else if (p->engine == NDF_ENGINE_NETWORK) {
         if ((e = upload(p->n_mapz, p->net_mapz)) != hipSuccess || (e = upload(p->n_mapy, p->net_mapy)) != hipSuccess ||
-            (e = upload(p->n_mapx, p->net_mapx)) != hipSuccess || (e = upload(p->n_taps, p->net_taps)) != hipSuccess) {
+            (e = upload(p->n_mapx, p->net_mapx)) != hipSuccess ||
+            (e = upload(p->n_taps, p->network.pair_c ? p->net_pair_taps : p->net_taps)) != hipSuccess) {
             return fail_hip(e, err, err_len);
         }
         NetworkArgs &n = p->network;
@@ -1416,8 +1462,9 @@ ndf_status ndf_plan_describe(const ndf_plan *plan, char *buffer, size_t buffer_l
     if (plan->engine == NDF_ENGINE_NETWORK) {
         snprintf(buffer, buffer_len,
                  "{\"engine\": \"%s\", \"count\": %zu, \"tile\": [%d, 64], \"network_wires\": %d, "
-                 "\"lds_bytes\": %zu, \"tiles\": %lld, \"bracket_tile_rows\": %d}",
-                 engine, plan->count, plan->network.ty, plan->network.live, plan->network_lds,
+                 "\"row_pair\": [%d, %d], \"lds_bytes\": %zu, \"tiles\": %lld, \"bracket_tile_rows\": %d}",
+                 engine, plan->count, plan->network.ty, plan->network.live, plan->network.pair_c,
+                 plan->network.pair_e, plan->network_lds,
                  (long long)plan->network.tiles_x * plan->network.tiles_y * plan->network.nz, plan->bracket_ty);
     } else if (plan->engine == NDF_ENGINE_SLIDE) {
         snprintf(buffer, buffer_len,
@@ -2674,7 +2721,7 @@ hipError_t static_layout(ndf_plan *p, char *base, hipStream_t stream, LaunchSet 
         n.mapz = static_cast<const int64_t *>(put(p->net_mapz));
         n.mapy = static_cast<const int64_t *>(put(p->net_mapy));
         n.mapx = static_cast<const int64_t *>(put(p->net_mapx));
-        n.taps = static_cast<const uint32_t *>(put(p->net_taps));
+        n.taps = static_cast<const uint32_t *>(put(p->network.pair_c ? p->net_pair_taps : p->net_taps));
     } else {
         SlideArgs &sl = t->slide;
         sl = p->slide;

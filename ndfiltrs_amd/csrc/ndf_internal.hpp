@@ -111,6 +111,9 @@ struct NetworkArgs {
     int32_t sx, sy, rows, s;
     int32_t ty, tiles_x, tiles_y;
     int32_t count, live;
+    // Two-row step (f32): the windows at rows r and r + 1 share pair_c core
+    // taps and have pair_e taps each beyond them; 0 for one output per step.
+    int32_t pair_c, pair_e;
     int32_t pad;  // sentinel keys after the support (taps past count read there)
     const uint32_t *taps;
     uint64_t cval_bits;
@@ -179,6 +182,11 @@ hipError_t launch_extrema(int dtype_bytes, const ExtremaArgs &args, size_t lds_b
 
 // Network size (live wires) for a window of `count` taps, 0 if too large.
 int network_live(int dtype_bytes, int count);
+// Two-row step class (*c, *e) for windows whose rows r and r + 1 share
+// `core` taps and have `extra` taps each beyond them: true when a compiled
+// class holds them and costs fewer comparators per output than the
+// one-output network of `live` wires (median or full statistics).
+bool network_pair_class(int dtype_bytes, int core, int extra, int live, bool median, int *c, int *e);
 size_t network_lds_bytes(int dtype_bytes, int s, int pad, int live);
 hipError_t launch_network(int dtype_bytes, const NetworkArgs &args, size_t lds_bytes, hipStream_t stream);
 // Batched network kernel for images whose windows have at most `live` taps:

@@ -51,6 +51,39 @@ struct SortHalf {
 // only, or either (chosen per image at run time).
 enum StatSet { STATS_MEDIAN, STATS_OTHER, STATS_ANY };
 
+// Output bits of a median from the sorted lower half h (H ranks), n retained.
+template <typename T, int H>
+__device__ __forceinline__ typename KeyTraits<T>::Key median_from_half(const NetworkArgs &a,
+                                                                       const typename KeyTraits<T>::Key (&h)[H], int n)
+{
+    using KT = KeyTraits<T>;
+    using Key = typename KT::Key;
+    const bool ok = n >= a.min_count && n > 0;
+    const int hi = n >> 1;
+    const bool pair = (n & 1) == 0;
+    const int lo = pair ? hi - 1 : hi;
+    Key klo, khi;
+    median_ranks<Key, H>(h, ok, lo, hi, klo, khi);
+    return ok ? (pair ? median_pair_bits<T>(klo, khi) : KT::unkey(khi)) : KT::CANONICAL_NAN;
+}
+
+// Output bits of a rank or percentile from the sorted keys f, n retained.
+template <typename T, int W>
+__device__ __forceinline__ typename KeyTraits<T>::Key statistic_from_sorted(const NetworkArgs &a,
+                                                                            const typename KeyTraits<T>::Key (&f)[W],
+                                                                            int n)
+{
+    using KT = KeyTraits<T>;
+    using Key = typename KT::Key;
+    Pick pick;
+    const bool ok = pick_ranks(a.stat, n, a.min_count, a.rank_param, a.q100, pick);
+    const uint32_t hi_index = ok ? (uint32_t)pick.hi : 0u;
+    const Key hi = select_index<Key, W>(f, hi_index);
+    Key lo = hi;
+    if (a.stat != STAT_RANK) lo = select_index<Key, W>(f, ok && pick.pair ? hi_index - 1u : hi_index);
+    return ok ? (pick.pair ? pair_bits<T>(a.stat, lo, hi, pick.frac) : KT::unkey(hi)) : KT::CANONICAL_NAN;
+}
+
 // The output bits of one window (k: its LIVE keys, n of them retained).
 template <typename T, int LIVE, int STATS>
 __device__ __forceinline__ typename KeyTraits<T>::Key window_result(const NetworkArgs &a,
@@ -66,35 +99,24 @@ __device__ __forceinline__ typename KeyTraits<T>::Key window_result(const Networ
             Key h[Sort::H];
 #pragma unroll
             for (int i = 0; i < Sort::H; ++i) h[i] = k[Sort::half_out[i]];
-            const bool ok = n >= a.min_count && n > 0;
-            const int hi = n >> 1;
-            const bool pair = (n & 1) == 0;
-            const int lo = pair ? hi - 1 : hi;
-            Key klo, khi;
-            median_ranks<Key, Sort::H>(h, ok, lo, hi, klo, khi);
-            if (ok) out = pair ? median_pair_bits<T>(klo, khi) : KT::unkey(khi);
+            out = median_from_half<T, Sort::H>(a, h, n);
         }
     } else if constexpr (STATS != STATS_MEDIAN) {
         apply_net<SortFull<LIVE>>(k);
         Key f[LIVE];
 #pragma unroll
         for (int i = 0; i < LIVE; ++i) f[i] = k[Sort::full_out[i]];
-        Pick pick;
-        const bool ok = pick_ranks(a.stat, n, a.min_count, a.rank_param, a.q100, pick);
-        const uint32_t hi_index = ok ? (uint32_t)pick.hi : 0u;
-        const Key hi = select_index<Key, LIVE>(f, hi_index);
-        Key lo = hi;
-        if (a.stat != STAT_RANK) lo = select_index<Key, LIVE>(f, ok && pick.pair ? hi_index - 1u : hi_index);
-        if (ok) out = pick.pair ? pair_bits<T>(a.stat, lo, hi, pick.frac) : KT::unkey(hi);
+        out = statistic_from_sorted<T, LIVE>(a, f, n);
     }
     return out;
 }
 
-// One tile: stage the support (classified keys) into LDS, then one output
-// per thread and row step. `taps` holds the window's support offsets.
-template <typename T, int LIVE, int STATS>
-__device__ __forceinline__ void network_tile(const NetworkArgs &a, int tile,
-                                             typename KeyTraits<T>::Key *support, const uint32_t *taps)
+// Stages one tile's support (classified keys) into LDS: (z0, y0, x0) is the
+// tile's first output.
+template <typename T>
+__device__ __forceinline__ void stage_network_tile(const NetworkArgs &a, int tile,
+                                                   typename KeyTraits<T>::Key *support, int64_t &z0, int64_t &y0,
+                                                   int64_t &x0)
 {
     using KT = KeyTraits<T>;
     using Key = typename KT::Key;
@@ -102,15 +124,14 @@ __device__ __forceinline__ void network_tile(const NetworkArgs &a, int tile,
     const int lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const Key *input = static_cast<const Key *>(a.input);
-    Key *output = static_cast<Key *>(a.output);
     const Key cval_key = classify<T>((Key)a.cval_bits, a.ignore_inf);
     int rest = tile;
     const int tx = rest % a.tiles_x;
     rest /= a.tiles_x;
     const int tyi = rest % a.tiles_y;
-    const int64_t z0 = rest / a.tiles_y;
-    const int64_t x0 = (int64_t)tx * 64;
-    const int64_t y0 = (int64_t)tyi * a.ty;
+    z0 = rest / a.tiles_y;
+    x0 = (int64_t)tx * 64;
+    y0 = (int64_t)tyi * a.ty;
     __syncthreads();  // the previous tile's reads of `support` are done
     for (int row = wave; row < a.rows; row += 4) {
         const int z = row / a.sy;
@@ -131,6 +152,21 @@ __device__ __forceinline__ void network_tile(const NetworkArgs &a, int tile,
         support[a.s + i] = KT::SENTINEL;
     }
     __syncthreads();
+}
+
+// One tile: stage the support (classified keys) into LDS, then one output
+// per thread and row step. `taps` holds the window's support offsets.
+template <typename T, int LIVE, int STATS>
+__device__ __forceinline__ void network_tile(const NetworkArgs &a, int tile,
+                                             typename KeyTraits<T>::Key *support, const uint32_t *taps)
+{
+    using KT = KeyTraits<T>;
+    using Key = typename KT::Key;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    Key *output = static_cast<Key *>(a.output);
+    int64_t z0, y0, x0;
+    stage_network_tile<T>(a, tile, support, z0, y0, x0);
     const int64_t x = x0 + lane;
     for (int r = wave; r < a.ty; r += 4) {
         const int64_t y = y0 + r;
@@ -150,6 +186,141 @@ __device__ __forceinline__ void network_tile(const NetworkArgs &a, int tile,
         const Key out = window_result<T, LIVE, STATS>(a, k, n);
         output[(z0 * a.ny + y) * a.nx + x] = out;
     }
+}
+
+// Parts of the two-row networks (PairSort<C, E> in ndf_stack_networks.inc).
+enum PairNet { PAIR_CORE_FULL, PAIR_CORE_HALF, PAIR_EXTRA_FULL, PAIR_EXTRA_HALF, PAIR_FIN_FULL, PAIR_FIN_HALF };
+template <int C, int E, int PART>
+struct PairPart {
+    using PS = stack_net::PairSort<C, E>;
+    static constexpr int count = PART == PAIR_CORE_FULL    ? PS::core_full_count
+                                 : PART == PAIR_CORE_HALF  ? PS::core_half_count
+                                 : PART == PAIR_EXTRA_FULL ? PS::extra_full_count
+                                 : PART == PAIR_EXTRA_HALF ? PS::extra_half_count
+                                 : PART == PAIR_FIN_FULL   ? PS::fin_full_count
+                                                           : PS::fin_half_count;
+    static constexpr int wire(int i, int side)
+    {
+        return PART == PAIR_CORE_FULL    ? PS::core_full[i][side]
+               : PART == PAIR_CORE_HALF  ? PS::core_half[i][side]
+               : PART == PAIR_EXTRA_FULL ? PS::extra_full[i][side]
+               : PART == PAIR_EXTRA_HALF ? PS::extra_half[i][side]
+               : PART == PAIR_FIN_FULL   ? PS::fin_full[i][side]
+                                         : PS::fin_half[i][side];
+    }
+    static constexpr int lo(int i) { return wire(i, 0); }
+    static constexpr int hi(int i) { return wire(i, 1); }
+};
+
+// One tile with two outputs per thread and step: the windows at rows r and
+// r + 1 share the C core taps (taps[0 .. C)); each has E taps of its own
+// (taps[C .. C + E) for row r, taps[C + E .. C + 2E) for row r + 1, all
+// relative to row r; entries past the real counts read the sentinel run).
+// The core and the two extras are sorted once each, then each window merges
+// the sorted core with its sorted extra.
+template <typename T, int C, int E, int STATS>
+__device__ __forceinline__ void network_pair_tile(const NetworkArgs &a, int tile,
+                                                  typename KeyTraits<T>::Key *support, const uint32_t *taps)
+{
+    using KT = KeyTraits<T>;
+    using Key = typename KT::Key;
+    using PS = stack_net::PairSort<C, E>;
+    constexpr int W = C + E;
+    constexpr bool MEDIAN = STATS == STATS_MEDIAN;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    Key *output = static_cast<Key *>(a.output);
+    int64_t z0, y0, x0;
+    stage_network_tile<T>(a, tile, support, z0, y0, x0);
+    const int64_t x = x0 + lane;
+    for (int r = 2 * wave; r < a.ty; r += 8) {
+        const int64_t y = y0 + r;
+        if (y >= a.ny || x >= a.nx) {
+            continue;
+        }
+        const Key *window = support + r * a.sx + lane;
+        Key core[C];
+        int ncore = 0;
+#pragma unroll
+        for (int i = 0; i < C; ++i) {
+            core[i] = window[taps[i]];
+            ncore += core[i] != KT::SENTINEL;
+        }
+        Key extra[2][E];
+        int nextra[2] = {0, 0};
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+#pragma unroll
+            for (int i = 0; i < E; ++i) {
+                extra[j][i] = window[taps[C + j * E + i]];
+                nextra[j] += extra[j][i] != KT::SENTINEL;
+            }
+        }
+        apply_net<PairPart<C, E, MEDIAN ? PAIR_CORE_HALF : PAIR_CORE_FULL>>(core);
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            apply_net<PairPart<C, E, MEDIAN ? PAIR_EXTRA_HALF : PAIR_EXTRA_FULL>>(extra[j]);
+            Key u[W];
+#pragma unroll
+            for (int i = 0; i < C; ++i) u[i] = core[PS::core_out[i]];
+#pragma unroll
+            for (int i = 0; i < E; ++i) u[C + i] = extra[j][PS::extra_out[i]];
+            const int n = ncore + nextra[j];
+            Key out;
+            if constexpr (MEDIAN) {
+                apply_net<PairPart<C, E, PAIR_FIN_HALF>>(u);
+                Key h[PS::H];
+#pragma unroll
+                for (int i = 0; i < PS::H; ++i) h[i] = u[PS::fin_half_out[i]];
+                out = median_from_half<T, PS::H>(a, h, n);
+            } else {
+                apply_net<PairPart<C, E, PAIR_FIN_FULL>>(u);
+                Key f[W];
+#pragma unroll
+                for (int i = 0; i < W; ++i) f[i] = u[PS::fin_full_out[i]];
+                out = statistic_from_sorted<T, W>(a, f, n);
+            }
+            if (y + j < a.ny) output[(z0 * a.ny + y + j) * a.nx + x] = out;
+        }
+    }
+}
+
+template <typename T, int C, int E, int STATS>
+__global__ void __launch_bounds__(NETWORK_THREADS) k_network_pair(NetworkArgs a)
+{
+    using Key = typename KeyTraits<T>::Key;
+    extern __shared__ __align__(16) unsigned char lds[];
+    Key *support = reinterpret_cast<Key *>(lds);
+    uint32_t *taps = reinterpret_cast<uint32_t *>(support + a.s + a.pad);
+    for (int t = threadIdx.x; t < C + 2 * E; t += NETWORK_THREADS) {
+        taps[t] = a.taps[t];
+    }
+    const int tiles = a.tiles_x * a.tiles_y * (int)a.nz;
+    for (int tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+        network_pair_tile<T, C, E, STATS>(a, tile, support, taps);
+    }
+}
+
+template <typename T, int C, int E, int STATS>
+hipError_t launch_pair_stats(const NetworkArgs &args, size_t lds_bytes, hipStream_t stream)
+{
+    const void *fn = reinterpret_cast<const void *>(&k_network_pair<T, C, E, STATS>);
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+    if (e != hipSuccess) {
+        return e;
+    }
+    const int64_t tiles = (int64_t)args.tiles_x * args.tiles_y * args.nz;
+    const int64_t blocks = tiles < ((int64_t)1 << 24) ? tiles : ((int64_t)1 << 24);
+    hipLaunchKernelGGL((k_network_pair<T, C, E, STATS>), dim3((unsigned)(blocks < 1 ? 1 : blocks)),
+                       dim3(NETWORK_THREADS), lds_bytes, stream, args);
+    return hipGetLastError();
+}
+
+template <int C, int E>
+hipError_t launch_pair(const NetworkArgs &args, size_t lds_bytes, hipStream_t stream)
+{
+    return args.stat == STAT_MEDIAN ? launch_pair_stats<float, C, E, STATS_MEDIAN>(args, lds_bytes, stream)
+                                    : launch_pair_stats<float, C, E, STATS_OTHER>(args, lds_bytes, stream);
 }
 
 // One image: workgroups stride over the tiles.
@@ -268,6 +439,68 @@ int network_live(int dtype_bytes, int count)
     return 0;
 }
 
+namespace {
+
+struct PairCost {
+    int c, e, half, full;
+};
+
+// Comparators per output of each compiled two-row class (median / full).
+template <int C, int E>
+constexpr PairCost pair_cost()
+{
+    using PS = stack_net::PairSort<C, E>;
+    return PairCost{C, E, (PS::core_half_count + 2 * PS::extra_half_count + 2 * PS::fin_half_count) / 2,
+                    (PS::core_full_count + 2 * PS::extra_full_count + 2 * PS::fin_full_count) / 2};
+}
+
+constexpr PairCost PAIR_COSTS[] = {pair_cost<16, 8>(),  pair_cost<32, 12>(), pair_cost<48, 20>(),
+                                   pair_cost<64, 24>(), pair_cost<96, 32>(), pair_cost<128, 32>()};
+
+template <int LIVE>
+constexpr int sort_cost(bool median)
+{
+    return median ? stack_net::Sort<LIVE>::half_count : stack_net::Sort<LIVE>::full_count;
+}
+
+int single_cost(int live, bool median)
+{
+    switch (live) {
+    case 9: return sort_cost<9>(median);
+    case 16: return sort_cost<16>(median);
+    case 25: return sort_cost<25>(median);
+    case 32: return sort_cost<32>(median);
+    case 49: return sort_cost<49>(median);
+    case 64: return sort_cost<64>(median);
+    case 72: return sort_cost<72>(median);
+    case 96: return sort_cost<96>(median);
+    case 128: return sort_cost<128>(median);
+    case 160: return sort_cost<160>(median);
+    default: return 1 << 30;
+    }
+}
+
+}  // namespace
+
+bool network_pair_class(int dtype_bytes, int core, int extra, int live, bool median, int *c, int *e)
+{
+    if (dtype_bytes != 4) return false;
+    for (const PairCost &k : PAIR_COSTS) {
+        if (core <= k.c && extra <= k.e) {
+            // Fewer comparators per output than one output per step, with a
+            // tenth to spare for the second extra's loads and merge set-up.
+            const int pair = median ? k.half : k.full;
+            if (10 * pair < 9 * single_cost(live, median)) {
+                *c = k.c;
+                *e = k.e;
+                return true;
+            }
+            return false;
+        }
+    }
+    return false;
+}
+
 size_t network_lds_bytes(int dtype_bytes, int s, int pad, int live)
 {
     return ((size_t)s + (size_t)pad) * (size_t)dtype_bytes + (size_t)live * 4;
@@ -339,6 +572,19 @@ hipError_t launch_network_batch(int dtype_bytes, int live, const NetworkArgs *im
 
 hipError_t launch_network(int dtype_bytes, const NetworkArgs &args, size_t lds_bytes, hipStream_t stream)
 {
+    if (args.pair_c > 0) {
+        if (dtype_bytes != 4) return hipErrorInvalidValue;
+#define NDF_PAIR_CASE(C, E) \
+    if (args.pair_c == C && args.pair_e == E) return launch_pair<C, E>(args, lds_bytes, stream);
+        NDF_PAIR_CASE(16, 8)
+        NDF_PAIR_CASE(32, 12)
+        NDF_PAIR_CASE(48, 20)
+        NDF_PAIR_CASE(64, 24)
+        NDF_PAIR_CASE(96, 32)
+        NDF_PAIR_CASE(128, 32)
+#undef NDF_PAIR_CASE
+        return hipErrorInvalidValue;
+    }
 #define NDF_NETWORK_CASE(LIVE)                                                                     \
     case LIVE:                                                                                     \
         return dtype_bytes == 4 ? launch_live<float, LIVE>(args, lds_bytes, stream)                \
