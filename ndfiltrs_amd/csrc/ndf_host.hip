@@ -1804,6 +1804,18 @@ struct ndf_batch {
         int nwork = 0;
     };
     std::vector<NetworkGroup> groups;
+    // Images with box windows the stack engine has networks for: one group
+    // (and launch) per box and statistic.
+    struct StackGroup {
+        StackArgs shape{};
+        std::vector<StackArgs> args;
+        std::vector<int64_t> offsets;
+        std::vector<int32_t> pairs;
+        DeviceBuffer d_args, d_offsets, d_work;
+        int grid = 0;
+        int nwork = 0;
+    };
+    std::vector<StackGroup> sgroups;
     // Captured launch (for one input/output pair).
     hipStream_t capture_stream = nullptr;
     hipGraph_t graph = nullptr;
@@ -1828,6 +1840,9 @@ struct ndf_batch {
         for (ndf_plan *p : plans) ndf_plan_destroy(p);
         for (DeviceBuffer *b : {&d_images, &d_offsets, &d_work, &d_counter, &d_error, &d_maps, &d_taps}) b->release();
         for (NetworkGroup &g : groups) {
+            for (DeviceBuffer *b : {&g.d_args, &g.d_offsets, &g.d_work}) b->release();
+        }
+        for (StackGroup &g : sgroups) {
             for (DeviceBuffer *b : {&g.d_args, &g.d_offsets, &g.d_work}) b->release();
         }
     }
@@ -1855,6 +1870,11 @@ ndf_status batch_prepare(ndf_batch *b, char *err, size_t err_len)
         for (DeviceBuffer *buffer : {&g.d_args, &g.d_offsets, &g.d_work}) buffer->release();
     }
     b->groups.clear();
+    for (ndf_batch::StackGroup &g : b->sgroups) {
+        for (DeviceBuffer *buffer : {&g.d_args, &g.d_offsets, &g.d_work}) buffer->release();
+    }
+    b->sgroups.clear();
+    std::vector<std::array<int64_t, 4>> stack_at;  // image, maps offset, group, index in group
     std::vector<uint32_t> taps;
     std::vector<std::array<int64_t, 5>> net_at;  // image, maps offset, taps offset, group, index in group
     for (size_t i = 0; i < b->plans.size(); ++i) {
@@ -1863,6 +1883,33 @@ ndf_status batch_prepare(ndf_batch *b, char *err, size_t err_len)
         if (p->len == 0) continue;
         if (p->all_nan) {
             b->fill_images.push_back((int32_t)i);
+            continue;
+        }
+        if (p->stack_ok) {
+            const StackArgs &k = p->stack;
+            size_t gi = 0;
+            while (gi < b->sgroups.size() &&
+                   !(b->sgroups[gi].shape.k0 == k.k0 && b->sgroups[gi].shape.sh == k.sh &&
+                     b->sgroups[gi].shape.sw == k.sw && b->sgroups[gi].shape.stat == k.stat)) {
+                ++gi;
+            }
+            if (gi == b->sgroups.size()) {
+                b->sgroups.emplace_back();
+                b->sgroups.back().shape = k;
+            }
+            ndf_batch::StackGroup &g = b->sgroups[gi];
+            stack_at.push_back({(int64_t)i, (int64_t)maps.size(), (int64_t)gi, (int64_t)g.args.size()});
+            maps.insert(maps.end(), p->stk_maps.begin(), p->stk_maps.end());
+            maps.insert(maps.end(), p->stk_mapr.begin(), p->stk_mapr.end());
+            maps.insert(maps.end(), p->stk_mapx.begin(), p->stk_mapx.end());
+            const int index = (int)g.args.size();
+            g.args.push_back(k);
+            g.offsets.push_back(b->offsets[i]);
+            const int64_t blocks = (k.units + 3) / 4;
+            for (int64_t t = 0; t < blocks; ++t) {
+                g.pairs.push_back(index);
+                g.pairs.push_back((int32_t)t);
+            }
             continue;
         }
         if (p->network_ok) {
@@ -1931,6 +1978,24 @@ ndf_status batch_prepare(ndf_batch *b, char *err, size_t err_len)
         n.input = nullptr;
         n.output = nullptr;
     }
+    for (const std::array<int64_t, 4> &at : stack_at) {
+        const ndf_plan *p = b->plans[(size_t)at[0]];
+        StackArgs &k = b->sgroups[(size_t)at[2]].args[(size_t)at[3]];
+        k.maps = base + at[1];
+        k.mapr = k.maps + p->stk_maps.size();
+        k.mapx = k.mapr + p->stk_mapr.size();
+        k.input = nullptr;
+        k.output = nullptr;
+    }
+    for (ndf_batch::StackGroup &g : b->sgroups) {
+        g.nwork = (int)(g.pairs.size() / 2);
+        if ((e = upload(g.d_args, g.args)) != hipSuccess || (e = upload(g.d_offsets, g.offsets)) != hipSuccess ||
+            (e = upload(g.d_work, g.pairs)) != hipSuccess ||
+            (e = stack_batch_grid(b->dbytes, g.shape, &g.grid)) != hipSuccess) {
+            return fail_hip(e, err, err_len);
+        }
+        g.grid = std::max(1, std::min(g.grid, g.nwork));
+    }
     for (ndf_batch::NetworkGroup &g : b->groups) {
         g.nwork = (int)(g.pairs.size() / 2);
         if ((e = upload(g.d_args, g.args)) != hipSuccess || (e = upload(g.d_offsets, g.offsets)) != hipSuccess ||
@@ -1955,7 +2020,7 @@ ndf_status batch_prepare(ndf_batch *b, char *err, size_t err_len)
     }
     if ((e = upload(b->d_images, b->images)) != hipSuccess || (e = upload(b->d_offsets, b->offsets)) != hipSuccess ||
         (e = upload(b->d_work, pairs)) != hipSuccess ||
-        (e = b->d_counter.ensure(sizeof(int32_t) * (1 + b->groups.size()))) != hipSuccess ||
+        (e = b->d_counter.ensure(sizeof(int32_t) * (1 + b->groups.size() + b->sgroups.size()))) != hipSuccess ||
         (e = b->d_error.ensure(sizeof(int32_t))) != hipSuccess) {
         return fail_hip(e, err, err_len);
     }
@@ -1981,7 +2046,18 @@ hipError_t batch_enqueue(ndf_batch *b, const void *d_in, void *d_out, hipStream_
         void *out = static_cast<char *>(d_out) + b->offsets[(size_t)i] * b->dbytes;
         e = launch_fill(b->dbytes, out, (int64_t)b->plans[(size_t)i]->len, nan_bits, stream);
     }
-    if (e == hipSuccess) e = launch_fill(4, b->d_counter.ptr, (int64_t)(1 + b->groups.size()), 0, stream);
+    if (e == hipSuccess) {
+        e = launch_fill(4, b->d_counter.ptr, (int64_t)(1 + b->groups.size() + b->sgroups.size()), 0, stream);
+    }
+    for (size_t gi = 0; gi < b->sgroups.size() && e == hipSuccess; ++gi) {
+        const ndf_batch::StackGroup &g = b->sgroups[gi];
+        if (g.nwork == 0) continue;
+        e = launch_stack_batch(b->dbytes, g.shape, static_cast<const StackArgs *>(g.d_args.ptr),
+                               static_cast<const int64_t *>(g.d_offsets.ptr),
+                               static_cast<const int32_t *>(g.d_work.ptr), g.nwork,
+                               static_cast<int32_t *>(b->d_counter.ptr) + 1 + b->groups.size() + gi, d_in, d_out,
+                               g.grid, stream);
+    }
     for (size_t gi = 0; gi < b->groups.size() && e == hipSuccess; ++gi) {
         const ndf_batch::NetworkGroup &g = b->groups[gi];
         if (g.nwork == 0) continue;
@@ -2203,7 +2279,16 @@ ndf_status ndf_batch_describe(ndf_batch *batch, char *buffer, size_t buffer_len)
         snprintf(buffer + used, buffer_len - used, "%s{\"wires\": %d, \"images\": %zu, \"tiles\": %d}",
                  gi ? ", " : "", batch->groups[gi].live, batch->groups[gi].args.size(), batch->groups[gi].nwork);
     }
-    const size_t used = strlen(buffer);
+    size_t used = strlen(buffer);
+    if (used + 24 < buffer_len) snprintf(buffer + used, buffer_len - used, "], \"stack_groups\": [");
+    for (size_t gi = 0; gi < batch->sgroups.size(); ++gi) {
+        used = strlen(buffer);
+        if (used + 96 >= buffer_len) break;
+        const ndf_batch::StackGroup &g = batch->sgroups[gi];
+        snprintf(buffer + used, buffer_len - used, "%s{\"slices\": [%d, %d, %d], \"images\": %zu, \"blocks\": %d}",
+                 gi ? ", " : "", g.shape.k0, g.shape.sh, g.shape.sw, g.args.size(), g.nwork);
+    }
+    used = strlen(buffer);
     if (used + 3 < buffer_len) snprintf(buffer + used, buffer_len - used, "]}");
     return st;
 }

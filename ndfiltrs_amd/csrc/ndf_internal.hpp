@@ -157,6 +157,14 @@ bool stack_supported(int dtype_bytes, int k0, int sh, int sw);
 size_t stack_lds_bytes(int dtype_bytes, int sh, int sx);
 int64_t stack_max_units();
 hipError_t launch_stack(int dtype_bytes, const StackArgs &args, hipStream_t stream);
+// Batched stack kernel for images sharing the box (k0, sh, sw) and the
+// statistic of `shape`: `images` one StackArgs per image (input / output
+// unused), `work` nwork (image, block of 4 units) pairs taken through
+// `counter`.
+hipError_t stack_batch_grid(int dtype_bytes, const StackArgs &shape, int *grid);
+hipError_t launch_stack_batch(int dtype_bytes, const StackArgs &shape, const StackArgs *images,
+                              const int64_t *offsets, const int32_t *work, int nwork, int32_t *counter,
+                              const void *input, void *output, int grid, hipStream_t stream);
 
 // Engine 4: rectangular minimum / maximum (rank <= 3) by separable passes
 // over an LDS tile of tx columns x ty rows of one plane; its support is kz

@@ -93,11 +93,13 @@ __device__ __forceinline__ int64_t combine(int64_t a, int64_t b)
     return a + b;
 }
 
-// MEDIAN: the statistic is the median (the lower-half network); otherwise
-// rank or percentile (the full network). Separate kernels keep the median's
-// register count to what its network needs.
+// One unit: 64 columns of one row of the row axis, a run along the slide
+// axis. MEDIAN: the statistic is the median (the lower-half network);
+// otherwise rank or percentile (the full network). Separate kernels keep
+// the median's register count to what its network needs. `lds` is the
+// workgroup's dynamic LDS (one region per wave).
 template <typename T, int K0, int SH, int SW, bool MEDIAN>
-__global__ void __launch_bounds__(STACK_THREADS) k_stack(StackArgs a)
+__device__ __forceinline__ void stack_unit(const StackArgs &a, int64_t unit, unsigned char *lds)
 {
     using Net = stack_net::Net<K0, SH, SW>;
     using KT = KeyTraits<T>;
@@ -107,19 +109,10 @@ __global__ void __launch_bounds__(STACK_THREADS) k_stack(StackArgs a)
     // LDS rows per wave: the SH sorted keys of each column, and its retained
     // count (one row of single keys when SH == 1).
     constexpr int ROWS = SH > 1 ? SH + 1 : 1;
-    extern __shared__ __align__(16) unsigned char lds[];
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int sx = a.sx;
     Key *col = reinterpret_cast<Key *>(lds) + (size_t)wave * ROWS * sx;
-
-    // XCD-aware order: the grid is a multiple of 8 blocks, dispatched round
-    // robin over the 8 XCDs; XCD j runs the contiguous block range
-    // [j * nb / 8, (j + 1) * nb / 8), whose units share input rows in its L2.
-    const int nb8 = (int)(gridDim.x >> 3);
-    const int64_t block = (int64_t)(blockIdx.x & 7) * nb8 + (blockIdx.x >> 3);
-    const int64_t unit = block * STACK_WAVES + wave;
-    if (unit >= a.units) return;  // wave-uniform; the kernel has no barrier
     const int64_t xc = unit % a.tiles_x;
     const int64_t rest = unit / a.tiles_x;
     const int64_t r = rest % a.nr;
@@ -354,6 +347,48 @@ __global__ void __launch_bounds__(STACK_THREADS) k_stack(StackArgs a)
     }
 }
 
+template <typename T, int K0, int SH, int SW, bool MEDIAN>
+__global__ void __launch_bounds__(STACK_THREADS) k_stack(StackArgs a)
+{
+    extern __shared__ __align__(16) unsigned char lds[];
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    // XCD-aware order: the grid is a multiple of 8 blocks, dispatched round
+    // robin over the 8 XCDs; XCD j runs the contiguous block range
+    // [j * nb / 8, (j + 1) * nb / 8), whose units share input rows in its L2.
+    const int nb8 = (int)(gridDim.x >> 3);
+    const int64_t block = (int64_t)(blockIdx.x & 7) * nb8 + (blockIdx.x >> 3);
+    const int64_t unit = block * STACK_WAVES + wave;
+    if (unit >= a.units) return;  // wave-uniform; the kernel has no barrier
+    stack_unit<T, K0, SH, SW, MEDIAN>(a, unit, lds);
+}
+
+// A batch of images whose windows share the box: workgroups take (image,
+// block of 4 units) pairs from a work queue; images[i] holds image i's
+// arguments (input / output unused: its data starts at offsets[i]).
+template <typename T, int K0, int SH, int SW, bool MEDIAN>
+__global__ void __launch_bounds__(STACK_THREADS) k_stack_batch(const StackArgs *images, const int64_t *offsets,
+                                                               const int32_t *work, int nwork, int32_t *counter,
+                                                               const void *input, void *output)
+{
+    using Key = typename KeyTraits<T>::Key;
+    extern __shared__ __align__(16) unsigned char lds[];
+    __shared__ int next_shared;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    for (;;) {
+        if (threadIdx.x == 0) next_shared = atomicAdd(counter, 1);
+        __syncthreads();
+        const int w = __builtin_amdgcn_readfirstlane(next_shared);
+        __syncthreads();
+        if (w >= nwork) break;
+        const int image = work[2 * w];
+        StackArgs a = images[image];
+        a.input = static_cast<const Key *>(input) + offsets[image];
+        a.output = static_cast<Key *>(output) + offsets[image];
+        const int64_t unit = (int64_t)work[2 * w + 1] * STACK_WAVES + wave;
+        if (unit < a.units) stack_unit<T, K0, SH, SW, MEDIAN>(a, unit, lds);
+    }
+}
+
 template <typename T, int K0, int SH, int SW>
 hipError_t launch_shape(const StackArgs &args, hipStream_t stream)
 {
@@ -368,6 +403,39 @@ hipError_t launch_shape(const StackArgs &args, hipStream_t stream)
                            args);
     }
     return hipGetLastError();
+}
+
+template <typename T, int K0, int SH, int SW>
+hipError_t batch_shape(const StackArgs *images, const int64_t *offsets, const int32_t *work, int nwork,
+                       int32_t *counter, const void *input, void *output, bool median, int sx, int grid,
+                       hipStream_t stream)
+{
+    const size_t lds = stack_lds_bytes((int)sizeof(T), SH, sx);
+    if (median) {
+        hipLaunchKernelGGL((k_stack_batch<T, K0, SH, SW, true>), dim3((unsigned)grid), dim3(STACK_THREADS), lds,
+                           stream, images, offsets, work, nwork, counter, input, output);
+    } else {
+        hipLaunchKernelGGL((k_stack_batch<T, K0, SH, SW, false>), dim3((unsigned)grid), dim3(STACK_THREADS), lds,
+                           stream, images, offsets, work, nwork, counter, input, output);
+    }
+    return hipGetLastError();
+}
+
+template <typename T, int K0, int SH, int SW>
+hipError_t batch_grid_shape(bool median, int sx, int *grid)
+{
+    const void *fn = median ? reinterpret_cast<const void *>(&k_stack_batch<T, K0, SH, SW, true>)
+                            : reinterpret_cast<const void *>(&k_stack_batch<T, K0, SH, SW, false>);
+    const size_t lds = stack_lds_bytes((int)sizeof(T), SH, sx);
+    int per_cu = 0, device = 0, cus = 0;
+    hipError_t e;
+    if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, STACK_THREADS, lds)) != hipSuccess ||
+        (e = hipGetDevice(&device)) != hipSuccess ||
+        (e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device)) != hipSuccess) {
+        return e;
+    }
+    *grid = (per_cu < 1 ? 1 : per_cu) * (cus < 1 ? 1 : cus);
+    return hipSuccess;
 }
 
 }  // namespace
@@ -407,6 +475,54 @@ hipError_t launch_stack(int dtype_bytes, const StackArgs &args, hipStream_t stre
     NDF_STACK_CASE_F32(3, 7, 7)
 #undef NDF_STACK_CASE
 #undef NDF_STACK_CASE_F32
+    return hipErrorInvalidValue;
+}
+
+// Shapes compiled for both dtypes (B) and for float32 only (F).
+#define NDF_STACK_SHAPES(B, F) B(3, 1, 3) B(5, 1, 5) B(7, 1, 7) F(9, 1, 9) B(3, 3, 3) F(3, 5, 5) F(3, 7, 7)
+
+hipError_t stack_batch_grid(int dtype_bytes, const StackArgs &shape, int *grid)
+{
+    if (!stack_supported(dtype_bytes, shape.k0, shape.sh, shape.sw)) return hipErrorInvalidValue;
+    const bool median = shape.stat == STAT_MEDIAN;
+#define NDF_GRID_BOTH(K0, SH, SW)                                                                        \
+    if (shape.k0 == K0 && shape.sh == SH && shape.sw == SW) {                                           \
+        return dtype_bytes == 4 ? batch_grid_shape<float, K0, SH, SW>(median, shape.sx, grid)            \
+                                : batch_grid_shape<double, K0, SH, SW>(median, shape.sx, grid);          \
+    }
+#define NDF_GRID_F32(K0, SH, SW)                                                                         \
+    if (shape.k0 == K0 && shape.sh == SH && shape.sw == SW) {                                           \
+        return batch_grid_shape<float, K0, SH, SW>(median, shape.sx, grid);                               \
+    }
+    NDF_STACK_SHAPES(NDF_GRID_BOTH, NDF_GRID_F32)
+#undef NDF_GRID_BOTH
+#undef NDF_GRID_F32
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_stack_batch(int dtype_bytes, const StackArgs &shape, const StackArgs *images,
+                              const int64_t *offsets, const int32_t *work, int nwork, int32_t *counter,
+                              const void *input, void *output, int grid, hipStream_t stream)
+{
+    if (!stack_supported(dtype_bytes, shape.k0, shape.sh, shape.sw)) return hipErrorInvalidValue;
+    const bool median = shape.stat == STAT_MEDIAN;
+#define NDF_BATCH_BOTH(K0, SH, SW)                                                                        \
+    if (shape.k0 == K0 && shape.sh == SH && shape.sw == SW) {                                            \
+        return dtype_bytes == 4 ? batch_shape<float, K0, SH, SW>(images, offsets, work, nwork, counter,   \
+                                                                 input, output, median, shape.sx, grid,   \
+                                                                 stream)                                  \
+                                : batch_shape<double, K0, SH, SW>(images, offsets, work, nwork, counter,  \
+                                                                  input, output, median, shape.sx, grid,  \
+                                                                  stream);                                \
+    }
+#define NDF_BATCH_F32(K0, SH, SW)                                                                         \
+    if (shape.k0 == K0 && shape.sh == SH && shape.sw == SW) {                                            \
+        return batch_shape<float, K0, SH, SW>(images, offsets, work, nwork, counter, input, output, median, \
+                                              shape.sx, grid, stream);                                    \
+    }
+    NDF_STACK_SHAPES(NDF_BATCH_BOTH, NDF_BATCH_F32)
+#undef NDF_BATCH_BOTH
+#undef NDF_BATCH_F32
     return hipErrorInvalidValue;
 }
 
