@@ -28,34 +28,61 @@ constexpr int MAX_TILE_BLOCKS = 1024;
 // One thread per 8x8 block; consecutive threads take consecutive blocks of
 // a block row, so each image row segment is read by 64 lanes as 2 KB of
 // contiguous memory. Output per block: retained count, sum, sum of squares
-// (f64).
+// (f64), accumulated in row-major order within the block. A whole block
+// whose rows start on 16-byte boundaries reads each row segment with 16-byte
+// vector loads, all eight rows before the first use; the sums are the same
+// additions in the same order either way.
 template <typename T>
 __global__ void __launch_bounds__(256) k_block_stats(const T *image, int64_t ny, int64_t nx, int64_t by, int64_t bx,
                                                      int ignore_inf, double *out)
 {
     using KT = KeyTraits<T>;
+    using Key = typename KT::Key;
     const int64_t block = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (block >= by * bx) {
         return;
     }
     const int64_t y0 = (block / bx) * BLOCK, x0 = (block % bx) * BLOCK;
     double n = 0.0, s = 0.0, q = 0.0;
-    for (int dy = 0; dy < BLOCK; ++dy) {
-        const int64_t y = y0 + dy;
-        if (y >= ny) {
-            break;
-        }
-        const T *row = image + y * nx;
+    auto add = [&](Key bits) {
+        // Omitted samples add +0.0, which leaves every sum unchanged (the
+        // sums start at +0.0 and never become -0.0).
+        const bool keep = classify<T>(bits, ignore_inf) != KT::SENTINEL;
+        const double v = KT::to_double(bits);
+        n += keep ? 1.0 : 0.0;
+        s += keep ? v : 0.0;
+        q += keep ? v * v : 0.0;
+    };
+    constexpr int PER16 = 16 / (int)sizeof(T);  // elements per 16-byte load
+    const bool whole = y0 + BLOCK <= ny && x0 + BLOCK <= nx && ((nx * (int64_t)sizeof(T)) % 16) == 0 &&
+                       (reinterpret_cast<uintptr_t>(image) % 16) == 0;
+    if (whole) {
+        uint4 raw[BLOCK][BLOCK / PER16];
 #pragma unroll
-        for (int dx = 0; dx < BLOCK; ++dx) {
-            if (x0 + dx < nx) {
-                const typename KT::Key bits = KT::bits(row[x0 + dx]);
-                if (classify<T>(bits, ignore_inf) != KT::SENTINEL) {
-                    const double v = KT::to_double(bits);
-                    n += 1.0;
-                    s += v;
-                    q += v * v;
-                }
+        for (int dy = 0; dy < BLOCK; ++dy) {
+            const uint4 *row = reinterpret_cast<const uint4 *>(image + (y0 + dy) * nx + x0);
+#pragma unroll
+            for (int c = 0; c < BLOCK / PER16; ++c) raw[dy][c] = row[c];
+        }
+#pragma unroll
+        for (int dy = 0; dy < BLOCK; ++dy) {
+#pragma unroll
+            for (int c = 0; c < BLOCK / PER16; ++c) {
+                const Key *keys = reinterpret_cast<const Key *>(&raw[dy][c]);
+#pragma unroll
+                for (int e = 0; e < PER16; ++e) add(keys[e]);
+            }
+        }
+    } else {
+        for (int dy = 0; dy < BLOCK; ++dy) {
+            const int64_t y = y0 + dy;
+            if (y >= ny) {
+                break;
+            }
+            const T *row = image + y * nx;
+#pragma unroll
+            for (int dx = 0; dx < BLOCK; ++dx) {
+                if (x0 + dx < nx) add(KT::bits(row[x0 + dx]));
             }
         }
     }
