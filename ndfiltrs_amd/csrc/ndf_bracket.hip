@@ -4,13 +4,14 @@
 // walks. The bracket comes from a tiny MLP (6 -> 32 -> 32 -> 2, ReLU, fp32)
 // that reads cheap box-sum features of the tile's support:
 //   1. k_block_stats: one pass over the image, per 8x8 block: retained
-//      count, sum, sum of squares (f64), min, max;
+//      count, mean and standard deviation (f64);
 //   2. k_tile_features: per tile, robust level c and noise scale sigma
 //      (medians of the block means and block std devs), a plane slope of
 //      the block means, the retained fraction and the fraction of bright
 //      blocks, standardised so that the features are scale free;
-//   3. k_bracket_predict: the MLP's two outputs are standardised offsets:
-//      bracket = [c + sigma*o0, c + sigma*o1], converted to keys.
+//   3. the MLP (run by the same wave as the tile's features): its two outputs
+//      are standardised offsets: bracket = [c + sigma*o0, c + sigma*o1],
+//      converted to keys.
 // Training (k_tile_labels, k_mlp_grad, k_mlp_adam) fits the offsets of the
 // smallest and largest window result per tile with pinball losses, so that
 // the bracket contains every target of a tile with high probability.
@@ -23,12 +24,15 @@ namespace ndf {
 namespace {
 
 constexpr int BLOCK = 8;
-constexpr int MAX_TILE_BLOCKS = 1024;
+// Blocks per tile support the features read (at most 4 per lane of the
+// wave); wider supports are subsampled by every 2nd, 4th, ... block row and
+// column.
+constexpr int MAX_TILE_BLOCKS = 256;
 
 // One thread per 8x8 block; consecutive threads take consecutive blocks of
 // a block row, so each image row segment is read by 64 lanes as 2 KB of
-// contiguous memory. Output per block: retained count, sum, sum of squares
-// (f64), accumulated in row-major order within the block. A whole block
+// contiguous memory. The block's retained count, sum and sum of squares are
+// accumulated in f64 in row-major order within the block. A whole block
 // whose rows start on 16-byte boundaries reads each row segment with 16-byte
 // vector loads, all eight rows before the first use; the sums are the same
 // additions in the same order either way.
@@ -86,10 +90,17 @@ __global__ void __launch_bounds__(256) k_block_stats(const T *image, int64_t ny,
             }
         }
     }
+    // Block moments for the tile features: retained count, mean and
+    // standard deviation (f64, computed once per block here rather than per
+    // tile that reads the block). A block with n == 0 stores a NaN mean,
+    // which no tile reads (blocks with fewer than 16 retained samples only
+    // contribute their count).
+    const double mu = s / n;
+    const double var = q / n - mu * mu;
     double *o = out + block * 3;
     o[0] = n;
-    o[1] = s;
-    o[2] = q;
+    o[1] = mu;
+    o[2] = var > 0.0 ? __builtin_sqrt(var) : 0.0;
 }
 
 // asinh(v) = sign(v) * log(|v| + sqrt(v^2 + 1)); log(2|v|) once v^2 would overflow.
@@ -100,24 +111,216 @@ __device__ inline float feature_asinh(double v)
     return (float)(v < 0.0 ? -r : r);
 }
 
-// Ascending bitonic sort of v[0 .. n) in LDS by one 64-lane workgroup
-// (n a power of two, 64 <= n <= MAX_TILE_BLOCKS).
-__device__ void wave_sort(float *v, int n, int lane)
+// Order-preserving 32-bit key of a float (total order: -NaN < -inf < ... <
+// -0 < +0 < ... < +inf < +NaN) and its inverse.
+__device__ inline uint32_t order_key(float v)
 {
-    for (int size = 2; size <= n; size <<= 1) {
-        for (int stride = size >> 1; stride > 0; stride >>= 1) {
-            for (int i = lane; i < n / 2; i += 64) {
-                const int lo = 2 * i - (i & (stride - 1));
-                const int hi = lo + stride;
-                const bool asc = (lo & size) == 0;
-                const float a = v[lo], b = v[hi];
-                const bool swap = asc ? (b < a) : (a < b);
-                v[lo] = swap ? b : a;
-                v[hi] = swap ? a : b;
+    const uint32_t u = __builtin_bit_cast(uint32_t, v);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+__device__ inline float order_value(uint32_t k)
+{
+    return __builtin_bit_cast(float, (k & 0x80000000u) ? (k ^ 0x80000000u) : ~k);
+}
+
+// v broadcast from lane i (i wave-uniform).
+__device__ inline float lane_value(float v, int i)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), i));
+}
+
+// Wave-wide minimum and maximum of the PER keys per lane.
+template <int PER>
+__device__ inline void wave_key_range(const uint32_t (&key)[PER], uint32_t &lo, uint32_t &hi)
+{
+    lo = key[0];
+    hi = key[0];
+#pragma unroll
+    for (int j = 1; j < PER; ++j) {
+        lo = key[j] < lo ? key[j] : lo;
+        hi = key[j] > hi ? key[j] : hi;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        const uint32_t l = (uint32_t)__shfl_xor((int)lo, off), h = (uint32_t)__shfl_xor((int)hi, off);
+        lo = l < lo ? l : lo;
+        hi = h > hi ? h : hi;
+    }
+}
+
+// The ka-th smallest of the wave's keys a[] and the kb-th smallest of b[]
+// (0-based; PER keys per lane each), by bitwise searches from the highest
+// bit in which a key range's minimum and maximum differ (the bits above are
+// common to every key in the range): each answer is the largest v with at
+// most k keys below it. A step counts the keys below each candidate with one
+// wave ballot per register; the two searches share the steps.
+template <int PER>
+__device__ inline void wave_select2(const uint32_t (&a)[PER], int ka, const uint32_t (&b)[PER], int kb, uint32_t &ra,
+                                    uint32_t &rb)
+{
+    uint32_t alo, ahi, blo, bhi;
+    wave_key_range<PER>(a, alo, ahi);
+    wave_key_range<PER>(b, blo, bhi);
+    const uint32_t adiff = alo ^ ahi, bdiff = blo ^ bhi;
+    const int abit = adiff ? 31 - __builtin_clz(adiff) : -1;
+    const int bbit = bdiff ? 31 - __builtin_clz(bdiff) : -1;
+    // Common high bits (all of them when every key is equal).
+    uint32_t pa = abit < 0 ? alo : abit >= 31 ? 0u : alo & ~((2u << abit) - 1u);
+    uint32_t pb = bbit < 0 ? blo : bbit >= 31 ? 0u : blo & ~((2u << bbit) - 1u);
+    for (int bit = abit > bbit ? abit : bbit; bit >= 0; --bit) {
+        const uint32_t ca = pa | (bit <= abit ? (1u << bit) : 0u);
+        const uint32_t cb = pb | (bit <= bbit ? (1u << bit) : 0u);
+        int na = 0, nb = 0;
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            na += (int)__popcll(__ballot(a[j] < ca));
+            nb += (int)__popcll(__ballot(b[j] < cb));
+        }
+        pa = na <= ka ? ca : pa;
+        pb = nb <= kb ? cb : pb;
+    }
+    ra = pa;
+    rb = pb;
+}
+
+// The support's 8x8 blocks as one row-major list of `count` entries
+// (w_used per row, every step-th block row and column).
+struct BlockList {
+    int64_t br0, bc0, bx;
+    int w_used, step, count;
+    __device__ const double *at(const double *blocks, int i) const
+    {
+        const int64_t br = br0 + (int64_t)(i / w_used) * step, bc = bc0 + (int64_t)(i % w_used) * step;
+        return blocks + (br * bx + bc) * 3;
+    }
+};
+
+// A tile's block statistics, summed over the wave (every lane holds the
+// totals): retained count n over `cells` pixels, `usable` blocks (at least
+// 16 retained samples), level c and scale sigma (medians of the usable
+// blocks' means and standard deviations), the plane-fit sums over the blocks
+// within 0.75 sigma of c, and the bright-block sums.
+struct TileSums {
+    double n, cells;
+    int usable;
+    float c, sigma;
+    double sw, sr, sq, sm, srr, sqq, srq, srm, sqm, extended, spiky;
+};
+
+// Block i = j * 64 + lane of the list sits in register j of its lane; the
+// three moments of all PER blocks are loaded before the first use.
+template <int PER>
+__device__ inline TileSums tile_sums(const double *blocks, const BlockList &list, int lane)
+{
+    double o0[PER], o1[PER], o2[PER];
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+        const int i = j * 64 + lane;
+        const double *o = list.at(blocks, i < list.count ? i : 0);
+        o0[j] = o[0];
+        o1[j] = o[1];
+        o2[j] = o[2];
+    }
+    TileSums t{};
+    uint32_t mean_key[PER], sd_key[PER];
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+        const bool valid = j * 64 + lane < list.count;
+        const bool usable = valid && o0[j] >= 16.0;
+        t.n += valid ? o0[j] : 0.0;
+        t.cells += valid ? 64.0 : 0.0;
+        t.usable += usable ? 1 : 0;
+        // Unusable blocks and padding sort last (above every float key).
+        mean_key[j] = usable ? order_key((float)o1[j]) : 0xFFFFFFFFu;
+        sd_key[j] = usable ? order_key((float)o2[j]) : 0xFFFFFFFFu;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        t.n += __shfl_xor(t.n, off);
+        t.cells += __shfl_xor(t.cells, off);
+        t.usable += __shfl_xor(t.usable, off);
+    }
+    t.c = 0.0f;
+    t.sigma = 1.0f;
+    if (t.usable > 0) {
+        uint32_t mk, sk;
+        wave_select2<PER>(mean_key, (t.usable - 1) / 2, sd_key, (t.usable - 1) / 2, mk, sk);
+        t.c = order_value(mk);
+        t.sigma = order_value(sk);
+    }
+    const float floor_sigma = 1e-6f * (__builtin_fabsf(t.c) + 1e-30f);
+    t.sigma = t.sigma > floor_sigma ? t.sigma : floor_sigma;
+    const double c = (double)t.c, sigma = (double)t.sigma;
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+        const int i = j * 64 + lane;
+        if (i < list.count && o0[j] >= 16.0) {
+            const double mu = o1[j] - c;
+            // A bright block is "spiky" when its spread dwarfs its mean
+            // shift (one hot pixel: sd ~ 8 x shift) and "extended" otherwise
+            // (a source covering the block); only extended brightness moves
+            // window ranks much.
+            if (mu > 0.5 * sigma) {
+                if (o2[j] < 4.0 * mu) {
+                    t.extended += mu;
+                } else {
+                    t.spiky += 1.0;
+                }
             }
-            __syncthreads();
+            if (__builtin_fabs(mu) <= 0.75 * sigma) {
+                const double r = (double)((i / list.w_used) * list.step * BLOCK);
+                const double q = (double)((i % list.w_used) * list.step * BLOCK);
+                t.sw += 1.0;
+                t.sr += r;
+                t.sq += q;
+                t.sm += mu;
+                t.srr += r * r;
+                t.sqq += q * q;
+                t.srq += r * q;
+                t.srm += r * mu;
+                t.sqm += q * mu;
+            }
         }
     }
+    for (int off = 32; off > 0; off >>= 1) {
+        t.sw += __shfl_xor(t.sw, off);
+        t.sr += __shfl_xor(t.sr, off);
+        t.sq += __shfl_xor(t.sq, off);
+        t.sm += __shfl_xor(t.sm, off);
+        t.srr += __shfl_xor(t.srr, off);
+        t.sqq += __shfl_xor(t.sqq, off);
+        t.srq += __shfl_xor(t.srq, off);
+        t.srm += __shfl_xor(t.srm, off);
+        t.sqm += __shfl_xor(t.sqm, off);
+        t.extended += __shfl_xor(t.extended, off);
+        t.spiky += __shfl_xor(t.spiky, off);
+    }
+    return t;
+}
+
+// The tile's key bracket [c + sigma * o0, c + sigma * o1] (offsets ordered),
+// and the standardised offsets themselves.
+template <typename T>
+__device__ inline void store_bracket(const BracketArgs &b, int64_t tile, double mu, double sigma, float out0,
+                                     float out1)
+{
+    using KT = KeyTraits<T>;
+    using Key = typename KT::Key;
+    const float o0 = out0 < out1 ? out0 : out1;
+    const float o1 = out0 < out1 ? out1 : out0;
+    const double lo = mu + sigma * (double)o0;
+    const double hi = mu + sigma * (double)o1;
+    // The bracket is a key range of the dtype; rounding lo down and hi up
+    // is not needed for exactness (a miss only costs a fallback).
+    Key lo_key = KT::key(KT::from_double(lo));
+    Key hi_key = KT::key(KT::from_double(hi));
+    if (!(lo == lo) || !(hi == hi)) {
+        lo_key = 0;
+        hi_key = KT::SENTINEL;
+    }
+    static_cast<Key *>(b.lo_keys)[tile] = lo_key;
+    static_cast<Key *>(b.hi_keys)[tile] = hi_key;
+    b.predicted[tile * 2 + 0] = o0;
+    b.predicted[tile * 2 + 1] = o1;
 }
 
 // Per tile, from the 8x8 blocks covering its support (clipped to the image):
@@ -131,143 +334,119 @@ __device__ void wave_sort(float *v, int n, int lane)
 // asinh(gx * 64 / sigma), retained fraction, target quantile, fraction of
 // spiky bright blocks (spread at least 4 x the shift: isolated hot pixels).
 // Targets are standardised as (value - c) / sigma.
-__global__ void __launch_bounds__(64) k_tile_features(BracketArgs b, const double *blocks)
+//
+// One wave per tile, four tiles per workgroup. With PREDICT the wave also
+// runs the MLP (lane j < 32 computes hidden unit j; the sums run in the same
+// order as mlp_forward's, so the outputs are the same bits) and writes the
+// tile's key bracket; workgroup 0 also clears `reset` (the fallback count
+// the bracket kernel appends to).
+template <typename T, bool PREDICT>
+__global__ void __launch_bounds__(256) k_tile_features(BracketArgs b, const double *blocks, const float *weights,
+                                                       int64_t tiles)
 {
-    __shared__ float means[MAX_TILE_BLOCKS];
-    __shared__ float sds[MAX_TILE_BLOCKS];
-    const int tile = blockIdx.x;
-    const int lane = threadIdx.x;
-    const int tyi = tile % b.tiles_y;
-    const int tx = tile / b.tiles_y;
+    __shared__ float w[PREDICT ? PARAMS : 1];
+    if (PREDICT && blockIdx.x == 0 && threadIdx.x == 0 && b.reset) *b.reset = 0;
+    // Waves past the last tile compute the last tile again and store nothing
+    // (every wave reaches the barrier before the MLP).
+    const int64_t wave_tile = (int64_t)blockIdx.x * 4 + threadIdx.x / 64;
+    const bool active = wave_tile < tiles;
+    const int64_t tile = active ? wave_tile : tiles - 1;
+    const int lane = threadIdx.x % 64;
+    const int tyi = (int)(tile % b.tiles_y);
+    const int tx = (int)(tile / b.tiles_y);
     int64_t r0 = (int64_t)tyi * b.ty + b.low_y, r1 = (int64_t)tyi * b.ty + b.ty - 1 + b.high_y;
     int64_t c0 = (int64_t)tx * 64 + b.low_x, c1 = (int64_t)tx * 64 + 63 + b.high_x;
     r0 = r0 < 0 ? 0 : r0;
     c0 = c0 < 0 ? 0 : c0;
     r1 = r1 >= b.ny ? b.ny - 1 : r1;
     c1 = c1 >= b.nx ? b.nx - 1 : c1;
-    const int64_t br0 = r0 / BLOCK, bc0 = c0 / BLOCK;
-    const int height = (int)(r1 / BLOCK - br0 + 1), width = (int)(c1 / BLOCK - bc0 + 1);
-    // Subsample block rows/columns when a support spans more blocks than the
-    // sort holds.
+    BlockList list;
+    list.br0 = r0 / BLOCK;
+    list.bc0 = c0 / BLOCK;
+    list.bx = b.bx;
+    const int height = (int)(r1 / BLOCK - list.br0 + 1), width = (int)(c1 / BLOCK - list.bc0 + 1);
+    // Subsample block rows/columns when a support spans more blocks than
+    // MAX_TILE_BLOCKS.
     int step = 1;
     while (((height + step - 1) / step) * ((width + step - 1) / step) > MAX_TILE_BLOCKS) {
         step <<= 1;
     }
-    const int h_used = (height + step - 1) / step, w_used = (width + step - 1) / step;
-    const int count = h_used * w_used;
-    int n_sort = 64;
-    while (n_sort < count) {
-        n_sort <<= 1;
+    list.step = step;
+    list.w_used = (width + step - 1) / step;
+    list.count = ((height + step - 1) / step) * list.w_used;
+    static_assert(MAX_TILE_BLOCKS == 4 * 64, "tile_sums holds at most 4 blocks per lane");
+    TileSums t;
+    if (list.count <= 64) {
+        t = tile_sums<1>(blocks, list, lane);
+    } else if (list.count <= 128) {
+        t = tile_sums<2>(blocks, list, lane);
+    } else {
+        t = tile_sums<4>(blocks, list, lane);
     }
-    double n = 0.0, cells = 0.0;
-    int usable = 0;
-    for (int i = lane; i < n_sort; i += 64) {
-        float mean = __builtin_huge_valf(), sd = __builtin_huge_valf();
-        if (i < count) {
-            const int64_t br = br0 + (int64_t)(i / w_used) * step, bc = bc0 + (int64_t)(i % w_used) * step;
-            const double *o = blocks + (br * b.bx + bc) * 3;
-            n += o[0];
-            cells += 64.0;
-            if (o[0] >= 16.0) {
-                const double mu = o[1] / o[0];
-                const double var = o[2] / o[0] - mu * mu;
-                mean = (float)mu;
-                sd = (float)(var > 0.0 ? __builtin_sqrt(var) : 0.0);
-                ++usable;
-            }
-        }
-        means[i] = mean;
-        sds[i] = sd;
-    }
-    for (int off = 32; off > 0; off >>= 1) {
-        n += __shfl_xor(n, off);
-        cells += __shfl_xor(cells, off);
-        usable += __shfl_xor(usable, off);
-    }
-    __syncthreads();
-    wave_sort(means, n_sort, lane);
-    wave_sort(sds, n_sort, lane);
-    float c = 0.0f, sigma = 1.0f;
-    if (usable > 0) {
-        c = means[(usable - 1) / 2];
-        sigma = sds[(usable - 1) / 2];
-    }
-    const float floor_sigma = 1e-6f * (__builtin_fabsf(c) + 1e-30f);
-    sigma = sigma > floor_sigma ? sigma : floor_sigma;
-    // Plane fit over unsorted block means (re-read from the block table).
-    double sw = 0, sr = 0, sq = 0, sm = 0, srr = 0, sqq = 0, srq = 0, srm = 0, sqm = 0, extended = 0, spiky = 0;
-    for (int i = lane; i < count; i += 64) {
-        const int64_t br = br0 + (int64_t)(i / w_used) * step, bc = bc0 + (int64_t)(i % w_used) * step;
-        const double *o = blocks + (br * b.bx + bc) * 3;
-        if (o[0] >= 16.0) {
-            const double mu = o[1] / o[0] - (double)c;
-            // A bright block is "spiky" when its spread dwarfs its mean
-            // shift (one hot pixel: sd ~ 8 x shift) and "extended" otherwise
-            // (a source covering the block); only extended brightness moves
-            // window ranks much.
-            if (mu > 0.5 * (double)sigma) {
-                const double var = o[2] / o[0] - (o[1] / o[0]) * (o[1] / o[0]);
-                const double sd = var > 0.0 ? __builtin_sqrt(var) : 0.0;
-                if (sd < 4.0 * mu) {
-                    extended += mu / (double)sigma;
-                } else {
-                    spiky += 1.0;
-                }
-            }
-            if (__builtin_fabs(mu) <= 0.75 * (double)sigma) {
-                const double r = (double)((i / w_used) * step * BLOCK), q = (double)((i % w_used) * step * BLOCK);
-                sw += 1.0;
-                sr += r;
-                sq += q;
-                sm += mu;
-                srr += r * r;
-                sqq += q * q;
-                srq += r * q;
-                srm += r * mu;
-                sqm += q * mu;
-            }
+    const float c = t.c, sigma = t.sigma;
+    const double n = t.n, cells = t.cells;
+    const int usable = t.usable;
+    const double sw = t.sw, sr = t.sr, sq = t.sq, sm = t.sm, srr = t.srr, sqq = t.sqq, srq = t.srq, srm = t.srm,
+                 sqm = t.sqm, extended = t.extended, spiky = t.spiky;
+    // Every lane holds the totals; each computes the features.
+    double gy = 0.0, gx = 0.0;
+    if (sw >= 3.0) {
+        const double vrr = srr - sr * sr / sw, vqq = sqq - sq * sq / sw, vrq = srq - sr * sq / sw;
+        const double vrm = srm - sr * sm / sw, vqm = sqm - sq * sm / sw;
+        const double det = vrr * vqq - vrq * vrq;
+        if (det > 1e-9 * (vrr * vqq + 1e-30)) {
+            gy = (vrm * vqq - vqm * vrq) / det;
+            gx = (vqm * vrr - vrm * vrq) / det;
+        } else if (vrr > 0.0) {
+            gy = vrm / vrr;
+        } else if (vqq > 0.0) {
+            gx = vqm / vqq;
         }
     }
-    for (int off = 32; off > 0; off >>= 1) {
-        sw += __shfl_xor(sw, off);
-        sr += __shfl_xor(sr, off);
-        sq += __shfl_xor(sq, off);
-        sm += __shfl_xor(sm, off);
-        srr += __shfl_xor(srr, off);
-        sqq += __shfl_xor(sqq, off);
-        srq += __shfl_xor(srq, off);
-        srm += __shfl_xor(srm, off);
-        sqm += __shfl_xor(sqm, off);
-        extended += __shfl_xor(extended, off);
-        spiky += __shfl_xor(spiky, off);
-    }
-    if (lane == 0) {
-        double gy = 0.0, gx = 0.0;
-        if (sw >= 3.0) {
-            const double vrr = srr - sr * sr / sw, vqq = sqq - sq * sq / sw, vrq = srq - sr * sq / sw;
-            const double vrm = srm - sr * sm / sw, vqm = sqm - sq * sm / sw;
-            const double det = vrr * vqq - vrq * vrq;
-            if (det > 1e-9 * (vrr * vqq + 1e-30)) {
-                gy = (vrm * vqq - vqm * vrq) / det;
-                gx = (vqm * vrr - vrm * vrq) / det;
-            } else if (vrr > 0.0) {
-                gy = vrm / vrr;
-            } else if (vqq > 0.0) {
-                gx = vqm / vqq;
-            }
-        }
-        const double sg = (double)sigma;
-        float *f = b.features + (int64_t)tile * FEATURES;
-        f[0] = usable > 0 ? feature_asinh(extended / (double)usable) : 0.0f;
-        f[1] = feature_asinh(gy * (double)b.ty / sg);
-        f[2] = feature_asinh(gx * 64.0 / sg);
-        f[3] = cells > 0.0 ? (float)(n / cells) : 0.0f;
-        f[4] = b.quantile;
-        f[5] = usable > 0 ? (float)(spiky / (double)usable) : 0.0f;
-        double *m = b.moments + (int64_t)tile * 3;
+    const double sg = (double)sigma;
+    // The three asinh features run on lanes 0, 1, 2 (one evaluation per
+    // wave instead of three), then are broadcast.
+    const double arg = lane == 0   ? (usable > 0 ? extended / sg / (double)usable : 0.0)
+                       : lane == 1 ? gy * (double)b.ty / sg
+                                   : gx * 64.0 / sg;
+    const float shaped = feature_asinh(arg);
+    float x[FEATURES];
+    x[0] = __shfl(shaped, 0);
+    x[1] = __shfl(shaped, 1);
+    x[2] = __shfl(shaped, 2);
+    x[3] = cells > 0.0 ? (float)(n / cells) : 0.0f;
+    x[4] = b.quantile;
+    x[5] = usable > 0 ? (float)(spiky / (double)usable) : 0.0f;
+    if (active && lane == 0) {
+        float *f = b.features + tile * FEATURES;
+        for (int k = 0; k < FEATURES; ++k) f[k] = x[k];
+        double *m = b.moments + tile * 3;
         m[0] = (double)c;
         m[1] = sg;
         m[2] = n;
+    }
+    if (!PREDICT) {
+        return;
+    }
+    // Weights staged after the features, so their loads overlap the block
+    // loads of the other waves.
+    for (int i = threadIdx.x; i < PARAMS; i += blockDim.x) w[i] = weights[i];
+    __syncthreads();
+    const int j = lane % HIDDEN;
+    float acc = w[B1_OFF + j];
+    for (int i = 0; i < FEATURES; ++i) acc += x[i] * w[W1_OFF + i * HIDDEN + j];
+    const float h1 = acc > 0.0f ? acc : 0.0f;
+    acc = w[B2_OFF + j];
+#pragma unroll
+    for (int i = 0; i < HIDDEN; ++i) acc += lane_value(h1, i) * w[W2_OFF + i * HIDDEN + j];
+    const float h2 = acc > 0.0f ? acc : 0.0f;
+    const int k = lane % 2;
+    acc = w[B3_OFF + k];
+#pragma unroll
+    for (int i = 0; i < HIDDEN; ++i) acc += lane_value(h2, i) * w[W3_OFF + i * 2 + k];
+    const float out0 = lane_value(acc, 0), out1 = lane_value(acc, 1);
+    if (active && lane == 0) {
+        store_bracket<T>(b, tile, (double)c, sg, out0, out1);
     }
 }
 
@@ -297,42 +476,6 @@ __device__ inline void mlp_forward(const float *w, const float *x, float h1[HIDD
         }
         out[j] = acc;
     }
-}
-
-template <typename T>
-__global__ void __launch_bounds__(64) k_bracket_predict(BracketArgs b, const float *weights, int64_t tiles)
-{
-    using KT = KeyTraits<T>;
-    using Key = typename KT::Key;
-    __shared__ float w[PARAMS];
-    for (int i = threadIdx.x; i < PARAMS; i += blockDim.x) {
-        w[i] = weights[i];
-    }
-    __syncthreads();
-    const int64_t tile = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (tile >= tiles) {
-        return;
-    }
-    float h1[HIDDEN], h2[HIDDEN], out[2];
-    mlp_forward(w, b.features + tile * FEATURES, h1, h2, out);
-    const double mu = b.moments[tile * 3 + 0];
-    const double sigma = b.moments[tile * 3 + 1];
-    const float o0 = out[0] < out[1] ? out[0] : out[1];
-    const float o1 = out[0] < out[1] ? out[1] : out[0];
-    const double lo = mu + sigma * (double)o0;
-    const double hi = mu + sigma * (double)o1;
-    // The bracket is a key range of the dtype; rounding lo down and hi up
-    // is not needed for exactness (a miss only costs a fallback).
-    Key lo_key = KT::key(KT::from_double(lo));
-    Key hi_key = KT::key(KT::from_double(hi));
-    if (!(lo == lo) || !(hi == hi)) {
-        lo_key = 0;
-        hi_key = KT::SENTINEL;
-    }
-    static_cast<Key *>(b.lo_keys)[tile] = lo_key;
-    static_cast<Key *>(b.hi_keys)[tile] = hi_key;
-    b.predicted[tile * 2 + 0] = o0;
-    b.predicted[tile * 2 + 1] = o1;
 }
 
 // Per tile: standardised smallest and largest non-NaN result of the tile's
@@ -500,18 +643,19 @@ hipError_t launch_block_stats(int dtype_bytes, const void *image, int64_t ny, in
 
 hipError_t launch_tile_features(const BracketArgs &args, const double *blocks, int64_t tiles, hipStream_t stream)
 {
-    hipLaunchKernelGGL(k_tile_features, dim3((unsigned)tiles), dim3(64), 0, stream, args, blocks);
+    hipLaunchKernelGGL((k_tile_features<float, false>), dim3((unsigned)((tiles + 3) / 4)), dim3(256), 0, stream, args,
+                       blocks, nullptr, tiles);
     return hipGetLastError();
 }
 
-hipError_t launch_bracket_predict(int dtype_bytes, const BracketArgs &args, const float *weights, int64_t tiles,
-                                  hipStream_t stream)
+hipError_t launch_bracket_proposal(int dtype_bytes, const BracketArgs &args, const double *blocks,
+                                   const float *weights, int64_t tiles, hipStream_t stream)
 {
-    const dim3 grid((unsigned)((tiles + 63) / 64));
+    const dim3 grid((unsigned)((tiles + 3) / 4));
     if (dtype_bytes == 4) {
-        hipLaunchKernelGGL(k_bracket_predict<float>, grid, dim3(64), 0, stream, args, weights, tiles);
+        hipLaunchKernelGGL((k_tile_features<float, true>), grid, dim3(256), 0, stream, args, blocks, weights, tiles);
     } else {
-        hipLaunchKernelGGL(k_bracket_predict<double>, grid, dim3(64), 0, stream, args, weights, tiles);
+        hipLaunchKernelGGL((k_tile_features<double, true>), grid, dim3(256), 0, stream, args, blocks, weights, tiles);
     }
     return hipGetLastError();
 }

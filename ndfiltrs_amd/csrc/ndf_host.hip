@@ -983,6 +983,7 @@ void bracket_bind(SlideArgs &slide_b, BracketArgs &b, void *const buffers[8])
     b.lo_keys = buffers[3];
     b.hi_keys = buffers[4];
     b.predicted = static_cast<float *>(buffers[5]);
+    b.reset = static_cast<int32_t *>(buffers[7]);
     slide_b.bracket_lo = buffers[3];
     slide_b.bracket_hi = buffers[4];
     slide_b.fallback_tiles = static_cast<int32_t *>(buffers[6]);
@@ -1076,10 +1077,9 @@ hipError_t enqueue(ndf_plan *p, const LaunchSet &t, const void *d_in, void *d_ou
         const int tiles = (int)bracket_tiles(p);
         p->bracket_launched = true;
         p->report_fcount = t.fcount;
+        // The proposal launch also clears the fallback count (t.bargs.reset).
         e = launch_block_stats(p->dbytes, d_in, p->slide.ny, p->slide.nx, p->eff_ignore_inf, t.blocks, stream);
-        if (e == hipSuccess) e = launch_tile_features(t.bargs, t.blocks, tiles, stream);
-        if (e == hipSuccess) e = launch_bracket_predict(p->dbytes, t.bargs, t.weights, tiles, stream);
-        if (e == hipSuccess) e = launch_fill(4, t.fcount, 1, 0, stream);
+        if (e == hipSuccess) e = launch_bracket_proposal(p->dbytes, t.bargs, t.blocks, t.weights, tiles, stream);
         if (e == hipSuccess) {
             SlideArgs args = t.slide_b;
             args.input = d_in;
