@@ -338,14 +338,14 @@ __device__ inline void store_bracket(const BracketArgs &b, int64_t tile, double 
 // One wave per tile, four tiles per workgroup. With PREDICT the wave also
 // runs the MLP (lane j < 32 computes hidden unit j; the sums run in the same
 // order as mlp_forward's, so the outputs are the same bits) and writes the
-// tile's key bracket; workgroup 0 also clears `reset` (the fallback count
-// the bracket kernel appends to).
+// tile's key bracket; workgroup 0 also clears the counters at `reset` (the
+// fallback count the bracket kernel appends to and its tile queues).
 template <typename T, bool PREDICT>
 __global__ void __launch_bounds__(256) k_tile_features(BracketArgs b, const double *blocks, const float *weights,
                                                        int64_t tiles)
 {
     __shared__ float w[PREDICT ? PARAMS : 1];
-    if (PREDICT && blockIdx.x == 0 && threadIdx.x == 0 && b.reset) *b.reset = 0;
+    if (PREDICT && blockIdx.x == 0 && b.reset && (int)threadIdx.x < b.reset_count) b.reset[threadIdx.x] = 0;
     // Waves past the last tile compute the last tile again and store nothing
     // (every wave reaches the barrier before the MLP).
     const int64_t wave_tile = (int64_t)blockIdx.x * 4 + threadIdx.x / 64;

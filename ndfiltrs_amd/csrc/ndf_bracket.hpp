@@ -29,14 +29,15 @@ struct BracketArgs {
     void *lo_keys;                          // tiles keys (uint32 / uint64)
     void *hi_keys;
     float *predicted;                       // tiles x 2 standardised offsets
-    int32_t *reset;                         // cleared by the proposal launch (fallback count), or null
+    int32_t *reset;                         // reset[0 .. reset_count) cleared by the proposal launch, or null
+    int32_t reset_count;                    // (the fallback count and the bracket kernel's tile queues)
 };
 
 hipError_t launch_block_stats(int dtype_bytes, const void *image, int64_t ny, int64_t nx, int ignore_inf,
                               double *blocks, hipStream_t stream);
 hipError_t launch_tile_features(const BracketArgs &args, const double *blocks, int64_t tiles, hipStream_t stream);
 // Features, MLP and key bracket of every tile in one launch; also clears
-// args.reset when it is set.
+// args.reset[0 .. args.reset_count) when it is set.
 hipError_t launch_bracket_proposal(int dtype_bytes, const BracketArgs &args, const double *blocks,
                                    const float *weights, int64_t tiles, hipStream_t stream);
 hipError_t launch_tile_labels(int dtype_bytes, const BracketArgs &args, const void *output, float *targets,

@@ -960,7 +960,8 @@ void bracket_geometry(ndf_plan *p)
 
 // Bytes of the bracket's per-launch buffers, in carve order: block moments
 // (double[3] per 8x8 block), features, tile moments, lo / hi keys,
-// predictions, fallback list, fallback count.
+// predictions, fallback list, counters (the fallback count, then the
+// bracket kernel's tile queues).
 void bracket_buffer_sizes(const ndf_plan *p, size_t sizes[8])
 {
     const int64_t tiles = bracket_tiles(p);
@@ -972,7 +973,7 @@ void bracket_buffer_sizes(const ndf_plan *p, size_t sizes[8])
     sizes[4] = (size_t)tiles * (size_t)p->dbytes;
     sizes[5] = (size_t)tiles * 2 * sizeof(float);
     sizes[6] = (size_t)tiles * sizeof(int32_t);
-    sizes[7] = sizeof(int32_t);
+    sizes[7] = (1 + BRACKET_QUEUES) * sizeof(int32_t);
 }
 
 // Points the bracket arguments at its buffers (in bracket_buffer_sizes order).
@@ -988,6 +989,8 @@ void bracket_bind(SlideArgs &slide_b, BracketArgs &b, void *const buffers[8])
     slide_b.bracket_hi = buffers[4];
     slide_b.fallback_tiles = static_cast<int32_t *>(buffers[6]);
     slide_b.fallback_count = static_cast<int32_t *>(buffers[7]);
+    slide_b.tile_queue = static_cast<int32_t *>(buffers[7]) + 1;
+    b.reset_count = 1 + BRACKET_QUEUES;
 }
 
 // Allocates the bracket buffers and fills the bracket launch arguments.

@@ -96,7 +96,13 @@ struct SlideArgs {
     const void *bracket_hi;
     int32_t *fallback_tiles;
     int32_t *fallback_count;
+    // Bracket mode: BRACKET_QUEUES tile counters (zero at launch), one per
+    // XCD-contiguous chunk of tiles, from which the persistent workgroups take
+    // their tiles.
+    int32_t *tile_queue;
 };
+
+constexpr int BRACKET_QUEUES = 8;
 
 // Engine 3: small windows (count <= 160 for f32, 64 for f64). A tile of 64
 // columns x ty rows of one plane; its support (kz planes of sy = ty + ky - 1

@@ -40,6 +40,7 @@ constexpr int NT = SLIDE_WAVES * 64;
 constexpr int BINS = 16;
 constexpr int BATCH = 16;
 constexpr int START = 8;  // start-cursor candidates per wave in the bracket kernel
+constexpr int MAX_PASSES = 6;  // key ranges the bracket kernel may run per tile
 constexpr int GUARD = BATCH;
 
 // Support positions are packed as (row << 8) | x with row = z * zrows + y
@@ -1307,17 +1308,23 @@ __device__ __forceinline__ uint64_t window_bits(const uint64_t w[2], int lane, u
 }
 
 // Bracket kernel for 2-D rectangular windows (support rows <= 128 samples).
-// One workgroup per tile, persistent over tiles. Support samples are
-// classified against the tile's key bracket [lo, hi]: only inside samples
-// are sorted and ranked; per support row, bit masks record the retained,
-// below and inside samples. Each lane keeps the retained / below / inside
-// counts of its window as popcounts of its window's bits, and a cursor
-// (m, c) over the inside ranks where c counts the window's below samples
-// plus its inside samples ranked under m. A row step therefore touches only
-// the inside samples of the leaving and entering rows. A wave starts at
-// m = 1 (c = below) and walks up. A window whose target rank is not inside
-// the bracket sends the tile to the fallback list, where the plain kernel
-// recomputes it.
+// One workgroup per tile, persistent over tiles. A tile runs one or more
+// passes, each over one key range (the first is the tile's learned bracket
+// [lo, hi]). In a pass, support samples are classified against the range:
+// only inside samples are sorted and ranked; per support row, bit masks
+// record the retained, below and inside samples. Each lane keeps the
+// retained / below / inside counts of its window as popcounts of its
+// window's bits, and a cursor (m, c) over the inside ranks where c counts the
+// window's below samples plus its inside samples ranked under m. A row step
+// therefore touches only the inside samples of the leaving and entering
+// rows. A window whose result ranks lie in the range is computed; one whose
+// ranks lie in other ranges waits for their passes (a median or percentile
+// pair split between two ranges parks its first element in the output).
+// More passes follow when the range holds more samples than the sort (the
+// range is split at its key midpoint) or when a target lies below or above
+// every range queued so far (an adjacent range doubles their span). A tile
+// that would exceed MAX_PASSES passes, or whose walk fails, goes to the
+// fallback list, where the plain kernel recomputes it.
 template <typename T, int E>
 __global__ void __launch_bounds__(NT, E >= 32 ? 1 : (sizeof(T) == 8 && E >= 16 ? 2 : 3)) k_bracket(SlideArgs a)
 {
@@ -1332,7 +1339,14 @@ __global__ void __launch_bounds__(NT, E >= 32 ? 1 : (sizeof(T) == 8 && E >= 16 ?
     uint16_t *codes = reinterpret_cast<uint16_t *>(lds + exchange_bytes<Key, E>());
     RowMasks *rowmasks = reinterpret_cast<RowMasks *>(codes + ((a.s + 7) & ~7));
     __shared__ int count_shared;
-    __shared__ int failed_shared;
+    // Pass bookkeeping of the current tile (written by thread 0 between
+    // barriers): pending key ranges (a stack, processed last-in first-out),
+    // the union of every range queued so far, passes started, the OR of the
+    // waves' pass flags and the tile's failure.
+    __shared__ Key pend_lo[MAX_PASSES], pend_hi[MAX_PASSES];
+    __shared__ Key union_lo, union_hi;
+    __shared__ int npend_shared, passes_shared, flags_shared, tile_failed;
+    __shared__ int next_tile;
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -1347,28 +1361,70 @@ __global__ void __launch_bounds__(NT, E >= 32 ? 1 : (sizeof(T) == 8 && E >= 16 ?
     const int64_t mz = uniform_entry(a.mapz, 0);
     const uint64_t lanemask_lt = (1ull << lane) - 1ull;
 
-    const int chunk = (ntiles + 7) / 8;
-    for (int slot = blockIdx.x; slot < 8 * chunk; slot += gridDim.x) {
-        const int t = (slot % 8) * chunk + slot / 8;
-        if (t >= ntiles) {
-            continue;
+    // Tiles come from BRACKET_QUEUES counters, one per contiguous chunk of
+    // tiles: workgroup b starts on chunk b % 8 (the XCD it runs on, so
+    // vertical neighbours share their halo rows in that XCD's L2) and moves on
+    // to the next chunk when its own is drained. Taking tiles on demand keeps
+    // workgroups that meet multi-pass tiles from setting the kernel's end.
+    const int chunk = (ntiles + BRACKET_QUEUES - 1) / BRACKET_QUEUES;
+    for (;;) {
+        if (tid == 0) {
+            int pick = -1;
+            for (int probe = 0; probe < BRACKET_QUEUES && pick < 0; ++probe) {
+                const int q = (int)((blockIdx.x + probe) % BRACKET_QUEUES);
+                const int size = min(chunk, ntiles - q * chunk);
+                if (size > 0 && __hip_atomic_load(&a.tile_queue[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < size) {
+                    const int i = atomicAdd(&a.tile_queue[q], 1);
+                    pick = i < size ? q * chunk + i : -1;
+                }
+            }
+            next_tile = pick;
+        }
+        __syncthreads();
+        const int t = next_tile;
+        if (t < 0) {
+            break;
         }
         const int tile = a.tile_list ? a.tile_list[t] : t;
         const int tyi = tile % a.tiles_y;
         const int tx = tile / a.tiles_y;
         const int64_t x0 = (int64_t)tx * SLIDE_TILE_X;
         const int64_t y0 = (int64_t)tyi * a.ty;
-        const Key lo_key = static_cast<const Key *>(a.bracket_lo)[tile];
-        const Key hi_key = static_cast<const Key *>(a.bracket_hi)[tile];
         if (tid == 0) {
+            pend_lo[0] = static_cast<const Key *>(a.bracket_lo)[tile];
+            pend_hi[0] = static_cast<const Key *>(a.bracket_hi)[tile];
+            union_lo = pend_lo[0];
+            union_hi = pend_hi[0];
+            npend_shared = 1;
+            passes_shared = 0;
+            tile_failed = 0;
+        }
+        __syncthreads();
+        const int64_t mx0 = a.mapx[x0 + lane];
+        const int64_t mx1 = lane + 64 < a.sx ? a.mapx[x0 + 64 + lane] : MAP_PAD;
+        // Bit r: the output of this lane's row y_first + r holds the key of
+        // one element of a pair whose other element lies in another range.
+        uint64_t halves = 0;
+
+        // ---- passes: one key range each, until none is pending ----
+        for (;;) {
+        const int np = npend_shared;
+        if (np == 0 || tile_failed) {
+            break;
+        }
+        const Key lo_key = pend_lo[np - 1], hi_key = pend_hi[np - 1];
+        const Key ulo = union_lo, uhi = union_hi;
+        const int pass = passes_shared;
+        __syncthreads();
+        if (tid == 0) {
+            npend_shared = np - 1;
+            passes_shared = pass + 1;
             count_shared = 0;
-            failed_shared = 0;
+            flags_shared = 0;
         }
         __syncthreads();
 
         // ---- stage: R rows per wave per batch, loads first ----
-        const int64_t mx0 = a.mapx[x0 + lane];
-        const int64_t mx1 = lane + 64 < a.sx ? a.mapx[x0 + 64 + lane] : MAP_PAD;
         for (int row0 = wave_u * R; row0 < a.rows; row0 += SLIDE_WAVES * R) {
             // Row map entries first (wave-uniform, scalar loads), then all
             // R * 2 sample loads of the batch before the first wait.
@@ -1442,8 +1498,21 @@ __global__ void __launch_bounds__(NT, E >= 32 ? 1 : (sizeof(T) == 8 && E >= 16 ?
         __syncthreads();
         const int retained = count_shared;
         if (retained > N) {
+            // More samples in the range than the sort holds: split the range
+            // at its key midpoint (lower half first), within the pass budget.
             if (tid == 0) {
-                a.fallback_tiles[atomicAdd(a.fallback_count, 1)] = tile;
+                // Passes started (pass + 1) plus pending after the split
+                // (np + 1) stay within the budget.
+                if (pass + np + 2 <= MAX_PASSES && hi_key > lo_key) {
+                    const Key mid = lo_key + (hi_key - lo_key) / 2;
+                    pend_lo[np - 1] = mid + 1;
+                    pend_hi[np - 1] = hi_key;
+                    pend_lo[np] = lo_key;
+                    pend_hi[np] = mid;
+                    npend_shared = np + 1;
+                } else {
+                    tile_failed = 1;
+                }
             }
             __syncthreads();
             continue;
@@ -1617,7 +1686,7 @@ __global__ void __launch_bounds__(NT, E >= 32 ? 1 : (sizeof(T) == 8 && E >= 16 ?
                     c = below + best_c;
                 }
             }
-            bool failed = false;
+            bool failed = false, need_low = false, need_high = false;
             for (int y = y_first; y < y_first + a.tw && y0 + y < a.ny && !failed; ++y) {
                 if (y > y_first) {
                     const RowMasks &leave = rowmasks[y - 1];
@@ -1644,36 +1713,114 @@ __global__ void __launch_bounds__(NT, E >= 32 ? 1 : (sizeof(T) == 8 && E >= 16 ?
                         c += (int)code_in[i] < m;
                     }
                 }
-                Key out = KT::CANONICAL_NAN;
                 Pick pick;
-                if (pick_ranks(a.stat, n, a.min_count, a.rank_param, a.q100, pick)) {
-                    const int k = (int)pick.hi;
-                    const uint32_t base = ((uint32_t)y << 8) | (uint32_t)lane;
-                    if (k < below || k >= below + inside || !walk_to(m, c, k, pos1, retained, g, base)) {
-                        failed = true;
-                    } else if (pick.pair) {
-                        const int m2 = previous_member(m, pos1, g, base);
-                        if (m2 < 1) {
-                            failed = true;
-                        } else {
-                            out = pair_bits<T>(a.stat, keys[m2 - 1], keys[m - 1], pick.frac);
-                        }
-                    } else {
-                        out = KT::unkey(keys[m - 1]);
+                if (!pick_ranks(a.stat, n, a.min_count, a.rank_param, a.q100, pick)) {
+                    if (col_valid) {
+                        output[(y0 + y) * a.nx + x0 + lane] = KT::CANONICAL_NAN;
                     }
+                    continue;
                 }
-                if (col_valid && !failed) {
+                // Ranks the result reads: k, and k - 1 for a pair. Each lies
+                // below this pass's range (rank < below), in it, or above it;
+                // a rank outside the union of the tile's ranges asks for an
+                // adjacent range.
+                const int k = (int)pick.hi;
+                const int first = pick.pair ? k - 1 : k;
+                const bool low_in = first >= below && first < below + inside;
+                const bool high_in = k >= below && k < below + inside;
+                need_low = need_low || (first < below && lo_key == ulo);
+                need_high = need_high || (k >= below + inside && hi_key == uhi);
+                if (!low_in && !high_in) {
+                    continue;
+                }
+                const uint32_t base = ((uint32_t)y << 8) | (uint32_t)lane;
+                if (!walk_to(m, c, high_in ? k : first, pos1, retained, g, base)) {
+                    failed = true;
+                    break;
+                }
+                Key out;
+                if (!pick.pair) {
+                    out = KT::unkey(keys[m - 1]);
+                } else if (low_in && high_in) {
+                    const int m2 = previous_member(m, pos1, g, base);
+                    if (m2 < 1) {
+                        failed = true;
+                        break;
+                    }
+                    out = pair_bits<T>(a.stat, keys[m2 - 1], keys[m - 1], pick.frac);
+                } else {
+                    // The pair straddles two ranges: the first pass to find
+                    // its element parks that key in the output, the second
+                    // combines both.
+                    if (!col_valid) {
+                        continue;
+                    }
+                    Key *slot = output + (y0 + y) * a.nx + x0 + lane;
+                    const uint64_t bit = 1ull << (y - y_first);
+                    if (!(halves & bit)) {
+                        *slot = keys[m - 1];
+                        halves |= bit;
+                        continue;
+                    }
+                    const Key other = *slot;
+                    halves &= ~bit;
+                    out = high_in ? pair_bits<T>(a.stat, other, keys[m - 1], pick.frac)
+                                  : pair_bits<T>(a.stat, keys[m - 1], other, pick.frac);
+                }
+                if (col_valid) {
                     output[(y0 + y) * a.nx + x0 + lane] = out;
                 }
             }
-            if (failed) {
-                failed_shared = 1;
+            const int wave_flags =
+                (__ballot(failed) ? 1 : 0) | (__ballot(need_low) ? 2 : 0) | (__ballot(need_high) ? 4 : 0);
+            if (lane == 0 && wave_flags) {
+                atomicOr(&flags_shared, wave_flags);
             }
         }
         __syncthreads();
-        if (tid == 0 && failed_shared) {
+        // Queue the adjacent ranges the waves asked for (each doubles the
+        // union's key span), within the pass budget; a failed walk, or a
+        // range that cannot be added, sends the tile to the fallback list.
+        if (tid == 0) {
+            const int f = flags_shared;
+            bool fail = (f & 1) != 0;
+            int n2 = npend_shared;
+            const int started = passes_shared;
+            const Key span = uhi - ulo;
+            const Key width = span < (Key)15 ? (Key)16 : (Key)(span + 1);
+            if (!fail && (f & 2)) {
+                if (ulo == 0 || started + n2 + 1 > MAX_PASSES) {
+                    fail = true;
+                } else {
+                    const Key nlo = ulo > width ? ulo - width : (Key)0;
+                    pend_lo[n2] = nlo;
+                    pend_hi[n2] = ulo - 1;
+                    ++n2;
+                    union_lo = nlo;
+                }
+            }
+            if (!fail && (f & 4)) {
+                const Key top = KT::SENTINEL - 1;  // the sentinel is never a retained key
+                if (uhi >= top || started + n2 + 1 > MAX_PASSES) {
+                    fail = true;
+                } else {
+                    const Key nhi = top - uhi > width ? uhi + width : top;
+                    pend_lo[n2] = uhi + 1;
+                    pend_hi[n2] = nhi;
+                    ++n2;
+                    union_hi = nhi;
+                }
+            }
+            npend_shared = n2;
+            tile_failed = fail ? 1 : 0;
+        }
+        __syncthreads();
+        }  // passes
+        if (tid == 0 && tile_failed) {
             a.fallback_tiles[atomicAdd(a.fallback_count, 1)] = tile;
         }
+        // Thread 0 rewrites the pass state at the next tile.
+        __syncthreads();
     }
 }
 
