@@ -115,6 +115,41 @@ def test_bracket_matches_oracle_small() -> None:
     assert 0 <= fallback <= tiles
 
 
+@pytest.mark.parametrize("kind,parameter", [("median", 0), ("percentile", 37.5), ("rank", 300)])
+def test_split_passes_match_oracle(kind, parameter) -> None:
+    """A +-3 sigma bracket holds every sample of a 31x31 support (94 x 62 >
+    4096 sort entries): each tile splits its range at the key midpoint and
+    runs two passes; median and percentile pairs whose two elements fall on
+    either side of the midpoint are combined across the passes. No tile may
+    fall back, and every output equals the oracle."""
+    image = synthetic.sky_small((192, 256), seed=23)
+    plan = make_plan(image, kind, (31, 31), parameter)
+    out, (tiles, fallback) = run(plan, image, constant_model(-3.0, 3.0))
+    assert bracket_support(plan, (31, 31)) > 4096
+    assert fallback == 0, (fallback, tiles)
+    expected = oracle.window_filter(image, size=(31, 31), mode="reflect", statistic=kind, omit_nan=True,
+                                    rank=parameter if kind == "rank" else 0,
+                                    q=parameter if kind == "percentile" else 50.0, method="direct")
+    assert_same_bits(out, expected)
+
+
+@pytest.mark.parametrize("lo,hi", [(0.3, 0.6), (-0.6, -0.3)])
+def test_extension_passes_match_oracle(lo, hi) -> None:
+    """A bracket entirely above (or below) the medians of a flat N(1000, 10)
+    sky: every target lies outside it, so each tile adds adjacent ranges
+    (each doubling the key span) until the targets are covered; no tile may
+    fall back and every output equals the oracle."""
+    rng = np.random.default_rng(24)
+    image = rng.normal(1000.0, 10.0, size=(160, 192)).astype(np.float32)
+    image[rng.random(image.shape) < 0.02] = np.nan
+    plan = make_plan(image, "median", (15, 15))
+    out, (tiles, fallback) = run(plan, image, constant_model(lo, hi))
+    assert fallback == 0, (fallback, tiles)
+    expected = oracle.window_filter(image, size=(15, 15), mode="reflect", statistic="median", omit_nan=True,
+                                    method="direct")
+    assert_same_bits(out, expected)
+
+
 def test_nan_model_is_an_unbounded_bracket() -> None:
     image = synthetic.sky_small((160, 192), seed=22)
     plan = make_plan(image, "median", (11, 11))
