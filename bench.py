@@ -149,7 +149,7 @@ def ideal_seconds(pixels: int, taps: int, itemsize: int) -> float:
 
 
 def bracket_pair(work: Workload, kind: str, size: int, mode: str, parameter: int, weights, steps: int, warmup: int,
-                 expected: np.ndarray | None) -> dict:
+                 expected: np.ndarray | None, sort_len: int = 4096) -> dict:
     """Plain and learned-bracket runs of one statistic. Each is verified by
     one more launch into a poisoned output buffer, compared bit for bit with
     the whole-image oracle result `expected` (and with each other)."""
@@ -157,7 +157,7 @@ def bracket_pair(work: Workload, kind: str, size: int, mode: str, parameter: int
     plain = work.run(plan, steps, warmup)
     plain_out = work.verify(plan)
     plain["unwritten_outputs"] = unwritten(plain_out)
-    plan.set_bracket(weights, 4096)
+    plan.set_bracket(weights, sort_len)
     bracketed = work.run(plan, steps, warmup)
     out = work.verify(plan)
     tiles, fallback = plan.bracket_report()
@@ -441,9 +441,11 @@ def profile_workload(name: str, steps: int, warmup: int) -> dict:
     if kwargs.get("footprint") == "ring":
         kwargs["footprint"] = ring_footprint()
     work = Workload(getattr(synthetic, factory)())
-    plan = work.plan(kwargs.pop("kind"), kwargs.pop("size"), kwargs.pop("mode"), **kwargs)
+    kind = kwargs.pop("kind")
+    plan = work.plan(kind, kwargs.pop("size"), kwargs.pop("mode"), **kwargs)
     if model:
-        plan.set_bracket(bracket.load(bracket.model_path(model))[0], 4096)
+        weights, meta = bracket.load(bracket.model_path(model))
+        plan.set_bracket(weights, bracket.sort_len(meta, kind))
     result = work.run(plan, steps, warmup)
     result.update(workload=name, engine=json.loads(plan.describe()))
     work.free()
@@ -502,10 +504,12 @@ def main() -> int:
         baseline, expected_median = cpu_baseline(c5.image, 63, "reflect")
         expected_rank, _ = oracle_full(c5.image, (63, 63), "reflect", "rank", 397, method="histogram")
     log("63x63 nanmedian: plain and learned bracket")
-    median63 = bracket_pair(c5, "median", 63, "reflect", 0, w63, args.steps, args.warmup, expected_median)
+    median63 = bracket_pair(c5, "median", 63, "reflect", 0, w63, args.steps, args.warmup, expected_median,
+                            bracket.sort_len(meta63, "median"))
     log(json.dumps(median63))
     log("63x63 rank 397 (10%): plain and learned bracket")
-    rank63 = bracket_pair(c5, "rank", 63, "reflect", 397, w63, args.steps, args.warmup, expected_rank)
+    rank63 = bracket_pair(c5, "rank", 63, "reflect", 397, w63, args.steps, args.warmup, expected_rank,
+                          bracket.sort_len(meta63, "rank"))
     log(json.dumps(rank63))
     del expected_median, expected_rank
     extra = {}
@@ -516,7 +520,7 @@ def main() -> int:
             expected, _ = oracle_full(c5.image, (63, 63), "reflect", "percentile", 10.0, method="histogram")
         log("63x63 nanpercentile 10: plain and learned bracket")
         extra["percentile10_63x63"] = bracket_pair(c5, "percentile", 63, "reflect", 10.0, w63, args.steps,
-                                                   args.warmup, expected)
+                                                   args.warmup, expected, bracket.sort_len(meta63, "percentile"))
         del expected
         log(json.dumps(extra["percentile10_63x63"]))
         for kind in ("min", "max"):
@@ -530,14 +534,14 @@ def main() -> int:
     if not args.quick:
         log("generating the 4096x4096 float64 sky (31x31 workload)")
         c2 = Workload(synthetic.sky_gradient_f64())
-        w31, _ = bracket.load(bracket.model_path("sky31_f64"))
+        w31, meta31 = bracket.load(bracket.model_path("sky31_f64"))
         for kind, parameter in (("median", 0), ("rank", 154), ("rank", 806)):
             log(f"31x31 float64 {kind} {parameter}: plain and learned bracket")
             expected = None
             if not args.no_cpu:
                 expected, _ = oracle_full(c2.image, (31, 31), "nearest", kind, parameter, method="histogram")
             extra[f"31x31_f64_{kind}{parameter if kind == 'rank' else ''}"] = bracket_pair(
-                c2, kind, 31, "nearest", parameter, w31, args.steps, args.warmup, expected)
+                c2, kind, 31, "nearest", parameter, w31, args.steps, args.warmup, expected, bracket.sort_len(meta31, kind))
             del expected
         if not args.no_cpu:
             extra["31x31_f64_median"]["cpu"] = cpu_sample(c2.image, 1024, "histogram", size=(31, 31),

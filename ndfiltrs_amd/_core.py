@@ -387,11 +387,13 @@ _SHIPPED_MODELS = {
     ("float32", (63, 63)): "bracket_sky_63x63.json",
     ("float64", (31, 31)): "bracket_sky_31x31_f64.json",
 }
-_MODEL_WEIGHTS: dict[str, np.ndarray] = {}
+_MODEL_WEIGHTS: dict[str, tuple[np.ndarray, dict]] = {}
 _BRACKET_MIN_PIXELS = 1 << 20
 
 
-def _shipped_model(dtype: np.dtype, size: Sequence[int]) -> np.ndarray | None:
+def _shipped_model(dtype: np.dtype, size: Sequence[int]):
+    """(weights, meta) of the shipped bracket model for this dtype and window,
+    or None."""
     name = _SHIPPED_MODELS.get((np.dtype(dtype).name, tuple(int(v) for v in size)))
     if name is None:
         return None
@@ -400,7 +402,8 @@ def _shipped_model(dtype: np.dtype, size: Sequence[int]) -> np.ndarray | None:
         if not os.path.exists(path):
             return None
         with open(path) as handle:
-            _MODEL_WEIGHTS[name] = np.asarray(json.load(handle)["weights"], dtype=np.float32)
+            data = json.load(handle)
+        _MODEL_WEIGHTS[name] = (np.asarray(data["weights"], dtype=np.float32), data.get("meta", {}))
     return _MODEL_WEIGHTS[name]
 
 
@@ -417,10 +420,12 @@ def _cached_apply(typed: np.ndarray, kind: str, size, footprint, origin, mode, c
                     min_count, parameter)
         if (footprint is None and typed.ndim == 2 and omit_nan and typed.size >= _BRACKET_MIN_PIXELS
                 and '"engine": "slide"' in plan.describe()):
-            weights = _shipped_model(typed.dtype, size)
-            if weights is not None:
+            model = _shipped_model(typed.dtype, size)
+            if model is not None:
+                from ndfiltrs_amd import bracket
+
                 try:
-                    plan.set_bracket(weights)
+                    plan.set_bracket(model[0], bracket.sort_len(model[1], kind))
                 except ValueError:
                     pass
     output = plan.apply(typed)

@@ -46,6 +46,9 @@ WORKLOADS = {
         "mode": "reflect",
         "ranks": [397],
         "tau": (0.003, 0.997),
+        # Sort capacity of the bracket kernel (inside samples per pass), per
+        # statistic.
+        "sort_len": {"median": 4096, "rank": 4096, "percentile": 4096},
         "law": "2048^2 crops (seeded offsets) of the 8192^2 field: sky N(1000,10) + quadratic background up to "
                "300 + 2000 sources per 8192^2 + 2% NaN (float32)",
     },
@@ -61,6 +64,10 @@ WORKLOADS = {
         # tile, so the 0.3 % tails gave 3.4-sigma brackets holding 40 % of a
         # tile's samples; 2 % tails halve the width (fallback about 1 %).
         "tau": (0.02, 0.98),
+        # A 2048-entry sort fits three float64 workgroups per CU; a tile whose
+        # range holds more samples splits it into extra passes, which the
+        # ranks' narrower ranges rarely need and the median's often would.
+        "sort_len": {"median": 4096, "rank": 2048, "percentile": 4096},
         "law": "1024^2 crops (seeded offsets) of the 4096^2 field: sky N(1000,10) + x-gradient 0..200 + 500 "
                "sources per 4096^2 + 0.1% hot pixels U(1e4,1e5) + 2% NaN + the field's 3 NaN columns (float64)",
     },
@@ -174,6 +181,13 @@ def save(path: str, weights: np.ndarray, meta: dict) -> None:
         handle.write("\n")
 
 
+def sort_len(meta: dict, kind: str = "median") -> int:
+    """The bracket kernel's sort capacity recorded with a model for one
+    statistic ("median", "rank" or "percentile"; default 4096)."""
+    value = meta.get("sort_len", 4096)
+    return int(value.get(kind.split("_")[0], 4096) if isinstance(value, dict) else value)
+
+
 def load(path: str = DEFAULT_MODEL) -> tuple[np.ndarray, dict]:
     with open(path) as handle:
         data = json.load(handle)
@@ -234,6 +248,7 @@ def train_main(argv=None) -> int:
         "steps": args.steps,
         "learning_rate": args.lr,
         "tau": [args.tau_lo, args.tau_hi],
+        "sort_len": spec["sort_len"],
         "final_loss": float(losses[-1]),
         "train": coverage(weights, x, t, w),
         "held_out": coverage(weights, xh, th, wh),

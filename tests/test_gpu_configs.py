@@ -56,13 +56,13 @@ def engines_match(image, expected, context, **plan_kwargs) -> None:
         assert_bits(plan.apply(image), expected, f"{context} {engine}")
 
 
-def device_run(plan, image, weights=None) -> tuple[np.ndarray, tuple[int, int]]:
+def device_run(plan, image, weights=None, sort_len=4096) -> tuple[np.ndarray, tuple[int, int]]:
     d_in = _core.DeviceArray(image.nbytes)
     d_out = _core.DeviceArray(image.nbytes)
     try:
         d_in.upload(image)
         d_out.fill(_cases.poison_bits(image.dtype), image.itemsize)
-        plan.set_bracket(weights) if weights is not None else plan.set_bracket(None)
+        plan.set_bracket(weights, sort_len) if weights is not None else plan.set_bracket(None)
         plan.apply_device(d_in.address, d_out.address)
         plan.last_error()
         return d_out.download(np.empty_like(image)), plan.bracket_report()
@@ -128,13 +128,13 @@ def test_c3b_cube_3x7x7_wrap_ignore_inf() -> None:
 @pytest.mark.parametrize("kind,rank", [("median", 0), ("rank", 397)])
 def test_c5_trained_bracket_matches_oracle(kind: str, rank: int) -> None:
     image = synthetic.sky_quadratic_crop((1024, 1024), seed=106)
-    weights, _ = bracket.load(bracket.model_path("sky63"))
+    weights, meta = bracket.load(bracket.model_path("sky63"))
     expected = oracle.window_filter(image, (63, 63), mode="reflect", statistic=kind, rank=rank, omit_nan=True,
                                     method="histogram")
     plan = plan_for(image, kind, (63, 63), "reflect", rank)
     plain, _ = device_run(plan, image)
     assert_bits(plain, expected, f"C5 {kind} plain")
-    out, (tiles, fallback) = device_run(plan, image, weights)
+    out, (tiles, fallback) = device_run(plan, image, weights, bracket.sort_len(meta, kind))
     assert_bits(out, expected, f"C5 {kind} bracket")
     assert tiles > 0 and fallback < tiles
 
@@ -142,11 +142,11 @@ def test_c5_trained_bracket_matches_oracle(kind: str, rank: int) -> None:
 @pytest.mark.parametrize("kind,rank", [("median", 0), ("rank", 154), ("rank", 806)])
 def test_c2_trained_bracket_matches_oracle(kind: str, rank: int) -> None:
     image = synthetic.sky_gradient_crop((1024, 1024), seed=107)
-    weights, _ = bracket.load(bracket.model_path("sky31_f64"))
+    weights, meta = bracket.load(bracket.model_path("sky31_f64"))
     expected = oracle.window_filter(image, (31, 31), mode="nearest", statistic=kind, rank=rank, omit_nan=True,
                                     method="histogram")
     plan = plan_for(image, kind, (31, 31), "nearest", rank)
-    out, (tiles, fallback) = device_run(plan, image, weights)
+    out, (tiles, fallback) = device_run(plan, image, weights, bracket.sort_len(meta, kind))
     assert_bits(out, expected, f"C2 {kind} {rank} bracket")
     assert tiles > 0 and fallback < tiles
 
