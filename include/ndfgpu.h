@@ -246,8 +246,10 @@ ndf_status ndf_plan_time_repair_device(ndf_plan *plan, const void *d_input, void
  * counterpart). A tiny MLP (6 -> 32 -> 32 -> 2, ReLU, fp32; NDF_BRACKET_PARAMS
  * floats: W1[6][32], b1[32], W2[32][32], b2[32], W3[32][2], b3[2]) proposes
  * a value bracket per tile from box-sum features of its support; only
- * in-bracket samples are sorted (sort_len entries: 1024, 2048, 4096 or 8192)
- * and any tile whose window targets leave the bracket is recomputed by the
+ * in-range samples are sorted (sort_len entries per pass: 1024, 2048, 4096
+ * or 8192). A tile whose range holds more samples than the sort splits it,
+ * and one whose window targets leave the ranges adds adjacent ranges, in
+ * further passes (at most 8); a tile that needs more is recomputed by the
  * plain kernel. Needs a 2-D image and a rectangular window at most 64 wide.
  * Results are bit-identical with and without a bracket. weights = NULL
  * disables the bracket. */
