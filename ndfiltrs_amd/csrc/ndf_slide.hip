@@ -1331,6 +1331,43 @@ __device__ __forceinline__ uint64_t window_bits(const uint64_t w[2], int lane, u
     return (low | high) & kxmask;
 }
 
+// (entering samples with a code below m) - (leaving samples with a code
+// below m), over the set bits of the lane's entering and leaving window bits
+// (bit i <-> code row[i]). Each round takes up to EDGE_BATCH set bits of
+// both rows and issues all their code reads before the first compare, so the
+// LDS latency is paid once per round rather than once per sample.
+constexpr int EDGE_BATCH = 4;
+__device__ __forceinline__ int edge_codes_below(uint64_t bits_in, const uint16_t *code_in, uint64_t bits_out,
+                                                const uint16_t *code_out, int m)
+{
+    int delta = 0;
+    while (bits_in | bits_out) {
+        int at_in[EDGE_BATCH], at_out[EDGE_BATCH];
+        bool ok_in[EDGE_BATCH], ok_out[EDGE_BATCH];
+#pragma unroll
+        for (int q = 0; q < EDGE_BATCH; ++q) {
+            ok_in[q] = bits_in != 0;
+            at_in[q] = ok_in[q] ? (int)__builtin_ctzll(bits_in) : 0;
+            bits_in &= bits_in - 1ull;
+            ok_out[q] = bits_out != 0;
+            at_out[q] = ok_out[q] ? (int)__builtin_ctzll(bits_out) : 0;
+            bits_out &= bits_out - 1ull;
+        }
+        int v_in[EDGE_BATCH], v_out[EDGE_BATCH];
+#pragma unroll
+        for (int q = 0; q < EDGE_BATCH; ++q) {
+            v_in[q] = code_in[at_in[q]];
+            v_out[q] = code_out[at_out[q]];
+        }
+#pragma unroll
+        for (int q = 0; q < EDGE_BATCH; ++q) {
+            delta += (ok_in[q] && v_in[q] < m) ? 1 : 0;
+            delta -= (ok_out[q] && v_out[q] < m) ? 1 : 0;
+        }
+    }
+    return delta;
+}
+
 // Bracket kernel for 2-D rectangular windows (support rows <= 128 samples).
 // One workgroup per tile, persistent over tiles. A tile runs one or more
 // passes, each over one key range (the first is the tile's learned bracket
@@ -1713,16 +1750,7 @@ __global__ void __launch_bounds__(NT, E >= 32 ? 1 : (sizeof(T) == 8 && E >= 16 ?
                     inside += (int)__popcll(bits_in) - (int)__popcll(bits_out);
                     const uint16_t *code_out = codes + (y - 1) * a.sx + lane;
                     const uint16_t *code_in = codes + (y - 1 + a.ky) * a.sx + lane;
-                    while (bits_out) {
-                        const int i = (int)__builtin_ctzll(bits_out);
-                        bits_out &= bits_out - 1ull;
-                        c -= (int)code_out[i] < m;
-                    }
-                    while (bits_in) {
-                        const int i = (int)__builtin_ctzll(bits_in);
-                        bits_in &= bits_in - 1ull;
-                        c += (int)code_in[i] < m;
-                    }
+                    c += edge_codes_below(bits_in, code_in, bits_out, code_out, m);
                 }
                 Pick pick;
                 if (!pick_ranks(a.stat, n, a.min_count, a.rank_param, a.q100, pick)) {
