@@ -39,7 +39,7 @@ constexpr uint16_t POS_NONE = 0xFFFF;
 constexpr int NT = SLIDE_WAVES * 64;
 constexpr int BINS = 16;
 constexpr int BATCH = 16;
-constexpr int START = 8;  // start-cursor candidates per wave in the bracket kernel
+constexpr int START = 4;  // start-cursor candidates per wave in the bracket kernel
 constexpr int MAX_PASSES = 8;  // key ranges the bracket kernel may run per tile
 constexpr int GUARD = BATCH;
 
