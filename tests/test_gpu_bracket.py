@@ -150,6 +150,57 @@ def test_extension_passes_match_oracle(lo, hi) -> None:
     assert_same_bits(out, expected)
 
 
+EDGE_STATS = [
+    # (dtype, kind, parameter, window, (lo, hi))
+    (np.float32, "rank", 0, (15, 15), (0.3, 0.6)),           # lowest rank, bracket above it: extensions down
+    (np.float32, "rank", 224, (15, 15), (-0.6, -0.3)),       # highest rank, bracket below it: extensions up
+    (np.float32, "percentile", 0.0, (15, 15), (-3.0, 3.0)),
+    (np.float32, "percentile", 100.0, (15, 15), (-3.0, 3.0)),
+    (np.float32, "percentile", 62.5, (31, 31), (-3.0, 3.0)),  # split passes, interpolated pairs
+    (np.float64, "median", 0, (31, 31), (-3.0, 3.0)),        # float64 split passes
+    (np.float64, "median", 0, (15, 15), (0.3, 0.6)),         # float64 extensions
+    (np.float64, "rank", 100, (31, 31), (-0.05, 0.05)),
+]
+
+
+@pytest.mark.parametrize("dtype,kind,parameter,size,bounds", EDGE_STATS,
+                         ids=[f"{np.dtype(c[0]).name}-{c[1]}{c[2]}-{c[3][0]}-{c[4][0]}" for c in EDGE_STATS])
+def test_passes_on_edge_statistics_match_plain_and_oracle(dtype, kind, parameter, size, bounds) -> None:
+    """Extreme ranks and percentiles, float64 and even-count pairs through
+    split and extension passes: bit-equal to the plain kernel and the oracle
+    (whatever the number of tiles the passes leave to the plain kernel)."""
+    rng = np.random.default_rng(31)
+    image = rng.normal(1000.0, 10.0, size=(130, 150)).astype(dtype)
+    image[rng.random(image.shape) < 0.03] = np.nan
+    image[::17, ::13] = np.inf
+    plan = make_plan(image, kind, size, parameter)
+    plain, _ = run(plan, image)
+    out, (tiles, fallback) = run(plan, image, constant_model(*bounds))
+    assert_same_bits(out, plain)
+    assert 0 <= fallback <= tiles
+    expected = oracle.window_filter(image, size=size, mode="reflect", statistic=kind, omit_nan=True,
+                                    rank=parameter if kind == "rank" else 0,
+                                    q=parameter if kind == "percentile" else 50.0, method="direct")
+    assert_same_bits(out, expected)
+
+
+@pytest.mark.parametrize("cval", [float("nan"), 1e6, 1000.0])
+def test_constant_mode_cval_through_passes(cval) -> None:
+    """cval is a sample of the border windows (NaN is omitted, 1e6 sits above
+    every bracket, 1000 inside it)."""
+    rng = np.random.default_rng(32)
+    image = rng.normal(1000.0, 10.0, size=(120, 140)).astype(np.float32)
+    image[rng.random(image.shape) < 0.02] = np.nan
+    plan = _core.Plan(image.dtype, image.shape, "median", (21, 21), None, (0, 0), ("constant",) * 2, cval, True,
+                      False, 1, 0)
+    plain, _ = run(plan, image)
+    out, _ = run(plan, image, constant_model(-0.2, 0.2))
+    assert_same_bits(out, plain)
+    expected = oracle.window_filter(image, size=(21, 21), mode="constant", cval=cval, statistic="median",
+                                    omit_nan=True, method="direct")
+    assert_same_bits(out, expected)
+
+
 def test_nan_model_is_an_unbounded_bracket() -> None:
     image = synthetic.sky_small((160, 192), seed=22)
     plan = make_plan(image, "median", (11, 11))
