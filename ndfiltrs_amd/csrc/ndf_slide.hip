@@ -1278,12 +1278,31 @@ __global__ void __launch_bounds__(NT, 1) k_slide_batch(const SlideArgs *images, 
     }
 }
 
+// Bit masks of one support row of a bracket tile (bit x <-> column x of the
+// support; rows are at most 128 samples wide).
+struct RowMasks {
+    uint64_t kept[2];    // retained samples
+    uint64_t below[2];   // retained samples below the tile's bracket
+    uint64_t inside[2];  // samples inside the bracket (ranked)
+};
+
+// Set bits of a 128-bit row mask below column `col` (col < 128).
+__device__ __forceinline__ int inside_before(const uint64_t w[2], uint32_t col)
+{
+    const uint64_t low = col >= 64 ? w[0] : w[0] & ((1ull << col) - 1ull);
+    const uint64_t high = col > 64 ? w[1] & ((1ull << (col - 64)) - 1ull) : 0ull;
+    return (int)__popcll(low) + (int)__popcll(high);
+}
+
 // Sorts the first `retained` staged (key, position) pairs (xk/xp, flat) with
 // a network of NT * E entries padded with sentinels, then writes the sorted
 // keys and positions from the start of the region (keys[NT*E], guard,
-// pos[NT*E], guard) and the 1-based rank of every sorted sample into codes.
+// pos[NT*E], guard) and the 1-based rank of every sorted sample into codes,
+// at the sample's compact index (row_offset[row] + the row's inside samples
+// left of its column).
 template <typename Key, int E>
-__device__ __forceinline__ void sort_inside(Key *xk, uint16_t *xp, uint16_t *codes, int retained, int sx, int tid)
+__device__ __forceinline__ void sort_inside(Key *xk, uint16_t *xp, uint16_t *codes, const uint16_t *row_offset,
+                                            const RowMasks *rowmasks, int retained, int tid)
 {
     using KT = KeyTraits<std::conditional_t<sizeof(Key) == 4, float, double>>;
     constexpr int N = NT * E;
@@ -1306,7 +1325,8 @@ __device__ __forceinline__ void sort_inside(Key *xk, uint16_t *xp, uint16_t *cod
         keys[tid * E + I] = chunk.k[I];
         pos0[tid * E + I] = (uint16_t)chunk.p[I];
         if (chunk.p[I] != POS_NONE) {
-            codes[(chunk.p[I] >> 8) * sx + (chunk.p[I] & 0xFFu)] = (uint16_t)(tid * E + I + 1);
+            const uint32_t row = chunk.p[I] >> 8, col = chunk.p[I] & 0xFFu;
+            codes[row_offset[row] + inside_before(rowmasks[row].inside, col)] = (uint16_t)(tid * E + I + 1);
         }
     });
     if (tid < GUARD) {
@@ -1314,14 +1334,6 @@ __device__ __forceinline__ void sort_inside(Key *xk, uint16_t *xp, uint16_t *cod
         pos0[N + tid] = POS_NONE;
     }
 }
-
-// Bit masks of one support row of a bracket tile (bit x <-> column x of the
-// support; rows are at most 128 samples wide).
-struct RowMasks {
-    uint64_t kept[2];    // retained samples
-    uint64_t below[2];   // retained samples below the tile's bracket
-    uint64_t inside[2];  // samples inside the bracket (ranked)
-};
 
 // Bits [lane, lane + kx) of a 128-bit row mask, shifted down to bit 0.
 __device__ __forceinline__ uint64_t window_bits(const uint64_t w[2], int lane, uint64_t kxmask)
@@ -1331,38 +1343,34 @@ __device__ __forceinline__ uint64_t window_bits(const uint64_t w[2], int lane, u
     return (low | high) & kxmask;
 }
 
+// Set bits of a 64-bit wave mask below this lane.
+__device__ __forceinline__ int lanes_below(uint64_t w)
+{
+    return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(w >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)w, 0u));
+}
+
 // (entering samples with a code below m) - (leaving samples with a code
-// below m), over the set bits of the lane's entering and leaving window bits
-// (bit i <-> code row[i]). Each round takes up to EDGE_BATCH set bits of
-// both rows and issues all their code reads before the first compare, so the
-// LDS latency is paid once per round rather than once per sample.
+// below m), over the n_in codes at code_in and the n_out codes at code_out.
+// Each round reads up to EDGE_BATCH codes of both runs before the first
+// compare, so the LDS latency is paid once per round rather than once per
+// sample.
 constexpr int EDGE_BATCH = 4;
-__device__ __forceinline__ int edge_codes_below(uint64_t bits_in, const uint16_t *code_in, uint64_t bits_out,
-                                                const uint16_t *code_out, int m)
+__device__ __forceinline__ int edge_codes_below(const uint16_t *code_in, int n_in, const uint16_t *code_out,
+                                                int n_out, int m)
 {
     int delta = 0;
-    while (bits_in | bits_out) {
-        int at_in[EDGE_BATCH], at_out[EDGE_BATCH];
-        bool ok_in[EDGE_BATCH], ok_out[EDGE_BATCH];
-#pragma unroll
-        for (int q = 0; q < EDGE_BATCH; ++q) {
-            ok_in[q] = bits_in != 0;
-            at_in[q] = ok_in[q] ? (int)__builtin_ctzll(bits_in) : 0;
-            bits_in &= bits_in - 1ull;
-            ok_out[q] = bits_out != 0;
-            at_out[q] = ok_out[q] ? (int)__builtin_ctzll(bits_out) : 0;
-            bits_out &= bits_out - 1ull;
-        }
+    const int n = n_in > n_out ? n_in : n_out;
+    for (int k = 0; k < n; k += EDGE_BATCH) {
         int v_in[EDGE_BATCH], v_out[EDGE_BATCH];
 #pragma unroll
         for (int q = 0; q < EDGE_BATCH; ++q) {
-            v_in[q] = code_in[at_in[q]];
-            v_out[q] = code_out[at_out[q]];
+            v_in[q] = code_in[k + q < n_in ? k + q : 0];
+            v_out[q] = code_out[k + q < n_out ? k + q : 0];
         }
 #pragma unroll
         for (int q = 0; q < EDGE_BATCH; ++q) {
-            delta += (ok_in[q] && v_in[q] < m) ? 1 : 0;
-            delta -= (ok_out[q] && v_out[q] < m) ? 1 : 0;
+            delta += (k + q < n_in && v_in[q] < m) ? 1 : 0;
+            delta -= (k + q < n_out && v_out[q] < m) ? 1 : 0;
         }
     }
     return delta;
@@ -1397,8 +1405,12 @@ __global__ void __launch_bounds__(NT, E >= 32 ? 1 : (sizeof(T) == 8 && E >= 16 ?
     Key *xk = reinterpret_cast<Key *>(lds);
     uint16_t *xp = exchange_positions<Key, E>(xk);
     Key *keys = reinterpret_cast<Key *>(lds);
+    // Rank codes of the inside samples only, row after row in column order:
+    // row r's start at row_offset[r] (N entries hold every pass's inside
+    // samples, since a pass never sorts more than N).
     uint16_t *codes = reinterpret_cast<uint16_t *>(lds + exchange_bytes<Key, E>());
-    RowMasks *rowmasks = reinterpret_cast<RowMasks *>(codes + ((a.s + 7) & ~7));
+    uint16_t *row_offset = codes + N;
+    RowMasks *rowmasks = reinterpret_cast<RowMasks *>(row_offset + ((a.rows + 1 + 7) & ~7));
     __shared__ int count_shared;
     // Pass bookkeeping of the current tile (written by thread 0 between
     // barriers): pending key ranges (a stack, processed last-in first-out),
@@ -1566,6 +1578,34 @@ __global__ void __launch_bounds__(NT, E >= 32 ? 1 : (sizeof(T) == 8 && E >= 16 ?
             continue;
         }
 
+        // ---- row offsets of the compact codes (wave 0; read after the sort's barriers) ----
+        if (wave == 0) {
+            // Lane l sums the inside samples of rows 4l .. 4l + 3, then a wave
+            // prefix scan gives every row's start.
+            int counts[4], total = 0;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int row = 4 * lane + j;
+                counts[j] = row < a.rows ? (int)__popcll(rowmasks[row].inside[0]) + (int)__popcll(rowmasks[row].inside[1])
+                                         : 0;
+                total += counts[j];
+            }
+            int scan = total;
+            for (int off = 1; off < 64; off <<= 1) {
+                const int u = __shfl_up(scan, off);
+                scan += lane >= off ? u : 0;
+            }
+            int start = scan - total;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int row = 4 * lane + j;
+                if (row <= a.rows) {
+                    row_offset[row] = (uint16_t)start;
+                }
+                start += counts[j];
+            }
+        }
+
         // ---- sort and rank the inside samples (smallest network that holds them) ----
         int nsort = N;
         if constexpr (E >= 16) {
@@ -1580,16 +1620,16 @@ __global__ void __launch_bounds__(NT, E >= 32 ? 1 : (sizeof(T) == 8 && E >= 16 ?
         }
         if constexpr (E >= 16) {
             if (nsort == NT * (E / 4)) {
-                sort_inside<Key, E / 4>(xk, xp, codes, retained, a.sx, tid);
+                sort_inside<Key, E / 4>(xk, xp, codes, row_offset, rowmasks, retained, tid);
             }
         }
         if constexpr (E >= 8) {
             if (nsort == NT * (E / 2)) {
-                sort_inside<Key, E / 2>(xk, xp, codes, retained, a.sx, tid);
+                sort_inside<Key, E / 2>(xk, xp, codes, row_offset, rowmasks, retained, tid);
             }
         }
         if (nsort == N) {
-            sort_inside<Key, E>(xk, xp, codes, retained, a.sx, tid);
+            sort_inside<Key, E>(xk, xp, codes, row_offset, rowmasks, retained, tid);
         }
         __syncthreads();
         const uint16_t *pos1 = reinterpret_cast<uint16_t *>(keys + nsort) + GUARD - 1;  // 1-based ranks
@@ -1650,9 +1690,11 @@ __global__ void __launch_bounds__(NT, E >= 32 ? 1 : (sizeof(T) == 8 && E >= 16 ?
                     const bool has1 = lane + 64 < a.sx;
                     // Samples that are not inside get 0x7FFF, above every rank,
                     // so a 16-bit difference's sign is `code < m_j`. The masks
-                    // and codes of four rows are read before any is used (the
-                    // code reads are unconditional; codes of samples that are
-                    // not inside are stale and replaced).
+                    // and row offsets of four rows are read before any is
+                    // used, then their codes (columns lane and lane + 64 sit
+                    // at the row offset plus the row's inside samples left of
+                    // them; for a sample that is not inside that index reads
+                    // a neighbouring code, which is replaced).
                     auto add_row = [&](uint64_t in0, uint64_t in1, unsigned short c0, unsigned short c1) {
                         const unsigned short e0 = ((in0 >> lane) & 1ull) ? c0 : (unsigned short)0x7FFF;
                         const unsigned short e1 = (has1 && ((in1 >> lane) & 1ull)) ? c1 : (unsigned short)0x7FFF;
@@ -1666,14 +1708,20 @@ __global__ void __launch_bounds__(NT, E >= 32 ? 1 : (sizeof(T) == 8 && E >= 16 ?
                     int dy = 0;
                     for (; dy + 4 <= a.ky; dy += 4) {
                         uint64_t in0[4], in1[4];
-                        unsigned short c0[4], c1[4];
+                        int off[4];
 #pragma unroll
                         for (int r = 0; r < 4; ++r) {
                             const int row = y_first + dy + r;
                             in0[r] = rowmasks[row].inside[0];
                             in1[r] = rowmasks[row].inside[1];
-                            c0[r] = codes[row * a.sx + lane];
-                            c1[r] = codes[row * a.sx + lane + 64];
+                            off[r] = row_offset[row];
+                        }
+                        unsigned short c0[4], c1[4];
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) {
+                            const int i0 = off[r] + lanes_below(in0[r]);
+                            c0[r] = codes[i0];
+                            c1[r] = codes[off[r] + (int)__popcll(in0[r]) + lanes_below(in1[r])];
                         }
 #pragma unroll
                         for (int r = 0; r < 4; ++r) {
@@ -1682,8 +1730,10 @@ __global__ void __launch_bounds__(NT, E >= 32 ? 1 : (sizeof(T) == 8 && E >= 16 ?
                     }
                     for (; dy < a.ky; ++dy) {
                         const int row = y_first + dy;
-                        add_row(rowmasks[row].inside[0], rowmasks[row].inside[1], codes[row * a.sx + lane],
-                                codes[row * a.sx + lane + 64]);
+                        const uint64_t w0 = rowmasks[row].inside[0], w1 = rowmasks[row].inside[1];
+                        const int off = row_offset[row];
+                        add_row(w0, w1, codes[off + lanes_below(w0)],
+                                codes[off + (int)__popcll(w0) + lanes_below(w1)]);
                     }
                     // Inclusive prefix over columns 0..127 (two candidates per
                     // register; counts stay below 2^16).
@@ -1745,12 +1795,15 @@ __global__ void __launch_bounds__(NT, E >= 32 ? 1 : (sizeof(T) == 8 && E >= 16 ?
                          (int)__popcll(window_bits(leave.kept, lane, kxmask));
                     below += db;
                     c += db;
-                    uint64_t bits_out = window_bits(leave.inside, lane, kxmask);
-                    uint64_t bits_in = window_bits(enter.inside, lane, kxmask);
-                    inside += (int)__popcll(bits_in) - (int)__popcll(bits_out);
-                    const uint16_t *code_out = codes + (y - 1) * a.sx + lane;
-                    const uint16_t *code_in = codes + (y - 1 + a.ky) * a.sx + lane;
-                    c += edge_codes_below(bits_in, code_in, bits_out, code_out, m);
+                    // The window's inside samples of a row are consecutive
+                    // in the compact codes: from the row offset plus the
+                    // row's inside samples left of the lane's column.
+                    const int n_out = (int)__popcll(window_bits(leave.inside, lane, kxmask));
+                    const int n_in = (int)__popcll(window_bits(enter.inside, lane, kxmask));
+                    inside += n_in - n_out;
+                    const uint16_t *code_out = codes + row_offset[y - 1] + lanes_below(leave.inside[0]);
+                    const uint16_t *code_in = codes + row_offset[y - 1 + a.ky] + lanes_below(enter.inside[0]);
+                    c += edge_codes_below(code_in, n_in, code_out, n_out, m);
                 }
                 Pick pick;
                 if (!pick_ranks(a.stat, n, a.min_count, a.rank_param, a.q100, pick)) {
@@ -2022,7 +2075,10 @@ size_t slide_lds_bytes(int dtype_bytes, int s, int sort_e, int hist_stride, int 
 
 size_t bracket_lds_bytes(int dtype_bytes, int s, int rows, int sort_e)
 {
-    return exchange_region(dtype_bytes, sort_e) + (size_t)((s + 7) & ~7) * 2 + (size_t)rows * sizeof(RowMasks);
+    // Exchange, compact codes (one per sort entry), row offsets, row masks.
+    (void)s;
+    return exchange_region(dtype_bytes, sort_e) + (size_t)NT * sort_e * 2 + (size_t)((rows + 1 + 7) & ~7) * 2 +
+           (size_t)rows * sizeof(RowMasks);
 }
 
 hipError_t launch_bracket_slide(int dtype_bytes, const SlideArgs &args, size_t lds_bytes, int max_tiles,
