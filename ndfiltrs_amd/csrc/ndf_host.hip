@@ -766,9 +766,9 @@ bool plan_slide(ndf_plan *p, int force_e = 0, int max_ty = 256)
             }
             const size_t need = bracket_lds_bytes(p->dbytes, v.s, v.rows, 16);
             const size_t granule = 512;
-            // The bracket kernel's launch bounds allow 3 workgroups per CU for
-            // float32 and 2 for float64 (register budget).
-            const int by_registers = p->dbytes == 4 ? 3 : 2;
+            // The bracket kernel's launch bounds allow 3 workgroups per CU
+            // (register budget).
+            const int by_registers = 3;
             const int per_cu =
                 std::min(by_registers, (int)((size_t)limit / ((need + 16 + granule - 1) / granule * granule)));
             if (per_cu > best_per_cu) {
