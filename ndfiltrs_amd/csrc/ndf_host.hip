@@ -753,8 +753,8 @@ bool plan_slide(ndf_plan *p, int force_e = 0, int max_ty = 256)
 
     // Bracket tiles (2-D rectangles with support rows of at most 128
     // samples): the tallest tile of 24 to 32 rows among those that keep the
-    // most bracket workgroups resident per CU (at most 3 counted); shorter
-    // tiles only when none of those fits.
+    // most bracket workgroups resident per CU (at most 4 counted for float32,
+    // 3 for float64); shorter tiles only when none of those fits.
     p->bracket_ty = 0;
     if (force_e == 0 && rect && kz == 1 && dims[0] == 1 && kx <= 64 && p->rank >= 2) {
         int best_per_cu = 0;
@@ -766,9 +766,9 @@ bool plan_slide(ndf_plan *p, int force_e = 0, int max_ty = 256)
             }
             const size_t need = bracket_lds_bytes(p->dbytes, v.s, v.rows, 16);
             const size_t granule = 512;
-            // The bracket kernel's launch bounds allow 3 workgroups per CU
-            // (register budget).
-            const int by_registers = 3;
+            // The bracket kernel's launch bounds allow 4 workgroups per CU
+            // for float32 and 3 for float64 (register budget).
+            const int by_registers = p->dbytes == 4 ? 4 : 3;
             const int per_cu =
                 std::min(by_registers, (int)((size_t)limit / ((need + 16 + granule - 1) / granule * granule)));
             if (per_cu > best_per_cu) {

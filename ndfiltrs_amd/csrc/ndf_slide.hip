@@ -1395,11 +1395,12 @@ __device__ __forceinline__ int edge_codes_below(const uint16_t *code_in, int n_i
 // that would exceed MAX_PASSES passes, or whose walk fails, goes to the
 // fallback list, where the plain kernel recomputes it.
 template <typename T, int E>
-// Three workgroups per CU up to a 4096-entry sort, float64 included: with
-// compact codes a float64 workgroup needs 52 KB of LDS, and the 168-VGPR
-// budget costs the float64 kernel some spilled sort registers but measured
-// faster than two workgroups per CU (31x31 median 1.71 -> 1.48 ms).
-__global__ void __launch_bounds__(NT, E >= 32 ? 1 : 3) k_bracket(SlideArgs a)
+// Up to a 4096-entry sort, four workgroups per CU for float32 (37 KB of LDS
+// each with compact codes; 128 VGPRs) and three for float64 (52 KB; 168
+// VGPRs). The register budgets spill a few sort registers (96 B/lane for
+// float32, 312 B/lane for float64) but measured faster than one workgroup
+// fewer: C5 median 5.06 -> 4.46 ms, 31x31 float64 median 1.71 -> 1.48 ms.
+__global__ void __launch_bounds__(NT, E >= 32 ? 1 : (sizeof(T) == 4 ? 4 : 3)) k_bracket(SlideArgs a)
 {
     using KT = KeyTraits<T>;
     using Key = typename KT::Key;
