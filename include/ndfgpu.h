@@ -143,8 +143,9 @@ ndf_status ndf_plan_upload(ndf_plan *plan, const void *input, size_t len, char *
 ndf_status ndf_plan_compute_download(ndf_plan *plan, void *output, size_t len, char *err, size_t err_len);
 
 /* Filter device-resident buffers on `stream` (a hipStream_t, NULL = default
- * stream). Asynchronous: returns after the kernels are enqueued. No
- * reference counterpart (the reference has no device). */
+ * stream). Asynchronous: returns after the kernels are enqueued. The output
+ * must not overlap the input (INVALID otherwise). No reference counterpart
+ * (the reference has no device; it always writes a new array). */
 ndf_status ndf_plan_apply_device(ndf_plan *plan, const void *d_input, void *d_output, size_t len,
                                  void *stream, char *err, size_t err_len);
 
@@ -296,7 +297,8 @@ ndf_status ndf_batch_configure(ndf_batch *batch, int32_t sort_len, int32_t max_t
  * (`repeats` launches each) and keeps the fastest. */
 ndf_status ndf_batch_autotune(ndf_batch *batch, const void *d_input, void *d_output, size_t len, int32_t repeats,
                               char *err, size_t err_len);
-/* Asynchronous on `stream` (a hipStream_t, NULL = default stream). */
+/* Asynchronous on `stream` (a hipStream_t, NULL = default stream). The
+ * output must not overlap the input (INVALID otherwise). */
 ndf_status ndf_batch_apply_device(ndf_batch *batch, const void *d_input, void *d_output, size_t len, void *stream,
                                   char *err, size_t err_len);
 /* Mean milliseconds per launch over `iterations` (HIP events). */

@@ -1342,6 +1342,20 @@ ndf_status build_plan(ndf_plan *p, int32_t dtype_bytes, size_t rank, const size_
     return NDF_OK;
 }
 
+// True when the len-element input and output ranges share a byte: the
+// filters read every input sample of a tile's support after other tiles have
+// written their outputs, so they need disjoint buffers.
+bool ranges_overlap(const void *d_input, const void *d_output, size_t bytes)
+{
+    const uintptr_t a = reinterpret_cast<uintptr_t>(d_input), b = reinterpret_cast<uintptr_t>(d_output);
+    return a < b + bytes && b < a + bytes;
+}
+
+bool buffers_overlap(const ndf_plan *plan, const void *d_input, const void *d_output, size_t len)
+{
+    return ranges_overlap(d_input, d_output, len * (size_t)plan->dbytes);
+}
+
 }  // namespace
 
 extern "C" {
@@ -1489,6 +1503,9 @@ ndf_status ndf_plan_apply_device(ndf_plan *plan, const void *d_input, void *d_ou
 {
     if (len != plan->len) return fail_code(NDF_ERR_LENGTH_MISMATCH, err, err_len);
     if (len == 0) return NDF_OK;
+    if (buffers_overlap(plan, d_input, d_output, len)) {
+        return fail(NDF_ERR_INVALID, "the output buffer must not overlap the input buffer", err, err_len);
+    }
     std::lock_guard<std::mutex> guard(plan->mutex);
     const ndf_status st = prepare_tables(plan, err, err_len);
     if (st != NDF_OK) return st;
@@ -2216,6 +2233,9 @@ ndf_status ndf_batch_apply_device(ndf_batch *batch, const void *d_input, void *d
 {
     if (len != (size_t)batch->total) return fail_code(NDF_ERR_LENGTH_MISMATCH, err, err_len);
     if (len == 0) return NDF_OK;
+    if (ranges_overlap(d_input, d_output, len * (size_t)batch->dbytes)) {
+        return fail(NDF_ERR_INVALID, "the output buffer must not overlap the input buffer", err, err_len);
+    }
     std::lock_guard<std::mutex> guard(batch->mutex);
     return batch_launch(batch, d_input, d_output, static_cast<hipStream_t>(stream), err, err_len);
 }
@@ -2848,6 +2868,9 @@ ndf_status ndf_plan_apply_device_static(ndf_plan *plan, const void *d_input, voi
 {
     if (len != plan->len) return fail_code(NDF_ERR_LENGTH_MISMATCH, err, err_len);
     if (len == 0) return NDF_OK;
+    if (buffers_overlap(plan, d_input, d_output, len)) {
+        return fail(NDF_ERR_INVALID, "the output buffer must not overlap the input buffer", err, err_len);
+    }
     std::lock_guard<std::mutex> guard(plan->mutex);
     hipStream_t s = static_cast<hipStream_t>(stream);
     LaunchSet t;

@@ -140,6 +140,16 @@ def test_bracket_validation_without_a_device() -> None:
         cube.set_bracket(weights)
 
 
+def test_overlapping_device_buffers_are_rejected_without_a_device() -> None:
+    """The filters need disjoint input and output buffers; the check runs
+    before any device call (the addresses here are never dereferenced)."""
+    plan = _core.Plan(np.float32, (64, 64), "median", (5, 5), None, (0, 0), ("reflect",) * 2, 0.0, True, False, 1, 0)
+    base = 1 << 40
+    for d_in, d_out in ((base, base), (base, base + 4 * 64 * 63), (base + 4, base)):
+        with pytest.raises(ValueError, match="must not overlap"):
+            plan.apply_device(d_in, d_out)
+
+
 def test_batch_validation_without_a_device() -> None:
     from ndfiltrs_amd import batch
 
