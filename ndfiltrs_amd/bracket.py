@@ -47,8 +47,9 @@ WORKLOADS = {
         "ranks": [397],
         "tau": (0.003, 0.997),
         # Sort capacity of the bracket kernel (inside samples per pass), per
-        # statistic.
-        "sort_len": {"median": 4096, "rank": 4096, "percentile": 4096},
+        # statistic: the 10 % rank's narrow ranges fit 2048 entries, which
+        # run six workgroups per CU instead of four.
+        "sort_len": {"median": 4096, "rank": 2048, "percentile": 2048},
         "law": "2048^2 crops (seeded offsets) of the 8192^2 field: sky N(1000,10) + quadratic background up to "
                "300 + 2000 sources per 8192^2 + 2% NaN (float32)",
     },

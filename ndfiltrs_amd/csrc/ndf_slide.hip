@@ -1395,12 +1395,15 @@ __device__ __forceinline__ int edge_codes_below(const uint16_t *code_in, int n_i
 // that would exceed MAX_PASSES passes, or whose walk fails, goes to the
 // fallback list, where the plain kernel recomputes it.
 template <typename T, int E>
-// Up to a 4096-entry sort, four workgroups per CU for float32 (37 KB of LDS
-// each with compact codes; 128 VGPRs) and three for float64 (52 KB; 168
-// VGPRs). The register budgets spill a few sort registers (96 B/lane for
-// float32, 312 B/lane for float64) but measured faster than one workgroup
-// fewer: C5 median 5.06 -> 4.46 ms, 31x31 float64 median 1.71 -> 1.48 ms.
-__global__ void __launch_bounds__(NT, E >= 32 ? 1 : (sizeof(T) == 4 ? 4 : 3)) k_bracket(SlideArgs a)
+// Workgroups per CU: at a 4096-entry sort four for float32 (37 KB of LDS each
+// with compact codes; 128 VGPRs) and three for float64 (52 KB; 168 VGPRs);
+// at 2048 entries or fewer six for float32 and four for float64. The
+// register budgets spill a few sort registers (96 B/lane for float32 and 312
+// for float64 at 4096) but measured faster than one workgroup fewer: C5
+// median 5.06 -> 4.46 ms, 31x31 float64 median 1.71 -> 1.48 ms, C5 rank 397
+// at 2048 entries 3.20 -> 3.04 ms, 31x31 float64 rank 154 1.14 -> 1.07 ms.
+__global__ void __launch_bounds__(NT, E >= 32 ? 1 : (sizeof(T) == 4 ? (E <= 8 ? 6 : 4) : (E <= 8 ? 4 : 3)))
+    k_bracket(SlideArgs a)
 {
     using KT = KeyTraits<T>;
     using Key = typename KT::Key;
