@@ -605,6 +605,9 @@ def main() -> int:
         "kernel_ms": main_bracket["event_main_kernel_ms"],
         "algorithmic_units": "3969 32-bit key compares per output (one exact counting pass over the 63x63 "
                              "window) x 67108864 outputs per launch",
+        "note": "a work-model ratio: the bracket kernel reuses work between neighbouring windows and sorts only "
+                "in-bracket samples, so it does fewer than W compares per output and can pass 1.0; its hardware "
+                "VALU issue rate and HBM bytes per launch are in profiles/c5_nanmedian63_8192_f32.md",
         "hbm_view": {
             "achieved": pixels * 8 / t_main / 1e9,
             "peak": HBM_PEAK / 1e9,
