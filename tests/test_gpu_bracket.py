@@ -201,6 +201,51 @@ def test_constant_mode_cval_through_passes(cval) -> None:
     assert_same_bits(out, expected)
 
 
+RANDOM_SEEDS = list(range(24))
+
+
+@pytest.mark.parametrize("seed", RANDOM_SEEDS)
+def test_random_configurations_bits_equal_plain(seed) -> None:
+    """Seeded random geometry, boundary mode, dtype, statistic, policy, sort
+    capacity and constant bracket (narrow, offset or wide, so tiles take
+    single, split and extension passes): bracketed launches equal the plain
+    kernel bit for bit."""
+    rng = np.random.default_rng(1000 + seed)
+    dtype = np.float32 if rng.random() < 0.6 else np.float64
+    ky, kx = int(rng.integers(3, 40)), int(rng.integers(3, 65))
+    shape = (int(rng.integers(ky + 20, 200)), int(rng.integers(70, 260)))
+    image = rng.normal(1000.0, 10.0, size=shape)
+    image += np.linspace(0.0, 50.0 * rng.random(), shape[1])[None, :]
+    image[rng.random(shape) < rng.choice([0.0, 0.02, 0.2])] = np.nan
+    if rng.random() < 0.5:
+        image[rng.random(shape) < 0.005] = np.inf
+    image = image.astype(dtype)
+    mode = str(rng.choice(["reflect", "constant", "nearest", "mirror", "wrap"]))
+    kind = str(rng.choice(["median", "rank", "percentile"]))
+    parameter = 0
+    if kind == "rank":
+        parameter = int(rng.integers(0, ky * kx))
+    elif kind == "percentile":
+        parameter = float(rng.choice([0.0, 10.0, 37.5, 50.0, 90.0, 100.0]))
+    omit = rng.random() < 0.85
+    # Propagate has no ignore_inf or min_count (the reference rejects them).
+    ignore_inf = omit and bool(rng.random() < 0.5)
+    min_count = int(rng.integers(1, max(2, ky * kx // 2))) if omit else 1
+    plan = _core.Plan(image.dtype, image.shape, kind, (ky, kx), None, (0, 0), (mode, mode),
+                      float(rng.choice([0.0, 1000.0, np.nan])), omit, ignore_inf, min_count, parameter)
+    if json.loads(plan.describe())["bracket_tile_rows"] == 0:
+        pytest.skip("no bracket geometry for this window")
+    plain, _ = run(plan, image)
+    centre = float(rng.normal(0.0, 0.5))
+    half = float(rng.choice([0.02, 0.3, 1.0, 4.0]))
+    sort_len = int(rng.choice([1024, 2048, 4096]))
+    out, (tiles, fallback) = run(plan, image, constant_model(centre - half, centre + half), sort_len)
+    print(f"seed {seed}: {dtype.__name__} {shape} {ky}x{kx} {mode} {kind}({parameter}) omit={omit} "
+          f"min_count={min_count} sort {sort_len} bracket {centre:+.2f}+-{half}: fallback {fallback}/{tiles}",
+          flush=True)
+    assert_same_bits(out, plain)
+
+
 def test_nan_model_is_an_unbounded_bracket() -> None:
     image = synthetic.sky_small((160, 192), seed=22)
     plan = make_plan(image, "median", (11, 11))
