@@ -52,6 +52,9 @@ __device__ inline void wave_lds_sync()
 constexpr int STAGE_COLUMNS = 8;
 // Rows a wave stages per batch (all loads of a batch are issued first).
 constexpr int STAGE_ROWS = 4;
+// Rows a wave reads before writing, per doubling step of the x and y passes.
+constexpr int X_ROWS = 8;
+constexpr int Y_ROWS = 8;
 
 template <typename T, bool MAXIMUM>
 __global__ void __launch_bounds__(EXTREMA_THREADS) k_extrema(ExtremaArgs a)
@@ -131,38 +134,74 @@ __global__ void __launch_bounds__(EXTREMA_THREADS) k_extrema(ExtremaArgs a)
             }
         }
         // ---- x pass, in place: each wave doubles the rows it staged ----
-        // Chunks of 64 consecutive columns go in increasing order and every
-        // lane of a chunk reads before any writes, so a chunk never reads a
-        // value a lower chunk already overwrote; only wave-level ordering is
-        // needed between steps.
+        // Batches of X_ROWS rows, one 64-column chunk at a time: a wave reads
+        // the chunk's values of every row of the batch before it writes any,
+        // so the LDS latency is paid once per batch. Rows are wave-private
+        // and a wave's LDS operations complete in order, so each read sees
+        // the previous step's value; chunks go in increasing order, so a
+        // chunk never reads a value a lower chunk already overwrote. Indices
+        // are clamped instead of branched on (the loop stays one basic
+        // block): a clamped lane or row repeats an in-range element's read
+        // and writes back the value that element's own lane writes.
         wave_lds_sync();
         for (int s = 1; s <= px; s <<= 1) {
             const int shift = s < px ? s : a.kx - px;
             const int columns = s < px ? a.sx : a.tx;
-            for (int r = wave; r < rows; r += EXTREMA_THREADS / 64) {
-                Key *row = stage + r * a.sx;
-                for (int i = lane; i < columns; i += 64) {
-                    const Key v = row[i];
-                    const Key w = i + shift < a.sx ? row[i + shift] : v;
-                    row[i] = pick_extreme<Key, MAXIMUM>(v, w);
+            for (int r0 = wave; r0 < rows; r0 += X_ROWS * (EXTREMA_THREADS / 64)) {
+                Key *row[X_ROWS];
+#pragma unroll
+                for (int u = 0; u < X_ROWS; ++u) {
+                    const int r = r0 + u * (EXTREMA_THREADS / 64);
+                    row[u] = stage + (r < rows ? r : r0) * a.sx;
+                }
+                for (int c = 0; c < columns; c += 64) {
+                    const int i = min(c + lane, columns - 1);
+                    const int iw = i + shift < a.sx ? i + shift : i;
+                    Key v[X_ROWS], w[X_ROWS];
+#pragma unroll
+                    for (int u = 0; u < X_ROWS; ++u) {
+                        v[u] = row[u][i];
+                        w[u] = row[u][iw];
+                    }
+#pragma unroll
+                    for (int u = 0; u < X_ROWS; ++u) row[u][i] = pick_extreme<Key, MAXIMUM>(v[u], w[u]);
                 }
             }
             wave_lds_sync();
         }
         __syncthreads();
         // ---- y pass: extreme over ky along columns, ping-pong ----
+        // Each wave reads one chunk of Y_ROWS source rows before writing them
+        // to the other buffer (tx is a multiple of 64; a clamped row repeats
+        // the batch's first row).
         const Key *src = stage;
         int src_stride = a.sx, src_plane = plane;
         Key *dst = partner;
         for (int s = 1; s < py; s <<= 1) {
-            for (int zr = wave; zr < rows; zr += EXTREMA_THREADS / 64) {
-                const int z = zr / a.sy;
-                const int r = zr - z * a.sy;
-                const Key *in = src + z * src_plane + r * src_stride;
-                Key *out = dst + z * yplane + r * a.tx;
-                for (int i = lane; i < a.tx; i += 64) {
-                    const Key v = in[i];
-                    out[i] = r + s < a.sy ? pick_extreme<Key, MAXIMUM>(v, in[i + s * src_stride]) : v;
+            for (int zr0 = wave; zr0 < rows; zr0 += Y_ROWS * (EXTREMA_THREADS / 64)) {
+                const Key *in[Y_ROWS];
+                Key *out[Y_ROWS];
+                int step[Y_ROWS];
+#pragma unroll
+                for (int u = 0; u < Y_ROWS; ++u) {
+                    const int zu = zr0 + u * (EXTREMA_THREADS / 64);
+                    const int zr = zu < rows ? zu : zr0;
+                    const int z = zr / a.sy;
+                    const int r = zr - z * a.sy;
+                    in[u] = src + z * src_plane + r * src_stride;
+                    out[u] = dst + z * yplane + r * a.tx;
+                    step[u] = r + s < a.sy ? s * src_stride : 0;
+                }
+                for (int c = 0; c < a.tx; c += 64) {
+                    const int i = c + lane;
+                    Key v[Y_ROWS], w[Y_ROWS];
+#pragma unroll
+                    for (int u = 0; u < Y_ROWS; ++u) {
+                        v[u] = in[u][i];
+                        w[u] = in[u][i + step[u]];
+                    }
+#pragma unroll
+                    for (int u = 0; u < Y_ROWS; ++u) out[u][i] = pick_extreme<Key, MAXIMUM>(v[u], w[u]);
                 }
             }
             __syncthreads();
