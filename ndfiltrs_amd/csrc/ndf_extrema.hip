@@ -14,9 +14,10 @@
 //   * the x pass and the y pass each take the extreme over k samples by
 //     doubling: log2(k) steps of m_2s[i] = op(m_s[i], m_s[i + s]), then
 //     op(m_P[i], m_P[i + k - P]) for the largest power of two P <= k (the
-//     two windows overlap, which an idempotent op allows); the steps ping-
-//     pong between two LDS buffers, one element per thread, with a barrier
-//     per step;
+//     two windows overlap, which an idempotent op allows); the x steps run
+//     in place on wave-private rows, the y steps ping-pong between two LDS
+//     buffers with a barrier per step, and each wave reads a batch of rows
+//     before writing them, so it waits on LDS once per batch;
 //   * the z pass folds the kz planes per output.
 // The extreme key is a retained sample's key (its bits unchanged), or one of
 // the two marks (canonical NaN). Equal keys are equal bits, so which of two
