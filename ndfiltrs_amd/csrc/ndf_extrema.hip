@@ -14,10 +14,11 @@
 //   * the x pass and the y pass each take the extreme over k samples by
 //     doubling: log2(k) steps of m_2s[i] = op(m_s[i], m_s[i + s]), then
 //     op(m_P[i], m_P[i + k - P]) for the largest power of two P <= k (the
-//     two windows overlap, which an idempotent op allows); the x steps run
-//     in place on wave-private rows, the y steps ping-pong between two LDS
-//     buffers with a barrier per step, and each wave reads a batch of rows
-//     before writing them, so it waits on LDS once per batch;
+//     two windows overlap, which an idempotent op allows); every step runs
+//     in place, the x steps on wave-private rows and the y steps on
+//     wave-private columns, so one barrier per tile separates the passes,
+//     and each wave reads a batch of rows before writing them, so it waits
+//     on LDS once per batch;
 //   * the z pass folds the kz planes per output.
 // The extreme key is a retained sample's key (its bits unchanged), or one of
 // the two marks (canonical NaN). Equal keys are equal bits, so which of two
@@ -63,10 +64,8 @@ __global__ void __launch_bounds__(EXTREMA_THREADS) k_extrema(ExtremaArgs a)
     using KT = KeyTraits<T>;
     using Key = typename KT::Key;
     extern __shared__ __align__(16) unsigned char lds[];
-    const int plane = a.sy * a.sx;   // staging buffer: kz planes of sy x sx
-    const int yplane = a.sy * a.tx;  // y-pass partner: kz planes of sy x tx
+    const int plane = a.sy * a.sx;  // staging buffer: kz planes of sy x sx
     Key *stage = reinterpret_cast<Key *>(lds);
-    Key *partner = stage + a.kz * plane;
     const Key *input = static_cast<const Key *>(a.input);
     Key *output = static_cast<Key *>(a.output);
     // Marks: an omitted (or padding) sample never wins; a propagated NaN always does.
@@ -99,7 +98,7 @@ __global__ void __launch_bounds__(EXTREMA_THREADS) k_extrema(ExtremaArgs a)
             const int x = lane + 64 * j;
             mx[j] = x < a.sx ? a.mapx[x0 + x] : MAP_PAD;
         }
-        __syncthreads();  // the previous tile is done with both buffers
+        __syncthreads();  // every wave of the previous tile is done with the buffer
         // ---- stage the support as keys: each wave owns rows wave + 4 i ----
         for (int row0 = wave; row0 < rows; row0 += STAGE_ROWS * (EXTREMA_THREADS / 64)) {
             Key raw[STAGE_ROWS][STAGE_COLUMNS];
@@ -170,63 +169,55 @@ __global__ void __launch_bounds__(EXTREMA_THREADS) k_extrema(ExtremaArgs a)
             }
             wave_lds_sync();
         }
-        __syncthreads();
-        // ---- y pass: extreme over ky along columns, ping-pong ----
-        // Each wave reads one chunk of Y_ROWS source rows before writing them
-        // to the other buffer (tx is a multiple of 64; a clamped row repeats
-        // the batch's first row).
-        const Key *src = stage;
-        int src_stride = a.sx, src_plane = plane;
-        Key *dst = partner;
+        __syncthreads();  // the y pass reads rows the other waves staged
+        // ---- y pass, in place on wave-private columns ----
+        // Wave w owns columns [w cw, (w + 1) cw) of the tile (cw = tx / 4);
+        // a lane covers column lane % cw of rows lane / cw + rp i. Rows go in
+        // increasing order, and a batch reads all of its rows (and the rows s
+        // below them, not yet written this step) before writing any, so
+        // every read sees the previous step's value; only wave-level ordering
+        // is needed. A clamped row repeats the plane's last row, which no
+        // step changes (rows r + s >= sy keep their value).
+        const int cw = a.tx >> 2;
+        const int rp = 64 / cw;
+        const int col = wave * cw + (lane & (cw - 1));
+        const int ro = lane / cw;
         for (int s = 1; s < py; s <<= 1) {
-            for (int zr0 = wave; zr0 < rows; zr0 += Y_ROWS * (EXTREMA_THREADS / 64)) {
-                const Key *in[Y_ROWS];
-                Key *out[Y_ROWS];
-                int step[Y_ROWS];
-#pragma unroll
-                for (int u = 0; u < Y_ROWS; ++u) {
-                    const int zu = zr0 + u * (EXTREMA_THREADS / 64);
-                    const int zr = zu < rows ? zu : zr0;
-                    const int z = zr / a.sy;
-                    const int r = zr - z * a.sy;
-                    in[u] = src + z * src_plane + r * src_stride;
-                    out[u] = dst + z * yplane + r * a.tx;
-                    step[u] = r + s < a.sy ? s * src_stride : 0;
-                }
-                for (int c = 0; c < a.tx; c += 64) {
-                    const int i = c + lane;
+            for (int z = 0; z < a.kz; ++z) {
+                Key *base = stage + z * plane + col;
+                for (int r0 = 0; r0 < a.sy; r0 += Y_ROWS * rp) {
+                    int at[Y_ROWS];
                     Key v[Y_ROWS], w[Y_ROWS];
 #pragma unroll
                     for (int u = 0; u < Y_ROWS; ++u) {
-                        v[u] = in[u][i];
-                        w[u] = in[u][i + step[u]];
+                        const int r = min(r0 + u * rp + ro, a.sy - 1);
+                        at[u] = r * a.sx;
+                        v[u] = base[at[u]];
+                        w[u] = base[(r + s < a.sy ? r + s : r) * a.sx];
                     }
 #pragma unroll
-                    for (int u = 0; u < Y_ROWS; ++u) out[u][i] = pick_extreme<Key, MAXIMUM>(v[u], w[u]);
+                    for (int u = 0; u < Y_ROWS; ++u) base[at[u]] = pick_extreme<Key, MAXIMUM>(v[u], w[u]);
                 }
             }
-            __syncthreads();
-            // Later steps ping-pong between the partner and the staging
-            // buffer, both laid out sy x tx from here on.
-            Key *next = dst == partner ? stage : partner;
-            src = dst;
-            src_stride = a.tx;
-            src_plane = yplane;
-            dst = next;
+            wave_lds_sync();
         }
-        // ---- final y step, z fold and output ----
+        // ---- final y step, z fold and output (the same wave-private columns) ----
         const int dy = a.ky - py;
-        for (int e = threadIdx.x; e < a.ty * a.tx; e += EXTREMA_THREADS) {
-            const int r = e / a.tx;
-            const int i = e - r * a.tx;
-            const int64_t y = y0 + r, x = x0 + i;
-            Key best = lose;
-            for (int z = 0; z < a.kz; ++z) {
-                const Key *col = src + z * src_plane + r * src_stride + i;
-                best = pick_extreme<Key, MAXIMUM>(best, pick_extreme<Key, MAXIMUM>(col[0], col[dy * src_stride]));
-            }
-            if (y < a.ny && x < a.nx) {
-                output[(z0 * a.ny + y) * a.nx + x] = (best == win || best == lose) ? KT::CANONICAL_NAN : KT::unkey(best);
+        const int64_t x = x0 + col;
+        for (int r0 = 0; r0 < a.ty; r0 += Y_ROWS * rp) {
+#pragma unroll
+            for (int u = 0; u < Y_ROWS; ++u) {
+                const int r = r0 + u * rp + ro;
+                const int rc = min(r, a.ty - 1);
+                Key best = lose;
+                for (int z = 0; z < a.kz; ++z) {
+                    const Key *at = stage + z * plane + rc * a.sx + col;
+                    best = pick_extreme<Key, MAXIMUM>(best, pick_extreme<Key, MAXIMUM>(at[0], at[dy * a.sx]));
+                }
+                const int64_t y = y0 + r;
+                if (r < a.ty && y < a.ny && x < a.nx) {
+                    output[(z0 * a.ny + y) * a.nx + x] = (best == win || best == lose) ? KT::CANONICAL_NAN : KT::unkey(best);
+                }
             }
         }
     }
@@ -256,7 +247,8 @@ hipError_t launch_typed(const ExtremaArgs &args, size_t lds_bytes, hipStream_t s
 
 size_t extrema_lds_bytes(int dtype_bytes, int kz, int sy, int sx, int tx)
 {
-    return (size_t)kz * (size_t)sy * ((size_t)sx + (size_t)tx) * (size_t)dtype_bytes;
+    (void)tx;  // every pass works in place in the staging buffer
+    return (size_t)kz * (size_t)sy * (size_t)sx * (size_t)dtype_bytes;
 }
 
 int extrema_max_support_columns() { return 64 * STAGE_COLUMNS; }
