@@ -31,7 +31,10 @@
 namespace ndf {
 namespace {
 
-constexpr int EXTREMA_THREADS = 256;
+// Eight waves (two per SIMD) share one staged tile: the passes have no
+// barriers of their own, so the second wave hides the first one's LDS waits.
+constexpr int EXTREMA_THREADS = 512;
+constexpr int EXTREMA_WAVES = EXTREMA_THREADS / 64;
 
 template <typename Key, bool MAXIMUM>
 __device__ __forceinline__ Key pick_extreme(Key a, Key b)
@@ -99,7 +102,7 @@ __global__ void __launch_bounds__(EXTREMA_THREADS) k_extrema(ExtremaArgs a)
             mx[j] = x < a.sx ? a.mapx[x0 + x] : MAP_PAD;
         }
         __syncthreads();  // every wave of the previous tile is done with the buffer
-        // ---- stage the support as keys: each wave owns rows wave + 4 i ----
+        // ---- stage the support as keys: each wave owns rows wave + 8 i ----
         for (int row0 = wave; row0 < rows; row0 += STAGE_ROWS * (EXTREMA_THREADS / 64)) {
             Key raw[STAGE_ROWS][STAGE_COLUMNS];
             int64_t my[STAGE_ROWS];
@@ -171,14 +174,14 @@ __global__ void __launch_bounds__(EXTREMA_THREADS) k_extrema(ExtremaArgs a)
         }
         __syncthreads();  // the y pass reads rows the other waves staged
         // ---- y pass, in place on wave-private columns ----
-        // Wave w owns columns [w cw, (w + 1) cw) of the tile (cw = tx / 4);
+        // Wave w owns columns [w cw, (w + 1) cw) of the tile (cw = tx / 8);
         // a lane covers column lane % cw of rows lane / cw + rp i. Rows go in
         // increasing order, and a batch reads all of its rows (and the rows s
         // below them, not yet written this step) before writing any, so
         // every read sees the previous step's value; only wave-level ordering
         // is needed. A clamped row repeats the plane's last row, which no
         // step changes (rows r + s >= sy keep their value).
-        const int cw = a.tx >> 2;
+        const int cw = a.tx / EXTREMA_WAVES;
         const int rp = 64 / cw;
         const int col = wave * cw + (lane & (cw - 1));
         const int ro = lane / cw;
