@@ -373,6 +373,22 @@ def repair_bench(image: np.ndarray, steps: int, warmup: int, check: bool) -> dic
                            "unwritten_outputs": unwritten(outputs[path])}
         entry["paths_bit_equal"] = bool(np.array_equal(outputs["sparse"].view(np.uint32),
                                                        outputs["dense"].view(np.uint32)))
+        model = _core._shipped_model(image.dtype, (size, size))
+        if model is not None:
+            # The dense path with the shipped bracket, as the public call runs it.
+            from ndfiltrs_amd import bracket
+
+            plan.set_bracket(model[0], bracket.sort_len(model[1], "median"))
+            ms = plan.time_repair_device(d_in.address, d_out.address, d_snap.address, d_pos.address,
+                                         positions.size, "dense", warmup, steps)
+            d_out.fill(POISON[image.itemsize], image.itemsize)
+            plan.repair_device(d_in.address, d_out.address, d_snap.address, d_pos.address, positions.size, "dense")
+            bracketed = d_out.download(np.empty_like(image))
+            entry["dense_bracket"] = {"ms": ms, "masked_mpix_s": positions.size / (ms * 1e-3) / 1e6,
+                                      "image_mpix_s": image.size / (ms * 1e-3) / 1e6,
+                                      "unwritten_outputs": unwritten(bracketed),
+                                      "bits_equal_sparse": bool(np.array_equal(bracketed.view(np.uint32),
+                                                                               outputs["sparse"].view(np.uint32)))}
         import ndfiltrs_amd as nf
 
         calls = []
@@ -384,7 +400,8 @@ def repair_bench(image: np.ndarray, steps: int, warmup: int, check: bool) -> dic
                                "bits_equal_device_path": bool(np.array_equal(public.view(np.uint32),
                                                                              outputs["sparse"].view(np.uint32))),
                                "what": "nf.fixpix on host arrays (validation, snapshot copies, H2D of data and "
-                                       "positions, automatic path, D2H), median of 3"}
+                                       "mask, mask positions compacted on the device, automatic path with the "
+                                       "shipped bracket where one matches, D2H), median of 3"}
         if check:
             from oracle import oracle
 

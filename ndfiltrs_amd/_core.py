@@ -407,6 +407,22 @@ def _shipped_model(dtype: np.dtype, size: Sequence[int]):
     return _MODEL_WEIGHTS[name]
 
 
+def _use_shipped_bracket(plan: "Plan", typed: np.ndarray, kind: str, size, footprint, omit_nan: bool) -> None:
+    """Give a new plan the shipped bracket model when one matches: a 2-D
+    rectangular NaN-omitting window on the slide engine, at least 2^20
+    pixels. Results never depend on it (tiles it misses are recomputed)."""
+    if (footprint is None and typed.ndim == 2 and omit_nan and typed.size >= _BRACKET_MIN_PIXELS
+            and '"engine": "slide"' in plan.describe()):
+        model = _shipped_model(typed.dtype, size)
+        if model is not None:
+            from ndfiltrs_amd import bracket
+
+            try:
+                plan.set_bracket(model[0], bracket.sort_len(model[1], kind))
+            except ValueError:
+                pass
+
+
 def _cached_apply(typed: np.ndarray, kind: str, size, footprint, origin, mode, cval: float, omit_nan: bool,
                   ignore_inf: bool, min_count: int, parameter) -> np.ndarray:
     """Run one public call on a cached (or new) plan."""
@@ -418,16 +434,7 @@ def _cached_apply(typed: np.ndarray, kind: str, size, footprint, origin, mode, c
     if plan is None:
         plan = Plan(typed.dtype, typed.shape, kind, size, footprint, origin, mode, cval, omit_nan, ignore_inf,
                     min_count, parameter)
-        if (footprint is None and typed.ndim == 2 and omit_nan and typed.size >= _BRACKET_MIN_PIXELS
-                and '"engine": "slide"' in plan.describe()):
-            model = _shipped_model(typed.dtype, size)
-            if model is not None:
-                from ndfiltrs_amd import bracket
-
-                try:
-                    plan.set_bracket(model[0], bracket.sort_len(model[1], kind))
-                except ValueError:
-                    pass
+        _use_shipped_bracket(plan, typed, kind, size, footprint, omit_nan)
     output = plan.apply(typed)
     _PLANS.give(key, plan, 2 * typed.nbytes)
     return output
@@ -563,6 +570,8 @@ def fixpix_median(
     plan = _PLANS.take(key)
     if plan is None:
         plan = Plan(data.dtype, data.shape, "median", size, fp, origin, mode, cval, True, ignore_inf, min_count, 0)
+        # The dense repair runs the plan's whole-array launch, bracket included.
+        _use_shipped_bracket(plan, data, "median", size, fp, True)
     err = ctypes.create_string_buffer(256)
     # Snapshot (GIL held): the data and the mask go to the device; the mask's
     # positions are compacted there.

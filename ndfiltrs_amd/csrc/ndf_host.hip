@@ -2451,9 +2451,10 @@ namespace {
 // affects time). Sparse, per position, by the keys each lane holds: 1.0 ns
 // (<= 64 taps), 2 ns (<= 256), 4.3 ns (<= 1024), 24 ns (<= 4096), and a
 // thread per position beyond. Dense, per element: the slide engine about
-// 0.2 ns at any window size (31x31 and 63x63 alike), the network engine
-// growing with the taps W (0.023 ns at 25 taps, 0.11 ns at 68, 0.37 ns at
-// 147). Plans without a tiled engine always run sparse.
+// 0.2 ns at any window size (31x31 and 63x63 alike), 0.07 ns with a learned
+// bracket (63x63: 4.6 ms for 8192^2), the network engine growing with the
+// taps W (0.023 ns at 25 taps, 0.11 ns at 68, 0.37 ns at 147). Plans without
+// a tiled engine always run sparse.
 double repair_sparse_ns(size_t count)
 {
     return count <= 64 ? 1.0 : count <= 256 ? 2.0 : count <= 1024 ? 4.3 : count <= 4096 ? 24.0 : 1000.0;
@@ -2465,7 +2466,7 @@ double repair_dense_ns(const ndf_plan *p)
     const double scale = p->dbytes == 8 ? 1.5 : 1.0;
     if (p->engine == NDF_ENGINE_NETWORK) return scale * 0.0009 * w * (1.0 + w / 60.0);
     if (p->engine == NDF_ENGINE_STACK) return scale * 0.0009 * w * (1.0 + w / 150.0);
-    return scale * 0.2;
+    return scale * (p->bracket_on ? 0.07 : 0.2);
 }
 
 bool repair_sparse(const ndf_plan *p, size_t npos, int32_t path)

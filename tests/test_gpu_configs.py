@@ -232,3 +232,18 @@ def test_public_fixpix_reuses_plans_and_matches_oracle() -> None:
     # A mask given as an integer array takes the converting path.
     got = nf.fixpix(image, mask.astype(np.int32) * 7, method="median", size=5)
     assert_bits(got, want, "fixpix integer mask")
+
+
+def test_public_fixpix_63x63_runs_the_shipped_bracket_and_matches_oracle() -> None:
+    """fixpix(method="median", size=63) on a float32 image of at least 2^20
+    pixels gives its plan the shipped bracket model; the automatic path then
+    estimates the dense form (the whole-array launch, bracket included) below
+    the sparse one, and the result is still the oracle's bits."""
+    _core.clear_plan_cache()
+    image = synthetic.sky_quadratic_crop((1024, 1088), seed=114)
+    mask = np.random.default_rng(115).random(image.shape) < 0.02
+    got = nf.fixpix(image, mask, method="median", size=63)
+    want = oracle.fixpix_median(image, mask, (63, 63), mode="reflect")
+    assert_bits(got, want, "fixpix 63x63 bracket")
+    plans = [plan for entries in _core._PLANS._idle.values() for plan, _ in entries]
+    assert len(plans) == 1 and '"bracket": true' in plans[0].describe(), [p.describe() for p in plans]
