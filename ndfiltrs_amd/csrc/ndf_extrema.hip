@@ -11,10 +11,11 @@
 //     becomes the key that dominates the reduction (0 for a minimum, all
 //     ones for a maximum) and an omitted sample the one that loses, so the
 //     policy rides inside the reduction;
-//   * the x pass and the y pass each take the extreme over k samples by
-//     doubling: log2(k) steps of m_2s[i] = op(m_s[i], m_s[i + s]), then
-//     op(m_P[i], m_P[i + k - P]) for the largest power of two P <= k (the
-//     two windows overlap, which an idempotent op allows); every step runs
+//   * the x pass and the y pass each take the extreme over k samples in
+//     radix-4 steps: m_4w[i] = op(m_w[i], m_w[i + w], m_w[i + 2w],
+//     m_w[i + 3w]) while 4w <= k, then op over the ceil(k / w) <= 4 windows
+//     of width w at i, i + w, ..., i + k - w (they overlap, which an
+//     idempotent op allows), about log4(k) steps in all; every step runs
 //     in place, the x steps on wave-private rows and the y steps on
 //     wave-private columns, so one barrier per tile separates the passes,
 //     and each wave reads a batch of rows before writing them, so it waits
@@ -34,7 +35,7 @@ namespace {
 // Waves per workgroup: several waves per SIMD share one staged tile (the
 // passes have no barriers of their own, so they hide each other's LDS
 // waits). Windows of at least EXTREMA_WIDE_TAPS taps run 16 waves, smaller
-// ones (fewer doubling steps per staged sample) 8: 63x63 on 8192^2 float32
+// ones (fewer steps per staged sample) 8: 63x63 on 8192^2 float32
 // takes 0.76 ms at 16 against 0.95 ms at 8, 5x5 0.40 against 0.33 ms.
 constexpr int EXTREMA_WIDE_WAVES = 16;
 constexpr int EXTREMA_NARROW_WAVES = 8;
@@ -61,9 +62,32 @@ __device__ inline void wave_lds_sync()
 constexpr int STAGE_COLUMNS = 8;
 // Rows a wave stages per batch (all loads of a batch are issued first).
 constexpr int STAGE_ROWS = 4;
-// Rows a wave reads before writing, per doubling step of the x and y passes.
+// Rows a wave reads before writing, per step of the x and y passes.
 constexpr int X_ROWS = 8;
 constexpr int Y_ROWS = 8;
+
+// One radix-4 step along an axis with window k: from extremes over w
+// consecutive samples to extremes over 4w (reads at +0, w, 2w, 3w) while
+// 4w <= k; else the final combine of the n = ceil(k / w) <= 4 windows of
+// width w that cover k (reads at +0, w, ..., the last at k - w; reads past
+// the n-th repeat k - w, which an idempotent op allows).
+struct Radix {
+    int o1, o2, o3;
+    bool final;
+};
+__device__ inline Radix radix_step(int w, int k)
+{
+    if (4 * w <= k) return {w, 2 * w, 3 * w, false};
+    const int last = k - w;
+    const int n = (k + w - 1) / w;
+    return {n > 2 ? w : last, n > 3 ? 2 * w : last, last, true};
+}
+
+template <typename Key, bool MAXIMUM>
+__device__ __forceinline__ Key pick4(Key a, Key b, Key c, Key d)
+{
+    return pick_extreme<Key, MAXIMUM>(pick_extreme<Key, MAXIMUM>(a, b), pick_extreme<Key, MAXIMUM>(c, d));
+}
 
 template <typename T, bool MAXIMUM, int WAVES>
 __global__ void __launch_bounds__(WAVES * 64) k_extrema(ExtremaArgs a)
@@ -84,10 +108,6 @@ __global__ void __launch_bounds__(WAVES * 64) k_extrema(ExtremaArgs a)
         return KT::key(bits);
     };
     const Key cval_key = classify_key((Key)a.cval_bits);
-    int px = 1;
-    while (2 * px <= a.kx) px <<= 1;
-    int py = 1;
-    while (2 * py <= a.ky) py <<= 1;
     const int tiles = a.tiles_x * a.tiles_y * (int)a.nz;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int rows = a.kz * a.sy;
@@ -140,7 +160,7 @@ __global__ void __launch_bounds__(WAVES * 64) k_extrema(ExtremaArgs a)
                 }
             }
         }
-        // ---- x pass, in place: each wave doubles the rows it staged ----
+        // ---- x pass, in place: each wave reduces the rows it staged ----
         // Batches of X_ROWS rows, one 64-column chunk at a time: a wave reads
         // the chunk's values of every row of the batch before it writes any,
         // so the LDS latency is paid once per batch. Rows are wave-private
@@ -151,9 +171,9 @@ __global__ void __launch_bounds__(WAVES * 64) k_extrema(ExtremaArgs a)
         // block): a clamped lane or row repeats an in-range element's read
         // and writes back the value that element's own lane writes.
         wave_lds_sync();
-        for (int s = 1; s <= px; s <<= 1) {
-            const int shift = s < px ? s : a.kx - px;
-            const int columns = s < px ? a.sx : a.tx;
+        for (int w = 1; w < a.kx; w *= 4) {
+            const Radix st = radix_step(w, a.kx);
+            const int columns = st.final ? a.tx : a.sx;
             for (int r0 = wave; r0 < rows; r0 += X_ROWS * WAVES) {
                 Key *row[X_ROWS];
 #pragma unroll
@@ -163,53 +183,62 @@ __global__ void __launch_bounds__(WAVES * 64) k_extrema(ExtremaArgs a)
                 }
                 for (int c = 0; c < columns; c += 64) {
                     const int i = min(c + lane, columns - 1);
-                    const int iw = i + shift < a.sx ? i + shift : i;
-                    Key v[X_ROWS], w[X_ROWS];
+                    const int i1 = i + st.o1 < a.sx ? i + st.o1 : i;
+                    const int i2 = i + st.o2 < a.sx ? i + st.o2 : i;
+                    const int i3 = i + st.o3 < a.sx ? i + st.o3 : i;
+                    Key v0[X_ROWS], v1[X_ROWS], v2[X_ROWS], v3[X_ROWS];
 #pragma unroll
                     for (int u = 0; u < X_ROWS; ++u) {
-                        v[u] = row[u][i];
-                        w[u] = row[u][iw];
+                        v0[u] = row[u][i];
+                        v1[u] = row[u][i1];
+                        v2[u] = row[u][i2];
+                        v3[u] = row[u][i3];
                     }
 #pragma unroll
-                    for (int u = 0; u < X_ROWS; ++u) row[u][i] = pick_extreme<Key, MAXIMUM>(v[u], w[u]);
+                    for (int u = 0; u < X_ROWS; ++u) row[u][i] = pick4<Key, MAXIMUM>(v0[u], v1[u], v2[u], v3[u]);
                 }
             }
             wave_lds_sync();
+            if (st.final) break;
         }
         __syncthreads();  // the y pass reads rows the other waves staged
         // ---- y pass, in place on wave-private columns ----
         // Wave w owns columns [w cw, (w + 1) cw) of the tile (cw = tx / WAVES);
         // a lane covers column lane % cw of rows lane / cw + rp i. Rows go in
-        // increasing order, and a batch reads all of its rows (and the rows s
-        // below them, not yet written this step) before writing any, so
-        // every read sees the previous step's value; only wave-level ordering
-        // is needed. A clamped row repeats the plane's last row, which no
-        // step changes (rows r + s >= sy keep their value).
+        // increasing order, and a batch reads all of its rows (and the rows
+        // w, 2w, 3w below them, not yet written this step) before writing
+        // any, so every read sees the previous step's value; only wave-level
+        // ordering is needed. A read below the plane falls back to the row
+        // itself, so the plane's last row never changes, and a clamped row
+        // repeats it. The final combine happens in the output step.
         const int cw = a.tx / WAVES;
         const int rp = 64 / cw;
         const int col = wave * cw + (lane & (cw - 1));
         const int ro = lane / cw;
-        for (int s = 1; s < py; s <<= 1) {
+        int wy = 1;
+        for (; 4 * wy <= a.ky; wy *= 4) {
             for (int z = 0; z < a.kz; ++z) {
                 Key *base = stage + z * plane + col;
                 for (int r0 = 0; r0 < a.sy; r0 += Y_ROWS * rp) {
                     int at[Y_ROWS];
-                    Key v[Y_ROWS], w[Y_ROWS];
+                    Key v0[Y_ROWS], v1[Y_ROWS], v2[Y_ROWS], v3[Y_ROWS];
 #pragma unroll
                     for (int u = 0; u < Y_ROWS; ++u) {
                         const int r = min(r0 + u * rp + ro, a.sy - 1);
                         at[u] = r * a.sx;
-                        v[u] = base[at[u]];
-                        w[u] = base[(r + s < a.sy ? r + s : r) * a.sx];
+                        v0[u] = base[at[u]];
+                        v1[u] = base[(r + wy < a.sy ? r + wy : r) * a.sx];
+                        v2[u] = base[(r + 2 * wy < a.sy ? r + 2 * wy : r) * a.sx];
+                        v3[u] = base[(r + 3 * wy < a.sy ? r + 3 * wy : r) * a.sx];
                     }
 #pragma unroll
-                    for (int u = 0; u < Y_ROWS; ++u) base[at[u]] = pick_extreme<Key, MAXIMUM>(v[u], w[u]);
+                    for (int u = 0; u < Y_ROWS; ++u) base[at[u]] = pick4<Key, MAXIMUM>(v0[u], v1[u], v2[u], v3[u]);
                 }
             }
             wave_lds_sync();
         }
         // ---- final y step, z fold and output (the same wave-private columns) ----
-        const int dy = a.ky - py;
+        const Radix fy = radix_step(wy, a.ky);
         const int64_t x = x0 + col;
         for (int r0 = 0; r0 < a.ty; r0 += Y_ROWS * rp) {
 #pragma unroll
@@ -219,7 +248,8 @@ __global__ void __launch_bounds__(WAVES * 64) k_extrema(ExtremaArgs a)
                 Key best = lose;
                 for (int z = 0; z < a.kz; ++z) {
                     const Key *at = stage + z * plane + rc * a.sx + col;
-                    best = pick_extreme<Key, MAXIMUM>(best, pick_extreme<Key, MAXIMUM>(at[0], at[dy * a.sx]));
+                    best = pick_extreme<Key, MAXIMUM>(
+                        best, pick4<Key, MAXIMUM>(at[0], at[fy.o1 * a.sx], at[fy.o2 * a.sx], at[fy.o3 * a.sx]));
                 }
                 const int64_t y = y0 + r;
                 if (r < a.ty && y < a.ny && x < a.nx) {
