@@ -34,12 +34,13 @@ namespace {
 
 // Waves per workgroup: several waves per SIMD share one staged tile (the
 // passes have no barriers of their own, so they hide each other's LDS
-// waits). Windows of at least EXTREMA_WIDE_TAPS taps run 16 waves, smaller
-// ones (fewer steps per staged sample) 8: 63x63 on 8192^2 float32
-// takes 0.76 ms at 16 against 0.95 ms at 8, 5x5 0.40 against 0.33 ms.
+// waits). A tile above half of the CU's 160 KiB of LDS leaves one workgroup
+// per CU, which then runs 16 waves; smaller tiles run 8 and share the CU
+// with other workgroups (8192^2 float32: 63x63 0.68 ms at 16 waves against
+// 0.95 at 8; 15x15 0.46 ms at 8 against 0.63 at 16; 9x9 0.39 against 0.43).
 constexpr int EXTREMA_WIDE_WAVES = 16;
 constexpr int EXTREMA_NARROW_WAVES = 8;
-constexpr int64_t EXTREMA_WIDE_TAPS = 64;
+constexpr size_t EXTREMA_WIDE_LDS = 80 * 1024;
 
 template <typename Key, bool MAXIMUM>
 __device__ __forceinline__ Key pick_extreme(Key a, Key b)
@@ -284,9 +285,8 @@ hipError_t launch_waves(const ExtremaArgs &args, size_t lds_bytes, hipStream_t s
 template <typename T, bool MAXIMUM>
 hipError_t launch_typed(const ExtremaArgs &args, size_t lds_bytes, hipStream_t stream)
 {
-    const int64_t taps = (int64_t)args.kz * args.ky * args.kx;
-    return taps >= EXTREMA_WIDE_TAPS ? launch_waves<T, MAXIMUM, EXTREMA_WIDE_WAVES>(args, lds_bytes, stream)
-                                     : launch_waves<T, MAXIMUM, EXTREMA_NARROW_WAVES>(args, lds_bytes, stream);
+    return lds_bytes > EXTREMA_WIDE_LDS ? launch_waves<T, MAXIMUM, EXTREMA_WIDE_WAVES>(args, lds_bytes, stream)
+                                        : launch_waves<T, MAXIMUM, EXTREMA_NARROW_WAVES>(args, lds_bytes, stream);
 }
 
 }  // namespace

@@ -249,10 +249,12 @@ EXTREMA_CASES = [
     ((6, 50, 70), (3, 7, 7), (1, 0, -3), np.float32),
     ((5, 40, 33), (5, 4, 1), (-2, 1, 0), np.float64),
     ((2, 3, 700), (2, 3, 5), (0, 1, 2), np.float32),
-    # Images narrower than 65 columns take 64-column tiles; with 64 taps or
-    # more, 16 waves then share the y pass at 4 columns (16 rows) per wave.
+    # Images narrower than 65 columns take 64-column tiles; a tile above half
+    # of the LDS runs 16 waves, which then share the y pass at 4 columns (16
+    # rows) per wave (the float64 63x63 case), smaller ones 8 waves.
     ((200, 60), (15, 15), (3, -2), np.float32),
     ((90, 50), (9, 9), (0, 0), np.float64),
+    ((150, 60), (63, 63), (2, -5), np.float64),
     ((4, 70, 40), (2, 9, 5), (0, 1, -1), np.float32),
 ]
 
